@@ -1,0 +1,141 @@
+/*
+ * mumag_hip.h -- C-ABI of the MI355X-native micromagnetics hot path.
+ *
+ * One shared library, libmumag_hip.so, loaded with ctypes. Every entry point returns an
+ * int status: 0 on success, a negative value on error; mm_last_error() returns the
+ * message. All pointers are caller-owned host arrays (float64 / int32, C-contiguous)
+ * unless noted. A context owns its device buffers and one HIP stream; calls on one
+ * context are serialised by the caller, distinct contexts may be used concurrently.
+ *
+ * Units and meaning follow the reference: lengths in mesh units (nm), material constants
+ * already reduced (A_red, K1_red, Js_red, src/loop.py:862-877), fields B in units of
+ * Js_ref, energies and gradients divided by the magnetic volume V_mag.
+ */
+#ifndef MUMAG_HIP_H
+#define MUMAG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Minimiser parameters: the ``params`` dict of ``minimize(m0, B_ext, **params)``
+ * (src/minimizers.py:3437-3452) and the LoopParams fields it reads
+ * (src/hysteresis_loop.py:34-125). method: 0 = pcohen_hs, 1 = lbfgs. */
+typedef struct mm_params {
+  int method;
+  int max_iter;
+  double tau_f, eps_a, tau0;
+  double ls_eta1, ls_eta2, ls_C, ls_c;
+  int ls_max_evals;
+  int pc_iters, pc_auto;
+  double pc_force_eta, pc_force_alpha, pc_stagnation_nu, pc_reg;
+  int L;
+  double cg_tol, phi_tol;
+  int cg_maxiter;
+  int memory;
+} mm_params;
+
+/* Per-field-step statistics: the ``info`` dict returned by ``minimize``
+ * (src/minimizers.py:4084-4096) and by ``cpp_minimize`` (src/cpp_minimizer.py:181-189). */
+typedef struct mm_info {
+  int iters, evals, demag_iters, preco_iters, converged;
+  double E, gnorm, time;
+} mm_info;
+
+/* Last error message of the calling process. Returns the message length. */
+int mm_last_error(char* buf, int len);
+
+/* Library / device description ("gfx950 ..."). */
+int mm_device_info(char* buf, int len);
+
+/* Context creation: replaces the host set-up of src/loop.py:856-939 (compute_volume_JinvT,
+ * TetGeom, M_nodal) and the operator construction of src/loop.py:1089-1176.
+ * knt (N,3); conn (E,4) 0-based; mat_id (E,) 1-based; A_red, K1_red, Js_red (G,);
+ * k_easy (G,3); bnd_mask (N,) 1.0 interior / 0.0 Dirichlet, or NULL (no Dirichlet nodes);
+ * reg: Poisson diagonal regularisation (src/amg_utils.py:73).
+ * flags bit 0..1: geometry source of the Poisson apply (0 on-the-fly from coordinates,
+ * 1 stored FP64 gradients, 2 stored FP32 gradients). */
+int mm_ctx_create(int N, int E, const double* knt, const int32_t* conn, const int32_t* mat_id, int G,
+                  const double* A_red, const double* K1_red, const double* Js_red, const double* k_easy,
+                  const double* bnd_mask, double reg, int flags, int64_t* out_ctx);
+int mm_ctx_destroy(int64_t ctx);
+
+/* info[0..7] = V_mag, N, E, element blocks (all), element blocks (magnetic),
+ * largest node list of a block, magnetic elements, 0 */
+int mm_ctx_info(int64_t ctx, double* info);
+
+/* Oriented connectivity (E,4), Ve (E,) and shape-function gradients of corners 1..3 (E,3,3)
+ * in the context's internal element order (src/loop.py:56-102). */
+int mm_ctx_get_geometry(int64_t ctx, int32_t* conn, double* Ve, double* grad);
+
+/* M_nodal, inv_M_rel and inv_M_prec (N,) (src/loop.py:929-939, src/hysteresis_loop.py:340-353). */
+int mm_ctx_get_nodal(int64_t ctx, double* M_nodal, double* inv_M_rel, double* inv_M_prec);
+
+/* Select the geometry source of the Poisson apply (see flags of mm_ctx_create). */
+int mm_set_geometry_mode(int64_t ctx, int mode);
+
+/* y = A x: assembled-mode apply_A (src/poisson_solve.py:166-171) computed matrix-free. */
+int mm_poisson_apply(int64_t ctx, const double* x, double* y);
+/* b = D m: rhs_from_m (src/poisson_solve.py:173-186, 254-280); m (N,3). */
+int mm_poisson_rhs(int64_t ctx, const double* m, double* b);
+/* diag(A): assemble_diag (src/poisson_solve.py:282-299) of the masked, regularised A. */
+int mm_poisson_diag(int64_t ctx, double* d);
+
+/* Preconditioner set-up. Level 0 uses the matrix-free apply; levels l >= 1 carry CSR A.
+ * mm_amg_set_level(l): n rows; M (n,) smoother multiplier (SPAI0 diagonal for amgcl,
+ * 0.6667/diag for amg); P (n x n_next) and R (n_next x n) in CSR (NULL on the last level);
+ * A in CSR for l >= 1 (NULL for l == 0); dense_inv (n,n) row-major on the last level.
+ * Replaces the hierarchy transfer of src/poisson_solve.py:600-639. */
+int mm_amg_set_level(int64_t ctx, int l, int n, int n_next, const int32_t* Ap, const int32_t* Ai, const double* Ax,
+                     const double* M, const int32_t* Pp, const int32_t* Pi, const double* Px, const int32_t* Rp,
+                     const int32_t* Ri, const double* Rx, const double* dense_inv);
+/* kind 1 = amgcl (SPAI0), 2 = amg (damped Jacobi); nlev levels set above. */
+int mm_amg_commit(int64_t ctx, int kind, int nlev);
+/* Jacobi preconditioner on diag(A) (precond_type 'jacobi', src/poisson_solve.py:409-421). */
+int mm_precond_jacobi(int64_t ctx);
+
+/* One preconditioner application z = M^-1 r (src/amg_utils.py:557-623). */
+int mm_precond_apply(int64_t ctx, const double* r, double* z);
+
+/* solve_U (src/poisson_solve.py:748-771): U_inout holds x0 on entry. */
+int mm_solve_U(int64_t ctx, const double* m, double* U_inout, double tol, int maxiter, int* it, double* rel_res);
+
+/* energy_and_grad (src/energy_kernels.py:340-459): E scalar, g (N,3). */
+int mm_energy_grad(int64_t ctx, const double* m, const double* U, const double* B, double* E, double* g);
+/* local_grad_only (src/energy_kernels.py:272-338): g = (K_ex + K_an) v / V_mag. */
+int mm_local_grad(int64_t ctx, const double* v, double* g);
+
+/* Volume-averaged polarisation along h (src/hysteresis_loop.py:153-190), reduced units. */
+int mm_j_parallel(int64_t ctx, const double* m, const double* h, double* J);
+
+/* Device-resident state for a field sweep: upload / download m (N,3) and U (N,). */
+int mm_state_set(int64_t ctx, const double* m, const double* U);
+int mm_state_get(int64_t ctx, double* m, double* U);
+/* Minimise the resident state at field B (one field step, src/minimizers.py:3402-3452). */
+int mm_minimize_resident(int64_t ctx, const mm_params* p, const double* B, mm_info* info);
+/* mm_state_set + mm_minimize_resident + mm_state_get; the role of
+ * run_cpp_pcohen_hs_minimization (src/cpp/cpp_mkl_minimizer.cpp:542-763). */
+int mm_minimize(int64_t ctx, const mm_params* p, double* m_inout, double* U_inout, const double* B, mm_info* info);
+/* J_parallel of the resident state. */
+int mm_state_j_parallel(int64_t ctx, const double* h, double* J);
+
+/* Host-side aggregation for the smoothed-aggregation AMG set-up (the role of
+ * pyamg.smoothed_aggregation_solver's aggregation step, src/poisson_solve.py:596-598):
+ * S is the strength graph in CSR (n rows); agg[i] receives the aggregate of node i or -1 for
+ * an isolated node; n_agg the number of aggregates. No device work. */
+int mm_sa_aggregate(int n, const int32_t* Sp, const int32_t* Sj, int32_t* agg, int32_t* n_agg);
+
+/* Counters of kernel launches since creation: applies, rhs, energy, local, V-cycles, PCG iterations. */
+int mm_counters(int64_t ctx, int64_t* out6);
+
+/* Benchmark hook: `reps` back-to-back Poisson applies on device-resident vectors, timed with
+ * HIP events on the context's stream; returns the mean milliseconds per apply. */
+int mm_bench_apply(int64_t ctx, int geom, int reps, double* ms_per_apply);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MUMAG_HIP_H */

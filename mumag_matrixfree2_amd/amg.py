@@ -1,0 +1,151 @@
+"""Host set-up of the smoothed-aggregation AMG hierarchy used as the PCG preconditioner.
+
+The reference builds it with ``pyamg.smoothed_aggregation_solver(A, strength='evolution',
+smooth='energy', max_levels=10, max_coarse=1000)`` on the masked, regularised Poisson
+matrix (``src/poisson_solve.py:570-639``). pyamg is not available here, so this module
+builds a classical SA hierarchy itself:
+
+* symmetric strength of connection |a_ij| >= theta sqrt(|a_ii a_jj|) (theta = 0 keeps every
+  off-diagonal entry), Dirichlet identity rows are isolated and never aggregated;
+* greedy three-pass aggregation (``mm_sa_aggregate`` in the library, host code);
+* tentative prolongator with normalised constant columns, smoothed once with damped
+  Jacobi, omega = 4/3 / rho(D^-1 A);
+* R = P^T, A_c = R A P, down to at most ``max_coarse`` rows or ``max_levels`` levels;
+* per-level smoother multiplier: SPAI0 diagonal a_ii / sum_j a_ij^2 for 'amgcl'
+  (``src/amg_utils.py:87-111``), 0.6667 / a_ii for 'amg' (``src/amg_utils.py:377-404``);
+* the coarsest level is inverted once (dense), instead of the reference's per-call
+  ``jnp.linalg.solve`` (``src/amg_utils.py:577-578``): the same solution.
+
+Level 0 is applied matrix-free on the GPU; only its P, R and smoother diagonal are uploaded.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from ._hip import call, f64, i32, ptr
+
+
+def fine_matrix(ctx) -> sp.csr_matrix:
+    """Assembled Poisson matrix of a context: Ve grad phi_a . grad phi_b, Dirichlet rows and
+    columns removed with unit diagonal, reg on the interior diagonal (``src/amg_utils.py:21-84``)."""
+    conn, Ve, grad = ctx.geometry()
+    N = ctx.N
+    g = np.empty((conn.shape[0], 4, 3))
+    g[:, 1:, :] = grad
+    g[:, 0, :] = -grad.sum(axis=1)
+    Ke = Ve[:, None, None] * np.einsum("eai,ebi->eab", g, g)
+    c64 = conn.astype(np.int64)
+    rows = np.repeat(c64, 4, axis=1).ravel()
+    cols = np.tile(c64, (1, 4)).ravel()
+    A = sp.csr_matrix((Ke.ravel(), (rows, cols)), shape=(N, N))
+    A.sum_duplicates()
+    diag_extra = np.full(N, ctx.reg)
+    if ctx.boundary_mask is not None:
+        b = ctx.boundary_mask == 0
+        coo = A.tocoo()
+        keep = ~(b[coo.row] | b[coo.col])
+        A = sp.csr_matrix((coo.data[keep], (coo.row[keep], coo.col[keep])), shape=(N, N))
+        diag_extra[b] = 1.0
+    A = (A + sp.diags(diag_extra)).tocsr()
+    A.sort_indices()
+    return A
+
+
+def strength(A: sp.csr_matrix, theta: float = 0.0) -> sp.csr_matrix:
+    """Symmetric strength graph without the diagonal."""
+    C_ = A.tocoo()
+    d = np.abs(A.diagonal())
+    off = C_.row != C_.col
+    keep = off & (C_.data != 0)
+    if theta > 0:
+        keep &= np.abs(C_.data) >= theta * np.sqrt(d[C_.row] * d[C_.col])
+    S = sp.csr_matrix((np.ones(int(keep.sum())), (C_.row[keep], C_.col[keep])), shape=A.shape)
+    S.sort_indices()
+    return S
+
+
+def aggregate(S: sp.csr_matrix) -> tuple[np.ndarray, int]:
+    Sp = i32(S.indptr)
+    Sj = i32(S.indices)
+    agg = np.empty(S.shape[0], dtype=np.int32)
+    na = C.c_int32(0)
+    call("mm_sa_aggregate", S.shape[0], ptr(Sp), ptr(Sj), ptr(agg), C.byref(na))
+    return agg, int(na.value)
+
+
+def spectral_radius_DinvA(A: sp.csr_matrix) -> float:
+    d = A.diagonal()
+    s = 1.0 / np.sqrt(np.where(d > 0, d, 1.0))
+    B = sp.diags(s) @ A @ sp.diags(s)
+    n = A.shape[0]
+    if n < 50:
+        return float(np.max(np.abs(np.linalg.eigvalsh(B.toarray()))))
+    v0 = np.random.default_rng(0).standard_normal(n)
+    lam = spla.eigsh(B, k=1, which="LA", tol=1e-3, maxiter=200, v0=v0, return_eigenvectors=False)
+    return float(lam[0])
+
+
+def build_hierarchy(A: sp.csr_matrix, smoother: str = "amgcl", max_levels: int = 10, max_coarse: int = 1000,
+                    theta: float = 0.0, omega: float = 4.0 / 3.0) -> list[dict]:
+    """List of level dicts: ``A`` (CSR), ``M`` (smoother multiplier), ``P``, ``R`` (all but last),
+    ``A_dense_inv`` (last)."""
+    levels = []
+    A = A.tocsr()
+    while A.shape[0] > max_coarse and len(levels) < max_levels - 1:
+        S = strength(A, theta)
+        agg, na = aggregate(S)
+        if na == 0 or na >= A.shape[0]:
+            break
+        rows = np.where(agg >= 0)[0]
+        cnt = np.bincount(agg[rows], minlength=na).astype(np.float64)
+        T = sp.csr_matrix((1.0 / np.sqrt(cnt[agg[rows]]), (rows, agg[rows])), shape=(A.shape[0], na))
+        d = A.diagonal()
+        Dinv = 1.0 / np.where(d != 0, d, 1.0)
+        rho = spectral_radius_DinvA(A)
+        P = (T - (omega / rho) * (sp.diags(Dinv) @ (A @ T))).tocsr()
+        P.eliminate_zeros()
+        P.sort_indices()
+        R = P.T.tocsr()
+        R.sort_indices()
+        levels.append({"A": A, "P": P, "R": R})
+        A = (R @ A @ P).tocsr()
+        A.sort_indices()
+    levels.append({"A": A})
+    for lv in levels:
+        Al = lv["A"]
+        if smoother == "amgcl":
+            A2 = sp.csr_matrix((Al.data**2, Al.indices, Al.indptr), shape=Al.shape)
+            lv["M"] = Al.diagonal() / (np.asarray(A2.sum(axis=1)).ravel() + 1e-30)
+        else:
+            lv["M"] = 0.6667 / (Al.diagonal() + 1e-30)
+    levels[-1]["A_dense_inv"] = np.linalg.inv(levels[-1]["A"].toarray())
+    return levels
+
+
+def upload_hierarchy(ctx, levels: list[dict], kind: str) -> None:
+    """Copy the hierarchy to the device (level 0 keeps its matrix-free operator)."""
+    n_lev = len(levels)
+    for l, lv in enumerate(levels):
+        A = lv["A"]
+        n = A.shape[0]
+        last = l == n_lev - 1
+        Ap = Ai = Ax = None
+        if l > 0:
+            Ap, Ai, Ax = i32(A.indptr), i32(A.indices), f64(A.data)
+        P = lv.get("P")
+        R = lv.get("R")
+        n_next = 0 if last else P.shape[1]
+        Pp = Pi = Px = Rp = Ri = Rx = None
+        if not last:
+            Pp, Pi, Px = i32(P.indptr), i32(P.indices), f64(P.data)
+            Rp, Ri, Rx = i32(R.indptr), i32(R.indices), f64(R.data)
+        dinv = f64(lv["A_dense_inv"]) if last else None
+        M = f64(lv["M"])
+        call("mm_amg_set_level", ctx.handle, l, n, n_next, ptr(Ap), ptr(Ai), ptr(Ax), ptr(M), ptr(Pp), ptr(Pi),
+             ptr(Px), ptr(Rp), ptr(Ri), ptr(Rx), ptr(dinv))
+    call("mm_amg_commit", ctx.handle, 1 if kind == "amgcl" else 2, n_lev)

@@ -1,0 +1,246 @@
+"""``MeshContext``: one mesh + material set resident on the GPU, behind the C-ABI.
+
+It owns the device context (element-block partitions, node coordinates, nodal
+scalings, AMG hierarchy, minimiser state) and exposes thin NumPy-in / NumPy-out
+wrappers. The reference-shaped factories (``make_poisson_ops``, ``make_solve_U``,
+``make_energy_kernels``, ``make_minimizer``) are built on it.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _hip
+from ._hip import call, f64, i32, ptr
+
+GEOM_MODES = {"on_the_fly": 0, "stored_f64": 1, "stored_f32": 2}
+METHODS = {"pcohen_hs": 0, "lbfgs": 1}
+
+
+class MeshContext:
+    """Device-resident mesh, materials and solver state.
+
+    Args:
+        knt: node coordinates (N, 3) in mesh units.
+        conn: connectivity (E, 4), 0-based.
+        mat_id: material ids (E,), 1-based.
+        A_red, K1_red, Js_red: reduced material constants (G,).
+        k_easy: easy axes (G, 3).
+        boundary_mask: (N,) 1.0 interior / 0.0 Dirichlet, or None.
+        reg: Poisson diagonal regularisation.
+        geom: geometry source of the Poisson apply ('on_the_fly', 'stored_f64', 'stored_f32').
+    """
+
+    def __init__(self, knt, conn, mat_id, A_red, K1_red, Js_red, k_easy, boundary_mask=None, reg=1e-12,
+                 geom="on_the_fly"):
+        self.knt = f64(knt).reshape(-1, 3)
+        conn = i32(np.asarray(conn)[:, :4])
+        mat_id = i32(mat_id)
+        self.N = self.knt.shape[0]
+        self.E = conn.shape[0]
+        A_red, K1_red, Js_red = f64(A_red), f64(K1_red), f64(Js_red)
+        k_easy = f64(np.asarray(k_easy, dtype=np.float64).reshape(-1, 3))
+        G = A_red.size
+        if not (K1_red.size == G and Js_red.size == G and k_easy.shape[0] == G):
+            raise ValueError("material tables must all have G entries")
+        if conn.size and (conn.min() < 0 or conn.max() >= self.N):
+            raise ValueError("connectivity out of range")
+        bnd = None if boundary_mask is None else f64(boundary_mask)
+        if bnd is not None and bnd.size != self.N:
+            raise ValueError("boundary_mask must have N entries")
+        self.boundary_mask = bnd
+        self.reg = float(reg)
+        self.Js_red = Js_red
+        h = C.c_int64(0)
+        call("mm_ctx_create", self.N, self.E, ptr(self.knt), ptr(conn), ptr(mat_id), G, ptr(A_red), ptr(K1_red),
+             ptr(Js_red), ptr(k_easy), ptr(bnd), self.reg, GEOM_MODES[geom], C.byref(h))
+        self.handle = h.value
+        info = np.zeros(8)
+        call("mm_ctx_info", self.handle, ptr(info))
+        self.V_mag = float(info[0])
+        self.n_blocks_all = int(info[3])
+        self.n_blocks_mag = int(info[4])
+        self.max_block_nodes = int(info[5])
+        self.n_mag_elems = int(info[6])
+        self.precond_type = None
+        self.amg_levels = None
+
+    # ------------------------------------------------------------------ lifetime
+    def close(self):
+        if getattr(self, "handle", 0):
+            _hip.load().mm_ctx_destroy(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ set-up data
+    def geometry(self):
+        """(conn (E,4), Ve (E,), grad (E,3,3)) in the context's element order."""
+        conn = np.zeros((self.E, 4), dtype=np.int32)
+        Ve = np.zeros(self.E)
+        grad = np.zeros((self.E, 3, 3))
+        call("mm_ctx_get_geometry", self.handle, ptr(conn), ptr(Ve), ptr(grad))
+        return conn, Ve, grad
+
+    def nodal(self):
+        """(M_nodal, inv_M_rel, inv_M_prec), each (N,)."""
+        M = np.zeros(self.N)
+        a = np.zeros(self.N)
+        b = np.zeros(self.N)
+        call("mm_ctx_get_nodal", self.handle, ptr(M), ptr(a), ptr(b))
+        return M, a, b
+
+    def set_geometry_mode(self, geom: str):
+        call("mm_set_geometry_mode", self.handle, GEOM_MODES[geom])
+
+    # ------------------------------------------------------------------ operators
+    def apply_A(self, x):
+        x = f64(x)
+        y = np.empty(self.N)
+        call("mm_poisson_apply", self.handle, ptr(x), ptr(y))
+        return y
+
+    def rhs(self, m):
+        m = f64(m).reshape(-1)
+        b = np.empty(self.N)
+        call("mm_poisson_rhs", self.handle, ptr(m), ptr(b))
+        return b
+
+    def diag(self):
+        d = np.empty(self.N)
+        call("mm_poisson_diag", self.handle, ptr(d))
+        return d
+
+    def energy_grad(self, m, U, B):
+        m = f64(m).reshape(-1)
+        U = f64(U)
+        B = f64(B).reshape(3)
+        E = np.zeros(1)
+        g = np.empty(3 * self.N)
+        call("mm_energy_grad", self.handle, ptr(m), ptr(U), ptr(B), ptr(E), ptr(g))
+        return float(E[0]), g.reshape(-1, 3)
+
+    def local_grad(self, v):
+        v = f64(v).reshape(-1)
+        g = np.empty(3 * self.N)
+        call("mm_local_grad", self.handle, ptr(v), ptr(g))
+        return g.reshape(-1, 3)
+
+    def j_parallel(self, m, h):
+        m = f64(m).reshape(-1)
+        h = f64(h).reshape(3)
+        J = np.zeros(1)
+        call("mm_j_parallel", self.handle, ptr(m), ptr(h), ptr(J))
+        return float(J[0])
+
+    # ------------------------------------------------------------------ preconditioner
+    def setup_precond(self, precond_type="amgcl", **amg_kwargs):
+        """'jacobi', 'amgcl' (SPAI0 V-cycle) or 'amg' (damped-Jacobi V-cycle) on a host SA hierarchy."""
+        if precond_type in ("jacobi", "none"):
+            call("mm_precond_jacobi", self.handle)
+            self.precond_type = "jacobi"
+            return
+        from .amg import build_hierarchy, fine_matrix, upload_hierarchy
+
+        A = fine_matrix(self)
+        levels = build_hierarchy(A, smoother=precond_type, **amg_kwargs)
+        upload_hierarchy(self, levels, precond_type)
+        self.precond_type = precond_type
+        self.amg_levels = levels
+
+    def precond_apply(self, r):
+        r = f64(r)
+        z = np.empty(self.N)
+        call("mm_precond_apply", self.handle, ptr(r), ptr(z))
+        return z
+
+    def solve_U(self, m, x0, tol, maxiter=2000):
+        """(U, iterations, relative residual)."""
+        m = f64(m).reshape(-1)
+        U = f64(np.array(x0, dtype=np.float64, copy=True))
+        it = C.c_int32(0)
+        rr = C.c_double(0.0)
+        call("mm_solve_U", self.handle, ptr(m), ptr(U), float(tol), int(maxiter), C.byref(it), C.byref(rr))
+        return U, int(it.value), float(rr.value)
+
+    # ------------------------------------------------------------------ minimiser
+    def state_set(self, m, U=None):
+        m = f64(m).reshape(-1)
+        call("mm_state_set", self.handle, ptr(m), ptr(None if U is None else f64(U)))
+
+    def state_get(self):
+        m = np.empty(3 * self.N)
+        U = np.empty(self.N)
+        call("mm_state_get", self.handle, ptr(m), ptr(U))
+        return m.reshape(-1, 3), U
+
+    def minimize_resident(self, params: "_hip.MMParams", B):
+        B = f64(B).reshape(3)
+        info = _hip.MMInfo()
+        call("mm_minimize_resident", self.handle, C.byref(params), ptr(B), C.byref(info))
+        return {
+            "iters": info.iters,
+            "evals": info.evals,
+            "demag_iters": info.demag_iters,
+            "preco_iters": info.preco_iters,
+            "converged": bool(info.converged),
+            "E": info.E,
+            "gnorm": info.gnorm,
+            "time": info.time,
+        }
+
+    def state_j_parallel(self, h):
+        h = f64(h).reshape(3)
+        J = np.zeros(1)
+        call("mm_state_j_parallel", self.handle, ptr(h), ptr(J))
+        return float(J[0])
+
+    def counters(self):
+        out = np.zeros(6, dtype=np.int64)
+        call("mm_counters", self.handle, ptr(out))
+        return dict(zip(("apply", "rhs", "energy", "local", "vcycle", "pcg_iters"), out.tolist()))
+
+    def bench_apply(self, geom="on_the_fly", reps=50):
+        ms = C.c_double(0.0)
+        call("mm_bench_apply", self.handle, GEOM_MODES[geom], int(reps), C.byref(ms))
+        return float(ms.value)
+
+
+def make_params(method="pcohen_hs", *, max_iter=2000, tau_f=1e-8, eps_a=1e-12, tau0=1e-2, ls_eta1=0.1,
+                ls_eta2=0.1, ls_C=2.0, ls_c=0.5, ls_max_evals=15, pc_iters=10, pc_auto=True, pc_force_eta=0.5,
+                pc_force_alpha=0.5, pc_stagnation_nu=0.01, pc_reg=0.0, L=None, cg_tol=1e-8, phi_tol=None,
+                cg_maxiter=2000, memory=10, **_ignored) -> "_hip.MMParams":
+    """C-ABI parameter struct with the reference's defaults (src/loop.py:528-700, src/minimizers.py:3447-3452).
+
+    ``phi_tol`` defaults to min(cg_tol, 0.1 tau_f); the line search uses at most 15
+    evaluations per phase (src/minimizers.py:1035,1335).
+    """
+    p = _hip.MMParams()
+    p.method = METHODS[method]
+    p.max_iter = int(max_iter)
+    p.tau_f = float(tau_f)
+    p.eps_a = float(eps_a)
+    p.tau0 = float(tau0)
+    p.ls_eta1 = float(ls_eta1)
+    p.ls_eta2 = float(ls_eta2)
+    p.ls_C = float(ls_C)
+    p.ls_c = float(ls_c)
+    p.ls_max_evals = int(ls_max_evals)
+    p.pc_iters = int(pc_iters)
+    p.pc_auto = int(bool(pc_auto))
+    p.pc_force_eta = float(pc_force_eta)
+    p.pc_force_alpha = float(pc_force_alpha)
+    p.pc_stagnation_nu = float(pc_stagnation_nu)
+    p.pc_reg = float(pc_reg)
+    p.L = int(L or 0)
+    p.cg_tol = float(cg_tol)
+    p.phi_tol = float(min(cg_tol, 0.1 * tau_f) if phi_tol is None else phi_tol)
+    p.cg_maxiter = int(cg_maxiter)
+    p.memory = int(memory)
+    return p

@@ -1,0 +1,800 @@
+// C-ABI of the MI355X micromagnetics library: context set-up (orientation, element
+// ordering, block partition, device uploads) and the operator / solver entry points.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+
+#include "mm_device.h"
+#include "mm_solver.h"
+#include "../../include/mumag_hip.h"
+
+namespace mm {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+static std::mutex g_mu;
+static std::vector<Ctx*> g_ctx;  // index + 1 is the public handle
+
+static Ctx* get_ctx(int64_t h) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (h <= 0 || h > (int64_t)g_ctx.size() || g_ctx[h - 1] == nullptr) return nullptr;
+  return g_ctx[h - 1];
+}
+
+template <typename T>
+static int dalloc(Ctx& c, T** p, size_t n) {
+  if (n == 0) n = 1;
+  void* q = nullptr;
+  MM_HIP(hipMalloc(&q, n * sizeof(T)));
+  c.allocs.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+template <typename T>
+static int upload(Ctx& c, T** p, const T* h, size_t n) {
+  MM_TRY(dalloc(c, p, n));
+  if (n) MM_HIP(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Host-side description of a partition before upload.
+struct HostPart {
+  std::vector<int> eptr, nptr;
+  std::vector<uint32_t> nodes, lconn;
+  std::vector<uint8_t> mat;
+  int max_nodes = 0;
+};
+
+// Greedy cut of the ordered element list `elems` into blocks of consecutive elements that
+// touch at most kMaxBlockNodes distinct nodes; node lists are sorted; flags mark the first
+// block touching a node (owner), nodes touched by one block only (private) and Dirichlet nodes.
+static void build_partition(int N, const std::vector<int>& conn, const std::vector<int>& mat0,
+                            const std::vector<int>& elems, const std::vector<char>& is_bnd, HostPart& hp) {
+  std::vector<int> stamp(N, -1), local(N, 0), touches(N, 0);
+  std::vector<char> owned(N, 0);
+  hp.eptr.assign(1, 0);
+  hp.nptr.assign(1, 0);
+  hp.nodes.clear();
+  hp.lconn.clear();
+  hp.mat.clear();
+  hp.lconn.reserve(elems.size());
+  hp.mat.reserve(elems.size());
+  std::vector<int> blk_nodes;
+  int blk = 0;
+  size_t blk_e0 = 0;
+  auto close_block = [&]() {
+    // sort node list, remap local indices of the block's elements
+    const int nb = (int)blk_nodes.size();
+    std::vector<int> order(nb);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return blk_nodes[a] < blk_nodes[b]; });
+    std::vector<int> remap(nb);
+    for (int i = 0; i < nb; ++i) remap[order[i]] = i;
+    for (size_t e = blk_e0; e < hp.lconn.size(); ++e) {
+      uint32_t lc = hp.lconn[e], out = 0;
+      for (int a = 0; a < 4; ++a) out |= (uint32_t)remap[(lc >> (8 * a)) & 255u] << (8 * a);
+      hp.lconn[e] = out;
+    }
+    for (int i = 0; i < nb; ++i) {
+      const int v = blk_nodes[order[i]];
+      uint32_t f = (uint32_t)v;
+      if (!owned[v]) {
+        owned[v] = 1;
+        f |= kOwner;
+      }
+      if (is_bnd[v]) f |= kBoundary;
+      touches[v]++;
+      hp.nodes.push_back(f);
+    }
+    hp.max_nodes = std::max(hp.max_nodes, nb);
+    hp.eptr.push_back((int)hp.lconn.size());
+    hp.nptr.push_back((int)hp.nodes.size());
+    blk_nodes.clear();
+    blk_e0 = hp.lconn.size();
+    ++blk;
+  };
+  for (size_t k = 0; k < elems.size(); ++k) {
+    const int e = elems[k];
+    int fresh = 0;
+    for (int a = 0; a < 4; ++a) {
+      const int v = conn[4 * e + a];
+      bool dup = false;
+      for (int b2 = 0; b2 < a; ++b2) dup |= conn[4 * e + b2] == v;
+      if (!dup && stamp[v] != blk) ++fresh;
+    }
+    if ((int)blk_nodes.size() + fresh > kMaxBlockNodes || (int)(hp.lconn.size() - blk_e0) >= kMaxBlockElems)
+      close_block();
+    uint32_t lc = 0;
+    for (int a = 0; a < 4; ++a) {
+      const int v = conn[4 * e + a];
+      if (stamp[v] != blk) {
+        stamp[v] = blk;
+        local[v] = (int)blk_nodes.size();
+        blk_nodes.push_back(v);
+      }
+      lc |= (uint32_t)local[v] << (8 * a);
+    }
+    hp.lconn.push_back(lc);
+    hp.mat.push_back((uint8_t)mat0[e]);
+  }
+  if (!blk_nodes.empty()) close_block();
+  for (auto& f : hp.nodes)
+    if (touches[f & kIdMask] == 1) f |= kPrivate;
+}
+
+static int upload_part(Ctx& c, const HostPart& hp, int E, Part& p) {
+  p.nblk = (int)hp.eptr.size() - 1;
+  p.E = E;
+  p.Es = ((E + 63) / 64) * 64;
+  int *ep, *np;
+  uint32_t *nodes, *lconn;
+  uint8_t* mat;
+  MM_TRY(upload(c, &ep, hp.eptr.data(), hp.eptr.size()));
+  MM_TRY(upload(c, &np, hp.nptr.data(), hp.nptr.size()));
+  MM_TRY(upload(c, &nodes, hp.nodes.data(), hp.nodes.size()));
+  MM_TRY(upload(c, &lconn, hp.lconn.data(), hp.lconn.size()));
+  MM_TRY(upload(c, &mat, hp.mat.data(), hp.mat.size()));
+  p.eptr = ep;
+  p.nptr = np;
+  p.nodes = nodes;
+  p.lconn = lconn;
+  p.mat = mat;
+  return 0;
+}
+
+static int create(int N, int E, const double* knt, const int32_t* conn_in, const int32_t* mat_id, int G,
+                  const double* A_red, const double* K1_red, const double* Js_red, const double* k_easy,
+                  const double* bnd_mask, double reg, int flags, Ctx& c) {
+  MM_CHECK(N > 0 && E > 0, "N and E must be positive");
+  MM_CHECK(N < (int)kIdMask, "too many nodes");
+  MM_CHECK(G >= 1 && G <= kMaxMat, "material count must be in [1, 64]");
+  MM_CHECK(knt && conn_in && mat_id && A_red && K1_red && Js_red && k_easy, "null input array");
+  c.N = N;
+  c.E = E;
+  c.G = G;
+  c.reg = reg;
+  c.geom_mode = flags & 3;
+  MM_CHECK(c.geom_mode <= 2, "invalid geometry mode");
+  MM_HIP(hipGetDevice(&c.device));
+  MM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  MM_HIP(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming));
+
+  // orientation fix (src/loop.py:70-85): swap corners 1 and 2 where det J < 0
+  std::vector<int> conn(conn_in, conn_in + 4 * (size_t)E);
+  std::vector<int> mat0(E);
+  for (int e = 0; e < E; ++e) {
+    for (int a = 0; a < 4; ++a)
+      MM_CHECK(conn[4 * e + a] >= 0 && conn[4 * e + a] < N, "connectivity index out of range");
+    MM_CHECK(mat_id[e] >= 1 && mat_id[e] <= G, "material id out of range [1, G]");
+    mat0[e] = mat_id[e] - 1;
+    const double* x0 = knt + 3 * (size_t)conn[4 * e];
+    double cc[3][3];
+    for (int b = 0; b < 3; ++b)
+      for (int k = 0; k < 3; ++k) cc[b][k] = knt[3 * (size_t)conn[4 * e + b + 1] + k] - x0[k];
+    const double det = cc[0][0] * (cc[1][1] * cc[2][2] - cc[1][2] * cc[2][1]) -
+                       cc[0][1] * (cc[1][0] * cc[2][2] - cc[1][2] * cc[2][0]) +
+                       cc[0][2] * (cc[1][0] * cc[2][1] - cc[1][1] * cc[2][0]);
+    if (det < 0) std::swap(conn[4 * e + 1], conn[4 * e + 2]);
+  }
+  // element order: by (smallest node id, largest node id), stable
+  std::vector<long long> key(E);
+  for (int e = 0; e < E; ++e) {
+    int lo = conn[4 * e], hi = conn[4 * e];
+    for (int a = 1; a < 4; ++a) {
+      lo = std::min(lo, conn[4 * e + a]);
+      hi = std::max(hi, conn[4 * e + a]);
+    }
+    key[e] = ((long long)lo << 31) | hi;
+  }
+  std::vector<int> order(E);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+
+  std::vector<char> is_bnd(N, 0);
+  if (bnd_mask)
+    for (int i = 0; i < N; ++i) is_bnd[i] = bnd_mask[i] == 0.0;
+
+  // material table
+  std::memset(&c.hmt, 0, sizeof(c.hmt));
+  for (int g = 0; g < G; ++g) {
+    c.hmt.A[g] = A_red[g];
+    c.hmt.K1[g] = K1_red[g];
+    c.hmt.Js[g] = Js_red[g];
+    for (int k = 0; k < 3; ++k) c.hmt.k[g][k] = k_easy[3 * g + k];
+  }
+
+  HostPart hp_all, hp_mag;
+  build_partition(N, conn, mat0, order, is_bnd, hp_all);
+  std::vector<int> mag_elems;
+  mag_elems.reserve(E);
+  for (int k = 0; k < E; ++k) {
+    const int g = mat0[order[k]];
+    if (A_red[g] != 0.0 || K1_red[g] != 0.0 || Js_red[g] != 0.0) mag_elems.push_back(order[k]);
+  }
+  const std::vector<char> no_bnd(N, 0);
+  build_partition(N, conn, mat0, mag_elems, no_bnd, hp_mag);
+
+  MM_TRY(upload_part(c, hp_all, E, c.part_all));
+  MM_TRY(upload_part(c, hp_mag, (int)mag_elems.size(), c.part_mag));
+  c.max_block_nodes = std::max(hp_all.max_nodes, hp_mag.max_nodes);
+  c.n_mag = (int)mag_elems.size();
+
+  // device geometry in part_all order
+  c.h_conn.resize(4 * (size_t)E);
+  for (int k = 0; k < E; ++k)
+    for (int a = 0; a < 4; ++a) c.h_conn[4 * (size_t)k + a] = conn[4 * (size_t)order[k] + a];
+  MM_TRY(upload(c, &c.knt, knt, 3 * (size_t)N));
+  int* conn_dev = nullptr;
+  MM_TRY(upload(c, &conn_dev, c.h_conn.data(), c.h_conn.size()));
+  MatTable* mt = nullptr;
+  MM_TRY(upload(c, &mt, &c.hmt, 1));
+  const size_t Es = c.part_all.Es;
+  double *grad, *Ve;
+  float *grad32, *Ve32;
+  MM_TRY(dalloc(c, &grad, 9 * Es));
+  MM_TRY(dalloc(c, &Ve, Es));
+  MM_TRY(dalloc(c, &grad32, 9 * Es));
+  MM_TRY(dalloc(c, &Ve32, Es));
+  c.geo.knt = c.knt;
+  c.geo.grad = grad;
+  c.geo.Ve = Ve;
+  c.geo.grad32 = grad32;
+  c.geo.Ve32 = Ve32;
+  c.geo.mt = mt;
+  MM_TRY(launch_geometry(c, conn_dev, c.stream));
+  MM_HIP(hipGetLastError());
+  std::vector<double> hg(9 * Es);
+  c.h_Ve.resize(Es);
+  MM_HIP(hipMemcpyAsync(hg.data(), grad, 9 * Es * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipMemcpyAsync(c.h_Ve.data(), Ve, Es * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  c.h_Ve.resize(E);
+  c.h_grad.resize(9 * (size_t)E);
+  for (int e = 0; e < E; ++e)
+    for (int j = 0; j < 9; ++j) c.h_grad[9 * (size_t)e + j] = hg[(size_t)j * Es + e];
+
+  // nodal set-up: V_mag, M_nodal, exchange diagonal (src/loop.py:862-865,929-939;
+  // src/energy_kernels.py:492-566; src/hysteresis_loop.py:340-353)
+  double V_mag = 0.0;
+  for (int k = 0; k < E; ++k)
+    if (Js_red[mat0[order[k]]] > 0) V_mag += c.h_Ve[k];
+  c.V_mag = V_mag == 0.0 ? 1.0 : V_mag;
+  c.h_M_nodal.assign(N, 0.0);
+  std::vector<double> dex(N, 0.0);
+  for (int k = 0; k < E; ++k) {
+    const int g = mat0[order[k]];
+    const double ve = c.h_Ve[k];
+    const double* gr = &c.h_grad[9 * (size_t)k];
+    double G4[4][3];
+    for (int j = 0; j < 3; ++j) {
+      G4[1][j] = gr[j];
+      G4[2][j] = gr[3 + j];
+      G4[3][j] = gr[6 + j];
+      G4[0][j] = -(G4[1][j] + G4[2][j] + G4[3][j]);
+    }
+    for (int a = 0; a < 4; ++a) {
+      const int v = c.h_conn[4 * (size_t)k + a];
+      c.h_M_nodal[v] += Js_red[g] * ve * 0.25;
+      dex[v] += 2.0 * A_red[g] * ve * (G4[a][0] * G4[a][0] + G4[a][1] * G4[a][1] + G4[a][2] * G4[a][2]);
+    }
+  }
+  c.h_inv_M_rel.resize(N);
+  c.h_inv_M_prec.resize(N);
+  for (int i = 0; i < N; ++i) {
+    const double M = c.h_M_nodal[i];
+    c.h_inv_M_rel[i] = M > 1e-20 ? c.V_mag / M : 0.0;
+    const double dd = dex[i] / c.V_mag;
+    c.h_inv_M_prec[i] = dd > 1e-20 ? 1.0 / dd : 1.0;
+  }
+  MM_TRY(upload(c, &c.M_nodal, c.h_M_nodal.data(), N));
+  MM_TRY(upload(c, &c.inv_M_rel, c.h_inv_M_rel.data(), N));
+  MM_TRY(upload(c, &c.inv_M_prec, c.h_inv_M_prec.data(), N));
+  c.geo.M_nodal = c.M_nodal;
+
+  // reductions and scalars
+  const int max_grid = std::max({kRedBlocks, 8 * ((c.part_all.nblk + 7) / 8), 8 * ((c.part_mag.nblk + 7) / 8)});
+  MM_TRY(dalloc(c, &c.partial, 4 * (size_t)max_grid));
+  MM_TRY(dalloc(c, &c.counter, 16));
+  MM_HIP(hipMemset(c.counter, 0, 16 * sizeof(unsigned)));
+  MM_TRY(dalloc(c, &c.dscal, kNumScal));
+  MM_HIP(hipMemset(c.dscal, 0, kNumScal * sizeof(double)));
+  MM_HIP(hipHostMalloc((void**)&c.hscal, kNumScal * sizeof(double), hipHostMallocDefault));
+
+  // Poisson work vectors
+  for (double** v : {&c.pb, &c.pr, &c.pz, &c.pp, &c.pq, &c.pt, &c.px0, &c.work_U, &c.diagA}) MM_TRY(dalloc(c, v, N));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+static void destroy(Ctx* c) {
+  if (!c) return;
+  (void)hipDeviceSynchronize();
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->ev) (void)hipEventDestroy(c->ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int alloc_min_state(Ctx& c) {
+  if (c.m) return 0;
+  const size_t n3 = 3 * (size_t)c.N, n = c.N;
+  for (double** v : {&c.m, &c.g_raw, &c.g_tan, &c.g_ext, &c.g_prev, &c.y, &c.d, &c.d_prev, &c.H, &c.m_t, &c.g_t,
+                     &c.r3, &c.z3, &c.p3, &c.q3, &c.cv3})
+    MM_TRY(dalloc(c, v, n3));
+  MM_TRY(dalloc(c, &c.U, n));
+  MM_TRY(dalloc(c, &c.U_t, n));
+  return 0;
+}
+
+}  // namespace mm
+
+using namespace mm;
+
+#define GET_CTX(h)                                  \
+  Ctx* cp = get_ctx(h);                             \
+  if (!cp) {                                        \
+    set_error("invalid context handle");            \
+    return -1;                                      \
+  }                                                 \
+  Ctx& c = *cp;                                     \
+  MM_HIP(hipSetDevice(c.device));
+
+extern "C" {
+
+int mm_last_error(char* buf, int len) {
+  if (buf && len > 0) {
+    std::strncpy(buf, g_err.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  return (int)g_err.size();
+}
+
+int mm_device_info(char* buf, int len) {
+  hipDeviceProp_t prop;
+  int dev = 0;
+  MM_HIP(hipGetDevice(&dev));
+  MM_HIP(hipGetDeviceProperties(&prop, dev));
+  std::string s = std::string(prop.gcnArchName) + " CUs=" + std::to_string(prop.multiProcessorCount) +
+                  " mem=" + std::to_string(prop.totalGlobalMem >> 20) + "MiB";
+  if (buf && len > 0) {
+    std::strncpy(buf, s.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  return 0;
+}
+
+int mm_ctx_create(int N, int E, const double* knt, const int32_t* conn, const int32_t* mat_id, int G,
+                  const double* A_red, const double* K1_red, const double* Js_red, const double* k_easy,
+                  const double* bnd_mask, double reg, int flags, int64_t* out_ctx) {
+  if (!out_ctx) {
+    set_error("out_ctx is null");
+    return -1;
+  }
+  Ctx* c = new Ctx();
+  const int r = create(N, E, knt, conn, mat_id, G, A_red, K1_red, Js_red, k_easy, bnd_mask, reg, flags, *c);
+  if (r != 0) {
+    destroy(c);
+    return r;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_ctx.push_back(c);
+  *out_ctx = (int64_t)g_ctx.size();
+  return 0;
+}
+
+int mm_ctx_destroy(int64_t h) {
+  Ctx* c = get_ctx(h);
+  if (!c) {
+    set_error("invalid context handle");
+    return -1;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_ctx[h - 1] = nullptr;
+  }
+  destroy(c);
+  return 0;
+}
+
+int mm_ctx_info(int64_t h, double* info) {
+  GET_CTX(h);
+  info[0] = c.V_mag;
+  info[1] = c.N;
+  info[2] = c.E;
+  info[3] = c.part_all.nblk;
+  info[4] = c.part_mag.nblk;
+  info[5] = c.max_block_nodes;
+  info[6] = c.n_mag;
+  info[7] = 0;
+  return 0;
+}
+
+int mm_ctx_get_geometry(int64_t h, int32_t* conn, double* Ve, double* grad) {
+  GET_CTX(h);
+  if (conn) std::memcpy(conn, c.h_conn.data(), c.h_conn.size() * sizeof(int));
+  if (Ve) std::memcpy(Ve, c.h_Ve.data(), c.h_Ve.size() * sizeof(double));
+  if (grad) std::memcpy(grad, c.h_grad.data(), c.h_grad.size() * sizeof(double));
+  return 0;
+}
+
+int mm_ctx_get_nodal(int64_t h, double* M_nodal, double* inv_M_rel, double* inv_M_prec) {
+  GET_CTX(h);
+  if (M_nodal) std::memcpy(M_nodal, c.h_M_nodal.data(), c.N * sizeof(double));
+  if (inv_M_rel) std::memcpy(inv_M_rel, c.h_inv_M_rel.data(), c.N * sizeof(double));
+  if (inv_M_prec) std::memcpy(inv_M_prec, c.h_inv_M_prec.data(), c.N * sizeof(double));
+  return 0;
+}
+
+int mm_set_geometry_mode(int64_t h, int mode) {
+  GET_CTX(h);
+  MM_CHECK(mode >= 0 && mode <= 2, "geometry mode must be 0, 1 or 2");
+  c.geom_mode = mode;
+  return 0;
+}
+
+int mm_poisson_apply(int64_t h, const double* x, double* y) {
+  GET_CTX(h);
+  MM_CHECK(x && y, "null array");
+  MM_HIP(hipMemcpyAsync(c.pp, x, c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  vfill(c.pq, 0.0, c.N, c.stream);
+  MM_TRY(launch_apply(c, c.geom_mode, c.pp, c.pq, c.stream));
+  MM_HIP(hipGetLastError());
+  MM_HIP(hipMemcpyAsync(y, c.pq, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  c.n_apply++;
+  return 0;
+}
+
+int mm_poisson_rhs(int64_t h, const double* m, double* b) {
+  GET_CTX(h);
+  MM_CHECK(m && b, "null array");
+  MM_TRY(alloc_min_state(c));
+  MM_HIP(hipMemcpyAsync(c.m_t, m, 3 * c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  vfill(c.pb, 0.0, c.N, c.stream);
+  MM_TRY(launch_rhs(c, c.m_t, c.pb, c.stream));
+  MM_HIP(hipGetLastError());
+  MM_HIP(hipMemcpyAsync(b, c.pb, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int mm_poisson_diag(int64_t h, double* d) {
+  GET_CTX(h);
+  vfill(c.diagA, 0.0, c.N, c.stream);
+  MM_TRY(launch_diag(c, c.diagA, c.stream));
+  MM_HIP(hipGetLastError());
+  if (d) MM_HIP(hipMemcpyAsync(d, c.diagA, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, const int32_t* Ai, const double* Ax,
+                     const double* M, const int32_t* Pp, const int32_t* Pi, const double* Px, const int32_t* Rp,
+                     const int32_t* Ri, const double* Rx, const double* dense_inv) {
+  GET_CTX(h);
+  MM_CHECK(l >= 0 && l < 32, "level index out of range");
+  MM_CHECK(n > 0 && M, "level needs n > 0 and the smoother diagonal");
+  MM_CHECK(l != 0 || n == c.N, "level 0 must have N rows");
+  if ((int)c.lv.size() <= l) c.lv.resize(l + 1);
+  Level L;
+  L.n = n;
+  L.n_next = n_next;
+  MM_TRY(upload(c, &L.M, M, n));
+  if (l > 0 && Ap) {
+    MM_TRY(upload(c, &L.Ap, Ap, n + 1));
+    MM_TRY(upload(c, &L.Ai, Ai, Ap[n]));
+    MM_TRY(upload(c, &L.Ax, Ax, Ap[n]));
+  }
+  if (n_next > 0) {
+    MM_CHECK(Pp && Pi && Px && Rp && Ri && Rx, "P and R are required above the last level");
+    MM_TRY(upload(c, &L.Pp, Pp, n + 1));
+    MM_TRY(upload(c, &L.Pi, Pi, Pp[n]));
+    MM_TRY(upload(c, &L.Px, Px, Pp[n]));
+    MM_TRY(upload(c, &L.Rp, Rp, n_next + 1));
+    MM_TRY(upload(c, &L.Ri, Ri, Rp[n_next]));
+    MM_TRY(upload(c, &L.Rx, Rx, Rp[n_next]));
+  }
+  if (dense_inv) MM_TRY(upload(c, &L.dense_inv, dense_inv, (size_t)n * n));
+  if (l > 0) {
+    MM_TRY(dalloc(c, &L.b, n));
+    MM_TRY(dalloc(c, &L.x, n));
+    MM_TRY(dalloc(c, &L.t, n));
+  }
+  c.lv[l] = L;
+  return 0;
+}
+
+int mm_amg_commit(int64_t h, int kind, int nlev) {
+  GET_CTX(h);
+  MM_CHECK(nlev >= 1 && (int)c.lv.size() >= nlev, "levels missing");
+  c.lv.resize(nlev);
+  for (int l = 0; l < nlev; ++l) {
+    MM_CHECK(c.lv[l].M != nullptr, "level not set");
+    if (l < nlev - 1) {
+      MM_CHECK(c.lv[l].n_next == c.lv[l + 1].n, "P/R shape does not match the next level");
+      MM_CHECK(l == 0 || c.lv[l].Ap, "coarse level needs A");
+    } else {
+      MM_CHECK(c.lv[l].dense_inv != nullptr, "last level needs the dense inverse");
+    }
+  }
+  c.amg_kind = kind;
+  return 0;
+}
+
+int mm_precond_jacobi(int64_t h) {
+  GET_CTX(h);
+  MM_TRY(mm_poisson_diag(h, nullptr));
+  std::vector<double> d(c.N);
+  MM_HIP(hipMemcpy(d.data(), c.diagA, c.N * sizeof(double), hipMemcpyDeviceToHost));
+  for (auto& v : d) v = 1.0 / (v + 1e-30);
+  c.lv.clear();
+  c.lv.resize(1);
+  c.lv[0].n = c.N;
+  MM_TRY(upload(c, &c.lv[0].M, d.data(), c.N));
+  c.amg_kind = 0;
+  return 0;
+}
+
+int mm_precond_apply(int64_t h, const double* r, double* z) {
+  GET_CTX(h);
+  MM_CHECK(r && z, "null array");
+  MM_HIP(hipMemcpyAsync(c.pr, r, c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  MM_TRY(precond_apply(c, c.pr, c.pz, c.dscal + 6, c.stream));
+  MM_HIP(hipGetLastError());
+  MM_HIP(hipMemcpyAsync(z, c.pz, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int mm_solve_U(int64_t h, const double* m, double* U_inout, double tol, int maxiter, int* it, double* rel_res) {
+  GET_CTX(h);
+  MM_CHECK(m && U_inout, "null array");
+  MM_TRY(alloc_min_state(c));
+  MM_HIP(hipMemcpyAsync(c.m_t, m, 3 * c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  MM_HIP(hipMemcpyAsync(c.work_U, U_inout, c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  int iters = 0;
+  double rr = 0;
+  MM_TRY(solve_U_dev(c, c.m_t, c.work_U, tol, maxiter, &iters, &rr));
+  MM_HIP(hipGetLastError());
+  MM_HIP(hipMemcpyAsync(U_inout, c.work_U, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  if (it) *it = iters;
+  if (rel_res) *rel_res = rr;
+  return 0;
+}
+
+int mm_energy_grad(int64_t h, const double* m, const double* U, const double* B, double* E, double* g) {
+  GET_CTX(h);
+  MM_CHECK(m && U && B && E && g, "null array");
+  MM_TRY(alloc_min_state(c));
+  MM_HIP(hipMemcpyAsync(c.m_t, m, 3 * c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  MM_HIP(hipMemcpyAsync(c.U_t, U, c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  vfill(c.g_t, 0.0, 3L * c.N, c.stream);
+  MM_TRY(launch_energy(c, c.m_t, c.U_t, B, c.g_t, c.dscal + 16, c.stream));
+  MM_HIP(hipGetLastError());
+  MM_HIP(hipMemcpyAsync(g, c.g_t, 3 * c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipMemcpyAsync(E, c.dscal + 16, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  c.n_energy++;
+  return 0;
+}
+
+int mm_local_grad(int64_t h, const double* v, double* g) {
+  GET_CTX(h);
+  MM_CHECK(v && g, "null array");
+  MM_TRY(alloc_min_state(c));
+  MM_HIP(hipMemcpyAsync(c.p3, v, 3 * c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  vfill(c.cv3, 0.0, 3L * c.N, c.stream);
+  MM_TRY(launch_local(c, c.p3, c.cv3, nullptr, c.stream));
+  MM_HIP(hipGetLastError());
+  MM_HIP(hipMemcpyAsync(g, c.cv3, 3 * c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  c.n_local++;
+  return 0;
+}
+
+int mm_j_parallel(int64_t h, const double* m, const double* hdir, double* J) {
+  GET_CTX(h);
+  MM_TRY(alloc_min_state(c));
+  MM_HIP(hipMemcpyAsync(c.m_t, m, 3 * c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  double hn = std::sqrt(hdir[0] * hdir[0] + hdir[1] * hdir[1] + hdir[2] * hdir[2]) + 1e-30;
+  const double hh[3] = {hdir[0] / hn, hdir[1] / hn, hdir[2] / hn};
+  MM_TRY(launch_jpar(c, c.m_t, hh, c.dscal + 20, c.stream));
+  MM_HIP(hipMemcpyAsync(c.hscal + 20, c.dscal + 20, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  *J = c.hscal[20] / (c.hscal[21] + 1e-30);
+  return 0;
+}
+
+int mm_state_j_parallel(int64_t h, const double* hdir, double* J) {
+  GET_CTX(h);
+  MM_CHECK(c.have_state, "no resident state");
+  double hn = std::sqrt(hdir[0] * hdir[0] + hdir[1] * hdir[1] + hdir[2] * hdir[2]) + 1e-30;
+  const double hh[3] = {hdir[0] / hn, hdir[1] / hn, hdir[2] / hn};
+  MM_TRY(launch_jpar(c, c.m, hh, c.dscal + 20, c.stream));
+  MM_HIP(hipMemcpyAsync(c.hscal + 20, c.dscal + 20, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  *J = c.hscal[20] / (c.hscal[21] + 1e-30);
+  return 0;
+}
+
+int mm_state_set(int64_t h, const double* m, const double* U) {
+  GET_CTX(h);
+  MM_TRY(alloc_min_state(c));
+  MM_HIP(hipMemcpyAsync(c.m, m, 3 * c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  if (U)
+    MM_HIP(hipMemcpyAsync(c.U, U, c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  else
+    vfill(c.U, 0.0, c.N, c.stream);
+  MM_HIP(hipStreamSynchronize(c.stream));
+  c.have_state = true;
+  return 0;
+}
+
+int mm_state_get(int64_t h, double* m, double* U) {
+  GET_CTX(h);
+  MM_CHECK(c.have_state, "no resident state");
+  if (m) MM_HIP(hipMemcpyAsync(m, c.m, 3 * c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  if (U) MM_HIP(hipMemcpyAsync(U, c.U, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+static void to_minparams(const mm_params* p, MinParams& q) {
+  q.method = p->method;
+  q.max_iter = p->max_iter;
+  q.tau_f = p->tau_f;
+  q.eps_a = p->eps_a;
+  q.tau0 = p->tau0;
+  q.ls_eta1 = p->ls_eta1;
+  q.ls_eta2 = p->ls_eta2;
+  q.ls_C = p->ls_C;
+  q.ls_c = p->ls_c;
+  q.ls_max_evals = p->ls_max_evals;
+  q.pc_iters = p->pc_iters;
+  q.pc_auto = p->pc_auto;
+  q.pc_force_eta = p->pc_force_eta;
+  q.pc_force_alpha = p->pc_force_alpha;
+  q.pc_stagnation_nu = p->pc_stagnation_nu;
+  q.pc_reg = p->pc_reg;
+  q.L = p->L;
+  q.cg_tol = p->cg_tol;
+  q.phi_tol = p->phi_tol;
+  q.cg_maxiter = p->cg_maxiter;
+  q.memory = p->memory;
+}
+
+int mm_minimize_resident(int64_t h, const mm_params* p, const double* B, mm_info* info) {
+  GET_CTX(h);
+  MM_CHECK(p && B && info, "null argument");
+  MM_CHECK(c.have_state, "no resident state (call mm_state_set)");
+  MM_CHECK(p->method == 0 || p->method == 1, "method must be 0 (pcohen_hs) or 1 (lbfgs)");
+  MinParams q;
+  to_minparams(p, q);
+  MinInfo mi;
+  MM_TRY(minimize_resident(c, q, B, mi));
+  info->iters = mi.iters;
+  info->evals = mi.evals;
+  info->demag_iters = mi.demag_iters;
+  info->preco_iters = mi.preco_iters;
+  info->converged = mi.converged;
+  info->E = mi.E;
+  info->gnorm = mi.gnorm;
+  info->time = mi.time;
+  return 0;
+}
+
+int mm_minimize(int64_t h, const mm_params* p, double* m_inout, double* U_inout, const double* B, mm_info* info) {
+  MM_TRY(mm_state_set(h, m_inout, U_inout));
+  MM_TRY(mm_minimize_resident(h, p, B, info));
+  return mm_state_get(h, m_inout, U_inout);
+}
+
+int mm_sa_aggregate(int n, const int32_t* Sp, const int32_t* Sj, int32_t* agg, int32_t* n_agg) {
+  // Greedy three-pass aggregation on the strength graph S (CSR, diagonal ignored):
+  // 1) a node whose neighbours are all free becomes a root and takes its whole neighbourhood;
+  // 2) remaining free nodes join the aggregate of an aggregated neighbour;
+  // 3) still-free nodes start new aggregates with their free neighbours.
+  // Nodes without neighbours stay unaggregated (agg = -1).
+  if (n < 0 || !Sp || !Sj || !agg || !n_agg) {
+    set_error("mm_sa_aggregate: bad arguments");
+    return -1;
+  }
+  std::vector<int> state(n, 0);  // 0 free, >0 aggregate+1 (pass 1/3), <0 joined -(aggregate+1), isolated -n-1
+  int next = 0;
+  const int iso = -n - 1;
+  for (int i = 0; i < n; ++i) {
+    if (state[i] != 0) continue;
+    bool has_nb = false, nb_taken = false;
+    for (int k = Sp[i]; k < Sp[i + 1]; ++k) {
+      const int j = Sj[k];
+      if (j == i) continue;
+      has_nb = true;
+      if (state[j] != 0) {
+        nb_taken = true;
+        break;
+      }
+    }
+    if (!has_nb) {
+      state[i] = iso;
+    } else if (!nb_taken) {
+      ++next;
+      state[i] = next;
+      for (int k = Sp[i]; k < Sp[i + 1]; ++k) state[Sj[k]] = next;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    if (state[i] != 0) continue;
+    for (int k = Sp[i]; k < Sp[i + 1]; ++k) {
+      const int sj = state[Sj[k]];
+      if (sj > 0) {
+        state[i] = -sj;
+        break;
+      }
+    }
+  }
+  int count = next;  // aggregates are numbered 0..count-1 so far
+  for (int i = 0; i < n; ++i) {
+    const int si = state[i];
+    if (si == iso) {
+      agg[i] = -1;
+    } else if (si > 0) {
+      agg[i] = si - 1;
+    } else if (si < 0) {
+      agg[i] = -si - 1;
+    } else {
+      agg[i] = count;
+      for (int k = Sp[i]; k < Sp[i + 1]; ++k) {
+        const int j = Sj[k];
+        if (j != i && state[j] == 0) {
+          state[j] = -(count + 1);
+          if (j < i) agg[j] = count;
+        }
+      }
+      state[i] = -(count + 1);
+      ++count;
+    }
+  }
+  *n_agg = count;
+  return 0;
+}
+
+int mm_counters(int64_t h, int64_t* out6) {
+  GET_CTX(h);
+  out6[0] = c.n_apply;
+  out6[1] = c.n_rhs;
+  out6[2] = c.n_energy;
+  out6[3] = c.n_local;
+  out6[4] = c.n_vcycle;
+  out6[5] = c.n_pcg_it;
+  return 0;
+}
+
+int mm_bench_apply(int64_t h, int geom, int reps, double* ms) {
+  GET_CTX(h);
+  MM_CHECK(geom >= 0 && geom <= 2 && reps > 0, "bad arguments");
+  hipEvent_t e0, e1;
+  MM_HIP(hipEventCreate(&e0));
+  MM_HIP(hipEventCreate(&e1));
+  vfill(c.pp, 1.0, c.N, c.stream);
+  vfill(c.pq, 0.0, c.N, c.stream);
+  MM_TRY(launch_apply(c, geom, c.pp, c.pq, c.stream));  // warm-up
+  MM_HIP(hipEventRecord(e0, c.stream));
+  for (int r = 0; r < reps; ++r) MM_TRY(launch_apply(c, geom, c.pp, c.pq, c.stream));
+  MM_HIP(hipEventRecord(e1, c.stream));
+  MM_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  MM_HIP(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,164 @@
+// Device helpers shared by the kernel translation units: block lookup, tet geometry,
+// LDS/global scatter, and the deterministic last-block reduction.
+#pragma once
+
+#include "mm_kernels.h"
+
+namespace mm {
+
+// Workgroup -> element block. Workgroups are dispatched round-robin over the 8 XCDs
+// (id mod 8), so XCD j gets the contiguous block range [j*chunk, (j+1)*chunk).
+__device__ __forceinline__ BlkInfo blk_info(const Part& p) {
+  BlkInfo bi;
+  const int chunk = (p.nblk + 7) >> 3;
+  const int b = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+  bi.valid = b < p.nblk;
+  if (bi.valid) {
+    bi.e0 = p.eptr[b];
+    bi.e1 = p.eptr[b + 1];
+    bi.n0 = p.nptr[b];
+    bi.nb = p.nptr[b + 1] - bi.n0;
+  } else {
+    bi.e0 = bi.e1 = bi.n0 = bi.nb = 0;
+  }
+  return bi;
+}
+
+// Shape-function gradients and volume of one tet from LDS-staged corner coordinates.
+// grad phi_b (b = 1..3) are the rows of J^-1, J = [x1-x0, x2-x0, x3-x0]:
+// (c2 x c3)/det, (c3 x c1)/det, (c1 x c2)/det; grad phi_0 = -(their sum); Ve = |det|/6.
+__device__ __forceinline__ void tet_grads(const double (*X)[kMaxBlockNodes], int l0, int l1, int l2, int l3,
+                                          double g[4][3], double& vol) {
+  const double ox = X[0][l0], oy = X[1][l0], oz = X[2][l0];
+  const double c1x = X[0][l1] - ox, c1y = X[1][l1] - oy, c1z = X[2][l1] - oz;
+  const double c2x = X[0][l2] - ox, c2y = X[1][l2] - oy, c2z = X[2][l2] - oz;
+  const double c3x = X[0][l3] - ox, c3y = X[1][l3] - oy, c3z = X[2][l3] - oz;
+  const double n1x = c2y * c3z - c2z * c3y, n1y = c2z * c3x - c2x * c3z, n1z = c2x * c3y - c2y * c3x;
+  const double n2x = c3y * c1z - c3z * c1y, n2y = c3z * c1x - c3x * c1z, n2z = c3x * c1y - c3y * c1x;
+  const double n3x = c1y * c2z - c1z * c2y, n3y = c1z * c2x - c1x * c2z, n3z = c1x * c2y - c1y * c2x;
+  const double det = c1x * n1x + c1y * n1y + c1z * n1z;
+  const double inv = 1.0 / det;
+  g[1][0] = n1x * inv; g[1][1] = n1y * inv; g[1][2] = n1z * inv;
+  g[2][0] = n2x * inv; g[2][1] = n2y * inv; g[2][2] = n2z * inv;
+  g[3][0] = n3x * inv; g[3][1] = n3y * inv; g[3][2] = n3z * inv;
+  g[0][0] = -(g[1][0] + g[2][0] + g[3][0]);
+  g[0][1] = -(g[1][1] + g[2][1] + g[3][1]);
+  g[0][2] = -(g[1][2] + g[2][2] + g[3][2]);
+  vol = fabs(det) * (1.0 / 6.0);
+}
+
+__device__ __forceinline__ void unpack(uint32_t lc, int& l0, int& l1, int& l2, int& l3) {
+  l0 = lc & 255u;
+  l1 = (lc >> 8) & 255u;
+  l2 = (lc >> 16) & 255u;
+  l3 = lc >> 24;
+}
+
+// Block-partial of node `f` into a global vector: plain store when no other block
+// touches the node, hardware FP64 atomic add otherwise (the vector is zeroed first).
+__device__ __forceinline__ void node_out(double* __restrict__ y, uint32_t f, double v) {
+  const uint32_t id = f & kIdMask;
+  if (f & kPrivate)
+    y[id] = v;
+  else
+    unsafeAtomicAdd(&y[id], v);
+}
+
+template <bool MAX>
+__device__ __forceinline__ double wave_red(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double u = __shfl_down(v, o, 64);
+    v = MAX ? fmax(v, u) : v + u;
+  }
+  return v;
+}
+
+// Returns true in exactly one workgroup: the last one to arrive. Every workgroup of the
+// grid must call it. The counter is reset by the last workgroup.
+__device__ __forceinline__ bool last_block(unsigned* counter) {
+  __shared__ int is_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    is_last = (t == gridDim.x - 1);
+    if (is_last) *counter = 0u;
+  }
+  __syncthreads();
+  if (is_last) __threadfence();
+  return is_last;
+}
+
+// Block-reduce K values into per-workgroup partials; the last workgroup reduces all
+// partials in a fixed order (deterministic for a fixed grid) and returns true with the
+// totals in v[] (valid in thread 0).
+template <int K, bool MAX>
+__device__ bool reduce_last(double (&v)[K], double* __restrict__ partial, unsigned* __restrict__ counter) {
+  __shared__ double sh[K][kBlock / 64];
+  __shared__ int is_last;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_red<MAX>(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k][wv] = v[k];
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = sh[k][0];
+      for (int w = 1; w < kBlock / 64; ++w) s = MAX ? fmax(s, sh[k][w]) : s + sh[k][w];
+      partial[blockIdx.x * K + k] = s;
+    }
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int i = tid; i < (int)gridDim.x; i += kBlock) {
+      const double p = __hip_atomic_load(&partial[i * K + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s = MAX ? fmax(s, p) : s + p;
+    }
+    v[k] = wave_red<MAX>(s);
+  }
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k][wv] = v[k];
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = sh[k][0];
+      for (int w = 1; w < kBlock / 64; ++w) s = MAX ? fmax(s, sh[k][w]) : s + sh[k][w];
+      v[k] = s;
+    }
+    *counter = 0u;
+  }
+  return true;
+}
+
+// reduce_last + store of the totals to out[0..K-1]
+template <int K, bool MAX>
+__device__ void reduce_store(double (&v)[K], double* __restrict__ partial, unsigned* __restrict__ counter,
+                             double* __restrict__ out) {
+  if (reduce_last<K, MAX>(v, partial, counter) && threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = v[k];
+}
+
+// grid size for grid-stride vector kernels with reductions
+inline int red_grid(long n) {
+  long g = (n + kBlock * 4 - 1) / (kBlock * 4);
+  if (g < 1) g = 1;
+  if (g > kRedBlocks) g = kRedBlocks;
+  return (int)g;
+}
+
+}  // namespace mm

@@ -1,0 +1,182 @@
+// Internal declarations of the MI355X micromagnetics library (not part of the C-ABI).
+//
+// Data layout in HBM (one context = one mesh on one GPU):
+//   * nodes keep the caller's (Morton) order; nodal vectors are N doubles (potential) or
+//     3N interleaved doubles (m, g, ...); node coordinates are 3N interleaved doubles;
+//   * elements are re-sorted by (smallest node id, largest node id) and cut greedily into
+//     blocks of consecutive elements touching at most 256 distinct nodes; one workgroup
+//     owns one block;
+//   * per block: the sorted list of its distinct nodes (`nodes`, owner/private/boundary
+//     flags in the top three bits) and, per element, the four corner indices into that
+//     list packed as 4 x uint8 (`lconn`);
+//   * two partitions: all elements (Poisson apply) and magnetic elements only (Js, A or
+//     K1 nonzero: right-hand side, energy, local Hessian), each with its own material
+//     index stream;
+//   * optional stored per-element geometry (gradients of corners 1..3, 9 x E, and Ve) in
+//     FP64 and FP32, structure-of-arrays, for the streamed-geometry apply variant.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mm {
+
+constexpr int kBlock = 256;              // threads per workgroup
+constexpr int kMaxBlockNodes = 256;      // distinct nodes per element block (uint8 indices)
+constexpr int kMaxBlockElems = 2048;     // elements per block cap
+constexpr uint32_t kOwner = 1u << 31;    // this block adds reg*x / writes the boundary row / Zeeman
+constexpr uint32_t kPrivate = 1u << 30;  // no other block touches the node: plain store
+constexpr uint32_t kBoundary = 1u << 29; // Dirichlet node (outer surface)
+constexpr uint32_t kIdMask = (1u << 29) - 1;
+constexpr int kMaxMat = 64;
+constexpr int kRedBlocks = 2048;         // max workgroups of a reducing kernel
+constexpr int kNumScal = 256;            // device scalar slots
+
+enum Geom { kGeomCoords = 0, kGeomStored64 = 1, kGeomStored32 = 2 };
+
+void set_error(const std::string& msg);
+
+#define MM_HIP(call)                                                                      \
+  do {                                                                                    \
+    hipError_t _e = (call);                                                               \
+    if (_e != hipSuccess) {                                                               \
+      mm::set_error(std::string(#call) + " failed: " + hipGetErrorString(_e) + " at " + \
+                    __FILE__ + ":" + std::to_string(__LINE__));                           \
+      return -2;                                                                          \
+    }                                                                                     \
+  } while (0)
+
+#define MM_CHECK(cond, msg) \
+  do {                      \
+    if (!(cond)) {          \
+      mm::set_error(msg);   \
+      return -1;            \
+    }                       \
+  } while (0)
+
+#define MM_TRY(call)        \
+  do {                      \
+    int _r = (call);        \
+    if (_r != 0) return _r; \
+  } while (0)
+
+struct MatTable {
+  double A[kMaxMat];
+  double K1[kMaxMat];
+  double Js[kMaxMat];
+  double k[kMaxMat][3];
+};
+
+struct Part {
+  int nblk = 0;
+  int E = 0;                          // elements in this partition
+  int Es = 0;                         // padded stride of stored-geometry streams
+  const int* eptr = nullptr;          // nblk + 1 element offsets
+  const int* nptr = nullptr;          // nblk + 1 node-list offsets
+  const uint32_t* nodes = nullptr;    // node ids | flags
+  const uint32_t* lconn = nullptr;    // E packed local corners
+  const uint8_t* mat = nullptr;       // E material index (0-based)
+};
+
+struct Geo {
+  const double* knt = nullptr;        // 3N
+  const double* grad = nullptr;       // 9 x Es (part_all order)
+  const double* Ve = nullptr;         // Es
+  const float* grad32 = nullptr;
+  const float* Ve32 = nullptr;
+  const MatTable* mt = nullptr;
+  const double* M_nodal = nullptr;    // N
+};
+
+struct BlkInfo {
+  int e0, e1, n0, nb;
+  bool valid;
+};
+
+// One AMG level; level 0 uses the matrix-free apply (A arrays unused there).
+struct Level {
+  int n = 0, n_next = 0;
+  int *Ap = nullptr, *Ai = nullptr;
+  double* Ax = nullptr;
+  int *Pp = nullptr, *Pi = nullptr;
+  double* Px = nullptr;
+  int *Rp = nullptr, *Ri = nullptr;
+  double* Rx = nullptr;
+  double* M = nullptr;                // smoother multiplier: x += M * (b - A x)
+  double* dense_inv = nullptr;        // coarsest level
+  double *b = nullptr, *x = nullptr, *t = nullptr;
+};
+
+struct Ctx {
+  int device = 0;
+  int N = 0, E = 0, G = 0;
+  double V_mag = 1.0, reg = 1e-12;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  Part part_all, part_mag;
+  Geo geo;
+  int n_mag = 0;                      // magnetic elements
+  int max_block_nodes = 0;
+  int geom_mode = kGeomCoords;        // geometry source of the Poisson apply
+  std::vector<void*> allocs;          // every device allocation, freed at destroy
+  MatTable hmt;
+
+  double* knt = nullptr;
+  double* M_nodal = nullptr;          // N
+  double* inv_M_rel = nullptr;        // N
+  double* inv_M_prec = nullptr;       // N
+  std::vector<int> h_conn;            // E x 4 (part_all order)
+  std::vector<double> h_Ve, h_grad;   // E, E x 9
+  std::vector<double> h_M_nodal, h_inv_M_rel, h_inv_M_prec;
+
+  // reductions / scalars
+  double* partial = nullptr;          // kRedBlocks x 4
+  unsigned* counter = nullptr;        // 16 last-block counters
+  double* dscal = nullptr;            // kNumScal device scalars
+  double* hscal = nullptr;            // pinned host mirror
+
+  // Poisson / PCG vectors (N)
+  double *pb = nullptr, *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *pt = nullptr, *px0 = nullptr;
+  double* work_U = nullptr;
+  // AMG
+  int amg_kind = 0;                   // 0: Jacobi on diag(A) only, 1: SPAI0 V-cycle, 2: damped-Jacobi V-cycle
+  std::vector<Level> lv;
+  double* diagA = nullptr;            // N
+
+  // minimiser state (3N unless noted)
+  double *m = nullptr, *U = nullptr /*N*/, *g_raw = nullptr, *g_tan = nullptr, *g_ext = nullptr, *g_prev = nullptr;
+  double *y = nullptr, *d = nullptr, *d_prev = nullptr, *H = nullptr;
+  double *m_t = nullptr, *g_t = nullptr, *U_t = nullptr /*N*/;
+  double *r3 = nullptr, *z3 = nullptr, *p3 = nullptr, *q3 = nullptr, *cv3 = nullptr;
+  int mem = 10;
+  double *S = nullptr, *Yh = nullptr;  // mem x 3N
+  bool have_state = false;
+
+  long long n_apply = 0, n_rhs = 0, n_energy = 0, n_local = 0, n_vcycle = 0, n_pcg_it = 0;
+};
+
+// element kernels (mm_kernels.hip)
+int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s);
+int launch_rhs(const Ctx& c, const double* m, double* b, hipStream_t s);
+int launch_energy(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,
+                  hipStream_t s);
+int launch_local(const Ctx& c, const double* v, double* out, const double* gate, hipStream_t s);
+int launch_geometry(const Ctx& c, const int* conn_dev, hipStream_t s);
+int launch_diag(const Ctx& c, double* d, hipStream_t s);
+int launch_jpar(const Ctx& c, const double* m, const double h[3], double* out, hipStream_t s);
+
+// vector kernels (mm_vec.hip)
+int vfill(double* x, double v, long n, hipStream_t s);
+int vcopy(double* dst, const double* src, long n, hipStream_t s);
+// out[0] = a.b (deterministic)
+int vdot(const Ctx& c, const double* a, const double* b, long n, double* out, hipStream_t s);
+int vmaxabs(const Ctx& c, const double* a, long n, double* out, hipStream_t s);
+
+// solver (mm_solver.hip)
+int precond_apply(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s);
+int solve_U_dev(Ctx& c, const double* m, double* U, double tol, int maxiter, int* it, double* rel_res);
+
+}  // namespace mm

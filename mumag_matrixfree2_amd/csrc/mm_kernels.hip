@@ -1,0 +1,459 @@
+// Element kernels of the MI355X micromagnetics library: Poisson apply, demag right-hand
+// side, energy + gradient, local (exchange + anisotropy) Hessian action, plus the
+// deterministic reductions and nodal vector kernels they share.
+//
+// Every element kernel works on one element block per workgroup:
+//   1. the block's distinct nodes (sorted list, flags in the top bits) are staged in LDS
+//      (node values and, for on-the-fly geometry, the coordinates);
+//   2. each thread processes elements of the block, gathering corner values from LDS and
+//      accumulating corner contributions with LDS FP64 atomics (ds_add_f64);
+//   3. the block writes each node once: a plain store for nodes no other block touches,
+//      a hardware FP64 atomic add (global_atomic_add_f64) for shared nodes.
+// Workgroup ids are remapped so that each XCD (ids i mod 8) runs a contiguous, spatially
+// compact range of blocks and keeps its node data in its own L2.
+#include "mm_device.h"
+
+namespace mm {
+
+template <typename TG>
+__device__ __forceinline__ void stored_grads(const TG* __restrict__ grad, const TG* __restrict__ Ve, int Es, int e,
+                                             double g[4][3], double& vol) {
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g[b + 1][k] = (double)grad[(3 * b + k) * Es + e];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) g[0][k] = -(g[1][k] + g[2][k] + g[3][k]);
+  vol = (double)Ve[e];
+}
+
+// ------------------------------------------------------------------ Poisson apply (K1)
+// y = A x with A the assembled Poisson matrix: element stiffness Ve grad phi_a . grad phi_b,
+// rows and columns of Dirichlet nodes removed (x masked to 0 at the gather, the boundary
+// row written as y = x by its owner), reg added on the diagonal of interior rows.
+template <int GEOM>
+__global__ __launch_bounds__(kBlock) void k_apply(Part p, Geo geo, const double* __restrict__ x,
+                                                  double* __restrict__ y, double reg) {
+  __shared__ double sx[kMaxBlockNodes];
+  __shared__ double sy[kMaxBlockNodes];
+  __shared__ double sX[GEOM == kGeomCoords ? 3 : 1][kMaxBlockNodes];
+  const BlkInfo bi = blk_info(p);
+  if (!bi.valid) return;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t f = p.nodes[bi.n0 + i];
+    const uint32_t id = f & kIdMask;
+    sx[i] = (f & kBoundary) ? 0.0 : x[id];
+    sy[i] = 0.0;
+    if constexpr (GEOM == kGeomCoords) {
+      sX[0][i] = geo.knt[3 * id + 0];
+      sX[1][i] = geo.knt[3 * id + 1];
+      sX[2][i] = geo.knt[3 * id + 2];
+    }
+  }
+  __syncthreads();
+  for (int e = bi.e0 + tid; e < bi.e1; e += kBlock) {
+    int l0, l1, l2, l3;
+    unpack(p.lconn[e], l0, l1, l2, l3);
+    double g[4][3], vol;
+    if constexpr (GEOM == kGeomCoords)
+      tet_grads(sX, l0, l1, l2, l3, g, vol);
+    else if constexpr (GEOM == kGeomStored64)
+      stored_grads(geo.grad, geo.Ve, p.Es, e, g, vol);
+    else
+      stored_grads(geo.grad32, geo.Ve32, p.Es, e, g, vol);
+    const double x0 = sx[l0];
+    const double d1 = sx[l1] - x0, d2 = sx[l2] - x0, d3 = sx[l3] - x0;
+    const double gx = (d1 * g[1][0] + d2 * g[2][0] + d3 * g[3][0]) * vol;
+    const double gy = (d1 * g[1][1] + d2 * g[2][1] + d3 * g[3][1]) * vol;
+    const double gz = (d1 * g[1][2] + d2 * g[2][2] + d3 * g[3][2]) * vol;
+    atomicAdd(&sy[l0], g[0][0] * gx + g[0][1] * gy + g[0][2] * gz);
+    atomicAdd(&sy[l1], g[1][0] * gx + g[1][1] * gy + g[1][2] * gz);
+    atomicAdd(&sy[l2], g[2][0] * gx + g[2][1] * gy + g[2][2] * gz);
+    atomicAdd(&sy[l3], g[3][0] * gx + g[3][1] * gy + g[3][2] * gz);
+  }
+  __syncthreads();
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t f = p.nodes[bi.n0 + i];
+    if (f & kBoundary) {
+      if (f & kOwner) y[f & kIdMask] = x[f & kIdMask];
+      continue;
+    }
+    double v = sy[i];
+    if (f & kOwner) v += reg * sx[i];
+    node_out(y, f, v);
+  }
+}
+
+// ------------------------------------------------------------- demag right-hand side (K2)
+// b_a = sum_e (Js_e Ve / 4) grad phi_a . (m_0 + m_1 + m_2 + m_3) over magnetic elements.
+__global__ __launch_bounds__(kBlock) void k_rhs(Part p, Geo geo, const double* __restrict__ m,
+                                                double* __restrict__ b) {
+  __shared__ double sm[3][kMaxBlockNodes];
+  __shared__ double sX[3][kMaxBlockNodes];
+  __shared__ double sb[kMaxBlockNodes];
+  const BlkInfo bi = blk_info(p);
+  if (!bi.valid) return;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t id = p.nodes[bi.n0 + i] & kIdMask;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      sm[k][i] = m[3 * id + k];
+      sX[k][i] = geo.knt[3 * id + k];
+    }
+    sb[i] = 0.0;
+  }
+  __syncthreads();
+  for (int e = bi.e0 + tid; e < bi.e1; e += kBlock) {
+    const double Js = geo.mt->Js[p.mat[e]];
+    if (Js == 0.0) continue;
+    int l0, l1, l2, l3;
+    unpack(p.lconn[e], l0, l1, l2, l3);
+    double g[4][3], vol;
+    tet_grads(sX, l0, l1, l2, l3, g, vol);
+    const double f = Js * vol * 0.25;
+    const double sx_ = (sm[0][l0] + sm[0][l1] + sm[0][l2] + sm[0][l3]) * f;
+    const double sy_ = (sm[1][l0] + sm[1][l1] + sm[1][l2] + sm[1][l3]) * f;
+    const double sz_ = (sm[2][l0] + sm[2][l1] + sm[2][l2] + sm[2][l3]) * f;
+    atomicAdd(&sb[l0], g[0][0] * sx_ + g[0][1] * sy_ + g[0][2] * sz_);
+    atomicAdd(&sb[l1], g[1][0] * sx_ + g[1][1] * sy_ + g[1][2] * sz_);
+    atomicAdd(&sb[l2], g[2][0] * sx_ + g[2][1] * sy_ + g[2][2] * sz_);
+    atomicAdd(&sb[l3], g[3][0] * sx_ + g[3][1] * sy_ + g[3][2] * sz_);
+  }
+  __syncthreads();
+  for (int i = tid; i < bi.nb; i += kBlock) node_out(b, p.nodes[bi.n0 + i], sb[i]);
+}
+
+// ------------------------------------------------------------ energy + gradient (K7)
+// Per magnetic element: exchange 2A Ve grad phi_a . grad m, uniaxial anisotropy
+// -(2 K1 Ve/20) (sum_b m_b.k + m_a.k) k, demag (2 Js Ve/4) grad U on all four corners.
+// Owners add the Zeeman term -2 M_i B. Energy E = 1/2 sum m.(g_quad) + sum m.g_z,
+// reduced per block and finalised by the last block. Output g and E are divided by V_mag.
+__global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double* __restrict__ m,
+                                                   const double* __restrict__ U, double Bx, double By, double Bz,
+                                                   double inv_V, double* __restrict__ g, double* __restrict__ partial,
+                                                   unsigned* __restrict__ counter, double* __restrict__ E_out) {
+  __shared__ double sm[3][kMaxBlockNodes];
+  __shared__ double sX[3][kMaxBlockNodes];
+  __shared__ double su[kMaxBlockNodes];
+  __shared__ double sg[3][kMaxBlockNodes];
+  const BlkInfo bi = blk_info(p);
+  const int tid = threadIdx.x;
+  double acc[1] = {0.0};
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t id = p.nodes[bi.n0 + i] & kIdMask;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      sm[k][i] = m[3 * id + k];
+      sX[k][i] = geo.knt[3 * id + k];
+      sg[k][i] = 0.0;
+    }
+    su[i] = U[id];
+  }
+  __syncthreads();
+  for (int e = bi.e0 + tid; e < bi.e1; e += kBlock) {
+    const int mi = p.mat[e];
+    const double A = geo.mt->A[mi], K1 = geo.mt->K1[mi], Js = geo.mt->Js[mi];
+    const double kx = geo.mt->k[mi][0], ky = geo.mt->k[mi][1], kz = geo.mt->k[mi][2];
+    int l[4];
+    unpack(p.lconn[e], l[0], l[1], l[2], l[3]);
+    double g4[4][3], vol;
+    tet_grads(sX, l[0], l[1], l[2], l[3], g4, vol);
+    const double a_ve = 2.0 * A * vol, q_ve = 2.0 * K1 * vol / 20.0, j_ve = 2.0 * Js * vol / 4.0;
+    // grad m_c (c = x,y,z) and grad U, relative to corner 0
+    double gm[3][3], gu[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double m0 = sm[c][l[0]];
+      const double d1 = sm[c][l[1]] - m0, d2 = sm[c][l[2]] - m0, d3 = sm[c][l[3]] - m0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) gm[c][i] = d1 * g4[1][i] + d2 * g4[2][i] + d3 * g4[3][i];
+    }
+    {
+      const double u0 = su[l[0]];
+      const double d1 = su[l[1]] - u0, d2 = su[l[2]] - u0, d3 = su[l[3]] - u0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) gu[i] = (d1 * g4[1][i] + d2 * g4[2][i] + d3 * g4[3][i]) * j_ve;
+    }
+    double v[4], vs = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      v[a] = sm[0][l[a]] * kx + sm[1][l[a]] * ky + sm[2][l[a]] * kz;
+      vs += v[a];
+    }
+    double e_el = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const double fa = -q_ve * (vs + v[a]);
+      double c3[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        c3[c] = a_ve * (g4[a][0] * gm[c][0] + g4[a][1] * gm[c][1] + g4[a][2] * gm[c][2]) + gu[c];
+      c3[0] += fa * kx;
+      c3[1] += fa * ky;
+      c3[2] += fa * kz;
+      e_el += sm[0][l[a]] * c3[0] + sm[1][l[a]] * c3[1] + sm[2][l[a]] * c3[2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) atomicAdd(&sg[c][l[a]], c3[c]);
+    }
+    acc[0] += 0.5 * e_el;
+  }
+  __syncthreads();
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t f = p.nodes[bi.n0 + i];
+    const uint32_t id = f & kIdMask;
+    double v0 = sg[0][i], v1 = sg[1][i], v2 = sg[2][i];
+    if (f & kOwner) {
+      const double Mi = geo.M_nodal[id];
+      const double z0 = -2.0 * Mi * Bx, z1 = -2.0 * Mi * By, z2 = -2.0 * Mi * Bz;
+      v0 += z0;
+      v1 += z1;
+      v2 += z2;
+      acc[0] += sm[0][i] * z0 + sm[1][i] * z1 + sm[2][i] * z2;
+    }
+    if (f & kPrivate) {
+      g[3 * id + 0] = v0 * inv_V;
+      g[3 * id + 1] = v1 * inv_V;
+      g[3 * id + 2] = v2 * inv_V;
+    } else {
+      unsafeAtomicAdd(&g[3 * id + 0], v0 * inv_V);
+      unsafeAtomicAdd(&g[3 * id + 1], v1 * inv_V);
+      unsafeAtomicAdd(&g[3 * id + 2], v2 * inv_V);
+    }
+  }
+  acc[0] *= inv_V;
+  reduce_store<1, false>(acc, partial, counter, E_out);
+}
+
+// --------------------------------------------------- local Hessian action C v (K8)
+// C v = (K_ex + K_an) v / V_mag: exchange and anisotropy parts of K7 without demag/Zeeman.
+__global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const double* __restrict__ vin,
+                                                  double inv_V, double* __restrict__ out,
+                                                  const double* __restrict__ gate) {
+  if (gate != nullptr && *gate == 0.0) return;
+  __shared__ double sm[3][kMaxBlockNodes];
+  __shared__ double sX[3][kMaxBlockNodes];
+  __shared__ double sg[3][kMaxBlockNodes];
+  const BlkInfo bi = blk_info(p);
+  if (!bi.valid) return;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t id = p.nodes[bi.n0 + i] & kIdMask;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      sm[k][i] = vin[3 * id + k];
+      sX[k][i] = geo.knt[3 * id + k];
+      sg[k][i] = 0.0;
+    }
+  }
+  __syncthreads();
+  for (int e = bi.e0 + tid; e < bi.e1; e += kBlock) {
+    const int mi = p.mat[e];
+    const double A = geo.mt->A[mi], K1 = geo.mt->K1[mi];
+    const double kx = geo.mt->k[mi][0], ky = geo.mt->k[mi][1], kz = geo.mt->k[mi][2];
+    int l[4];
+    unpack(p.lconn[e], l[0], l[1], l[2], l[3]);
+    double g4[4][3], vol;
+    tet_grads(sX, l[0], l[1], l[2], l[3], g4, vol);
+    const double a_ve = 2.0 * A * vol, q_ve = 2.0 * K1 * vol / 20.0;
+    double gm[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double m0 = sm[c][l[0]];
+      const double d1 = sm[c][l[1]] - m0, d2 = sm[c][l[2]] - m0, d3 = sm[c][l[3]] - m0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) gm[c][i] = (d1 * g4[1][i] + d2 * g4[2][i] + d3 * g4[3][i]) * a_ve;
+    }
+    double v[4], vs = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      v[a] = sm[0][l[a]] * kx + sm[1][l[a]] * ky + sm[2][l[a]] * kz;
+      vs += v[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const double fa = -q_ve * (vs + v[a]);
+      atomicAdd(&sg[0][l[a]], g4[a][0] * gm[0][0] + g4[a][1] * gm[0][1] + g4[a][2] * gm[0][2] + fa * kx);
+      atomicAdd(&sg[1][l[a]], g4[a][0] * gm[1][0] + g4[a][1] * gm[1][1] + g4[a][2] * gm[1][2] + fa * ky);
+      atomicAdd(&sg[2][l[a]], g4[a][0] * gm[2][0] + g4[a][1] * gm[2][1] + g4[a][2] * gm[2][2] + fa * kz);
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t f = p.nodes[bi.n0 + i];
+    const uint32_t id = f & kIdMask;
+    if (f & kPrivate) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) out[3 * id + k] = sg[k][i] * inv_V;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&out[3 * id + k], sg[k][i] * inv_V);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ geometry (set-up)
+// Stored-gradient streams for the streamed-geometry variant of the apply, and Ve.
+__global__ void k_geometry(int E, int Es, const double* __restrict__ knt, const int* __restrict__ conn,
+                           double* __restrict__ grad, double* __restrict__ Ve, float* __restrict__ grad32,
+                           float* __restrict__ Ve32) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double c[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[a][k] = knt[3 * conn[4 * e + a] + k];
+  const double c1x = c[1][0] - c[0][0], c1y = c[1][1] - c[0][1], c1z = c[1][2] - c[0][2];
+  const double c2x = c[2][0] - c[0][0], c2y = c[2][1] - c[0][1], c2z = c[2][2] - c[0][2];
+  const double c3x = c[3][0] - c[0][0], c3y = c[3][1] - c[0][1], c3z = c[3][2] - c[0][2];
+  const double n[3][3] = {{c2y * c3z - c2z * c3y, c2z * c3x - c2x * c3z, c2x * c3y - c2y * c3x},
+                          {c3y * c1z - c3z * c1y, c3z * c1x - c3x * c1z, c3x * c1y - c3y * c1x},
+                          {c1y * c2z - c1z * c2y, c1z * c2x - c1x * c2z, c1x * c2y - c1y * c2x}};
+  const double det = c1x * n[0][0] + c1y * n[0][1] + c1z * n[0][2];
+  const double inv = 1.0 / det;
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double v = n[b][k] * inv;
+      grad[(3 * b + k) * Es + e] = v;
+      grad32[(3 * b + k) * Es + e] = (float)v;
+    }
+  Ve[e] = fabs(det) / 6.0;
+  Ve32[e] = (float)(fabs(det) / 6.0);
+}
+
+// ------------------------------------------------------------- diag(A) (set-up)
+// d_a = sum_e Ve |grad phi_a|^2 + reg on interior rows, 1 on Dirichlet rows
+// (``assemble_diag`` semantics on the masked, regularised matrix).
+__global__ __launch_bounds__(kBlock) void k_diag(Part p, Geo geo, double reg, double* __restrict__ d) {
+  __shared__ double sX[3][kMaxBlockNodes];
+  __shared__ double sd[kMaxBlockNodes];
+  const BlkInfo bi = blk_info(p);
+  if (!bi.valid) return;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t id = p.nodes[bi.n0 + i] & kIdMask;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sX[k][i] = geo.knt[3 * id + k];
+    sd[i] = 0.0;
+  }
+  __syncthreads();
+  for (int e = bi.e0 + tid; e < bi.e1; e += kBlock) {
+    int l[4];
+    unpack(p.lconn[e], l[0], l[1], l[2], l[3]);
+    double g4[4][3], vol;
+    tet_grads(sX, l[0], l[1], l[2], l[3], g4, vol);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) atomicAdd(&sd[l[a]], vol * (g4[a][0] * g4[a][0] + g4[a][1] * g4[a][1] + g4[a][2] * g4[a][2]));
+  }
+  __syncthreads();
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t f = p.nodes[bi.n0 + i];
+    if (f & kBoundary) {
+      if (f & kOwner) d[f & kIdMask] = 1.0;
+      continue;
+    }
+    node_out(d, f, sd[i] + ((f & kOwner) ? reg : 0.0));
+  }
+}
+
+// ---------------------------------------------------------- volume-averaged J along h
+// out[0] = sum_e Ve Js_e mean_a(m_a).h, out[1] = sum_{Js_e > 0} Ve (``src/hysteresis_loop.py:153-190``).
+__global__ __launch_bounds__(kBlock) void k_jpar(Part p, Geo geo, const double* __restrict__ m, double hx, double hy,
+                                                 double hz, double* __restrict__ partial, unsigned* __restrict__ counter,
+                                                 double* __restrict__ out) {
+  __shared__ double sX[3][kMaxBlockNodes];
+  __shared__ double sh[kMaxBlockNodes];
+  const BlkInfo bi = blk_info(p);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t id = p.nodes[bi.n0 + i] & kIdMask;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sX[k][i] = geo.knt[3 * id + k];
+    sh[i] = m[3 * id] * hx + m[3 * id + 1] * hy + m[3 * id + 2] * hz;
+  }
+  __syncthreads();
+  double acc[2] = {0.0, 0.0};
+  for (int e = bi.e0 + tid; e < bi.e1; e += kBlock) {
+    const double Js = geo.mt->Js[p.mat[e]];
+    if (!(Js > 0.0)) continue;
+    int l[4];
+    unpack(p.lconn[e], l[0], l[1], l[2], l[3]);
+    double g4[4][3], vol;
+    tet_grads(sX, l[0], l[1], l[2], l[3], g4, vol);
+    acc[0] += vol * Js * 0.25 * (sh[l[0]] + sh[l[1]] + sh[l[2]] + sh[l[3]]);
+    acc[1] += vol;
+  }
+  reduce_store<2, false>(acc, partial, counter, out);
+}
+
+// ------------------------------------------------------------------------ launchers
+static inline int grid_for(const Part& p) { return 8 * ((p.nblk + 7) / 8); }
+
+int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s) {
+  const Part& p = c.part_all;
+  if (p.nblk == 0) return 0;
+  const int grid = grid_for(p);
+  switch (geom) {
+    case kGeomCoords:
+      k_apply<kGeomCoords><<<grid, kBlock, 0, s>>>(p, c.geo, x, y, c.reg);
+      break;
+    case kGeomStored64:
+      k_apply<kGeomStored64><<<grid, kBlock, 0, s>>>(p, c.geo, x, y, c.reg);
+      break;
+    default:
+      k_apply<kGeomStored32><<<grid, kBlock, 0, s>>>(p, c.geo, x, y, c.reg);
+  }
+  return 0;
+}
+
+int launch_rhs(const Ctx& c, const double* m, double* b, hipStream_t s) {
+  const Part& p = c.part_mag;
+  if (p.nblk == 0) return 0;
+  k_rhs<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, m, b);
+  return 0;
+}
+
+int launch_energy(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,
+                  hipStream_t s) {
+  const Part& p = c.part_mag;
+  if (p.nblk == 0) return 0;
+  k_energy<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, c.partial,
+                                           c.counter + 1, E_out);
+  return 0;
+}
+
+int launch_local(const Ctx& c, const double* v, double* out, const double* gate, hipStream_t s) {
+  const Part& p = c.part_mag;
+  if (p.nblk == 0) return 0;
+  k_local<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, v, 1.0 / c.V_mag, out, gate);
+  return 0;
+}
+
+int launch_geometry(const Ctx& c, const int* conn_dev, hipStream_t s) {
+  const int E = c.E;
+  if (E == 0) return 0;
+  k_geometry<<<(E + 255) / 256, 256, 0, s>>>(E, c.part_all.Es, c.geo.knt, conn_dev, const_cast<double*>(c.geo.grad),
+                                             const_cast<double*>(c.geo.Ve), const_cast<float*>(c.geo.grad32),
+                                             const_cast<float*>(c.geo.Ve32));
+  return 0;
+}
+
+int launch_diag(const Ctx& c, double* d, hipStream_t s) {
+  const Part& p = c.part_all;
+  if (p.nblk == 0) return 0;
+  k_diag<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, c.reg, d);
+  return 0;
+}
+
+int launch_jpar(const Ctx& c, const double* m, const double h[3], double* out, hipStream_t s) {
+  const Part& p = c.part_mag;
+  if (p.nblk == 0) return 0;
+  k_jpar<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, m, h[0], h[1], h[2], c.partial, c.counter + 2, out);
+  return 0;
+}
+
+}  // namespace mm

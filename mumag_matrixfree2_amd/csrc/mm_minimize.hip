@@ -1,0 +1,708 @@
+// Field-step minimiser on the unit sphere, device resident: projected L-BFGS
+// (src/minimizers.py:1216-1381) and preconditioned Cohen-HS conjugate gradients
+// (src/minimizers.py:950-1073) with the curvilinear Armijo line search
+// (src/minimizers.py:265-385), the Steihaug-PCG preconditioner solve
+// (src/minimizers.py:393-492) and the Gill-Murray convergence test (src/minimizers.py:160-172).
+//
+// Vectors (N x 3 interleaved) never leave the GPU. Scalars (dots, step lengths, flags)
+// live in the device scalar block and are consumed by the next kernel; the host reads
+// back only what drives control flow: the slope pg once per iteration, the trial energy
+// once per line-search evaluation, and the convergence quantities.
+#include <chrono>
+#include <cmath>
+#include <utility>
+
+#include "mm_device.h"
+#include "mm_solver.h"
+
+namespace mm {
+
+int alloc_min_state(Ctx& c);
+
+enum Slot {
+  S_E = 16,
+  S_GN = 24,   // max |g_tan|
+  S_GS = 25,   // max |y|
+  S_NUM = 26,  // y . (g_ext - g_prev)
+  S_DEN = 27,  // d_prev . (g_ext - g_prev)
+  S_DG = 28,   // d . g_ext
+  S_PG = 29,   // g_raw . d
+  S_DM = 30,   // max |m_new - m|
+  S_PCIT = 31, // inner iterations (copy for the host)
+  S_RHO = 32, S_Q = 33, S_IT = 34, S_DONE = 35, S_ACT = 36, S_ALPHA = 37, S_TGT = 38, S_DQ = 39, S_DN = 40,
+  S_PAP = 41, S_RHON = 42, S_YY = 43, S_ZZ = 44, S_YG = 45,
+  S_SY = 48, S_YMY = 49, S_CURV = 50, S_SS = 51, S_TMP = 52,
+  S_ALPHAS = 64,  // + history slot
+  S_RHOS = 96,    // + history slot
+};
+
+static inline int vg(long n) {
+  long g = (n + kBlock - 1) / kBlock;
+  if (g > 8192) g = 8192;
+  return (int)(g < 1 ? 1 : g);
+}
+
+#define NODE_LOOP(i, n) for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < (n); i += (long)gridDim.x * kBlock)
+
+__device__ __forceinline__ void tangent3(const double* m, const double* g, double s, double* out) {
+  const double md = m[0] * g[0] + m[1] * g[1] + m[2] * g[2];
+  out[0] = s * (g[0] - md * m[0]);
+  out[1] = s * (g[1] - md * m[1]);
+  out[2] = s * (g[2] - md * m[2]);
+}
+
+// m /= |m|
+__global__ void k_normalize(double* __restrict__ m, long N) {
+  NODE_LOOP(i, N) {
+    const double a = m[3 * i], b = m[3 * i + 1], c = m[3 * i + 2];
+    const double n = sqrt(a * a + b * b + c * c);
+    m[3 * i] = a / n;
+    m[3 * i + 1] = b / n;
+    m[3 * i + 2] = c / n;
+  }
+}
+
+// g_tan = T(m, g invMr), g_ext = T(m, g); out = max |g_tan|  (src/minimizers.py:94-96, 963-966)
+__global__ __launch_bounds__(kBlock) void k_tangents(const double* __restrict__ m, const double* __restrict__ g,
+                                                     const double* __restrict__ invMr, double* __restrict__ g_tan,
+                                                     double* __restrict__ g_ext, long N, double* __restrict__ partial,
+                                                     unsigned* __restrict__ counter, double* __restrict__ out) {
+  double v[1] = {0.0};
+  NODE_LOOP(i, N) {
+    double t[3];
+    tangent3(m + 3 * i, g + 3 * i, 1.0, t);
+    g_ext[3 * i] = t[0];
+    g_ext[3 * i + 1] = t[1];
+    g_ext[3 * i + 2] = t[2];
+    const double s = invMr[i];
+    double gs[3] = {g[3 * i] * s, g[3 * i + 1] * s, g[3 * i + 2] * s};
+    tangent3(m + 3 * i, gs, 1.0, t);
+    g_tan[3 * i] = t[0];
+    g_tan[3 * i + 1] = t[1];
+    g_tan[3 * i + 2] = t[2];
+    v[0] = fmax(v[0], fmax(fabs(t[0]), fmax(fabs(t[1]), fabs(t[2]))));
+  }
+  reduce_store<1, true>(v, partial, counter, out);
+}
+
+// ------------------------------------------------------------- Steihaug-PCG (solve_Py_g)
+// y = 0; r = g_ext; z = Mp r; p = z; rho = r.z
+__global__ __launch_bounds__(kBlock) void k_pc_init(const double* __restrict__ g_ext, const double* __restrict__ Mp,
+                                                    double* __restrict__ y, double* __restrict__ r,
+                                                    double* __restrict__ z, double* __restrict__ p,
+                                                    double* __restrict__ cv, long N, double* __restrict__ partial,
+                                                    unsigned* __restrict__ counter, double* __restrict__ out) {
+  double v[1] = {0.0};
+  NODE_LOOP(i, N) {
+    const double s = Mp[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double ri = g_ext[3 * i + k];
+      y[3 * i + k] = 0.0;
+      r[3 * i + k] = ri;
+      z[3 * i + k] = ri * s;
+      p[3 * i + k] = ri * s;
+      cv[3 * i + k] = 0.0;
+      v[0] += ri * ri * s;
+    }
+  }
+  reduce_store<1, false>(v, partial, counter, out);
+}
+
+__device__ __forceinline__ double pc_active(const double* sc, int max_iter) {
+  const double rho = sc[S_RHO];
+  const bool act = (sc[S_IT] < (double)max_iter) && (rho > sc[S_TGT]) && (rho > 1e-25) && (rho < 1e20) &&
+                   (sc[S_DONE] == 0.0);
+  return act ? 1.0 : 0.0;
+}
+
+// forcing-sequence tolerance and loop state (src/minimizers.py:968-972, 445-450)
+__global__ void k_pc_state_init(double* __restrict__ sc, int pc_auto, double eta, double alpha_f, int max_iter) {
+  const double tol = pc_auto ? fmin(eta, pow(sc[S_GN], alpha_f)) : 0.0;
+  sc[S_TGT] = tol * tol * sc[S_RHO];
+  sc[S_Q] = 0.0;
+  sc[S_IT] = 0.0;
+  sc[S_DONE] = 0.0;
+  sc[S_ACT] = pc_active(sc, max_iter);
+}
+
+// q = P p = Cv - (m.Cv) m - (m.g) p + reg p; p.q -> alpha, dq, negative curvature / stagnation
+__global__ __launch_bounds__(kBlock) void k_pc_ap(double* __restrict__ sc, const double* __restrict__ m,
+                                                  const double* __restrict__ g, const double* __restrict__ p,
+                                                  const double* __restrict__ cv, double* __restrict__ q, double reg,
+                                                  double nu, long N, double* __restrict__ partial,
+                                                  unsigned* __restrict__ counter) {
+  if (sc[S_ACT] == 0.0) return;
+  double v[1] = {0.0};
+  NODE_LOOP(i, N) {
+    const double* mi = m + 3 * i;
+    const double mc = mi[0] * cv[3 * i] + mi[1] * cv[3 * i + 1] + mi[2] * cv[3 * i + 2];
+    const double mg = mi[0] * g[3 * i] + mi[1] * g[3 * i + 1] + mi[2] * g[3 * i + 2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double pk = p[3 * i + k];
+      const double qk = cv[3 * i + k] - mc * mi[k] - mg * pk + reg * pk;
+      q[3 * i + k] = qk;
+      v[0] += pk * qk;
+    }
+  }
+  if (reduce_last<1, false>(v, partial, counter) && threadIdx.x == 0) {
+    const double pAp = v[0], rho = sc[S_RHO];
+    const double alpha = rho / (pAp + 1e-30);
+    const double dq = 0.5 * alpha * rho;
+    const bool stag = (sc[S_IT] > 0.0) && (dq <= nu * sc[S_Q]);
+    sc[S_PAP] = pAp;
+    sc[S_ALPHA] = alpha;
+    sc[S_DQ] = dq;
+    sc[S_DN] = (pAp <= 0.0 || stag) ? 1.0 : 0.0;
+  }
+}
+
+// y += alpha p; r -= alpha q (unless done now); z = Mp r; rho_n = r.z
+__global__ __launch_bounds__(kBlock) void k_pc_yr(double* __restrict__ sc, double* __restrict__ y,
+                                                  double* __restrict__ r, double* __restrict__ z,
+                                                  const double* __restrict__ p, const double* __restrict__ q,
+                                                  const double* __restrict__ Mp, long N, double* __restrict__ partial,
+                                                  unsigned* __restrict__ counter) {
+  if (sc[S_ACT] == 0.0) return;
+  const bool upd = sc[S_DN] == 0.0;
+  const double alpha = sc[S_ALPHA];
+  double v[1] = {0.0};
+  NODE_LOOP(i, N) {
+    const double s = Mp[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double rk = r[3 * i + k];
+      if (upd) {
+        y[3 * i + k] += alpha * p[3 * i + k];
+        rk -= alpha * q[3 * i + k];
+        r[3 * i + k] = rk;
+      }
+      const double zk = rk * s;
+      z[3 * i + k] = zk;
+      v[0] += rk * zk;
+    }
+  }
+  if (reduce_last<1, false>(v, partial, counter) && threadIdx.x == 0) sc[S_RHON] = v[0];
+}
+
+// p = z + beta p (unless done now); cv = 0 for the next iteration; loop-state update
+__global__ __launch_bounds__(kBlock) void k_pc_p(double* __restrict__ sc, const double* __restrict__ z,
+                                                 double* __restrict__ p, double* __restrict__ cv, int max_iter, long N,
+                                                 unsigned* __restrict__ counter) {
+  if (sc[S_ACT] == 0.0) return;
+  const bool upd = sc[S_DN] == 0.0;
+  const double beta = sc[S_RHON] / (sc[S_RHO] + 1e-30);
+  NODE_LOOP(i, N) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (upd) p[3 * i + k] = z[3 * i + k] + beta * p[3 * i + k];
+      cv[3 * i + k] = 0.0;
+    }
+  }
+  if (last_block(counter) && threadIdx.x == 0) {
+    sc[S_RHO] = sc[S_RHON];
+    sc[S_Q] += sc[S_DQ];
+    sc[S_IT] += 1.0;
+    sc[S_DONE] = sc[S_DN];
+    sc[S_ACT] = pc_active(sc, max_iter);
+  }
+}
+
+// y.y, zf.zf (zf = Mp g_ext), y.g_ext
+__global__ __launch_bounds__(kBlock) void k_pc_norms(double* __restrict__ sc, const double* __restrict__ y,
+                                                     const double* __restrict__ g, const double* __restrict__ Mp,
+                                                     long N, double* __restrict__ partial,
+                                                     unsigned* __restrict__ counter) {
+  double v[3] = {0.0, 0.0, 0.0};
+  NODE_LOOP(i, N) {
+    const double s = Mp[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double yk = y[3 * i + k], gk = g[3 * i + k];
+      v[0] += yk * yk;
+      v[1] += gk * s * gk * s;
+      v[2] += yk * gk;
+    }
+  }
+  reduce_store<3, false>(v, partial, counter, sc + S_YY);
+}
+
+// clip |y| <= 10 |zf|; fall back to zf when y is not a descent direction (src/minimizers.py:478-486)
+__global__ void k_pc_finish(double* __restrict__ sc, double* __restrict__ y, const double* __restrict__ g,
+                            const double* __restrict__ Mp, long N) {
+  const double yn = sqrt(sc[S_YY]), zn = sqrt(sc[S_ZZ]);
+  const double scale = (yn > 10.0 * zn) ? (10.0 * zn / (yn + 1e-30)) : 1.0;
+  const bool fallback = !(scale * sc[S_YG] > 1e-12);
+  NODE_LOOP(i, N) {
+    const double s = Mp[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) y[3 * i + k] = fallback ? g[3 * i + k] * s : y[3 * i + k] * scale;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_PCIT] = sc[S_IT];
+}
+
+// ----------------------------------------------------------------- direction kernels
+// HS beta dots: y.(g_ext - g_prev), d_prev.(g_ext - g_prev)
+__global__ __launch_bounds__(kBlock) void k_beta_dots(double* __restrict__ sc, const double* __restrict__ y,
+                                                      const double* __restrict__ ge, const double* __restrict__ gp,
+                                                      const double* __restrict__ dp, long n3,
+                                                      double* __restrict__ partial, unsigned* __restrict__ counter) {
+  double v[2] = {0.0, 0.0};
+  NODE_LOOP(i, n3) {
+    const double dg = ge[i] - gp[i];
+    v[0] += y[i] * dg;
+    v[1] += dp[i] * dg;
+  }
+  reduce_store<2, false>(v, partial, counter, sc + S_NUM);
+}
+
+// d = -y + beta T(m, d_prev); d.g_ext  (src/minimizers.py:997-1012)
+__global__ __launch_bounds__(kBlock) void k_direction(double* __restrict__ sc, int restart, const double* __restrict__ m,
+                                                      const double* __restrict__ y, const double* __restrict__ dp,
+                                                      const double* __restrict__ ge, double* __restrict__ d, long N,
+                                                      double* __restrict__ partial, unsigned* __restrict__ counter) {
+  const double beta = restart ? 0.0 : fmax(0.0, sc[S_NUM] / (sc[S_DEN] + 1e-30));
+  double v[1] = {0.0};
+  NODE_LOOP(i, N) {
+    double t[3];
+    tangent3(m + 3 * i, dp + 3 * i, 1.0, t);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double dk = -y[3 * i + k] + beta * t[k];
+      d[3 * i + k] = dk;
+      v[0] += dk * ge[3 * i + k];
+    }
+  }
+  reduce_store<1, false>(v, partial, counter, sc + S_DG);
+}
+
+// if (force || d.g_ext > 0) d = -alt; H = m x d; pg = g_raw . d
+__global__ __launch_bounds__(kBlock) void k_direction_fix(double* __restrict__ sc, int force,
+                                                          const double* __restrict__ m, const double* __restrict__ alt,
+                                                          const double* __restrict__ g, double* __restrict__ d,
+                                                          double* __restrict__ H, long N, double* __restrict__ partial,
+                                                          unsigned* __restrict__ counter) {
+  const bool flip = force || sc[S_DG] > 0.0;
+  double v[1] = {0.0};
+  NODE_LOOP(i, N) {
+    double dd[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      dd[k] = flip ? -alt[3 * i + k] : d[3 * i + k];
+      d[3 * i + k] = dd[k];
+      v[0] += g[3 * i + k] * dd[k];
+    }
+    const double* mi = m + 3 * i;
+    H[3 * i + 0] = mi[1] * dd[2] - mi[2] * dd[1];
+    H[3 * i + 1] = mi[2] * dd[0] - mi[0] * dd[2];
+    H[3 * i + 2] = mi[0] * dd[1] - mi[1] * dd[0];
+  }
+  reduce_store<1, false>(v, partial, counter, sc + S_PG);
+}
+
+// Cayley update m_t = cayley(m, H, s) (src/minimizers.py:71-79)
+__global__ void k_cayley(const double* __restrict__ m, const double* __restrict__ H, double s,
+                         double* __restrict__ out, long N) {
+  NODE_LOOP(i, N) {
+    const double k0 = 0.5 * s * H[3 * i], k1 = 0.5 * s * H[3 * i + 1], k2 = 0.5 * s * H[3 * i + 2];
+    const double m0 = m[3 * i], m1 = m[3 * i + 1], m2 = m[3 * i + 2];
+    const double kk = k0 * k0 + k1 * k1 + k2 * k2;
+    const double km0 = k1 * m2 - k2 * m1, km1 = k2 * m0 - k0 * m2, km2 = k0 * m1 - k1 * m0;
+    const double kd = k0 * m0 + k1 * m1 + k2 * m2;
+    const double inv = 1.0 / (1.0 + kk);
+    double a = ((1.0 - kk) * m0 + 2.0 * km0 + 2.0 * kd * k0) * inv;
+    double b = ((1.0 - kk) * m1 + 2.0 * km1 + 2.0 * kd * k1) * inv;
+    double c = ((1.0 - kk) * m2 + 2.0 * km2 + 2.0 * kd * k2) * inv;
+    const double n = sqrt(a * a + b * b + c * c);
+    out[3 * i] = a / n;
+    out[3 * i + 1] = b / n;
+    out[3 * i + 2] = c / n;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_maxdiff(const double* __restrict__ a, const double* __restrict__ b,
+                                                    long n, double* __restrict__ partial,
+                                                    unsigned* __restrict__ counter, double* __restrict__ out) {
+  double v[1] = {0.0};
+  NODE_LOOP(i, n) v[0] = fmax(v[0], fabs(a[i] - b[i]));
+  reduce_store<1, true>(v, partial, counter, out);
+}
+
+// ----------------------------------------------------------------------- L-BFGS kernels
+// out[slot] = a.b (generic 1-value dot into a scalar slot)
+__global__ __launch_bounds__(kBlock) void k_dot_slot(double* __restrict__ sc, int slot, const double* __restrict__ a,
+                                                     const double* __restrict__ b, long n, double* __restrict__ partial,
+                                                     unsigned* __restrict__ counter) {
+  double v[1] = {0.0};
+  NODE_LOOP(i, n) v[0] += a[i] * b[i];
+  reduce_store<1, false>(v, partial, counter, sc + slot);
+}
+
+// first loop: alpha_idx = rho_idx * sc[S_TMP]; q -= alpha_idx Y_idx
+__global__ void k_lb_first(double* __restrict__ sc, int idx, double* __restrict__ q, const double* __restrict__ Yi,
+                           long n) {
+  const double alpha = sc[S_RHOS + idx] * sc[S_TMP];
+  NODE_LOOP(i, n) q[i] -= alpha * Yi[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_ALPHAS + idx] = alpha;
+}
+
+// y.y_invM for gamma: S_last.Y_last and Y_last.(Y_last invMr)
+__global__ __launch_bounds__(kBlock) void k_lb_gamma_dots(double* __restrict__ sc, const double* __restrict__ S,
+                                                          const double* __restrict__ Y, const double* __restrict__ invMr,
+                                                          long N, double* __restrict__ partial,
+                                                          unsigned* __restrict__ counter) {
+  double v[2] = {0.0, 0.0};
+  NODE_LOOP(i, N) {
+    const double s = invMr[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      v[0] += S[3 * i + k] * Y[3 * i + k];
+      v[1] += Y[3 * i + k] * Y[3 * i + k] * s;
+    }
+  }
+  reduce_store<2, false>(v, partial, counter, sc + S_SY);
+}
+
+// r = gamma q invMr, gamma = clip(sy/(ymy + 1e-30), 1e-3, 1e3) or 1 without history
+__global__ void k_lb_scale(const double* __restrict__ sc, int have_hist, const double* __restrict__ q,
+                           const double* __restrict__ invMr, double* __restrict__ r, long N) {
+  double gam = have_hist ? sc[S_SY] / (sc[S_YMY] + 1e-30) : 1.0;
+  gam = fmin(fmax(gam, 1e-3), 1e3);
+  NODE_LOOP(i, N) {
+    const double s = gam * invMr[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[3 * i + k] = s * q[3 * i + k];
+  }
+}
+
+// second loop: beta = rho_idx * sc[S_TMP]; r += S_idx (alpha_idx - beta)
+__global__ void k_lb_second(const double* __restrict__ sc, int idx, double* __restrict__ r,
+                            const double* __restrict__ Si, long n) {
+  const double f = sc[S_ALPHAS + idx] - sc[S_RHOS + idx] * sc[S_TMP];
+  NODE_LOOP(i, n) r[i] += Si[i] * f;
+}
+
+// d = -r and d.g_ext
+__global__ __launch_bounds__(kBlock) void k_neg_dot(double* __restrict__ sc, const double* __restrict__ r,
+                                                    const double* __restrict__ ge, double* __restrict__ d, long n,
+                                                    double* __restrict__ partial, unsigned* __restrict__ counter) {
+  double v[1] = {0.0};
+  NODE_LOOP(i, n) {
+    const double dk = -r[i];
+    d[i] = dk;
+    v[0] += dk * ge[i];
+  }
+  reduce_store<1, false>(v, partial, counter, sc + S_DG);
+}
+
+// secant pair at m_new: s = T(m, d tau), y = T(m, g_new) - T(m, g_ext_old); curv = y.s, ss = s.s
+__global__ __launch_bounds__(kBlock) void k_lb_pair(double* __restrict__ sc, const double* __restrict__ m,
+                                                    const double* __restrict__ d, double tau,
+                                                    const double* __restrict__ gn, const double* __restrict__ ge,
+                                                    double* __restrict__ s_out, double* __restrict__ y_out, long N,
+                                                    double* __restrict__ partial, unsigned* __restrict__ counter) {
+  double v[2] = {0.0, 0.0};
+  NODE_LOOP(i, N) {
+    double s3[3], a[3], b[3];
+    tangent3(m + 3 * i, d + 3 * i, tau, s3);
+    // T(m, d tau) = tau T(m, d) up to rounding; compute exactly as written: T(m, d*tau)
+    const double dt[3] = {d[3 * i] * tau, d[3 * i + 1] * tau, d[3 * i + 2] * tau};
+    tangent3(m + 3 * i, dt, 1.0, s3);
+    tangent3(m + 3 * i, gn + 3 * i, 1.0, a);
+    tangent3(m + 3 * i, ge + 3 * i, 1.0, b);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double yk = a[k] - b[k];
+      s_out[3 * i + k] = s3[k];
+      y_out[3 * i + k] = yk;
+      v[0] += yk * s3[k];
+      v[1] += s3[k] * s3[k];
+    }
+  }
+  reduce_store<2, false>(v, partial, counter, sc + S_CURV);
+}
+
+// transport every history vector to the tangent space at m (src/minimizers.py:1342-1347)
+__global__ void k_lb_transport(const double* __restrict__ m, double* __restrict__ S, double* __restrict__ Y, int mem,
+                               long N) {
+  NODE_LOOP(i, N) {
+    const double m0 = m[3 * i], m1 = m[3 * i + 1], m2 = m[3 * i + 2];
+    for (int h = 0; h < mem; ++h) {
+      double* s = S + (long)h * 3 * N + 3 * i;
+      double* y = Y + (long)h * 3 * N + 3 * i;
+      const double ds = m0 * s[0] + m1 * s[1] + m2 * s[2];
+      s[0] -= ds * m0;
+      s[1] -= ds * m1;
+      s[2] -= ds * m2;
+      const double dy = m0 * y[0] + m1 * y[1] + m2 * y[2];
+      y[0] -= dy * m0;
+      y[1] -= dy * m1;
+      y[2] -= dy * m2;
+    }
+  }
+}
+
+__global__ void k_set_slot(double* __restrict__ sc, int slot, double v) { sc[slot] = v; }
+
+// ------------------------------------------------------------------------- host driver
+struct Red {
+  double* partial;
+  unsigned* counter;
+};
+
+static int read_slots(Ctx& c, int first, int count) {
+  MM_HIP(hipMemcpyAsync(c.hscal + first, c.dscal + first, count * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+// Energy and gradient of (m, U) into g (zeroed here); energy left in slot S_E.
+static int eval_energy(Ctx& c, const double* m, const double* U, const double B[3], double* g) {
+  vfill(g, 0.0, 3L * c.N, c.stream);
+  MM_TRY(launch_energy(c, m, U, B, g, c.dscal + S_E, c.stream));
+  c.n_energy++;
+  return 0;
+}
+
+// Steihaug-PCG solve of P y = g_ext into c.y; iterations in slot S_PCIT.
+static int solve_P(Ctx& c, const MinParams& p) {
+  const long N = c.N;
+  hipStream_t s = c.stream;
+  double* sc = c.dscal;
+  unsigned* cnt = c.counter + 8;
+  k_pc_init<<<red_grid(N), kBlock, 0, s>>>(c.g_ext, c.inv_M_prec, c.y, c.r3, c.z3, c.p3, c.cv3, N, c.partial, cnt,
+                                            sc + S_RHO);
+  k_pc_state_init<<<1, 1, 0, s>>>(sc, p.pc_auto, p.pc_force_eta, p.pc_force_alpha, p.pc_iters);
+  for (int k = 0; k < p.pc_iters; ++k) {
+    MM_TRY(launch_local(c, c.p3, c.cv3, sc + S_ACT, s));
+    c.n_local++;
+    k_pc_ap<<<red_grid(N), kBlock, 0, s>>>(sc, c.m, c.g_raw, c.p3, c.cv3, c.q3, p.pc_reg, p.pc_stagnation_nu, N,
+                                            c.partial, cnt);
+    k_pc_yr<<<red_grid(N), kBlock, 0, s>>>(sc, c.y, c.r3, c.z3, c.p3, c.q3, c.inv_M_prec, N, c.partial, cnt);
+    k_pc_p<<<red_grid(N), kBlock, 0, s>>>(sc, c.z3, c.p3, c.cv3, p.pc_iters, N, cnt);
+  }
+  k_pc_norms<<<red_grid(N), kBlock, 0, s>>>(sc, c.y, c.g_ext, c.inv_M_prec, N, c.partial, cnt);
+  k_pc_finish<<<vg(N), kBlock, 0, s>>>(sc, c.y, c.g_ext, c.inv_M_prec, N);
+  return 0;
+}
+
+struct LSResult {
+  double tau, E;
+  int evals, demag;
+  bool moved;  // the trial buffers hold the accepted state
+};
+
+// Curvilinear Armijo line search (make_armijo_ls_v2). Trials are written to m_t, U_t, g_t;
+// on return with moved == true they hold the accepted state.
+static int line_search(Ctx& c, const MinParams& p, const double B[3], double pg, double E0, double s0, LSResult& out) {
+  out.tau = 0.0;
+  out.E = E0;
+  out.evals = 0;
+  out.demag = 0;
+  out.moved = false;
+  if (!(pg < 0.0)) return 0;
+  MM_TRY(vcopy(c.U_t, c.U, c.N, c.stream));
+  auto D = [&](double s, double& d, double& E) -> int {
+    k_cayley<<<vg(c.N), kBlock, 0, c.stream>>>(c.m, c.H, s, c.m_t, c.N);
+    int it = 0;
+    double rr = 0;
+    MM_TRY(solve_U_dev(c, c.m_t, c.U_t, p.phi_tol, p.cg_maxiter, &it, &rr));
+    out.demag += it;
+    MM_TRY(eval_energy(c, c.m_t, c.U_t, B, c.g_t));
+    MM_TRY(read_slots(c, S_E, 1));
+    E = c.hscal[S_E];
+    if (!std::isfinite(E)) E = 1e20;
+    d = (E - E0) / (s * pg + 1e-30);
+    return 0;
+  };
+  const int maxe = p.ls_max_evals;
+  double s = s0, s_min = 0.0, d_v = 0.0, E_v = E0;
+  int it_exp = 0;
+  bool done = false;
+  while (it_exp < maxe && !done) {
+    double dn, En;
+    MM_TRY(D(s, dn, En));
+    const bool stop = std::fabs(1.0 - dn) >= p.ls_eta2 || dn < 0.0;
+    const double s_next = stop ? s : p.ls_C * s;
+    if (!stop) s_min = s;
+    s = s_next;
+    d_v = dn;
+    E_v = En;
+    ++it_exp;
+    done = stop;
+  }
+  const double s_min_exp = s_min;
+  done = d_v >= p.ls_eta1 && d_v < 1e10;
+  int it_con = 0;
+  while (it_con < maxe && !done) {
+    s = s_min_exp + p.ls_c * (s - s_min_exp);
+    MM_TRY(D(s, d_v, E_v));
+    done = d_v >= p.ls_eta1 && d_v < 1e10;
+    ++it_con;
+  }
+  out.evals = it_exp + it_con;
+  if (d_v >= 0.0) {
+    out.tau = s;
+    out.E = E_v;
+    out.moved = true;
+  }
+  return 0;
+}
+
+int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& info) {
+  MM_TRY(alloc_min_state(c));
+  const auto t0 = std::chrono::steady_clock::now();
+  const long N = c.N, n3 = 3 * N;
+  hipStream_t s = c.stream;
+  double* sc = c.dscal;
+  unsigned* cnt = c.counter + 9;
+  const int mem = p.memory > 0 ? p.memory : 10;
+  if (p.method == 1 && (!c.S || c.mem != mem)) {
+    c.mem = mem;
+    double* S = nullptr;
+    double* Y = nullptr;
+    MM_HIP(hipMalloc(&S, (size_t)mem * n3 * sizeof(double)));
+    MM_HIP(hipMalloc(&Y, (size_t)mem * n3 * sizeof(double)));
+    c.allocs.push_back(S);
+    c.allocs.push_back(Y);
+    c.S = S;
+    c.Yh = Y;
+  }
+
+  // solve_and_minimize (src/minimizers.py:3408-3435)
+  k_normalize<<<vg(N), kBlock, 0, s>>>(c.m, N);
+  int demag = 0;
+  {
+    int it = 0;
+    double rr = 0;
+    MM_TRY(solve_U_dev(c, c.m, c.U, p.cg_tol, p.cg_maxiter, &it, &rr));
+    demag = it;
+  }
+  MM_TRY(eval_energy(c, c.m, c.U, B, c.g_raw));
+  k_tangents<<<red_grid(N), kBlock, 0, s>>>(c.m, c.g_raw, c.inv_M_rel, c.g_tan, c.g_ext, N, c.partial, cnt, sc + S_GN);
+  MM_TRY(read_slots(c, S_E, 1));
+  double E = c.hscal[S_E];
+  MM_TRY(read_slots(c, S_GN, 1));
+  double gnorm = c.hscal[S_GN];
+  int evals = 1, preco = 0, it = 0;
+  bool conv = false;
+  const int Lp = p.L > 0 ? p.L : (int)N;
+
+  if (p.method == 0) {
+    // PCGState init: g = g_ext, y = g_ext, d = -g_ext
+    MM_TRY(vcopy(c.g_prev, c.g_ext, n3, s));
+    hipLaunchKernelGGL(k_neg_dot, dim3(red_grid(n3)), dim3(kBlock), 0, s, sc, c.g_ext, c.g_ext, c.d_prev, n3, c.partial,
+                       cnt);
+  } else {
+    vfill(c.S, 0.0, (long)mem * n3, s);
+    vfill(c.Yh, 0.0, (long)mem * n3, s);
+    for (int h = 0; h < mem; ++h) k_set_slot<<<1, 1, 0, s>>>(sc, S_RHOS + h, 0.0);
+  }
+  int hist = 0;
+  std::vector<double> h_rho(mem, 0.0);
+
+  while (!conv && it < p.max_iter) {
+    k_tangents<<<red_grid(N), kBlock, 0, s>>>(c.m, c.g_raw, c.inv_M_rel, c.g_tan, c.g_ext, N, c.partial, cnt,
+                                               sc + S_GN);
+    double s0 = 1.0;
+    if (p.method == 0) {
+      MM_TRY(solve_P(c, p));
+      vmaxabs(c, c.y, n3, sc + S_GS, s);
+      k_beta_dots<<<red_grid(n3), kBlock, 0, s>>>(sc, c.y, c.g_ext, c.g_prev, c.d_prev, n3, c.partial, cnt);
+      k_direction<<<red_grid(N), kBlock, 0, s>>>(sc, (it % Lp) == 0, c.m, c.y, c.d_prev, c.g_ext, c.d, N, c.partial,
+                                                  cnt);
+      k_direction_fix<<<red_grid(N), kBlock, 0, s>>>(sc, 0, c.m, c.y, c.g_raw, c.d, c.H, N, c.partial, cnt);
+    } else {
+      // two-loop recursion on q = g_ext (src/minimizers.py:1284-1312); q in r3, r in z3
+      MM_TRY(vcopy(c.r3, c.g_ext, n3, s));
+      const int nh = std::min(hist, mem);
+      for (int i = 0; i < nh; ++i) {
+        const int idx = ((hist - 1 - i) % mem + mem) % mem;
+        k_dot_slot<<<red_grid(n3), kBlock, 0, s>>>(sc, S_TMP, c.S + (long)idx * n3, c.r3, n3, c.partial, cnt);
+        k_lb_first<<<vg(n3), kBlock, 0, s>>>(sc, idx, c.r3, c.Yh + (long)idx * n3, n3);
+      }
+      if (hist > 0) {
+        const int last = ((hist - 1) % mem + mem) % mem;
+        k_lb_gamma_dots<<<red_grid(N), kBlock, 0, s>>>(sc, c.S + (long)last * n3, c.Yh + (long)last * n3, c.inv_M_rel,
+                                                        N, c.partial, cnt);
+      }
+      k_lb_scale<<<vg(N), kBlock, 0, s>>>(sc, hist > 0, c.r3, c.inv_M_rel, c.z3, N);
+      for (int i = 0; i < nh; ++i) {
+        const int idx = ((hist - nh + i) % mem + mem) % mem;
+        k_dot_slot<<<red_grid(n3), kBlock, 0, s>>>(sc, S_TMP, c.Yh + (long)idx * n3, c.z3, n3, c.partial, cnt);
+        k_lb_second<<<vg(n3), kBlock, 0, s>>>(sc, idx, c.z3, c.S + (long)idx * n3, n3);
+      }
+      k_neg_dot<<<red_grid(n3), kBlock, 0, s>>>(sc, c.z3, c.g_ext, c.d, n3, c.partial, cnt);
+      k_direction_fix<<<red_grid(N), kBlock, 0, s>>>(sc, it == 0, c.m, c.g_tan, c.g_raw, c.d, c.H, N, c.partial, cnt);
+      s0 = (it == 0) ? p.tau0 : 1.0;
+    }
+    MM_TRY(read_slots(c, S_GN, 8));  // GN, GS, NUM, DEN, DG, PG, DM, PCIT
+    const double gn_tan = c.hscal[S_GN];
+    const double gs = c.hscal[S_GS];
+    const double pg = c.hscal[S_PG];
+    const int pit = (p.method == 0) ? (int)c.hscal[S_PCIT] : 0;
+    const double gconv = (p.method == 0) ? gs : gn_tan;
+
+    LSResult ls;
+    MM_TRY(line_search(c, p, B, pg, E, s0, ls));
+    double dm = 0.0;
+    if (ls.moved) {
+      k_maxdiff<<<red_grid(n3), kBlock, 0, s>>>(c.m_t, c.m, n3, c.partial, cnt, sc + S_DM);
+      MM_TRY(read_slots(c, S_DM, 1));
+      dm = c.hscal[S_DM];
+    }
+    const double E_new = ls.E;
+    // check_convergence (src/minimizers.py:160-172)
+    if (it > 0) {
+      const bool u1 = (E - E_new) < p.tau_f * (1.0 + std::fabs(E_new));
+      const bool u2 = dm < std::sqrt(p.tau_f) * 2.0;
+      const bool u3 = gconv <= std::pow(p.tau_f, 1.0 / 3.0) * (1.0 + std::fabs(E_new));
+      const bool u4 = gconv < p.eps_a;
+      conv = (u1 && u2 && u3) || u4;
+    }
+    if (ls.moved) {
+      std::swap(c.m, c.m_t);
+      std::swap(c.U, c.U_t);
+      std::swap(c.g_raw, c.g_t);
+    }
+    if (p.method == 0) {
+      std::swap(c.g_prev, c.g_ext);
+      std::swap(c.d_prev, c.d);
+      gnorm = gs;
+    } else {
+      // secant pair, transport of the history, conditional insertion (src/minimizers.py:1330-1360)
+      k_lb_pair<<<red_grid(N), kBlock, 0, s>>>(sc, c.m, c.d, ls.tau, c.g_raw, c.g_ext, c.q3, c.cv3, N, c.partial, cnt);
+      MM_TRY(read_slots(c, S_CURV, 2));
+      const double curv = c.hscal[S_CURV], ss = c.hscal[S_SS];
+      const bool ok = curv > 1e-12 * ss;
+      const int idx = hist % mem;
+      k_lb_transport<<<vg(N), kBlock, 0, s>>>(c.m, c.S, c.Yh, mem, N);
+      if (ok) {
+        MM_TRY(vcopy(c.S + (long)idx * n3, c.q3, n3, s));
+        MM_TRY(vcopy(c.Yh + (long)idx * n3, c.cv3, n3, s));
+        k_set_slot<<<1, 1, 0, s>>>(sc, S_RHOS + idx, 1.0 / (curv + 1e-30));
+        ++hist;
+      }
+      gnorm = gn_tan;
+    }
+    E = E_new;
+    evals += ls.evals;
+    preco += pit;
+    demag += ls.demag;
+    ++it;
+  }
+  MM_HIP(hipStreamSynchronize(s));
+  MM_HIP(hipGetLastError());
+  info.iters = it;
+  info.evals = evals;
+  info.demag_iters = demag;
+  info.preco_iters = preco;
+  info.converged = conv ? 1 : 0;
+  info.E = E;
+  info.gnorm = gnorm;
+  info.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+}  // namespace mm

@@ -1,0 +1,289 @@
+// Nodal vector kernels, CSR SpMV for the coarse AMG levels, and the preconditioned CG
+// for the demagnetising potential (``make_pcg_solve`` + ``make_jax_amgcl_vcycle`` /
+// ``make_jax_amg_vcycle`` semantics, src/poisson_solve.py:422-478, src/amg_utils.py:491-623).
+//
+// All PCG scalars live in device memory: alpha and beta are formed inside the update
+// kernels from the dot products written by the previous kernel, so the host never waits
+// inside an iteration except to read r.r for the stopping test, and that read overlaps
+// the V-cycle of the same iteration.
+#include <cmath>
+
+#include "mm_device.h"
+#include "mm_solver.h"
+
+namespace mm {
+
+// ------------------------------------------------------------------------ vector kernels
+__global__ void k_fill(double* __restrict__ x, double v, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
+}
+
+__global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot(const double* __restrict__ a, const double* __restrict__ b, long n,
+                                                double* __restrict__ partial, unsigned* __restrict__ counter,
+                                                double* __restrict__ out) {
+  double v[1] = {0.0};
+  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) v[0] += a[i] * b[i];
+  reduce_store<1, false>(v, partial, counter, out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_maxabs(const double* __restrict__ a, long n, double* __restrict__ partial,
+                                                   unsigned* __restrict__ counter, double* __restrict__ out) {
+  double v[1] = {0.0};
+  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) v[0] = fmax(v[0], fabs(a[i]));
+  reduce_store<1, true>(v, partial, counter, out);
+}
+
+static inline int vgrid(long n) {
+  long g = (n + kBlock - 1) / kBlock;
+  if (g > 8192) g = 8192;
+  return (int)(g < 1 ? 1 : g);
+}
+
+int vfill(double* x, double v, long n, hipStream_t s) {
+  if (n > 0) k_fill<<<vgrid(n), kBlock, 0, s>>>(x, v, n);
+  return 0;
+}
+int vcopy(double* d, const double* src, long n, hipStream_t s) {
+  if (n > 0) k_copy<<<vgrid(n), kBlock, 0, s>>>(d, src, n);
+  return 0;
+}
+int vdot(const Ctx& c, const double* a, const double* b, long n, double* out, hipStream_t s) {
+  k_dot<<<red_grid(n), kBlock, 0, s>>>(a, b, n, c.partial, c.counter + 3, out);
+  return 0;
+}
+int vmaxabs(const Ctx& c, const double* a, long n, double* out, hipStream_t s) {
+  k_maxabs<<<red_grid(n), kBlock, 0, s>>>(a, n, c.partial, c.counter + 3, out);
+  return 0;
+}
+
+// ----------------------------------------------------------------------------- PCG kernels
+// r = b - A x0 (A x0 in q); z = M0 r handled by the preconditioner; here: r, r.r and b.b.
+__global__ __launch_bounds__(kBlock) void k_pcg_init(const double* __restrict__ b, const double* __restrict__ q,
+                                                     double* __restrict__ r, long n, double* __restrict__ partial,
+                                                     unsigned* __restrict__ counter, double* __restrict__ out) {
+  double v[2] = {0.0, 0.0};
+  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    const double ri = b[i] - q[i];
+    r[i] = ri;
+    v[0] += ri * ri;
+    v[1] += b[i] * b[i];
+  }
+  reduce_store<2, false>(v, partial, counter, out);  // out[0] = r.r, out[1] = b.b
+}
+
+// alpha = rz / (p.q + 1e-30); x += alpha p; r -= alpha q; out = r.r
+__global__ __launch_bounds__(kBlock) void k_pcg_xr(const double* __restrict__ sc, int rz_slot, int pq_slot,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   const double* __restrict__ p, const double* __restrict__ q, long n,
+                                                   double* __restrict__ partial, unsigned* __restrict__ counter,
+                                                   double* __restrict__ out) {
+  const double alpha = sc[rz_slot] / (sc[pq_slot] + 1e-30);
+  double v[1] = {0.0};
+  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    v[0] += ri * ri;
+  }
+  reduce_store<1, false>(v, partial, counter, out);
+}
+
+// beta = rz_new / (rz_old + 1e-30); p = z + beta p  (first == 1: p = z)
+__global__ void k_pcg_p(const double* __restrict__ sc, int new_slot, int old_slot, int first,
+                        const double* __restrict__ z, double* __restrict__ p, long n) {
+  const double beta = first ? 0.0 : sc[new_slot] / (sc[old_slot] + 1e-30);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = first ? z[i] : z[i] + beta * p[i];
+}
+
+// x = M .* b
+__global__ void k_scale(const double* __restrict__ M, const double* __restrict__ b, double* __restrict__ x, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = M[i] * b[i];
+}
+
+// z = x + M .* (b - t), and out = r.z with r == b (level-0 post-smoothing fused with the dot)
+__global__ __launch_bounds__(kBlock) void k_post_dot(const double* __restrict__ x, const double* __restrict__ M,
+                                                     const double* __restrict__ b, const double* __restrict__ t,
+                                                     double* __restrict__ z, long n, double* __restrict__ partial,
+                                                     unsigned* __restrict__ counter, double* __restrict__ out) {
+  double v[1] = {0.0};
+  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    const double zi = x[i] + M[i] * (b[i] - t[i]);
+    z[i] = zi;
+    v[0] += b[i] * zi;
+  }
+  reduce_store<1, false>(v, partial, counter, out);
+}
+
+// x += M .* (b - t)
+__global__ void k_post(double* __restrict__ x, const double* __restrict__ M, const double* __restrict__ b,
+                       const double* __restrict__ t, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] += M[i] * (b[i] - t[i]);
+}
+
+// CSR SpMV with 8 lanes per row. MODE 0: y = A x; 1: y += A x; 2: y = A (x - x2)
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv(int nrows, const int* __restrict__ Ap, const int* __restrict__ Ai,
+                                                 const double* __restrict__ Ax, const double* __restrict__ x,
+                                                 const double* __restrict__ x2, double* __restrict__ y) {
+  const int lane = threadIdx.x & 7;
+  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 3;
+  double s = 0.0;
+  if (row < nrows) {
+    const int j0 = Ap[row], j1 = Ap[row + 1];
+    for (int j = j0 + lane; j < j1; j += 8) {
+      const int c = Ai[j];
+      const double xv = (MODE == 2) ? (x[c] - x2[c]) : x[c];
+      s += Ax[j] * xv;
+    }
+  }
+  s += __shfl_down(s, 4, 8);
+  s += __shfl_down(s, 2, 8);
+  s += __shfl_down(s, 1, 8);
+  if (row < nrows && lane == 0) {
+    if (MODE == 1)
+      y[row] += s;
+    else
+      y[row] = s;
+  }
+}
+
+// y = Ainv b, dense row-major n x n: one wave per row
+__global__ __launch_bounds__(kBlock) void k_gemv(int n, const double* __restrict__ Ainv, const double* __restrict__ b,
+                                                 double* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (row >= n) return;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += Ainv[(long)row * n + j] * b[j];
+  s = wave_red<false>(s);
+  if (lane == 0) y[row] = s;
+}
+
+static void spmv(int mode, int nrows, const int* Ap, const int* Ai, const double* Ax, const double* x,
+                 const double* x2, double* y, hipStream_t s) {
+  if (nrows <= 0) return;
+  const int grid = (int)(((long)nrows * 8 + kBlock - 1) / kBlock);
+  if (mode == 0)
+    k_spmv<0><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y);
+  else if (mode == 1)
+    k_spmv<1><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y);
+  else
+    k_spmv<2><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y);
+}
+
+// ----------------------------------------------------------------------------- V-cycle
+// Coarse part of the V-cycle on levels l >= 1: returns the correction in lv[l].x.
+static void vcycle_coarse(Ctx& c, int l, hipStream_t s) {
+  Level& L = c.lv[l];
+  const int last = (int)c.lv.size() - 1;
+  if (l == last) {
+    k_gemv<<<(int)(((long)L.n * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>(L.n, L.dense_inv, L.b, L.x);
+    return;
+  }
+  Level& Nx = c.lv[l + 1];
+  k_scale<<<vgrid(L.n), kBlock, 0, s>>>(L.M, L.b, L.x, L.n);                     // pre-smooth from x = 0
+  spmv(0, L.n, L.Ap, L.Ai, L.Ax, L.x, nullptr, L.t, s);                           // t = A x
+  spmv(2, Nx.n, L.Rp, L.Ri, L.Rx, L.b, L.t, Nx.b, s);                             // b' = R (b - t)
+  vcycle_coarse(c, l + 1, s);
+  spmv(1, L.n, L.Pp, L.Pi, L.Px, Nx.x, nullptr, L.x, s);                          // x += P e
+  spmv(0, L.n, L.Ap, L.Ai, L.Ax, L.x, nullptr, L.t, s);                           // t = A x
+  k_post<<<vgrid(L.n), kBlock, 0, s>>>(L.x, L.M, L.b, L.t, L.n);                  // x += M (b - t)
+}
+
+// z = V-cycle(r) with the matrix-free level 0; also writes r.z into out.
+static int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
+  const long N = c.N;
+  Level& L0 = c.lv[0];
+  if (c.lv.size() <= 1) {
+    if (L0.dense_inv)  // single-level hierarchy: exact dense solve
+      k_gemv<<<(int)((N * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>((int)N, L0.dense_inv, r, z);
+    else  // Jacobi on diag(A)
+      k_scale<<<vgrid(N), kBlock, 0, s>>>(L0.M, r, z, N);
+    vdot(c, r, z, N, rz_out, s);
+    return 0;
+  }
+  Level& L1 = c.lv[1];
+  k_scale<<<vgrid(N), kBlock, 0, s>>>(L0.M, r, c.px0, N);                         // x = M0 r
+  vfill(c.pt, 0.0, N, s);
+  MM_TRY(launch_apply(c, c.geom_mode, c.px0, c.pt, s));                           // t = A x
+  spmv(2, L1.n, L0.Rp, L0.Ri, L0.Rx, r, c.pt, L1.b, s);                           // b1 = R0 (r - t)
+  vcycle_coarse(c, 1, s);
+  spmv(1, (int)N, L0.Pp, L0.Pi, L0.Px, L1.x, nullptr, c.px0, s);                  // x += P0 e1
+  vfill(c.pt, 0.0, N, s);
+  MM_TRY(launch_apply(c, c.geom_mode, c.px0, c.pt, s));                           // t = A x
+  k_post_dot<<<red_grid(N), kBlock, 0, s>>>(c.px0, L0.M, r, c.pt, z, N, c.partial, c.counter + 4, rz_out);
+  c.n_apply += 2;
+  c.n_vcycle += 1;
+  return 0;
+}
+
+int precond_apply(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
+  MM_CHECK(c.lv.size() >= 1 && c.lv[0].M != nullptr, "preconditioner not set up");
+  return precond(c, r, z, rz_out, s);
+}
+
+// Device-resident PCG: U (in/out, device) solves A U = b (device), stop when
+// it >= maxiter or r.r <= tol^2 b.b. Returns iterations and r.r / b.b via the host mirror.
+int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it_out, double* r2_out,
+            double* bb_out) {
+  hipStream_t s = c.stream;
+  const long N = c.N;
+  MM_CHECK(c.lv.size() >= 1 && c.lv[0].M != nullptr, "preconditioner not set up (call mm_amg_upload or mm_precond_jacobi)");
+  double* sc = c.dscal;
+  // slots: 0/1 rz ping-pong, 2 p.q, 3 r.r, 4 b.b
+  vfill(c.pq, 0.0, N, s);
+  MM_TRY(launch_apply(c, c.geom_mode, U, c.pq, s));
+  c.n_apply++;
+  k_pcg_init<<<red_grid(N), kBlock, 0, s>>>(b, c.pq, c.pr, N, c.partial, c.counter + 5, sc + 3);
+  MM_TRY(precond(c, c.pr, c.pz, sc + 0, s));
+  k_pcg_p<<<vgrid(N), kBlock, 0, s>>>(sc, 0, 0, 1, c.pz, c.pp, N);
+  MM_HIP(hipMemcpyAsync(c.hscal + 3, sc + 3, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  MM_HIP(hipStreamSynchronize(s));
+  const double bb = c.hscal[4];
+  const double tol2 = tol * tol * bb;
+  double r2 = c.hscal[3];
+  int it = 0;
+  int cur = 0;  // slot holding the current r.z
+  while (it < maxiter && r2 > tol2) {
+    vfill(c.pq, 0.0, N, s);
+    MM_TRY(launch_apply(c, c.geom_mode, c.pp, c.pq, s));
+    c.n_apply++;
+    vdot(c, c.pp, c.pq, N, sc + 2, s);
+    k_pcg_xr<<<red_grid(N), kBlock, 0, s>>>(sc, cur, 2, U, c.pr, c.pp, c.pq, N, c.partial, c.counter + 5, sc + 3);
+    MM_HIP(hipMemcpyAsync(c.hscal + 3, sc + 3, sizeof(double), hipMemcpyDeviceToHost, s));
+    MM_HIP(hipEventRecord(c.ev, s));
+    // preconditioner and direction update run while the host waits for r.r
+    MM_TRY(precond(c, c.pr, c.pz, sc + (1 - cur), s));
+    k_pcg_p<<<vgrid(N), kBlock, 0, s>>>(sc, 1 - cur, cur, 0, c.pz, c.pp, N);
+    cur = 1 - cur;
+    ++it;
+    MM_HIP(hipEventSynchronize(c.ev));
+    r2 = c.hscal[3];
+  }
+  MM_HIP(hipStreamSynchronize(s));
+  c.n_pcg_it += it;
+  *it_out = it;
+  *r2_out = r2;
+  *bb_out = bb;
+  return 0;
+}
+
+// solve_U (src/poisson_solve.py:748-771): b = D m, PCG from the initial guess in U.
+int solve_U_dev(Ctx& c, const double* m, double* U, double tol, int maxiter, int* it, double* rel_res) {
+  hipStream_t s = c.stream;
+  vfill(c.pb, 0.0, c.N, s);
+  MM_TRY(launch_rhs(c, m, c.pb, s));
+  c.n_rhs++;
+  double r2 = 0, bb = 0;
+  MM_TRY(pcg_dev(c, c.pb, U, tol, maxiter, it, &r2, &bb));
+  *rel_res = std::sqrt(r2 / (bb + 1e-30));
+  return 0;
+}
+
+}  // namespace mm

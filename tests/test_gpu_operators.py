@@ -1,0 +1,95 @@
+"""GPU parity of the element kernels against the oracle, through the C-ABI (FP64).
+
+Bar: relative L2 difference <= 1e-12 (the FP64 path's parity target); the only
+differences allowed are summation order (LDS / global FP64 atomics) and the on-the-fly
+gradient arithmetic.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import ndfeb_materials, tilted_m0
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def ctx_c0(cube_c0):
+    from mumag_matrixfree2_amd.context import MeshContext
+
+    A, K1, Js, k = ndfeb_materials()
+    m = cube_c0
+    print(f"[ctx_c0] N={m['knt'].shape[0]} E={m['ijk'].shape[0]}", flush=True)
+    return MeshContext(m["knt"], m["ijk"][:, :4], m["ijk"][:, 4], A, K1, Js, k, boundary_mask=m["boundary_mask"])
+
+
+def test_context_setup_matches_oracle(ctx_c0, oracle_c0):
+    assert abs(ctx_c0.V_mag - oracle_c0.V_mag) <= 1e-12 * oracle_c0.V_mag
+    M, invMr, invMp = ctx_c0.nodal()
+    assert rel(M, oracle_c0.M_nodal) <= TOL
+    assert rel(invMr, oracle_c0.inv_M_rel) <= TOL
+    assert rel(invMp, oracle_c0.inv_M_prec) <= TOL
+    assert ctx_c0.max_block_nodes <= 256
+
+
+@pytest.mark.parametrize("geom", ["on_the_fly", "stored_f64"])
+def test_poisson_apply(ctx_c0, oracle_c0, geom):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(ctx_c0.N)
+    ctx_c0.set_geometry_mode(geom)
+    try:
+        y = ctx_c0.apply_A(x)
+    finally:
+        ctx_c0.set_geometry_mode("on_the_fly")
+    assert rel(y, oracle_c0.A @ x) <= TOL
+
+
+def test_poisson_apply_fp32_geometry(ctx_c0, oracle_c0):
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(ctx_c0.N)
+    ctx_c0.set_geometry_mode("stored_f32")
+    try:
+        y = ctx_c0.apply_A(x)
+    finally:
+        ctx_c0.set_geometry_mode("on_the_fly")
+    assert rel(y, oracle_c0.A @ x) <= 1e-6
+
+
+def test_poisson_diag(ctx_c0, oracle_c0):
+    assert rel(ctx_c0.diag(), oracle_c0.A.diagonal()) <= TOL
+
+
+def test_rhs(ctx_c0, oracle_c0):
+    m = tilted_m0(ctx_c0.N, seed=3, sigma=0.3)
+    assert rel(ctx_c0.rhs(m), oracle_c0.rhs(m)) <= TOL
+
+
+def test_energy_and_grad(ctx_c0, oracle_c0):
+    m = tilted_m0(ctx_c0.N, seed=4, sigma=0.3)
+    U = np.random.default_rng(5).standard_normal(ctx_c0.N) * 0.1
+    B = np.array([0.01, -0.02, -0.3])
+    E, g = ctx_c0.energy_grad(m, U, B)
+    E_ref, g_ref = oracle_c0.energy_and_grad(m, U, B)
+    terms = oracle_c0.energy_terms(m, U, B)
+    scale = sum(abs(v) for v in terms.values())
+    assert abs(E - E_ref) <= TOL * scale
+    assert rel(g, g_ref) <= TOL
+
+
+def test_local_grad(ctx_c0, oracle_c0):
+    v = np.random.default_rng(6).standard_normal((ctx_c0.N, 3))
+    assert rel(ctx_c0.local_grad(v), oracle_c0.local_grad(v)) <= TOL
+
+
+def test_j_parallel(ctx_c0, oracle_c0):
+    m = tilted_m0(ctx_c0.N, seed=7, sigma=0.5)
+    h = np.array([0.017452, 0.0, 0.999848])
+    assert abs(ctx_c0.j_parallel(m, h) - oracle_c0.J_parallel(m, h)) <= 1e-13

@@ -1,0 +1,96 @@
+"""GPU parity of the preconditioner and solve_U (PCG + AMG V-cycle) against the oracle (FP64)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import ndfeb_materials, oracle_levels, tilted_m0
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+@pytest.fixture(scope="module", params=["amgcl", "amg"])
+def solver_ctx(request, cube_c0):
+    from mumag_matrixfree2_amd.context import MeshContext
+
+    A, K1, Js, k = ndfeb_materials()
+    m = cube_c0
+    ctx = MeshContext(m["knt"], m["ijk"][:, :4], m["ijk"][:, 4], A, K1, Js, k, boundary_mask=m["boundary_mask"])
+    ctx.setup_precond(request.param)
+    print(f"[solver_ctx] kind={request.param} levels={[l['A'].shape[0] for l in ctx.amg_levels]}", flush=True)
+    return request.param, ctx
+
+
+def test_vcycle_matches_oracle(solver_ctx, oracle_c0):
+    from oracle.solvers import vcycle
+
+    kind, ctx = solver_ctx
+    r = np.random.default_rng(11).standard_normal(ctx.N)
+    z = ctx.precond_apply(r)
+    z_ref = vcycle(oracle_levels(ctx.amg_levels, kind), oracle_c0.apply_A, r, kind)
+    assert rel(z, z_ref) <= 1e-11
+    assert float(r @ z) > 0.0
+
+
+def test_solve_U_matches_oracle_pcg(solver_ctx, oracle_c0):
+    from oracle.solvers import solve_U
+
+    kind, ctx = solver_ctx
+    m = tilted_m0(ctx.N, seed=12, sigma=0.4)
+    x0 = np.zeros(ctx.N)
+    U, it, rr = ctx.solve_U(m, x0, 1e-10)
+    U_ref, it_ref, rr_ref = solve_U(oracle_c0, m, x0, 1e-10, precond="amg",
+                                    levels=oracle_levels(ctx.amg_levels, kind), kind=kind)
+    print(f"[{kind}] it={it} it_ref={it_ref} rr={rr:.3e} rr_ref={rr_ref:.3e}", flush=True)
+    assert abs(it - it_ref) <= 1
+    assert rr <= 1e-10
+    assert rel(U, U_ref) <= 1e-9
+    # true residual of the GPU solution, computed by the oracle
+    b = oracle_c0.rhs(m)
+    assert np.linalg.norm(b - oracle_c0.A @ U) <= 1.0001e-10 * np.linalg.norm(b)
+
+
+def test_solve_U_matches_direct(solver_ctx, oracle_c0):
+    from oracle.solvers import solve_U
+
+    kind, ctx = solver_ctx
+    m = np.zeros((ctx.N, 3))
+    m[:, 0] = 1.0
+    U, it, rr = ctx.solve_U(m, np.zeros(ctx.N), 1e-12)
+    U_ref, _, _ = solve_U(oracle_c0, m, np.zeros(ctx.N), 1e-12, precond="direct")
+    assert rel(U, U_ref) <= 1e-9
+
+
+def test_solve_U_warm_start_zero_iterations(solver_ctx, oracle_c0):
+    """A converged initial guess returns after 0 iterations with U unchanged."""
+    from oracle.solvers import solve_U
+
+    kind, ctx = solver_ctx
+    m = tilted_m0(ctx.N, seed=13, sigma=0.2)
+    U_ref, _, _ = solve_U(oracle_c0, m, np.zeros(ctx.N), 1e-13, precond="direct")
+    U, it, rr = ctx.solve_U(m, U_ref, 1e-8)
+    assert it == 0
+    assert np.array_equal(U, U_ref)
+
+
+def test_jacobi_pcg(cube_c0, oracle_c0):
+    from mumag_matrixfree2_amd.context import MeshContext
+    from oracle.solvers import pcg
+
+    A, K1, Js, k = ndfeb_materials()
+    m = cube_c0
+    ctx = MeshContext(m["knt"], m["ijk"][:, :4], m["ijk"][:, 4], A, K1, Js, k, boundary_mask=m["boundary_mask"])
+    ctx.setup_precond("jacobi")
+    mm = tilted_m0(ctx.N, seed=14, sigma=0.4)
+    U, it, rr = ctx.solve_U(mm, np.zeros(ctx.N), 1e-9, maxiter=5000)
+    d = oracle_c0.A.diagonal()
+    b = oracle_c0.rhs(mm)
+    U_ref, it_ref, _ = pcg(oracle_c0.apply_A, lambda r: r / (d + 1e-30), b, np.zeros(ctx.N), 1e-9, 5000)
+    assert abs(it - it_ref) <= 2
+    assert rel(U, U_ref) <= 1e-7
+    ctx.close()
