@@ -73,6 +73,9 @@ int mm_ctx_get_geometry(int64_t ctx, int32_t* conn, double* Ve, double* grad);
 /* M_nodal, inv_M_rel and inv_M_prec (N,) (src/loop.py:929-939, src/hysteresis_loop.py:340-353). */
 int mm_ctx_get_nodal(int64_t ctx, double* M_nodal, double* inv_M_rel, double* inv_M_prec);
 
+/* diag(K_ex)/V_mag (N,): compute_exchange_diagonal (src/energy_kernels.py:492-566). */
+int mm_ctx_get_exchange_diag(int64_t ctx, double* d);
+
 /* Select the geometry source of the Poisson apply (see flags of mm_ctx_create). */
 int mm_set_geometry_mode(int64_t ctx, int mode);
 
@@ -98,6 +101,10 @@ int mm_precond_jacobi(int64_t ctx);
 
 /* One preconditioner application z = M^-1 r (src/amg_utils.py:557-623). */
 int mm_precond_apply(int64_t ctx, const double* r, double* z);
+
+/* PCG on A x = b for a caller-given b: make_pcg_solve's solve (src/poisson_solve.py:422-478);
+ * x_inout holds x0 on entry; r2 receives the final r.r. */
+int mm_pcg_solve(int64_t ctx, const double* b, double* x_inout, double tol, int maxiter, int* it, double* r2);
 
 /* solve_U (src/poisson_solve.py:748-771): U_inout holds x0 on entry. */
 int mm_solve_U(int64_t ctx, const double* m, double* U_inout, double tol, int maxiter, int* it, double* rel_res);

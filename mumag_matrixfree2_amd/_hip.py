@@ -76,6 +76,8 @@ SIGNATURES = {
     "mm_amg_commit": [C.c_int64, C.c_int, C.c_int],
     "mm_precond_jacobi": [C.c_int64],
     "mm_precond_apply": [C.c_int64, f64p, f64p],
+    "mm_ctx_get_exchange_diag": [C.c_int64, f64p],
+    "mm_pcg_solve": [C.c_int64, f64p, f64p, C.c_double, C.c_int, i32p, f64p],
     "mm_solve_U": [C.c_int64, f64p, f64p, C.c_double, C.c_int, i32p, f64p],
     "mm_energy_grad": [C.c_int64, f64p, f64p, f64p, f64p, f64p],
     "mm_local_grad": [C.c_int64, f64p, f64p],

@@ -96,6 +96,12 @@ class MeshContext:
         call("mm_ctx_get_nodal", self.handle, ptr(M), ptr(a), ptr(b))
         return M, a, b
 
+    def exchange_diagonal(self):
+        """diag(K_ex)/V_mag (N,)."""
+        d = np.zeros(self.N)
+        call("mm_ctx_get_exchange_diag", self.handle, ptr(d))
+        return d
+
     def set_geometry_mode(self, geom: str):
         call("mm_set_geometry_mode", self.handle, GEOM_MODES[geom])
 
@@ -159,6 +165,15 @@ class MeshContext:
         z = np.empty(self.N)
         call("mm_precond_apply", self.handle, ptr(r), ptr(z))
         return z
+
+    def pcg(self, b, x0, tol, maxiter=2000):
+        """PCG on A x = b: (x, iterations, r.r)."""
+        b = f64(b)
+        x = f64(np.array(x0, dtype=np.float64, copy=True))
+        it = C.c_int32(0)
+        r2 = C.c_double(0.0)
+        call("mm_pcg_solve", self.handle, ptr(b), ptr(x), float(tol), int(maxiter), C.byref(it), C.byref(r2))
+        return x, int(it.value), float(r2.value)
 
     def solve_U(self, m, x0, tol, maxiter=2000):
         """(U, iterations, relative residual)."""

@@ -285,10 +285,12 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   }
   c.h_inv_M_rel.resize(N);
   c.h_inv_M_prec.resize(N);
+  c.h_exdiag.resize(N);
   for (int i = 0; i < N; ++i) {
     const double M = c.h_M_nodal[i];
     c.h_inv_M_rel[i] = M > 1e-20 ? c.V_mag / M : 0.0;
     const double dd = dex[i] / c.V_mag;
+    c.h_exdiag[i] = dd;
     c.h_inv_M_prec[i] = dd > 1e-20 ? 1.0 / dd : 1.0;
   }
   MM_TRY(upload(c, &c.M_nodal, c.h_M_nodal.data(), N));
@@ -431,6 +433,13 @@ int mm_ctx_get_nodal(int64_t h, double* M_nodal, double* inv_M_rel, double* inv_
   return 0;
 }
 
+int mm_ctx_get_exchange_diag(int64_t h, double* d) {
+  GET_CTX(h);
+  MM_CHECK(d, "null array");
+  std::memcpy(d, c.h_exdiag.data(), c.N * sizeof(double));
+  return 0;
+}
+
 int mm_set_geometry_mode(int64_t h, int mode) {
   GET_CTX(h);
   MM_CHECK(mode >= 0 && mode <= 2, "geometry mode must be 0, 1 or 2");
@@ -549,6 +558,22 @@ int mm_precond_apply(int64_t h, const double* r, double* z) {
   MM_HIP(hipGetLastError());
   MM_HIP(hipMemcpyAsync(z, c.pz, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   MM_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int mm_pcg_solve(int64_t h, const double* b, double* x_inout, double tol, int maxiter, int* it, double* r2) {
+  GET_CTX(h);
+  MM_CHECK(b && x_inout, "null array");
+  MM_HIP(hipMemcpyAsync(c.pb, b, c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  MM_HIP(hipMemcpyAsync(c.work_U, x_inout, c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  int iters = 0;
+  double rr = 0, bb = 0;
+  MM_TRY(pcg_dev(c, c.pb, c.work_U, tol, maxiter, &iters, &rr, &bb));
+  MM_HIP(hipGetLastError());
+  MM_HIP(hipMemcpyAsync(x_inout, c.work_U, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  if (it) *it = iters;
+  if (r2) *r2 = rr;
   return 0;
 }
 

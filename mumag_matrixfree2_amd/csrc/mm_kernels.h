@@ -130,7 +130,7 @@ struct Ctx {
   double* inv_M_prec = nullptr;       // N
   std::vector<int> h_conn;            // E x 4 (part_all order)
   std::vector<double> h_Ve, h_grad;   // E, E x 9
-  std::vector<double> h_M_nodal, h_inv_M_rel, h_inv_M_prec;
+  std::vector<double> h_M_nodal, h_inv_M_rel, h_inv_M_prec, h_exdiag;
 
   // reductions / scalars
   double* partial = nullptr;          // kRedBlocks x 4

@@ -1,0 +1,123 @@
+"""Hysteresis sweep driver (``src/hysteresis_loop.py``) on the HIP backend.
+
+One device context per mesh; the magnetisation and potential stay resident on the GPU
+across field steps (U is carried from step to step as the reference does,
+``src/hysteresis_loop.py:383,504-507``); the host reads back per step only the statistics
+and the volume-averaged J along h.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .context import MeshContext, make_params
+
+
+@dataclass
+class LoopParams:
+    """Sweep and solver settings; field names and defaults of ``src/hysteresis_loop.py:34-125``
+    with the CLI defaults of ``src/loop.py`` where those override them (pc_iters 10)."""
+
+    h_dir: np.ndarray
+    B_start: float
+    B_end: float
+    dB: float
+    loop: bool = False
+    max_iter: int = 2000
+    tau_f: float = 1e-8
+    eps_a: float = 1e-12
+    tau0: float = 1e-2
+    ls_eta1: float = 0.1
+    ls_eta2: float = 0.1
+    ls_C: float = 2.0
+    ls_c: float = 0.5
+    ls_max_evals: int = 15
+    Js_ref: float = 1.0
+    cg_maxiter: int = 2000
+    cg_tol: float = 1e-8
+    poisson_reg: float = 1e-12
+    mfinal: float | None = None
+    method: str = "pcohen_hs"
+    pc_iters: int = 10
+    pc_auto: bool = True
+    pc_force_eta: float = 0.5
+    pc_force_alpha: float = 0.5
+    pc_stagnation_nu: float = 0.01
+    memory: int = 10
+    pc_reg: float = 0.0
+    L: int | None = None
+    precond_type: str = "amgcl"
+    verbose: bool = False
+    extra: dict = field(default_factory=dict)
+
+
+def field_values(H_start: float, H_end: float, dH: float, loop: bool) -> np.ndarray:
+    """Field magnitudes of the sweep (``src/hysteresis_loop.py:128-150``)."""
+    if dH == 0:
+        raise ValueError("dH must be nonzero")
+    if (H_end - H_start) * dH < 0:
+        dH = -dH
+    n = int(np.floor((H_end - H_start) / dH + 1e-12))
+    vals = H_start + dH * np.arange(n + 1)
+    if abs(vals[-1] - H_end) > 1e-12 * max(1.0, abs(H_end)):
+        vals = np.append(vals, H_end)
+    if not loop:
+        return vals
+    return np.concatenate([vals, vals[-2::-1]])
+
+
+def switching_step(J_par: np.ndarray) -> int:
+    """First field step with J_par < 0 (sweeping down from positive saturation); -1 if none."""
+    idx = np.where(np.asarray(J_par) < 0)[0]
+    return int(idx[0]) if idx.size else -1
+
+
+def run_hysteresis_loop(ctx: MeshContext, m0: np.ndarray, params: LoopParams, *, steps: int | None = None,
+                        on_step=None) -> dict:
+    """Sweep the field along ``params.h_dir`` and minimise at each value.
+
+    Returns a dict with ``history`` rows (B_ext [Js_ref units], J_par [Js_ref units], E, gnorm,
+    iters, evals, demag_iters, preco_iters, seconds) and the final ``m``, ``U``.
+    """
+    h = np.asarray(params.h_dir, dtype=np.float64)
+    h = h / (np.linalg.norm(h) + 1e-30)
+    if ctx.precond_type != params.precond_type:
+        ctx.setup_precond(params.precond_type)
+    p = make_params(params.method, max_iter=params.max_iter, tau_f=params.tau_f, eps_a=params.eps_a,
+                    tau0=params.tau0, ls_eta1=params.ls_eta1, ls_eta2=params.ls_eta2, ls_C=params.ls_C,
+                    ls_c=params.ls_c, ls_max_evals=params.ls_max_evals, pc_iters=params.pc_iters,
+                    pc_auto=params.pc_auto, pc_force_eta=params.pc_force_eta,
+                    pc_force_alpha=params.pc_force_alpha, pc_stagnation_nu=params.pc_stagnation_nu,
+                    pc_reg=params.pc_reg, L=params.L, cg_tol=params.cg_tol, cg_maxiter=params.cg_maxiter,
+                    memory=params.memory)
+    m = np.asarray(m0, dtype=np.float64)
+    m = m / np.linalg.norm(m, axis=1, keepdims=True)
+    ctx.state_set(m, np.zeros(ctx.N))
+    B_vals = field_values(params.B_start, params.B_end, params.dB, params.loop)
+    if steps is not None:
+        B_vals = B_vals[:steps]
+    hist = []
+    t_all = time.perf_counter()
+    for k, Bmag in enumerate(B_vals):
+        t0 = time.perf_counter()
+        info = ctx.minimize_resident(p, Bmag * h)
+        J = ctx.state_j_parallel(h)
+        dt = time.perf_counter() - t0
+        row = [Bmag, J, info["E"], info["gnorm"], info["iters"], info["evals"], info["demag_iters"],
+               info["preco_iters"], dt]
+        hist.append(row)
+        if params.verbose:
+            print(f"step {k:05d} B={Bmag * params.Js_ref:+.6e} T J_par={J * params.Js_ref:+.6e} T "
+                  f"E={info['E']:.6e} t={dt:.3f}s it={info['iters']} nf={info['evals']} "
+                  f"icg={info['demag_iters']}", flush=True)
+        if on_step is not None:
+            on_step(k, row)
+        if params.mfinal is not None and J <= params.mfinal:
+            break
+    m, U = ctx.state_get()
+    hist = np.asarray(hist)
+    return {"history": hist, "m": m, "U": U, "seconds": time.perf_counter() - t_all,
+            "switching_step": switching_step(hist[:, 1]) if len(hist) else -1}
