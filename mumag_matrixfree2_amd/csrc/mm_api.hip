@@ -44,6 +44,7 @@ static int upload(Ctx& c, T** p, const T* h, size_t n) {
 
 // Host-side description of a partition before upload.
 struct HostPart {
+  std::vector<int> elem_ids;  // original element id of each partition slot
   std::vector<int> eptr, nptr;
   std::vector<uint32_t> nodes, lconn;
   std::vector<uint8_t> mat;
@@ -62,6 +63,8 @@ static void build_partition(int N, const std::vector<int>& conn, const std::vect
   hp.nodes.clear();
   hp.lconn.clear();
   hp.mat.clear();
+  hp.elem_ids.clear();
+  hp.elem_ids.reserve(elems.size());
   hp.lconn.reserve(elems.size());
   hp.mat.reserve(elems.size());
   std::vector<int> blk_nodes;
@@ -79,6 +82,33 @@ static void build_partition(int N, const std::vector<int>& conn, const std::vect
       uint32_t lc = hp.lconn[e], out = 0;
       for (int a = 0; a < 4; ++a) out |= (uint32_t)remap[(lc >> (8 * a)) & 255u] << (8 * a);
       hp.lconn[e] = out;
+    }
+    // Interleave the block's elements (a 64 x s transpose) so that the 64 elements a wave
+    // processes together lie s positions apart in the (min node)-sorted order: they touch
+    // different nodes and their LDS atomics rarely collide.
+    {
+      const int n = (int)(hp.lconn.size() - blk_e0);
+      const int st = (n + 63) / 64;
+      std::vector<int> perm;
+      perm.reserve(n);
+      for (int cc = 0; cc < st; ++cc)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int old = lane * st + cc;
+          if (old < n) perm.push_back(old);
+        }
+      std::vector<uint32_t> lc2(n);
+      std::vector<uint8_t> mt2(n);
+      std::vector<int> id2(n);
+      for (int j = 0; j < n; ++j) {
+        lc2[j] = hp.lconn[blk_e0 + perm[j]];
+        mt2[j] = hp.mat[blk_e0 + perm[j]];
+        id2[j] = hp.elem_ids[blk_e0 + perm[j]];
+      }
+      for (int j = 0; j < n; ++j) {
+        hp.lconn[blk_e0 + j] = lc2[j];
+        hp.mat[blk_e0 + j] = mt2[j];
+        hp.elem_ids[blk_e0 + j] = id2[j];
+      }
     }
     for (int i = 0; i < nb; ++i) {
       const int v = blk_nodes[order[i]];
@@ -121,6 +151,7 @@ static void build_partition(int N, const std::vector<int>& conn, const std::vect
     }
     hp.lconn.push_back(lc);
     hp.mat.push_back((uint8_t)mat0[e]);
+    hp.elem_ids.push_back(e);
   }
   if (!blk_nodes.empty()) close_block();
   for (auto& f : hp.nodes)
@@ -226,6 +257,7 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
 
   // device geometry in part_all order
   c.h_conn.resize(4 * (size_t)E);
+  order = hp_all.elem_ids;
   for (int k = 0; k < E; ++k)
     for (int a = 0; a < 4; ++a) c.h_conn[4 * (size_t)k + a] = conn[4 * (size_t)order[k] + a];
   MM_TRY(upload(c, &c.knt, knt, 3 * (size_t)N));
@@ -510,6 +542,18 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
     MM_TRY(upload(c, &L.Rx, Rx, Rp[n_next]));
   }
   if (dense_inv) MM_TRY(upload(c, &L.dense_inv, dense_inv, (size_t)n * n));
+  // SpMV lanes per row: power of two near half the average row length, in [4, 64]
+  auto lanes = [](long nnz, long rows) {
+    const double avg = rows > 0 ? (double)nnz / (double)rows : 0.0;
+    int l = 4;
+    while (l < 64 && 2.0 * l < avg) l *= 2;
+    return l;
+  };
+  if (l > 0 && Ap) L.lanesA = lanes(Ap[n], n);
+  if (n_next > 0) {
+    L.lanesP = lanes(Pp[n], n);
+    L.lanesR = lanes(Rp[n_next], n_next);
+  }
   if (l > 0) {
     MM_TRY(dalloc(c, &L.b, n));
     MM_TRY(dalloc(c, &L.x, n));

@@ -155,7 +155,7 @@ __device__ void reduce_store(double (&v)[K], double* __restrict__ partial, unsig
 
 // grid size for grid-stride vector kernels with reductions
 inline int red_grid(long n) {
-  long g = (n + kBlock * 4 - 1) / (kBlock * 4);
+  long g = (n + kBlock - 1) / kBlock;
   if (g < 1) g = 1;
   if (g > kRedBlocks) g = kRedBlocks;
   return (int)g;

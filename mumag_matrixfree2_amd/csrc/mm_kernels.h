@@ -107,6 +107,7 @@ struct Level {
   double* Rx = nullptr;
   double* M = nullptr;                // smoother multiplier: x += M * (b - A x)
   double* dense_inv = nullptr;        // coarsest level
+  int lanesA = 8, lanesP = 8, lanesR = 8;  // SpMV lanes per row
   double *b = nullptr, *x = nullptr, *t = nullptr;
 };
 

@@ -92,17 +92,23 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xr(const double* __restrict__ sc
   reduce_store<1, false>(v, partial, counter, out);
 }
 
-// beta = rz_new / (rz_old + 1e-30); p = z + beta p  (first == 1: p = z)
+// beta = rz_new / (rz_old + 1e-30); p = z + beta p  (first == 1: p = z); q = 0 for the next apply
 __global__ void k_pcg_p(const double* __restrict__ sc, int new_slot, int old_slot, int first,
-                        const double* __restrict__ z, double* __restrict__ p, long n) {
+                        const double* __restrict__ z, double* __restrict__ p, double* __restrict__ q, long n) {
   const double beta = first ? 0.0 : sc[new_slot] / (sc[old_slot] + 1e-30);
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     p[i] = first ? z[i] : z[i] + beta * p[i];
+    q[i] = 0.0;
+  }
 }
 
-// x = M .* b
-__global__ void k_scale(const double* __restrict__ M, const double* __restrict__ b, double* __restrict__ x, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = M[i] * b[i];
+// x = M .* b (and t = 0 when t != nullptr)
+__global__ void k_scale(const double* __restrict__ M, const double* __restrict__ b, double* __restrict__ x,
+                        double* __restrict__ t, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    x[i] = M[i] * b[i];
+    if (t) t[i] = 0.0;
+  }
 }
 
 // z = x + M .* (b - t), and out = r.z with r == b (level-0 post-smoothing fused with the dot)
@@ -126,30 +132,33 @@ __global__ void k_post(double* __restrict__ x, const double* __restrict__ M, con
     x[i] += M[i] * (b[i] - t[i]);
 }
 
-// CSR SpMV with 8 lanes per row. MODE 0: y = A x; 1: y += A x; 2: y = A (x - x2)
-template <int MODE>
+// CSR SpMV with LANES lanes per row. MODE 0: y = A x; 1: y += A x (and z = 0 on the
+// same rows when z != nullptr); 2: y = A (x - x2)
+template <int MODE, int LANES>
 __global__ __launch_bounds__(kBlock) void k_spmv(int nrows, const int* __restrict__ Ap, const int* __restrict__ Ai,
                                                  const double* __restrict__ Ax, const double* __restrict__ x,
-                                                 const double* __restrict__ x2, double* __restrict__ y) {
-  const int lane = threadIdx.x & 7;
-  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 3;
+                                                 const double* __restrict__ x2, double* __restrict__ y,
+                                                 double* __restrict__ z) {
+  const int lane = threadIdx.x & (LANES - 1);
+  const int row = (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
   double s = 0.0;
   if (row < nrows) {
     const int j0 = Ap[row], j1 = Ap[row + 1];
-    for (int j = j0 + lane; j < j1; j += 8) {
+    for (int j = j0 + lane; j < j1; j += LANES) {
       const int c = Ai[j];
       const double xv = (MODE == 2) ? (x[c] - x2[c]) : x[c];
       s += Ax[j] * xv;
     }
   }
-  s += __shfl_down(s, 4, 8);
-  s += __shfl_down(s, 2, 8);
-  s += __shfl_down(s, 1, 8);
+#pragma unroll
+  for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LANES);
   if (row < nrows && lane == 0) {
-    if (MODE == 1)
+    if (MODE == 1) {
       y[row] += s;
-    else
+      if (z) z[row] = 0.0;
+    } else {
       y[row] = s;
+    }
   }
 }
 
@@ -165,16 +174,29 @@ __global__ __launch_bounds__(kBlock) void k_gemv(int n, const double* __restrict
   if (lane == 0) y[row] = s;
 }
 
-static void spmv(int mode, int nrows, const int* Ap, const int* Ai, const double* Ax, const double* x,
-                 const double* x2, double* y, hipStream_t s) {
+template <int MODE>
+static void spmv_lanes(int lanes, int nrows, const int* Ap, const int* Ai, const double* Ax, const double* x,
+                       const double* x2, double* y, double* z, hipStream_t s) {
+  const int grid = (int)(((long)nrows * lanes + kBlock - 1) / kBlock);
+  switch (lanes) {
+    case 4: k_spmv<MODE, 4><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
+    case 8: k_spmv<MODE, 8><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
+    case 16: k_spmv<MODE, 16><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
+    case 32: k_spmv<MODE, 32><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
+    default: k_spmv<MODE, 64><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
+  }
+}
+
+// lanes per row: the power of two nearest half the average row length, in [4, 64]
+static void spmv(int mode, int lanes, int nrows, const int* Ap, const int* Ai, const double* Ax, const double* x,
+                 const double* x2, double* y, hipStream_t s, double* z = nullptr) {
   if (nrows <= 0) return;
-  const int grid = (int)(((long)nrows * 8 + kBlock - 1) / kBlock);
   if (mode == 0)
-    k_spmv<0><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y);
+    spmv_lanes<0>(lanes, nrows, Ap, Ai, Ax, x, x2, y, nullptr, s);
   else if (mode == 1)
-    k_spmv<1><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y);
+    spmv_lanes<1>(lanes, nrows, Ap, Ai, Ax, x, x2, y, z, s);
   else
-    k_spmv<2><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y);
+    spmv_lanes<2>(lanes, nrows, Ap, Ai, Ax, x, x2, y, nullptr, s);
 }
 
 // ----------------------------------------------------------------------------- V-cycle
@@ -187,12 +209,12 @@ static void vcycle_coarse(Ctx& c, int l, hipStream_t s) {
     return;
   }
   Level& Nx = c.lv[l + 1];
-  k_scale<<<vgrid(L.n), kBlock, 0, s>>>(L.M, L.b, L.x, L.n);                     // pre-smooth from x = 0
-  spmv(0, L.n, L.Ap, L.Ai, L.Ax, L.x, nullptr, L.t, s);                           // t = A x
-  spmv(2, Nx.n, L.Rp, L.Ri, L.Rx, L.b, L.t, Nx.b, s);                             // b' = R (b - t)
+  k_scale<<<vgrid(L.n), kBlock, 0, s>>>(L.M, L.b, L.x, nullptr, L.n);            // pre-smooth from x = 0
+  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, nullptr, L.t, s);                 // t = A x
+  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rx, L.b, L.t, Nx.b, s);                   // b' = R (b - t)
   vcycle_coarse(c, l + 1, s);
-  spmv(1, L.n, L.Pp, L.Pi, L.Px, Nx.x, nullptr, L.x, s);                          // x += P e
-  spmv(0, L.n, L.Ap, L.Ai, L.Ax, L.x, nullptr, L.t, s);                           // t = A x
+  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Px, Nx.x, nullptr, L.x, s);                // x += P e
+  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, nullptr, L.t, s);                 // t = A x
   k_post<<<vgrid(L.n), kBlock, 0, s>>>(L.x, L.M, L.b, L.t, L.n);                  // x += M (b - t)
 }
 
@@ -204,18 +226,16 @@ static int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream
     if (L0.dense_inv)  // single-level hierarchy: exact dense solve
       k_gemv<<<(int)((N * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>((int)N, L0.dense_inv, r, z);
     else  // Jacobi on diag(A)
-      k_scale<<<vgrid(N), kBlock, 0, s>>>(L0.M, r, z, N);
+      k_scale<<<vgrid(N), kBlock, 0, s>>>(L0.M, r, z, nullptr, N);
     vdot(c, r, z, N, rz_out, s);
     return 0;
   }
   Level& L1 = c.lv[1];
-  k_scale<<<vgrid(N), kBlock, 0, s>>>(L0.M, r, c.px0, N);                         // x = M0 r
-  vfill(c.pt, 0.0, N, s);
+  k_scale<<<vgrid(N), kBlock, 0, s>>>(L0.M, r, c.px0, c.pt, N);                  // x = M0 r, t = 0
   MM_TRY(launch_apply(c, c.geom_mode, c.px0, c.pt, s));                           // t = A x
-  spmv(2, L1.n, L0.Rp, L0.Ri, L0.Rx, r, c.pt, L1.b, s);                           // b1 = R0 (r - t)
+  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rx, r, c.pt, L1.b, s);                // b1 = R0 (r - t)
   vcycle_coarse(c, 1, s);
-  spmv(1, (int)N, L0.Pp, L0.Pi, L0.Px, L1.x, nullptr, c.px0, s);                  // x += P0 e1
-  vfill(c.pt, 0.0, N, s);
+  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Px, L1.x, nullptr, c.px0, s, c.pt); // x += P0 e1, t = 0
   MM_TRY(launch_apply(c, c.geom_mode, c.px0, c.pt, s));                           // t = A x
   k_post_dot<<<red_grid(N), kBlock, 0, s>>>(c.px0, L0.M, r, c.pt, z, N, c.partial, c.counter + 4, rz_out);
   c.n_apply += 2;
@@ -242,7 +262,7 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
   c.n_apply++;
   k_pcg_init<<<red_grid(N), kBlock, 0, s>>>(b, c.pq, c.pr, N, c.partial, c.counter + 5, sc + 3);
   MM_TRY(precond(c, c.pr, c.pz, sc + 0, s));
-  k_pcg_p<<<vgrid(N), kBlock, 0, s>>>(sc, 0, 0, 1, c.pz, c.pp, N);
+  k_pcg_p<<<vgrid(N), kBlock, 0, s>>>(sc, 0, 0, 1, c.pz, c.pp, c.pq, N);
   MM_HIP(hipMemcpyAsync(c.hscal + 3, sc + 3, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   MM_HIP(hipStreamSynchronize(s));
   const double bb = c.hscal[4];
@@ -251,8 +271,7 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
   int it = 0;
   int cur = 0;  // slot holding the current r.z
   while (it < maxiter && r2 > tol2) {
-    vfill(c.pq, 0.0, N, s);
-    MM_TRY(launch_apply(c, c.geom_mode, c.pp, c.pq, s));
+    MM_TRY(launch_apply(c, c.geom_mode, c.pp, c.pq, s));  // q was cleared by k_pcg_p
     c.n_apply++;
     vdot(c, c.pp, c.pq, N, sc + 2, s);
     k_pcg_xr<<<red_grid(N), kBlock, 0, s>>>(sc, cur, 2, U, c.pr, c.pp, c.pq, N, c.partial, c.counter + 5, sc + 3);
@@ -260,7 +279,7 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
     MM_HIP(hipEventRecord(c.ev, s));
     // preconditioner and direction update run while the host waits for r.r
     MM_TRY(precond(c, c.pr, c.pz, sc + (1 - cur), s));
-    k_pcg_p<<<vgrid(N), kBlock, 0, s>>>(sc, 1 - cur, cur, 0, c.pz, c.pp, N);
+    k_pcg_p<<<vgrid(N), kBlock, 0, s>>>(sc, 1 - cur, cur, 0, c.pz, c.pp, c.pq, N);
     cur = 1 - cur;
     ++it;
     MM_HIP(hipEventSynchronize(c.ev));
