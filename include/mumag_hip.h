@@ -63,7 +63,7 @@ int mm_ctx_create(int N, int E, const double* knt, const int32_t* conn, const in
 int mm_ctx_destroy(int64_t ctx);
 
 /* info[0..7] = V_mag, N, E, element blocks (all), element blocks (magnetic),
- * largest node list of a block, magnetic elements, 0 */
+ * largest node list of a block, magnetic elements, node-list entries of all blocks */
 int mm_ctx_info(int64_t ctx, double* info);
 
 /* Oriented connectivity (E,4), Ve (E,) and shape-function gradients of corners 1..3 (E,3,3)
@@ -133,6 +133,11 @@ int mm_state_j_parallel(int64_t ctx, const double* h, double* J);
  * S is the strength graph in CSR (n rows); agg[i] receives the aggregate of node i or -1 for
  * an isolated node; n_agg the number of aggregates. No device work. */
 int mm_sa_aggregate(int n, const int32_t* Sp, const int32_t* Sj, int32_t* agg, int32_t* n_agg);
+
+/* Live timing of the Poisson apply kernel inside the PCG: HIP events on the context's stream
+ * bracket every launch. Returns the accumulated milliseconds and launch count since the last
+ * call, then resets them and enables (enable != 0) or disables the timing. */
+int mm_apply_timing(int64_t ctx, int enable, double* total_ms, int64_t* count);
 
 /* Counters of kernel launches since creation: applies, rhs, energy, local, V-cycles, PCG iterations. */
 int mm_counters(int64_t ctx, int64_t* out6);

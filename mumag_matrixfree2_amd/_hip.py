@@ -88,6 +88,7 @@ SIGNATURES = {
     "mm_minimize": [C.c_int64, C.POINTER(MMParams), f64p, f64p, f64p, C.POINTER(MMInfo)],
     "mm_state_j_parallel": [C.c_int64, f64p, f64p],
     "mm_sa_aggregate": [C.c_int, i32p, i32p, i32p, i32p],
+    "mm_apply_timing": [C.c_int64, C.c_int, f64p, i64p],
     "mm_counters": [C.c_int64, i64p],
     "mm_bench_apply": [C.c_int64, C.c_int, C.c_int, f64p],
 }

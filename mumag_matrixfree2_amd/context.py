@@ -64,6 +64,7 @@ class MeshContext:
         self.n_blocks_mag = int(info[4])
         self.max_block_nodes = int(info[5])
         self.n_mag_elems = int(info[6])
+        self.node_list_entries = int(info[7])
         self.precond_type = None
         self.amg_levels = None
 
@@ -220,6 +221,13 @@ class MeshContext:
         out = np.zeros(6, dtype=np.int64)
         call("mm_counters", self.handle, ptr(out))
         return dict(zip(("apply", "rhs", "energy", "local", "vcycle", "pcg_iters"), out.tolist()))
+
+    def apply_timing(self, enable: bool):
+        """(total ms, launches) of the apply kernel since the last call; then enable/disable timing."""
+        ms = C.c_double(0.0)
+        n = C.c_int64(0)
+        call("mm_apply_timing", self.handle, int(bool(enable)), C.byref(ms), C.byref(n))
+        return float(ms.value), int(n.value)
 
     def bench_apply(self, geom="on_the_fly", reps=50):
         ms = C.c_double(0.0)

@@ -251,6 +251,7 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   build_partition(N, conn, mat0, mag_elems, no_bnd, hp_mag);
 
   MM_TRY(upload_part(c, hp_all, E, c.part_all));
+  c.nodes_all = (long long)hp_all.nodes.size();
   MM_TRY(upload_part(c, hp_mag, (int)mag_elems.size(), c.part_mag));
   c.max_block_nodes = std::max(hp_all.max_nodes, hp_mag.max_nodes);
   c.n_mag = (int)mag_elems.size();
@@ -350,6 +351,7 @@ static void destroy(Ctx* c) {
   (void)hipDeviceSynchronize();
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
+  for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev) (void)hipEventDestroy(c->ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -445,7 +447,7 @@ int mm_ctx_info(int64_t h, double* info) {
   info[4] = c.part_mag.nblk;
   info[5] = c.max_block_nodes;
   info[6] = c.n_mag;
-  info[7] = 0;
+  info[7] = c.nodes_all;
   return 0;
 }
 
@@ -542,11 +544,11 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
     MM_TRY(upload(c, &L.Rx, Rx, Rp[n_next]));
   }
   if (dense_inv) MM_TRY(upload(c, &L.dense_inv, dense_inv, (size_t)n * n));
-  // SpMV lanes per row: power of two near half the average row length, in [4, 64]
+  // SpMV lanes per row: power of two near a quarter of the average row length, in [4, 64]
   auto lanes = [](long nnz, long rows) {
     const double avg = rows > 0 ? (double)nnz / (double)rows : 0.0;
     int l = 4;
-    while (l < 64 && 2.0 * l < avg) l *= 2;
+    while (l < 64 && 4.0 * l <= avg) l *= 2;
     return l;
   };
   if (l > 0 && Ap) L.lanesA = lanes(Ap[n], n);
@@ -831,6 +833,17 @@ int mm_sa_aggregate(int n, const int32_t* Sp, const int32_t* Sj, int32_t* agg, i
     }
   }
   *n_agg = count;
+  return 0;
+}
+
+int mm_apply_timing(int64_t h, int enable, double* total_ms, int64_t* count) {
+  GET_CTX(h);
+  MM_TRY(resolve_apply_timing(c));
+  if (total_ms) *total_ms = c.apply_ms;
+  if (count) *count = c.apply_timed;
+  c.apply_ms = 0.0;
+  c.apply_timed = 0;
+  c.time_apply = enable != 0;
   return 0;
 }
 

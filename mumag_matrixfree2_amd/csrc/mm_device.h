@@ -121,9 +121,17 @@ __device__ bool reduce_last(double (&v)[K], double* __restrict__ partial, unsign
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     double s = 0.0;
-    for (int i = tid; i < (int)gridDim.x; i += kBlock) {
-      const double p = __hip_atomic_load(&partial[i * K + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s = MAX ? fmax(s, p) : s + p;
+    for (int i0 = 0; i0 < (int)gridDim.x; i0 += 4 * kBlock) {
+      double q[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * kBlock + tid;
+        q[u] = i < (int)gridDim.x
+                   ? __hip_atomic_load(&partial[i * K + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                   : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s = MAX ? fmax(s, q[u]) : s + q[u];
     }
     v[k] = wave_red<MAX>(s);
   }
@@ -144,6 +152,27 @@ __device__ bool reduce_last(double (&v)[K], double* __restrict__ partial, unsign
   return true;
 }
 
+// Block sum of K values written to partial[blockIdx.x * K + k] (no cross-block traffic);
+// sum_partials then reduces them in a fixed order.
+template <int K>
+__device__ void block_partial(double (&v)[K], double* __restrict__ partial) {
+  __shared__ double sh[K][kBlock / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_red<false>(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k][wv] = v[k];
+  __syncthreads();
+  if (tid == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = sh[k][0];
+      for (int w = 1; w < kBlock / 64; ++w) s += sh[k][w];
+      partial[blockIdx.x * K + k] = s;
+    }
+}
+
 // reduce_last + store of the totals to out[0..K-1]
 template <int K, bool MAX>
 __device__ void reduce_store(double (&v)[K], double* __restrict__ partial, unsigned* __restrict__ counter,
@@ -153,11 +182,14 @@ __device__ void reduce_store(double (&v)[K], double* __restrict__ partial, unsig
     for (int k = 0; k < K; ++k) out[k] = v[k];
 }
 
-// grid size for grid-stride vector kernels with reductions
+// Grid size for grid-stride vector kernels with reductions. Every workgroup of a reducing
+// kernel ends with a fence and one atomic on a shared counter; those serialise (~20 ns each),
+// so the grid stays at <= 256 workgroups (one per CU) and each thread loops over more items.
+constexpr int kRedGridMax = 256;
 inline int red_grid(long n) {
   long g = (n + kBlock - 1) / kBlock;
   if (g < 1) g = 1;
-  if (g > kRedBlocks) g = kRedBlocks;
+  if (g > kRedGridMax) g = kRedGridMax;
   return (int)g;
 }
 

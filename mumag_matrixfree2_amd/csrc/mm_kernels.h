@@ -120,6 +120,7 @@ struct Ctx {
   Part part_all, part_mag;
   Geo geo;
   int n_mag = 0;                      // magnetic elements
+  long long nodes_all = 0;            // node-list entries of the all-element partition
   int max_block_nodes = 0;
   int geom_mode = kGeomCoords;        // geometry source of the Poisson apply
   std::vector<void*> allocs;          // every device allocation, freed at destroy
@@ -157,7 +158,18 @@ struct Ctx {
   bool have_state = false;
 
   long long n_apply = 0, n_rhs = 0, n_energy = 0, n_local = 0, n_vcycle = 0, n_pcg_it = 0;
+
+  // live timing of the Poisson apply kernel: event pairs recorded on `stream` around each
+  // launch inside the PCG, resolved at the PCG's own synchronisation points
+  bool time_apply = false;
+  std::vector<hipEvent_t> ev_pool;
+  int ev_used = 0;
+  double apply_ms = 0.0;
+  long long apply_timed = 0;
 };
+
+int timed_apply(Ctx& c, const double* x, double* y, hipStream_t s);
+int resolve_apply_timing(Ctx& c);
 
 // element kernels (mm_kernels.hip)
 int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s);

@@ -40,6 +40,14 @@ __global__ __launch_bounds__(kBlock) void k_apply(Part p, Geo geo, const double*
   const BlkInfo bi = blk_info(p);
   if (!bi.valid) return;
   const int tid = threadIdx.x;
+  // element words of the first kPre rounds are loaded before the staging barrier
+  constexpr int kPre = 4;
+  uint32_t lcv[kPre];
+#pragma unroll
+  for (int r = 0; r < kPre; ++r) {
+    const int e = bi.e0 + tid + r * kBlock;
+    lcv[r] = e < bi.e1 ? p.lconn[e] : 0u;
+  }
   for (int i = tid; i < bi.nb; i += kBlock) {
     const uint32_t f = p.nodes[bi.n0 + i];
     const uint32_t id = f & kIdMask;
@@ -52,9 +60,9 @@ __global__ __launch_bounds__(kBlock) void k_apply(Part p, Geo geo, const double*
     }
   }
   __syncthreads();
-  for (int e = bi.e0 + tid; e < bi.e1; e += kBlock) {
+  auto element = [&](int e, uint32_t lc) {
     int l0, l1, l2, l3;
-    unpack(p.lconn[e], l0, l1, l2, l3);
+    unpack(lc, l0, l1, l2, l3);
     double g[4][3], vol;
     if constexpr (GEOM == kGeomCoords)
       tet_grads(sX, l0, l1, l2, l3, g, vol);
@@ -71,7 +79,13 @@ __global__ __launch_bounds__(kBlock) void k_apply(Part p, Geo geo, const double*
     atomicAdd(&sy[l1], g[1][0] * gx + g[1][1] * gy + g[1][2] * gz);
     atomicAdd(&sy[l2], g[2][0] * gx + g[2][1] * gy + g[2][2] * gz);
     atomicAdd(&sy[l3], g[3][0] * gx + g[3][1] * gy + g[3][2] * gz);
+  };
+#pragma unroll
+  for (int r = 0; r < kPre; ++r) {
+    const int e = bi.e0 + tid + r * kBlock;
+    if (e < bi.e1) element(e, lcv[r]);
   }
+  for (int e = bi.e0 + tid + kPre * kBlock; e < bi.e1; e += kBlock) element(e, p.lconn[e]);
   __syncthreads();
   for (int i = tid; i < bi.nb; i += kBlock) {
     const uint32_t f = p.nodes[bi.n0 + i];
@@ -223,7 +237,33 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
     }
   }
   acc[0] *= inv_V;
-  reduce_store<1, false>(acc, partial, counter, E_out);
+  block_partial<1>(acc, partial);
+}
+
+// out[k] = sum_b partial[b * K + k], one workgroup, fixed order
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ partial, int n,
+                                                         double* __restrict__ out) {
+  __shared__ double sh[K][kBlock / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int i = tid; i < n; i += kBlock) s += partial[i * K + k];
+    v[k] = wave_red<false>(s);
+  }
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k][wv] = v[k];
+  __syncthreads();
+  if (tid == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = sh[k][0];
+      for (int w = 1; w < kBlock / 64; ++w) s += sh[k][w];
+      out[k] = s;
+    }
 }
 
 // --------------------------------------------------- local Hessian action C v (K8)
@@ -387,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_jpar(Part p, Geo geo, const double* 
     acc[0] += vol * Js * 0.25 * (sh[l[0]] + sh[l[1]] + sh[l[2]] + sh[l[3]]);
     acc[1] += vol;
   }
-  reduce_store<2, false>(acc, partial, counter, out);
+  block_partial<2>(acc, partial);
 }
 
 // ------------------------------------------------------------------------ launchers
@@ -421,8 +461,10 @@ int launch_energy(const Ctx& c, const double* m, const double* U, const double B
                   hipStream_t s) {
   const Part& p = c.part_mag;
   if (p.nblk == 0) return 0;
-  k_energy<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, c.partial,
-                                           c.counter + 1, E_out);
+  const int grid = grid_for(p);
+  k_energy<<<grid, kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, c.partial, c.counter + 1,
+                                   E_out);
+  k_sum_partials<1><<<1, kBlock, 0, s>>>(c.partial, grid, E_out);
   return 0;
 }
 
@@ -452,7 +494,9 @@ int launch_diag(const Ctx& c, double* d, hipStream_t s) {
 int launch_jpar(const Ctx& c, const double* m, const double h[3], double* out, hipStream_t s) {
   const Part& p = c.part_mag;
   if (p.nblk == 0) return 0;
-  k_jpar<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, m, h[0], h[1], h[2], c.partial, c.counter + 2, out);
+  const int grid = grid_for(p);
+  k_jpar<<<grid, kBlock, 0, s>>>(p, c.geo, m, h[0], h[1], h[2], c.partial, c.counter + 2, out);
+  k_sum_partials<2><<<1, kBlock, 0, s>>>(c.partial, grid, out);
   return 0;
 }
 

@@ -199,6 +199,34 @@ static void spmv(int mode, int lanes, int nrows, const int* Ap, const int* Ai, c
     spmv_lanes<2>(lanes, nrows, Ap, Ai, Ax, x, x2, y, nullptr, s);
 }
 
+// --------------------------------------------------------------------- apply timing
+int resolve_apply_timing(Ctx& c) {
+  if (c.ev_used == 0) return 0;
+  MM_HIP(hipEventSynchronize(c.ev_pool[2 * c.ev_used - 1]));
+  for (int k = 0; k < c.ev_used; ++k) {
+    float ms = 0.f;
+    MM_HIP(hipEventElapsedTime(&ms, c.ev_pool[2 * k], c.ev_pool[2 * k + 1]));
+    c.apply_ms += ms;
+  }
+  c.apply_timed += c.ev_used;
+  c.ev_used = 0;
+  return 0;
+}
+
+int timed_apply(Ctx& c, const double* x, double* y, hipStream_t s) {
+  if (!c.time_apply) return launch_apply(c, c.geom_mode, x, y, s);
+  if (c.ev_pool.empty()) {
+    c.ev_pool.resize(512);
+    for (auto& e : c.ev_pool) MM_HIP(hipEventCreate(&e));
+  }
+  if (2 * c.ev_used >= (int)c.ev_pool.size()) MM_TRY(resolve_apply_timing(c));
+  MM_HIP(hipEventRecord(c.ev_pool[2 * c.ev_used], s));
+  MM_TRY(launch_apply(c, c.geom_mode, x, y, s));
+  MM_HIP(hipEventRecord(c.ev_pool[2 * c.ev_used + 1], s));
+  ++c.ev_used;
+  return 0;
+}
+
 // ----------------------------------------------------------------------------- V-cycle
 // Coarse part of the V-cycle on levels l >= 1: returns the correction in lv[l].x.
 static void vcycle_coarse(Ctx& c, int l, hipStream_t s) {
@@ -232,11 +260,11 @@ static int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream
   }
   Level& L1 = c.lv[1];
   k_scale<<<vgrid(N), kBlock, 0, s>>>(L0.M, r, c.px0, c.pt, N);                  // x = M0 r, t = 0
-  MM_TRY(launch_apply(c, c.geom_mode, c.px0, c.pt, s));                           // t = A x
+  MM_TRY(timed_apply(c, c.px0, c.pt, s));                                         // t = A x
   spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rx, r, c.pt, L1.b, s);                // b1 = R0 (r - t)
   vcycle_coarse(c, 1, s);
   spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Px, L1.x, nullptr, c.px0, s, c.pt); // x += P0 e1, t = 0
-  MM_TRY(launch_apply(c, c.geom_mode, c.px0, c.pt, s));                           // t = A x
+  MM_TRY(timed_apply(c, c.px0, c.pt, s));                                         // t = A x
   k_post_dot<<<red_grid(N), kBlock, 0, s>>>(c.px0, L0.M, r, c.pt, z, N, c.partial, c.counter + 4, rz_out);
   c.n_apply += 2;
   c.n_vcycle += 1;
@@ -258,7 +286,7 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
   double* sc = c.dscal;
   // slots: 0/1 rz ping-pong, 2 p.q, 3 r.r, 4 b.b
   vfill(c.pq, 0.0, N, s);
-  MM_TRY(launch_apply(c, c.geom_mode, U, c.pq, s));
+  MM_TRY(timed_apply(c, U, c.pq, s));
   c.n_apply++;
   k_pcg_init<<<red_grid(N), kBlock, 0, s>>>(b, c.pq, c.pr, N, c.partial, c.counter + 5, sc + 3);
   MM_TRY(precond(c, c.pr, c.pz, sc + 0, s));
@@ -271,7 +299,7 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
   int it = 0;
   int cur = 0;  // slot holding the current r.z
   while (it < maxiter && r2 > tol2) {
-    MM_TRY(launch_apply(c, c.geom_mode, c.pp, c.pq, s));  // q was cleared by k_pcg_p
+    MM_TRY(timed_apply(c, c.pp, c.pq, s));  // q was cleared by k_pcg_p
     c.n_apply++;
     vdot(c, c.pp, c.pq, N, sc + 2, s);
     k_pcg_xr<<<red_grid(N), kBlock, 0, s>>>(sc, cur, 2, U, c.pr, c.pp, c.pq, N, c.partial, c.counter + 5, sc + 3);
@@ -286,6 +314,7 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
     r2 = c.hscal[3];
   }
   MM_HIP(hipStreamSynchronize(s));
+  MM_TRY(resolve_apply_timing(c));
   c.n_pcg_it += it;
   *it_out = it;
   *r2_out = r2;
