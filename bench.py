@@ -64,6 +64,18 @@ def reference_layout_bytes(ctx) -> int:
     return ctx.E * (16 + 10 * 8) + ctx.N * 2 * 8
 
 
+def measured_traffic():
+    """HBM bytes per k_apply launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (profiles/k_apply_cube2M_fp64.md), or None if absent."""
+    f = ROOT / "profiles" / "k_apply_cube2M_fp64.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(mesh, mats, steps_T, budget_s=60.0):
     """The oracle (NumPy/SciPy restatement, kind 'port') on the same mesh, hierarchy and method,
     timed on the host for the first field steps of the sweep."""
@@ -188,7 +200,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "k_apply<on_the_fly> (matrix-free Poisson apply)",
             "avg_launch_us": avg_apply_s * 1e6,
             "launches": apply_n,

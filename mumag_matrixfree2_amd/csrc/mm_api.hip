@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -55,7 +56,8 @@ struct HostPart {
 // touch at most kMaxBlockNodes distinct nodes; node lists are sorted; flags mark the first
 // block touching a node (owner), nodes touched by one block only (private) and Dirichlet nodes.
 static void build_partition(int N, const std::vector<int>& conn, const std::vector<int>& mat0,
-                            const std::vector<int>& elems, const std::vector<char>& is_bnd, HostPart& hp) {
+                            const std::vector<int>& elems, const std::vector<char>& is_bnd, HostPart& hp,
+                            int node_cap) {
   std::vector<int> stamp(N, -1), local(N, 0), touches(N, 0);
   std::vector<char> owned(N, 0);
   hp.eptr.assign(1, 0);
@@ -137,7 +139,7 @@ static void build_partition(int N, const std::vector<int>& conn, const std::vect
       for (int b2 = 0; b2 < a; ++b2) dup |= conn[4 * e + b2] == v;
       if (!dup && stamp[v] != blk) ++fresh;
     }
-    if ((int)blk_nodes.size() + fresh > kMaxBlockNodes || (int)(hp.lconn.size() - blk_e0) >= kMaxBlockElems)
+    if ((int)blk_nodes.size() + fresh > node_cap || (int)(hp.lconn.size() - blk_e0) >= kMaxBlockElems)
       close_block();
     uint32_t lc = 0;
     for (int a = 0; a < 4; ++a) {
@@ -239,8 +241,11 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
     for (int k = 0; k < 3; ++k) c.hmt.k[g][k] = k_easy[3 * g + k];
   }
 
+  // nodes per element block (<= 256, the LDS staging capacity); MM_BLOCK_NODES overrides it
+  int node_cap = kMaxBlockNodes;
+  if (const char* env = std::getenv("MM_BLOCK_NODES")) node_cap = std::max(16, std::min(kMaxBlockNodes, std::atoi(env)));
   HostPart hp_all, hp_mag;
-  build_partition(N, conn, mat0, order, is_bnd, hp_all);
+  build_partition(N, conn, mat0, order, is_bnd, hp_all, node_cap);
   std::vector<int> mag_elems;
   mag_elems.reserve(E);
   for (int k = 0; k < E; ++k) {
@@ -248,7 +253,7 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
     if (A_red[g] != 0.0 || K1_red[g] != 0.0 || Js_red[g] != 0.0) mag_elems.push_back(order[k]);
   }
   const std::vector<char> no_bnd(N, 0);
-  build_partition(N, conn, mat0, mag_elems, no_bnd, hp_mag);
+  build_partition(N, conn, mat0, mag_elems, no_bnd, hp_mag, node_cap);
 
   MM_TRY(upload_part(c, hp_all, E, c.part_all));
   c.nodes_all = (long long)hp_all.nodes.size();
