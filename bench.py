@@ -122,10 +122,12 @@ def main():
     ap.add_argument("--precond", default="amgcl", choices=["amgcl", "amg"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the hysteresis path has no multi-GPU form in this build "
-              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the hysteresis path runs on a single MI355X here "
+              "(element-block partitions would shard across GPUs with a halo exchange per apply); "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from mumag_matrixfree2_amd.context import MeshContext, make_params
@@ -139,6 +141,7 @@ def main():
     ctx = MeshContext(mesh["knt"], mesh["ijk"][:, :4], mesh["ijk"][:, 4], *mats,
                       boundary_mask=mesh["boundary_mask"])
     ctx.setup_precond(args.precond)
+    ctx.set_precision(args.precision)
     log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: N={ctx.N} E={ctx.E} "
         f"AMG levels={[l['A'].shape[0] for l in ctx.amg_levels]}")
 

@@ -76,6 +76,12 @@ int mm_ctx_get_nodal(int64_t ctx, double* M_nodal, double* inv_M_rel, double* in
 /* diag(K_ex)/V_mag (N,): compute_exchange_diagonal (src/energy_kernels.py:492-566). */
 int mm_ctx_get_exchange_diag(int64_t ctx, double* d);
 
+/* Precision of the Poisson preconditioner: 0 = FP64 V-cycle; 1 = mixed: the whole V-cycle in
+ * FP32 (FP32 matrix-free applies and coarse levels) while the PCG recurrence, its operator
+ * apply (the true residual), the dot products and alpha/beta stay FP64, so solutions still
+ * meet the FP64 tolerance. */
+int mm_set_precision(int64_t ctx, int precision);
+
 /* Select the geometry source of the Poisson apply (see flags of mm_ctx_create). */
 int mm_set_geometry_mode(int64_t ctx, int mode);
 

@@ -68,6 +68,7 @@ SIGNATURES = {
     "mm_ctx_get_geometry": [C.c_int64, i32p, f64p, f64p],
     "mm_ctx_get_nodal": [C.c_int64, f64p, f64p, f64p],
     "mm_set_geometry_mode": [C.c_int64, C.c_int],
+    "mm_set_precision": [C.c_int64, C.c_int],
     "mm_poisson_apply": [C.c_int64, f64p, f64p],
     "mm_poisson_rhs": [C.c_int64, f64p, f64p],
     "mm_poisson_diag": [C.c_int64, f64p],

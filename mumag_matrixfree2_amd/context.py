@@ -15,7 +15,7 @@ import numpy as np
 from . import _hip
 from ._hip import call, f64, i32, ptr
 
-GEOM_MODES = {"on_the_fly": 0, "stored_f64": 1, "stored_f32": 2}
+GEOM_MODES = {"on_the_fly": 0, "stored_f64": 1, "stored_f32": 2, "on_the_fly_f32": 3}
 METHODS = {"pcohen_hs": 0, "lbfgs": 1}
 
 
@@ -67,6 +67,7 @@ class MeshContext:
         self.node_list_entries = int(info[7])
         self.precond_type = None
         self.amg_levels = None
+        self.precision = "fp64"
 
     # ------------------------------------------------------------------ lifetime
     def close(self):
@@ -102,6 +103,11 @@ class MeshContext:
         d = np.zeros(self.N)
         call("mm_ctx_get_exchange_diag", self.handle, ptr(d))
         return d
+
+    def set_precision(self, precision: str):
+        """'fp64' or 'mixed' (FP32 V-cycle under an FP64 PCG)."""
+        call("mm_set_precision", self.handle, {"fp64": 0, "mixed": 1}[precision])
+        self.precision = precision
 
     def set_geometry_mode(self, geom: str):
         call("mm_set_geometry_mode", self.handle, GEOM_MODES[geom])

@@ -347,6 +347,8 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
 
   // Poisson work vectors
   for (double** v : {&c.pb, &c.pr, &c.pz, &c.pp, &c.pq, &c.pt, &c.px0, &c.work_U, &c.diagA}) MM_TRY(dalloc(c, v, N));
+  MM_TRY(dalloc(c, &c.px0f, N));
+  MM_TRY(dalloc(c, &c.ptf, N));
   MM_HIP(hipStreamSynchronize(c.stream));
   return 0;
 }
@@ -479,6 +481,13 @@ int mm_ctx_get_exchange_diag(int64_t h, double* d) {
   return 0;
 }
 
+int mm_set_precision(int64_t h, int precision) {
+  GET_CTX(h);
+  MM_CHECK(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32 preconditioner)");
+  c.precision = precision;
+  return 0;
+}
+
 int mm_set_geometry_mode(int64_t h, int mode) {
   GET_CTX(h);
   MM_CHECK(mode >= 0 && mode <= 2, "geometry mode must be 0, 1 or 2");
@@ -549,6 +558,15 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
     MM_TRY(upload(c, &L.Rx, Rx, Rp[n_next]));
   }
   if (dense_inv) MM_TRY(upload(c, &L.dense_inv, dense_inv, (size_t)n * n));
+  // FP32 copies for the mixed-precision V-cycle
+  auto to_f32 = [](const double* a, size_t k) { return std::vector<float>(a, a + k); };
+  MM_TRY(upload(c, &L.Mf, to_f32(M, n).data(), n));
+  if (l > 0 && Ap) MM_TRY(upload(c, &L.Axf, to_f32(Ax, Ap[n]).data(), Ap[n]));
+  if (n_next > 0) {
+    MM_TRY(upload(c, &L.Pxf, to_f32(Px, Pp[n]).data(), Pp[n]));
+    MM_TRY(upload(c, &L.Rxf, to_f32(Rx, Rp[n_next]).data(), Rp[n_next]));
+  }
+  if (dense_inv) MM_TRY(upload(c, &L.dense_invf, to_f32(dense_inv, (size_t)n * n).data(), (size_t)n * n));
   // SpMV lanes per row: power of two near a quarter of the average row length, in [4, 64]
   auto lanes = [](long nnz, long rows) {
     const double avg = rows > 0 ? (double)nnz / (double)rows : 0.0;
@@ -565,6 +583,9 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
     MM_TRY(dalloc(c, &L.b, n));
     MM_TRY(dalloc(c, &L.x, n));
     MM_TRY(dalloc(c, &L.t, n));
+    MM_TRY(dalloc(c, &L.bf, n));
+    MM_TRY(dalloc(c, &L.xf, n));
+    MM_TRY(dalloc(c, &L.tf, n));
   }
   c.lv[l] = L;
   return 0;
@@ -865,15 +886,18 @@ int mm_counters(int64_t h, int64_t* out6) {
 
 int mm_bench_apply(int64_t h, int geom, int reps, double* ms) {
   GET_CTX(h);
-  MM_CHECK(geom >= 0 && geom <= 2 && reps > 0, "bad arguments");
+  MM_CHECK(geom >= 0 && geom <= 3 && reps > 0, "bad arguments");
   hipEvent_t e0, e1;
   MM_HIP(hipEventCreate(&e0));
   MM_HIP(hipEventCreate(&e1));
   vfill(c.pp, 1.0, c.N, c.stream);
   vfill(c.pq, 0.0, c.N, c.stream);
-  MM_TRY(launch_apply(c, geom, c.pp, c.pq, c.stream));  // warm-up
+  MM_HIP(hipMemsetAsync(c.px0f, 0, c.N * sizeof(float), c.stream));
+  // geom 3: the FP32 apply of the mixed-precision preconditioner (x, y in the FP32 work vectors)
+  auto one = [&]() { return geom == 3 ? launch_apply_f32(c, c.px0f, c.ptf, c.stream) : launch_apply(c, geom, c.pp, c.pq, c.stream); };
+  MM_TRY(one());  // warm-up
   MM_HIP(hipEventRecord(e0, c.stream));
-  for (int r = 0; r < reps; ++r) MM_TRY(launch_apply(c, geom, c.pp, c.pq, c.stream));
+  for (int r = 0; r < reps; ++r) MM_TRY(one());
   MM_HIP(hipEventRecord(e1, c.stream));
   MM_HIP(hipEventSynchronize(e1));
   float t = 0;

@@ -47,6 +47,27 @@ __device__ __forceinline__ void tet_grads(const double (*X)[kMaxBlockNodes], int
   vol = fabs(det) * (1.0 / 6.0);
 }
 
+// FP32 twin of tet_grads on block-relative FP32 coordinates (mixed-precision preconditioner).
+__device__ __forceinline__ void tet_grads_f(const float (*X)[kMaxBlockNodes], int l0, int l1, int l2, int l3,
+                                            float g[4][3], float& vol) {
+  const float ox = X[0][l0], oy = X[1][l0], oz = X[2][l0];
+  const float c1x = X[0][l1] - ox, c1y = X[1][l1] - oy, c1z = X[2][l1] - oz;
+  const float c2x = X[0][l2] - ox, c2y = X[1][l2] - oy, c2z = X[2][l2] - oz;
+  const float c3x = X[0][l3] - ox, c3y = X[1][l3] - oy, c3z = X[2][l3] - oz;
+  const float n1x = c2y * c3z - c2z * c3y, n1y = c2z * c3x - c2x * c3z, n1z = c2x * c3y - c2y * c3x;
+  const float n2x = c3y * c1z - c3z * c1y, n2y = c3z * c1x - c3x * c1z, n2z = c3x * c1y - c3y * c1x;
+  const float n3x = c1y * c2z - c1z * c2y, n3y = c1z * c2x - c1x * c2z, n3z = c1x * c2y - c1y * c2x;
+  const float det = c1x * n1x + c1y * n1y + c1z * n1z;
+  const float inv = 1.0f / det;
+  g[1][0] = n1x * inv; g[1][1] = n1y * inv; g[1][2] = n1z * inv;
+  g[2][0] = n2x * inv; g[2][1] = n2y * inv; g[2][2] = n2z * inv;
+  g[3][0] = n3x * inv; g[3][1] = n3y * inv; g[3][2] = n3z * inv;
+  g[0][0] = -(g[1][0] + g[2][0] + g[3][0]);
+  g[0][1] = -(g[1][1] + g[2][1] + g[3][1]);
+  g[0][2] = -(g[1][2] + g[2][2] + g[3][2]);
+  vol = fabsf(det) * (1.0f / 6.0f);
+}
+
 __device__ __forceinline__ void unpack(uint32_t lc, int& l0, int& l1, int& l2, int& l3) {
   l0 = lc & 255u;
   l1 = (lc >> 8) & 255u;

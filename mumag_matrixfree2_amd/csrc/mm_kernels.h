@@ -109,6 +109,9 @@ struct Level {
   double* dense_inv = nullptr;        // coarsest level
   int lanesA = 8, lanesP = 8, lanesR = 8;  // SpMV lanes per row
   double *b = nullptr, *x = nullptr, *t = nullptr;
+  // FP32 copies for the mixed-precision V-cycle
+  float *Axf = nullptr, *Pxf = nullptr, *Rxf = nullptr, *Mf = nullptr, *dense_invf = nullptr;
+  float *bf = nullptr, *xf = nullptr, *tf = nullptr;
 };
 
 struct Ctx {
@@ -147,6 +150,8 @@ struct Ctx {
   int amg_kind = 0;                   // 0: Jacobi on diag(A) only, 1: SPAI0 V-cycle, 2: damped-Jacobi V-cycle
   std::vector<Level> lv;
   double* diagA = nullptr;            // N
+  int precision = 0;                  // 0: FP64 V-cycle, 1: FP32 V-cycle (mixed precision)
+  float *px0f = nullptr, *ptf = nullptr;  // level-0 FP32 work vectors
 
   // minimiser state (3N unless noted)
   double *m = nullptr, *U = nullptr /*N*/, *g_raw = nullptr, *g_tan = nullptr, *g_ext = nullptr, *g_prev = nullptr;
@@ -173,6 +178,7 @@ int resolve_apply_timing(Ctx& c);
 
 // element kernels (mm_kernels.hip)
 int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s);
+int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s);
 int launch_rhs(const Ctx& c, const double* m, double* b, hipStream_t s);
 int launch_energy(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,
                   hipStream_t s);

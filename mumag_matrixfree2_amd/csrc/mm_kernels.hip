@@ -99,6 +99,77 @@ __global__ __launch_bounds__(kBlock) void k_apply(Part p, Geo geo, const double*
   }
 }
 
+// ------------------------------------------------ Poisson apply, FP32 (preconditioner)
+// Same operator in FP32: node values, block-relative coordinates (origin = the block's first
+// node, so edge vectors keep full FP32 precision) and element arithmetic in FP32; x and y are
+// FP32. The LDS accumulator stays FP64: on gfx950 ds_add_f32 runs at a fraction of the rate of
+// ds_add_f64 (measured: 53 us vs 16 us per apply on the 2.06M-tet cube). Used inside the
+// mixed-precision V-cycle only.
+__global__ __launch_bounds__(kBlock) void k_apply_f32(Part p, Geo geo, const float* __restrict__ x,
+                                                      float* __restrict__ y, float reg) {
+  __shared__ float sx[kMaxBlockNodes];
+  __shared__ double sy[kMaxBlockNodes];
+  __shared__ float sX[3][kMaxBlockNodes];
+  const BlkInfo bi = blk_info(p);
+  if (!bi.valid) return;
+  const int tid = threadIdx.x;
+  constexpr int kPre = 4;
+  uint32_t lcv[kPre];
+#pragma unroll
+  for (int r = 0; r < kPre; ++r) {
+    const int e = bi.e0 + tid + r * kBlock;
+    lcv[r] = e < bi.e1 ? p.lconn[e] : 0u;
+  }
+  const uint32_t id0 = p.nodes[bi.n0] & kIdMask;
+  const double ox = geo.knt[3 * id0], oy = geo.knt[3 * id0 + 1], oz = geo.knt[3 * id0 + 2];
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t f = p.nodes[bi.n0 + i];
+    const uint32_t id = f & kIdMask;
+    sx[i] = (f & kBoundary) ? 0.0f : x[id];
+    sy[i] = 0.0;
+    sX[0][i] = (float)(geo.knt[3 * id + 0] - ox);
+    sX[1][i] = (float)(geo.knt[3 * id + 1] - oy);
+    sX[2][i] = (float)(geo.knt[3 * id + 2] - oz);
+  }
+  __syncthreads();
+  auto element = [&](uint32_t lc) {
+    int l0, l1, l2, l3;
+    unpack(lc, l0, l1, l2, l3);
+    float g[4][3], vol;
+    tet_grads_f(sX, l0, l1, l2, l3, g, vol);
+    const float x0 = sx[l0];
+    const float d1 = sx[l1] - x0, d2 = sx[l2] - x0, d3 = sx[l3] - x0;
+    const float gx = (d1 * g[1][0] + d2 * g[2][0] + d3 * g[3][0]) * vol;
+    const float gy = (d1 * g[1][1] + d2 * g[2][1] + d3 * g[3][1]) * vol;
+    const float gz = (d1 * g[1][2] + d2 * g[2][2] + d3 * g[3][2]) * vol;
+    atomicAdd(&sy[l0], (double)(g[0][0] * gx + g[0][1] * gy + g[0][2] * gz));
+    atomicAdd(&sy[l1], (double)(g[1][0] * gx + g[1][1] * gy + g[1][2] * gz));
+    atomicAdd(&sy[l2], (double)(g[2][0] * gx + g[2][1] * gy + g[2][2] * gz));
+    atomicAdd(&sy[l3], (double)(g[3][0] * gx + g[3][1] * gy + g[3][2] * gz));
+  };
+#pragma unroll
+  for (int r = 0; r < kPre; ++r) {
+    const int e = bi.e0 + tid + r * kBlock;
+    if (e < bi.e1) element(lcv[r]);
+  }
+  for (int e = bi.e0 + tid + kPre * kBlock; e < bi.e1; e += kBlock) element(p.lconn[e]);
+  __syncthreads();
+  for (int i = tid; i < bi.nb; i += kBlock) {
+    const uint32_t f = p.nodes[bi.n0 + i];
+    const uint32_t id = f & kIdMask;
+    if (f & kBoundary) {
+      if (f & kOwner) y[id] = x[id];
+      continue;
+    }
+    float v = (float)sy[i];
+    if (f & kOwner) v += reg * sx[i];
+    if (f & kPrivate)
+      y[id] = v;
+    else
+      unsafeAtomicAdd(&y[id], v);
+  }
+}
+
 // ------------------------------------------------------------- demag right-hand side (K2)
 // b_a = sum_e (Js_e Ve / 4) grad phi_a . (m_0 + m_1 + m_2 + m_3) over magnetic elements.
 __global__ __launch_bounds__(kBlock) void k_rhs(Part p, Geo geo, const double* __restrict__ m,
@@ -447,6 +518,13 @@ int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t
     default:
       k_apply<kGeomStored32><<<grid, kBlock, 0, s>>>(p, c.geo, x, y, c.reg);
   }
+  return 0;
+}
+
+int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s) {
+  const Part& p = c.part_all;
+  if (p.nblk == 0) return 0;
+  k_apply_f32<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, x, y, (float)c.reg);
   return 0;
 }
 

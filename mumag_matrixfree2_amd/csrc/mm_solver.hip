@@ -132,21 +132,20 @@ __global__ void k_post(double* __restrict__ x, const double* __restrict__ M, con
     x[i] += M[i] * (b[i] - t[i]);
 }
 
-// CSR SpMV with LANES lanes per row. MODE 0: y = A x; 1: y += A x (and z = 0 on the
-// same rows when z != nullptr); 2: y = A (x - x2)
-template <int MODE, int LANES>
+// CSR SpMV with LANES lanes per row, accumulation in the matrix value type TA.
+// MODE 0: y = A x; 1: y += A x (and z = 0 on the same rows when z != nullptr); 2: y = A (x - x2)
+template <int MODE, int LANES, typename TA, typename TX, typename TX2, typename TY>
 __global__ __launch_bounds__(kBlock) void k_spmv(int nrows, const int* __restrict__ Ap, const int* __restrict__ Ai,
-                                                 const double* __restrict__ Ax, const double* __restrict__ x,
-                                                 const double* __restrict__ x2, double* __restrict__ y,
-                                                 double* __restrict__ z) {
+                                                 const TA* __restrict__ Ax, const TX* __restrict__ x,
+                                                 const TX2* __restrict__ x2, TY* __restrict__ y, TY* __restrict__ z) {
   const int lane = threadIdx.x & (LANES - 1);
   const int row = (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
-  double s = 0.0;
+  TA s = 0;
   if (row < nrows) {
     const int j0 = Ap[row], j1 = Ap[row + 1];
     for (int j = j0 + lane; j < j1; j += LANES) {
       const int c = Ai[j];
-      const double xv = (MODE == 2) ? (x[c] - x2[c]) : x[c];
+      const TA xv = (MODE == 2) ? (TA)((TA)x[c] - (TA)x2[c]) : (TA)x[c];
       s += Ax[j] * xv;
     }
   }
@@ -154,10 +153,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrows, const int* __restric
   for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LANES);
   if (row < nrows && lane == 0) {
     if (MODE == 1) {
-      y[row] += s;
-      if (z) z[row] = 0.0;
+      y[row] += (TY)s;
+      if (z) z[row] = 0;
     } else {
-      y[row] = s;
+      y[row] = (TY)s;
     }
   }
 }
@@ -174,29 +173,73 @@ __global__ __launch_bounds__(kBlock) void k_gemv(int n, const double* __restrict
   if (lane == 0) y[row] = s;
 }
 
-template <int MODE>
-static void spmv_lanes(int lanes, int nrows, const int* Ap, const int* Ai, const double* Ax, const double* x,
-                       const double* x2, double* y, double* z, hipStream_t s) {
+template <int MODE, typename TA, typename TX, typename TX2, typename TY>
+static void spmv_lanes(int lanes, int nrows, const int* Ap, const int* Ai, const TA* Ax, const TX* x, const TX2* x2,
+                       TY* y, TY* z, hipStream_t s) {
   const int grid = (int)(((long)nrows * lanes + kBlock - 1) / kBlock);
   switch (lanes) {
-    case 4: k_spmv<MODE, 4><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
-    case 8: k_spmv<MODE, 8><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
-    case 16: k_spmv<MODE, 16><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
-    case 32: k_spmv<MODE, 32><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
-    default: k_spmv<MODE, 64><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
+    case 4: k_spmv<MODE, 4, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
+    case 8: k_spmv<MODE, 8, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
+    case 16: k_spmv<MODE, 16, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
+    case 32: k_spmv<MODE, 32, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
+    default: k_spmv<MODE, 64, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
   }
 }
 
-// lanes per row: the power of two nearest half the average row length, in [4, 64]
-static void spmv(int mode, int lanes, int nrows, const int* Ap, const int* Ai, const double* Ax, const double* x,
-                 const double* x2, double* y, hipStream_t s, double* z = nullptr) {
+// y = A x (mode 0), y += A x and z = 0 (mode 1), y = A (x - x2) (mode 2)
+template <typename TA, typename TX, typename TX2, typename TY>
+static void spmv(int mode, int lanes, int nrows, const int* Ap, const int* Ai, const TA* Ax, const TX* x,
+                 const TX2* x2, TY* y, hipStream_t s, TY* z = nullptr) {
   if (nrows <= 0) return;
   if (mode == 0)
-    spmv_lanes<0>(lanes, nrows, Ap, Ai, Ax, x, x2, y, nullptr, s);
+    spmv_lanes<0>(lanes, nrows, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, s);
   else if (mode == 1)
     spmv_lanes<1>(lanes, nrows, Ap, Ai, Ax, x, x2, y, z, s);
   else
-    spmv_lanes<2>(lanes, nrows, Ap, Ai, Ax, x, x2, y, nullptr, s);
+    spmv_lanes<2>(lanes, nrows, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, s);
+}
+
+// ------------------------------------------------------------ FP32 V-cycle helpers
+template <typename TB>
+__global__ void k_scale_f(const float* __restrict__ M, const TB* __restrict__ b, float* __restrict__ x,
+                          float* __restrict__ t, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    x[i] = M[i] * (float)b[i];
+    if (t) t[i] = 0.0f;
+  }
+}
+
+__global__ void k_post_f(float* __restrict__ x, const float* __restrict__ M, const float* __restrict__ b,
+                         const float* __restrict__ t, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] += M[i] * (b[i] - t[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gemv_f(int n, const float* __restrict__ Ainv, const float* __restrict__ b,
+                                                   float* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (row >= n) return;
+  float s = 0.0f;
+  for (int j = lane; j < n; j += 64) s += Ainv[(long)row * n + j] * b[j];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (lane == 0) y[row] = s;
+}
+
+// z = x + M (r - t) in FP64 from FP32 x, M, t and FP64 r; out = r.z (FP64)
+__global__ __launch_bounds__(kBlock) void k_post_dot_mixed(const float* __restrict__ x, const float* __restrict__ M,
+                                                           const double* __restrict__ r, const float* __restrict__ t,
+                                                           double* __restrict__ z, long n,
+                                                           double* __restrict__ partial,
+                                                           unsigned* __restrict__ counter, double* __restrict__ out) {
+  double v[1] = {0.0};
+  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    const double zi = (double)x[i] + (double)M[i] * (r[i] - (double)t[i]);
+    z[i] = zi;
+    v[0] += r[i] * zi;
+  }
+  reduce_store<1, false>(v, partial, counter, out);
 }
 
 // --------------------------------------------------------------------- apply timing
@@ -238,18 +281,54 @@ static void vcycle_coarse(Ctx& c, int l, hipStream_t s) {
   }
   Level& Nx = c.lv[l + 1];
   k_scale<<<vgrid(L.n), kBlock, 0, s>>>(L.M, L.b, L.x, nullptr, L.n);            // pre-smooth from x = 0
-  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, nullptr, L.t, s);                 // t = A x
+  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, (const double*)nullptr, L.t, s);                 // t = A x
   spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rx, L.b, L.t, Nx.b, s);                   // b' = R (b - t)
   vcycle_coarse(c, l + 1, s);
-  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Px, Nx.x, nullptr, L.x, s);                // x += P e
-  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, nullptr, L.t, s);                 // t = A x
+  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Px, Nx.x, (const double*)nullptr, L.x, s);                // x += P e
+  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, (const double*)nullptr, L.t, s);                 // t = A x
   k_post<<<vgrid(L.n), kBlock, 0, s>>>(L.x, L.M, L.b, L.t, L.n);                  // x += M (b - t)
+}
+
+// FP32 coarse part of the V-cycle (levels >= 1).
+static void vcycle_coarse_f(Ctx& c, int l, hipStream_t s) {
+  Level& L = c.lv[l];
+  const int last = (int)c.lv.size() - 1;
+  if (l == last) {
+    k_gemv_f<<<(int)(((long)L.n * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>(L.n, L.dense_invf, L.bf, L.xf);
+    return;
+  }
+  Level& Nx = c.lv[l + 1];
+  k_scale_f<float><<<vgrid(L.n), kBlock, 0, s>>>(L.Mf, L.bf, L.xf, (float*)nullptr, L.n);
+  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.xf, (const float*)nullptr, L.tf, s);
+  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rxf, L.bf, L.tf, Nx.bf, s);
+  vcycle_coarse_f(c, l + 1, s);
+  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Pxf, Nx.xf, (const float*)nullptr, L.xf, s);
+  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.xf, (const float*)nullptr, L.tf, s);
+  k_post_f<<<vgrid(L.n), kBlock, 0, s>>>(L.xf, L.Mf, L.bf, L.tf, L.n);
+}
+
+// Mixed-precision preconditioner: the whole V-cycle in FP32 (FP32 matrix-free fine applies,
+// FP32 coarse matrices and dense inverse); input r and output z in FP64, r.z in FP64.
+static int precond_f32(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
+  const long N = c.N;
+  Level& L0 = c.lv[0];
+  Level& L1 = c.lv[1];
+  k_scale_f<double><<<vgrid(N), kBlock, 0, s>>>(L0.Mf, r, c.px0f, c.ptf, N);      // x = M0 r, t = 0
+  MM_TRY(launch_apply_f32(c, c.px0f, c.ptf, s));                                  // t = A x
+  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rxf, r, c.ptf, L1.bf, s);             // b1 = R0 (r - t)
+  vcycle_coarse_f(c, 1, s);
+  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Pxf, L1.xf, (const float*)nullptr, c.px0f, s, c.ptf);
+  MM_TRY(launch_apply_f32(c, c.px0f, c.ptf, s));                                  // t = A x
+  k_post_dot_mixed<<<red_grid(N), kBlock, 0, s>>>(c.px0f, L0.Mf, r, c.ptf, z, N, c.partial, c.counter + 4, rz_out);
+  c.n_vcycle += 1;
+  return 0;
 }
 
 // z = V-cycle(r) with the matrix-free level 0; also writes r.z into out.
 static int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
   const long N = c.N;
   Level& L0 = c.lv[0];
+  if (c.precision == 1 && c.lv.size() > 1 && L0.Mf != nullptr) return precond_f32(c, r, z, rz_out, s);
   if (c.lv.size() <= 1) {
     if (L0.dense_inv)  // single-level hierarchy: exact dense solve
       k_gemv<<<(int)((N * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>((int)N, L0.dense_inv, r, z);
@@ -263,7 +342,7 @@ static int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream
   MM_TRY(timed_apply(c, c.px0, c.pt, s));                                         // t = A x
   spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rx, r, c.pt, L1.b, s);                // b1 = R0 (r - t)
   vcycle_coarse(c, 1, s);
-  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Px, L1.x, nullptr, c.px0, s, c.pt); // x += P0 e1, t = 0
+  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Px, L1.x, (const double*)nullptr, c.px0, s, c.pt); // x += P0 e1, t = 0
   MM_TRY(timed_apply(c, c.px0, c.pt, s));                                         // t = A x
   k_post_dot<<<red_grid(N), kBlock, 0, s>>>(c.px0, L0.M, r, c.pt, z, N, c.partial, c.counter + 4, rz_out);
   c.n_apply += 2;

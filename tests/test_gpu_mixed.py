@@ -1,0 +1,103 @@
+"""Mixed-precision form: FP32 V-cycle under the FP64 PCG.
+
+Bar: the FP32 preconditioner is the FP64 one to FP32 accuracy (<= 1e-5 relative); the solve
+still meets the FP64 tolerance (true residual computed by the oracle in FP64); energies of a
+field step agree with the FP64 GPU path to 1e-10 of the energy scale; a sweep switches at the
+same field step.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import ndfeb_materials, oracle_levels, tilted_m0
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def mixed_ctx(cube_c0):
+    from mumag_matrixfree2_amd.context import MeshContext
+
+    A, K1, Js, k = ndfeb_materials()
+    m = cube_c0
+    ctx = MeshContext(m["knt"], m["ijk"][:, :4], m["ijk"][:, 4], A, K1, Js, k, boundary_mask=m["boundary_mask"])
+    ctx.setup_precond("amgcl")
+    print(f"[mixed_ctx] levels={[l['A'].shape[0] for l in ctx.amg_levels]}", flush=True)
+    yield ctx
+    ctx.set_precision("fp64")
+
+
+def test_fp32_vcycle_is_fp64_vcycle_to_fp32_accuracy(mixed_ctx, oracle_c0):
+    from oracle.solvers import vcycle
+
+    ctx = mixed_ctx
+    r = np.random.default_rng(31).standard_normal(ctx.N)
+    ctx.set_precision("mixed")
+    z = ctx.precond_apply(r)
+    ctx.set_precision("fp64")
+    z_ref = vcycle(oracle_levels(ctx.amg_levels, "amgcl"), oracle_c0.apply_A, r, "amgcl")
+    assert rel(z, z_ref) <= 1e-5
+    assert float(r @ z) > 0
+
+
+def test_mixed_solve_meets_fp64_tolerance(mixed_ctx, oracle_c0):
+    ctx = mixed_ctx
+    m = tilted_m0(ctx.N, seed=32, sigma=0.4)
+    U64, it64, _ = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+    ctx.set_precision("mixed")
+    U, it, rr = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+    ctx.set_precision("fp64")
+    b = oracle_c0.rhs(m)
+    true_rr = np.linalg.norm(b - oracle_c0.A @ U) / np.linalg.norm(b)
+    print(f"[mixed solve] it={it} (fp64 {it64}) rr={rr:.3e} true={true_rr:.3e}", flush=True)
+    assert true_rr <= 1.001e-10
+    assert it <= it64 + 2
+    assert rel(U, U64) <= 1e-8
+
+
+@pytest.mark.parametrize("method", ["pcohen_hs", "lbfgs"])
+def test_mixed_field_step_energy(mixed_ctx, oracle_c0, method):
+    from mumag_matrixfree2_amd.context import make_params
+
+    ctx = mixed_ctx
+    m0 = tilted_m0(ctx.N, seed=33, sigma=0.1)
+    B = np.array([0.017452, 0.0, 0.999848]) * (-0.4)
+    p = make_params(method, tau_f=1e-10, cg_tol=1e-10)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        ctx.set_precision(prec)
+        ctx.state_set(m0, np.zeros(ctx.N))
+        out[prec] = ctx.minimize_resident(p, B)
+    ctx.set_precision("fp64")
+    m, U = ctx.state_get()
+    terms = oracle_c0.energy_terms(m, U, B)
+    scale = sum(abs(v) for v in terms.values())
+    print(f"[mixed {method}] E64={out['fp64']['E']:.15e} Emix={out['mixed']['E']:.15e} "
+          f"it={out['fp64']['iters']}/{out['mixed']['iters']}", flush=True)
+    assert abs(out["mixed"]["E"] - out["fp64"]["E"]) <= 1e-10 * scale
+
+
+def test_mixed_sweep_same_switching_step(small_cube):
+    from mumag_matrixfree2_amd.context import MeshContext
+    from mumag_matrixfree2_amd.hysteresis_loop import LoopParams, run_hysteresis_loop
+
+    A, K1, Js, k = ndfeb_materials()
+    m = small_cube
+    ctx = MeshContext(m["knt"], m["ijk"][:, :4], m["ijk"][:, 4], A, K1, Js, k, boundary_mask=m["boundary_mask"])
+    ctx.setup_precond("amgcl", max_coarse=50)
+    h = np.array([0.017452, 0.0, 0.999848])
+    params = LoopParams(h_dir=h, B_start=2.0 / 1.61, B_end=-8.0 / 1.61, dB=-0.5 / 1.61, tau_f=1e-10, cg_tol=1e-10)
+    m0 = tilted_m0(ctx.N, seed=0, sigma=0.01)
+    r64 = run_hysteresis_loop(ctx, m0, params)
+    ctx.set_precision("mixed")
+    rmx = run_hysteresis_loop(ctx, m0, params)
+    ctx.close()
+    E64, Emx = r64["history"][:, 2], rmx["history"][:, 2]
+    assert r64["switching_step"] == rmx["switching_step"] > 0
+    assert np.max(np.abs(Emx - E64) / np.maximum(np.abs(E64), 1.0)) <= 1e-8
