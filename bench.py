@@ -123,6 +123,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--max-coarse", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the hysteresis path runs on a single MI355X here "
@@ -140,7 +141,7 @@ def main():
     mats = materials()
     ctx = MeshContext(mesh["knt"], mesh["ijk"][:, :4], mesh["ijk"][:, 4], *mats,
                       boundary_mask=mesh["boundary_mask"])
-    ctx.setup_precond(args.precond)
+    ctx.setup_precond(args.precond, max_coarse=args.max_coarse)
     ctx.set_precision(args.precision)
     log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: N={ctx.N} E={ctx.E} "
         f"AMG levels={[l['A'].shape[0] for l in ctx.amg_levels]}")
@@ -148,36 +149,54 @@ def main():
     p = make_params("pcohen_hs", tau_f=1e-10, eps_a=1e-12, cg_tol=1e-10, max_iter=2000)
     B_vals = field_values(2.0 / JS, -8.0 / JS, -0.25 / JS, False)  # 41 values
     m0 = initial_m(ctx.N)
+    steps = min(args.steps, len(B_vals))
+
+    def sweep(precision, timed):
+        ctx.set_precision(precision)
+        ctx.state_set(m0, np.zeros(ctx.N))
+        if timed:
+            ctx.apply_timing(True)
+        J, E, rows = [], [], []
+        t_start = time.perf_counter()
+        for k in range(steps):
+            ts = time.perf_counter()
+            info = ctx.minimize_resident(p, B_vals[k] * H_DIR)
+            J.append(ctx.state_j_parallel(H_DIR))
+            E.append(info["E"])
+            rows.append(time.perf_counter() - ts)
+            log(f"[bench:{precision}] step {k:2d} B={B_vals[k] * JS:+.3f} T J={J[-1] * JS:+.5f} T "
+                f"E={info['E']:+.10e} it={info['iters']} nf={info['evals']} icg={info['demag_iters']} "
+                f"t={rows[-1]:.3f}s")
+        return time.perf_counter() - t_start, np.array(J), np.array(E), rows
 
     # warm-up on a separate state
+    ctx.set_precision(args.precision)
     ctx.state_set(m0, np.zeros(ctx.N))
     for k in range(args.warmup):
         ctx.minimize_resident(p, B_vals[k % len(B_vals)] * H_DIR)
 
-    steps = min(args.steps, len(B_vals))
-    ctx.state_set(m0, np.zeros(ctx.N))
-    ctx.apply_timing(True)
     c0 = ctx.counters()
-    J, rows = [], []
-    t_start = time.perf_counter()
-    for k in range(steps):
-        ts = time.perf_counter()
-        info = ctx.minimize_resident(p, B_vals[k] * H_DIR)
-        J.append(ctx.state_j_parallel(H_DIR))
-        rows.append((B_vals[k] * JS, J[-1] * JS, info["E"], info["iters"], info["evals"], info["demag_iters"],
-                     time.perf_counter() - ts))
-        log(f"[bench] step {k:2d} B={B_vals[k] * JS:+.3f} T J={J[-1] * JS:+.5f} T E={info['E']:+.8e} "
-            f"it={info['iters']} nf={info['evals']} icg={info['demag_iters']} t={rows[-1][-1]:.3f}s")
-    elapsed = time.perf_counter() - t_start
+    elapsed, J, Es, rows = sweep(args.precision, timed=True)
     apply_ms, apply_n = ctx.apply_timing(False)
     c1 = ctx.counters()
     pcg_its = c1["pcg_iters"] - c0["pcg_iters"]
+    errors = None
+    if args.precision == "mixed":
+        # reference FP64 sweep in the same run: per-step relative energy error and switching step
+        t64, J64, E64, _ = sweep("fp64", timed=False)
+        errors = {
+            "max_rel_energy_err_vs_fp64": float(np.max(np.abs(Es - E64) / np.maximum(np.abs(E64), 1.0))),
+            "switching_step_fp64": switching_step(J64),
+            "fp64_field_steps_per_s_same_run": steps / t64,
+            "speedup_vs_fp64": t64 / elapsed,
+        }
 
     avg_apply_s = apply_ms / max(apply_n, 1) * 1e-3
     algo = apply_algorithmic_bytes(ctx)
     achieved = algo / avg_apply_s / 1e9
+    label = "FP64" if args.precision == "fp64" else "mixed FP32 V-cycle / FP64 PCG"
     result = {
-        "metric": "hysteresis field-steps/s (FP64, 2.06M-tet cube + airbox)",
+        "metric": f"hysteresis field-steps/s ({label}, 2.06M-tet cube + airbox)",
         "value": steps / elapsed,
         "unit": "field-steps/s",
         "n_gpus": 1,
@@ -187,16 +206,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
         "data": "synthetic (generated mesh, seeded initial magnetisation)",
         "config": {
-            "workload": "cube80nm_60^3x6tets+5layer_airbox_2.06Mtet_fp64_sweep+2..-8T_41steps",
+            "workload": f"cube80nm_60^3x6tets+5layer_airbox_2.06Mtet_{args.precision}_sweep+2..-8T_41steps",
             "tets": ctx.E, "nodes": ctx.N, "method": "pcohen_hs", "precond": f"SA-AMG V-cycle ({args.precond})",
             "amg_levels": [int(l["A"].shape[0]) for l in ctx.amg_levels],
             "pcg_tol": 1e-10, "tau_f": 1e-10,
         },
         "pcg_iters_per_s": pcg_its / elapsed,
-        "switching_step": switching_step(np.array(J)),
+        "switching_step": switching_step(J),
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -212,10 +231,12 @@ def main():
         },
     }
 
+    if errors is not None:
+        result["mixed_vs_fp64"] = errors
     if not args.no_cpu_baseline:
         cpu_T = [2.0 - 0.25 * k for k in range(args.cpu_steps)]
         done, dt = cpu_baseline(mesh, mats, cpu_T)
-        gpu_same = sum(r[-1] for r in rows[:done])
+        gpu_same = sum(rows[:done])
         result["cpu_baseline"] = {
             "value": done / dt, "unit": "field-steps/s", "cores": 1, "kind": "port",
             "sample": f"oracle (NumPy/SciPy, single thread) pcohen_hs on the same mesh and AMG hierarchy, "

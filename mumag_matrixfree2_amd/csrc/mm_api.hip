@@ -583,9 +583,11 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
     MM_TRY(dalloc(c, &L.b, n));
     MM_TRY(dalloc(c, &L.x, n));
     MM_TRY(dalloc(c, &L.t, n));
+    MM_TRY(dalloc(c, &L.xs, n));
     MM_TRY(dalloc(c, &L.bf, n));
     MM_TRY(dalloc(c, &L.xf, n));
     MM_TRY(dalloc(c, &L.tf, n));
+    MM_TRY(dalloc(c, &L.xsf, n));
   }
   c.lv[l] = L;
   return 0;
@@ -870,6 +872,48 @@ int mm_apply_timing(int64_t h, int enable, double* total_ms, int64_t* count) {
   c.apply_ms = 0.0;
   c.apply_timed = 0;
   c.time_apply = enable != 0;
+  return 0;
+}
+
+int mm_bench_precond(int64_t h, int use_graph, int reps, double* ms) {
+  GET_CTX(h);
+  MM_CHECK(reps > 0 && c.lv.size() >= 1 && c.lv[0].M, "bad arguments or no preconditioner");
+  const bool timing = c.time_apply;
+  c.time_apply = false;
+  vfill(c.pr, 1.0, c.N, c.stream);
+  hipEvent_t e0, e1;
+  MM_HIP(hipEventCreate(&e0));
+  MM_HIP(hipEventCreate(&e1));
+  MM_TRY(precond_apply(c, c.pr, c.pz, c.dscal + 6, c.stream));  // warm-up
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (use_graph) {
+    MM_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    vfill(c.pt, 0.0, c.N, c.stream);
+    MM_TRY(precond(c, c.pr, c.pz, c.dscal + 6, c.stream));
+    MM_HIP(hipStreamEndCapture(c.stream, &graph));
+    MM_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    MM_HIP(hipGraphLaunch(exec, c.stream));
+  }
+  MM_HIP(hipEventRecord(e0, c.stream));
+  for (int r = 0; r < reps; ++r) {
+    if (use_graph) {
+      MM_HIP(hipGraphLaunch(exec, c.stream));
+    } else {
+      vfill(c.pt, 0.0, c.N, c.stream);
+      MM_TRY(precond(c, c.pr, c.pz, c.dscal + 6, c.stream));
+    }
+  }
+  MM_HIP(hipEventRecord(e1, c.stream));
+  MM_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  MM_HIP(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.time_apply = timing;
   return 0;
 }
 

@@ -108,10 +108,10 @@ struct Level {
   double* M = nullptr;                // smoother multiplier: x += M * (b - A x)
   double* dense_inv = nullptr;        // coarsest level
   int lanesA = 8, lanesP = 8, lanesR = 8;  // SpMV lanes per row
-  double *b = nullptr, *x = nullptr, *t = nullptr;
+  double *b = nullptr, *x = nullptr, *t = nullptr, *xs = nullptr;  // xs: post-smoothed result
   // FP32 copies for the mixed-precision V-cycle
   float *Axf = nullptr, *Pxf = nullptr, *Rxf = nullptr, *Mf = nullptr, *dense_invf = nullptr;
-  float *bf = nullptr, *xf = nullptr, *tf = nullptr;
+  float *bf = nullptr, *xf = nullptr, *tf = nullptr, *xsf = nullptr;
 };
 
 struct Ctx {
@@ -179,6 +179,9 @@ int resolve_apply_timing(Ctx& c);
 // element kernels (mm_kernels.hip)
 int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s);
 int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s);
+// y = A (w .* r) with xo = w .* r written by the node owners (fused pre-smoothing); y zeroed first
+int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s);
+int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float* xo, float* y, hipStream_t s);
 int launch_rhs(const Ctx& c, const double* m, double* b, hipStream_t s);
 int launch_energy(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,
                   hipStream_t s);
@@ -196,6 +199,7 @@ int vmaxabs(const Ctx& c, const double* a, long n, double* out, hipStream_t s);
 
 // solver (mm_solver.hip)
 int precond_apply(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s);
+int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s);
 int solve_U_dev(Ctx& c, const double* m, double* U, double tol, int maxiter, int* it, double* rel_res);
 
 }  // namespace mm

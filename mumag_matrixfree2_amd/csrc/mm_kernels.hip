@@ -31,8 +31,11 @@ __device__ __forceinline__ void stored_grads(const TG* __restrict__ grad, const 
 // y = A x with A the assembled Poisson matrix: element stiffness Ve grad phi_a . grad phi_b,
 // rows and columns of Dirichlet nodes removed (x masked to 0 at the gather, the boundary
 // row written as y = x by its owner), reg added on the diagonal of interior rows.
-template <int GEOM>
+// With SCALED the input is x = w .* r (the level-0 pre-smoothing step of the V-cycle from a
+// zero guess): it is formed during the staging and each node's owner also writes it to xo.
+template <int GEOM, bool SCALED>
 __global__ __launch_bounds__(kBlock) void k_apply(Part p, Geo geo, const double* __restrict__ x,
+                                                  const double* __restrict__ w, double* __restrict__ xo,
                                                   double* __restrict__ y, double reg) {
   __shared__ double sx[kMaxBlockNodes];
   __shared__ double sy[kMaxBlockNodes];
@@ -51,7 +54,12 @@ __global__ __launch_bounds__(kBlock) void k_apply(Part p, Geo geo, const double*
   for (int i = tid; i < bi.nb; i += kBlock) {
     const uint32_t f = p.nodes[bi.n0 + i];
     const uint32_t id = f & kIdMask;
-    sx[i] = (f & kBoundary) ? 0.0 : x[id];
+    double xv = x[id];
+    if constexpr (SCALED) {
+      xv *= w[id];
+      if (f & kOwner) xo[id] = xv;
+    }
+    sx[i] = (f & kBoundary) ? 0.0 : xv;
     sy[i] = 0.0;
     if constexpr (GEOM == kGeomCoords) {
       sX[0][i] = geo.knt[3 * id + 0];
@@ -90,7 +98,10 @@ __global__ __launch_bounds__(kBlock) void k_apply(Part p, Geo geo, const double*
   for (int i = tid; i < bi.nb; i += kBlock) {
     const uint32_t f = p.nodes[bi.n0 + i];
     if (f & kBoundary) {
-      if (f & kOwner) y[f & kIdMask] = x[f & kIdMask];
+      if (f & kOwner) {
+        const uint32_t id = f & kIdMask;
+        y[id] = SCALED ? x[id] * w[id] : x[id];
+      }
       continue;
     }
     double v = sy[i];
@@ -105,8 +116,10 @@ __global__ __launch_bounds__(kBlock) void k_apply(Part p, Geo geo, const double*
 // FP32. The LDS accumulator stays FP64: on gfx950 ds_add_f32 runs at a fraction of the rate of
 // ds_add_f64 (measured: 53 us vs 16 us per apply on the 2.06M-tet cube). Used inside the
 // mixed-precision V-cycle only.
+template <bool SCALED>
 __global__ __launch_bounds__(kBlock) void k_apply_f32(Part p, Geo geo, const float* __restrict__ x,
-                                                      float* __restrict__ y, float reg) {
+                                                      const double* __restrict__ r, const float* __restrict__ w,
+                                                      float* __restrict__ xo, float* __restrict__ y, float reg) {
   __shared__ float sx[kMaxBlockNodes];
   __shared__ double sy[kMaxBlockNodes];
   __shared__ float sX[3][kMaxBlockNodes];
@@ -125,7 +138,14 @@ __global__ __launch_bounds__(kBlock) void k_apply_f32(Part p, Geo geo, const flo
   for (int i = tid; i < bi.nb; i += kBlock) {
     const uint32_t f = p.nodes[bi.n0 + i];
     const uint32_t id = f & kIdMask;
-    sx[i] = (f & kBoundary) ? 0.0f : x[id];
+    float xv;
+    if constexpr (SCALED) {
+      xv = (float)(w[id] * r[id]);
+      if (f & kOwner) xo[id] = xv;
+    } else {
+      xv = x[id];
+    }
+    sx[i] = (f & kBoundary) ? 0.0f : xv;
     sy[i] = 0.0;
     sX[0][i] = (float)(geo.knt[3 * id + 0] - ox);
     sX[1][i] = (float)(geo.knt[3 * id + 1] - oy);
@@ -158,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_f32(Part p, Geo geo, const flo
     const uint32_t f = p.nodes[bi.n0 + i];
     const uint32_t id = f & kIdMask;
     if (f & kBoundary) {
-      if (f & kOwner) y[id] = x[id];
+      if (f & kOwner) y[id] = SCALED ? (float)(w[id] * r[id]) : x[id];
       continue;
     }
     float v = (float)sy[i];
@@ -510,21 +530,35 @@ int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t
   const int grid = grid_for(p);
   switch (geom) {
     case kGeomCoords:
-      k_apply<kGeomCoords><<<grid, kBlock, 0, s>>>(p, c.geo, x, y, c.reg);
+      k_apply<kGeomCoords, false><<<grid, kBlock, 0, s>>>(p, c.geo, x, nullptr, nullptr, y, c.reg);
       break;
     case kGeomStored64:
-      k_apply<kGeomStored64><<<grid, kBlock, 0, s>>>(p, c.geo, x, y, c.reg);
+      k_apply<kGeomStored64, false><<<grid, kBlock, 0, s>>>(p, c.geo, x, nullptr, nullptr, y, c.reg);
       break;
     default:
-      k_apply<kGeomStored32><<<grid, kBlock, 0, s>>>(p, c.geo, x, y, c.reg);
+      k_apply<kGeomStored32, false><<<grid, kBlock, 0, s>>>(p, c.geo, x, nullptr, nullptr, y, c.reg);
   }
+  return 0;
+}
+
+int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s) {
+  const Part& p = c.part_all;
+  if (p.nblk == 0) return 0;
+  k_apply<kGeomCoords, true><<<grid_for(p), kBlock, 0, s>>>(p, c.geo, r, w, xo, y, c.reg);
   return 0;
 }
 
 int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s) {
   const Part& p = c.part_all;
   if (p.nblk == 0) return 0;
-  k_apply_f32<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, x, y, (float)c.reg);
+  k_apply_f32<false><<<grid_for(p), kBlock, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, (float)c.reg);
+  return 0;
+}
+
+int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float* xo, float* y, hipStream_t s) {
+  const Part& p = c.part_all;
+  if (p.nblk == 0) return 0;
+  k_apply_f32<true><<<grid_for(p), kBlock, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg);
   return 0;
 }
 

@@ -62,28 +62,35 @@ int vmaxabs(const Ctx& c, const double* a, long n, double* out, hipStream_t s) {
 
 // ----------------------------------------------------------------------------- PCG kernels
 // r = b - A x0 (A x0 in q); z = M0 r handled by the preconditioner; here: r, r.r and b.b.
+// t / tf (the preconditioner's first apply output) are cleared on the way.
 __global__ __launch_bounds__(kBlock) void k_pcg_init(const double* __restrict__ b, const double* __restrict__ q,
-                                                     double* __restrict__ r, long n, double* __restrict__ partial,
+                                                     double* __restrict__ r, double* __restrict__ t,
+                                                     float* __restrict__ tf, long n, double* __restrict__ partial,
                                                      unsigned* __restrict__ counter, double* __restrict__ out) {
   double v[2] = {0.0, 0.0};
   for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
     const double ri = b[i] - q[i];
     r[i] = ri;
+    if (t) t[i] = 0.0;
+    if (tf) tf[i] = 0.0f;
     v[0] += ri * ri;
     v[1] += b[i] * b[i];
   }
   reduce_store<2, false>(v, partial, counter, out);  // out[0] = r.r, out[1] = b.b
 }
 
-// alpha = rz / (p.q + 1e-30); x += alpha p; r -= alpha q; out = r.r
+// alpha = rz / (p.q + 1e-30); x += alpha p; r -= alpha q; out = r.r; t / tf cleared
 __global__ __launch_bounds__(kBlock) void k_pcg_xr(const double* __restrict__ sc, int rz_slot, int pq_slot,
                                                    double* __restrict__ x, double* __restrict__ r,
-                                                   const double* __restrict__ p, const double* __restrict__ q, long n,
+                                                   const double* __restrict__ p, const double* __restrict__ q,
+                                                   double* __restrict__ t, float* __restrict__ tf, long n,
                                                    double* __restrict__ partial, unsigned* __restrict__ counter,
                                                    double* __restrict__ out) {
   const double alpha = sc[rz_slot] / (sc[pq_slot] + 1e-30);
   double v[1] = {0.0};
   for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    if (t) t[i] = 0.0;
+    if (tf) tf[i] = 0.0f;
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
@@ -133,7 +140,8 @@ __global__ void k_post(double* __restrict__ x, const double* __restrict__ M, con
 }
 
 // CSR SpMV with LANES lanes per row, accumulation in the matrix value type TA.
-// MODE 0: y = A x; 1: y += A x (and z = 0 on the same rows when z != nullptr); 2: y = A (x - x2)
+// MODE 0: y = A x; 1: y += A x (and z = 0 on the same rows when z != nullptr); 2: y = A (x - x2);
+// 3: y = A (x2 .* x) and z = x2 .* x on the same rows (pre-smoothing from a zero guess, fused)
 template <int MODE, int LANES, typename TA, typename TX, typename TX2, typename TY>
 __global__ __launch_bounds__(kBlock) void k_spmv(int nrows, const int* __restrict__ Ap, const int* __restrict__ Ai,
                                                  const TA* __restrict__ Ax, const TX* __restrict__ x,
@@ -145,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrows, const int* __restric
     const int j0 = Ap[row], j1 = Ap[row + 1];
     for (int j = j0 + lane; j < j1; j += LANES) {
       const int c = Ai[j];
-      const TA xv = (MODE == 2) ? (TA)((TA)x[c] - (TA)x2[c]) : (TA)x[c];
+      const TA xv = (MODE == 2) ? (TA)((TA)x[c] - (TA)x2[c]) : (MODE == 3) ? (TA)((TA)x2[c] * (TA)x[c]) : (TA)x[c];
       s += Ax[j] * xv;
     }
   }
@@ -157,19 +165,46 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrows, const int* __restric
       if (z) z[row] = 0;
     } else {
       y[row] = (TY)s;
+      if (MODE == 3) z[row] = (TY)((TA)x2[row] * (TA)x[row]);
     }
   }
 }
 
-// y = Ainv b, dense row-major n x n: one wave per row
+// Post-smoothing step as one SpMV: xs = x + M .* (b - A x), written to a separate vector.
+template <int LANES, typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv_post(int nrows, const int* __restrict__ Ap,
+                                                      const int* __restrict__ Ai, const T* __restrict__ Ax,
+                                                      const T* __restrict__ x, const T* __restrict__ M,
+                                                      const T* __restrict__ b, T* __restrict__ xs) {
+  const int lane = threadIdx.x & (LANES - 1);
+  const int row = (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
+  T s = 0;
+  if (row < nrows) {
+    const int j0 = Ap[row], j1 = Ap[row + 1];
+    for (int j = j0 + lane; j < j1; j += LANES) s += Ax[j] * x[Ai[j]];
+  }
+#pragma unroll
+  for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LANES);
+  if (row < nrows && lane == 0) xs[row] = x[row] + M[row] * (b[row] - s);
+}
+
+// y = Ainv b, dense row-major n x n: one wave per row, 4 independent loads per lane in flight
 __global__ __launch_bounds__(kBlock) void k_gemv(int n, const double* __restrict__ Ainv, const double* __restrict__ b,
                                                  double* __restrict__ y) {
   const int lane = threadIdx.x & 63;
   const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   if (row >= n) return;
-  double s = 0.0;
-  for (int j = lane; j < n; j += 64) s += Ainv[(long)row * n + j] * b[j];
-  s = wave_red<false>(s);
+  const double* a = Ainv + (long)row * n;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int j = lane;
+  for (; j + 192 < n; j += 256) {
+    s0 += a[j] * b[j];
+    s1 += a[j + 64] * b[j + 64];
+    s2 += a[j + 128] * b[j + 128];
+    s3 += a[j + 192] * b[j + 192];
+  }
+  for (; j < n; j += 64) s0 += a[j] * b[j];
+  double s = wave_red<false>((s0 + s1) + (s2 + s3));
   if (lane == 0) y[row] = s;
 }
 
@@ -195,8 +230,24 @@ static void spmv(int mode, int lanes, int nrows, const int* Ap, const int* Ai, c
     spmv_lanes<0>(lanes, nrows, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, s);
   else if (mode == 1)
     spmv_lanes<1>(lanes, nrows, Ap, Ai, Ax, x, x2, y, z, s);
-  else
+  else if (mode == 2)
     spmv_lanes<2>(lanes, nrows, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, s);
+  else
+    spmv_lanes<3>(lanes, nrows, Ap, Ai, Ax, x, x2, y, z, s);
+}
+
+template <typename T>
+static void spmv_post(int lanes, int nrows, const int* Ap, const int* Ai, const T* Ax, const T* x, const T* M,
+                      const T* b, T* xs, hipStream_t s) {
+  if (nrows <= 0) return;
+  const int grid = (int)(((long)nrows * lanes + kBlock - 1) / kBlock);
+  switch (lanes) {
+    case 4: k_spmv_post<4, T><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, M, b, xs); break;
+    case 8: k_spmv_post<8, T><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, M, b, xs); break;
+    case 16: k_spmv_post<16, T><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, M, b, xs); break;
+    case 32: k_spmv_post<32, T><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, M, b, xs); break;
+    default: k_spmv_post<64, T><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, M, b, xs); break;
+  }
 }
 
 // ------------------------------------------------------------ FP32 V-cycle helpers
@@ -270,54 +321,66 @@ int timed_apply(Ctx& c, const double* x, double* y, hipStream_t s) {
   return 0;
 }
 
+static int timed_apply_scaled(Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s) {
+  if (!c.time_apply) return launch_apply_scaled(c, r, w, xo, y, s);
+  if (c.ev_pool.empty()) {
+    c.ev_pool.resize(512);
+    for (auto& e : c.ev_pool) MM_HIP(hipEventCreate(&e));
+  }
+  if (2 * c.ev_used >= (int)c.ev_pool.size()) MM_TRY(resolve_apply_timing(c));
+  MM_HIP(hipEventRecord(c.ev_pool[2 * c.ev_used], s));
+  MM_TRY(launch_apply_scaled(c, r, w, xo, y, s));
+  MM_HIP(hipEventRecord(c.ev_pool[2 * c.ev_used + 1], s));
+  ++c.ev_used;
+  return 0;
+}
+
 // ----------------------------------------------------------------------------- V-cycle
-// Coarse part of the V-cycle on levels l >= 1: returns the correction in lv[l].x.
+// Coarse part of the V-cycle on levels l >= 1; the correction is returned in lv[l].xs.
+// Pre-smoothing from a zero guess is fused into the first SpMV (t = A (M b), x = M b) and the
+// post-smoothing x + M (b - A x) is one SpMV into xs.
 static void vcycle_coarse(Ctx& c, int l, hipStream_t s) {
   Level& L = c.lv[l];
   const int last = (int)c.lv.size() - 1;
   if (l == last) {
-    k_gemv<<<(int)(((long)L.n * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>(L.n, L.dense_inv, L.b, L.x);
+    k_gemv<<<(int)(((long)L.n * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>(L.n, L.dense_inv, L.b, L.xs);
     return;
   }
   Level& Nx = c.lv[l + 1];
-  k_scale<<<vgrid(L.n), kBlock, 0, s>>>(L.M, L.b, L.x, nullptr, L.n);            // pre-smooth from x = 0
-  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, (const double*)nullptr, L.t, s);                 // t = A x
-  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rx, L.b, L.t, Nx.b, s);                   // b' = R (b - t)
+  spmv(3, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.b, (const double*)L.M, L.t, s, L.x);   // t = A (M b), x = M b
+  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rx, L.b, L.t, Nx.b, s);                    // b' = R (b - t)
   vcycle_coarse(c, l + 1, s);
-  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Px, Nx.x, (const double*)nullptr, L.x, s);                // x += P e
-  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, (const double*)nullptr, L.t, s);                 // t = A x
-  k_post<<<vgrid(L.n), kBlock, 0, s>>>(L.x, L.M, L.b, L.t, L.n);                  // x += M (b - t)
+  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Px, Nx.xs, (const double*)nullptr, L.x, s); // x += P e
+  spmv_post(L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, L.M, L.b, L.xs, s);              // xs = x + M (b - A x)
 }
 
-// FP32 coarse part of the V-cycle (levels >= 1).
+// FP32 coarse part of the V-cycle (levels >= 1), same structure as vcycle_coarse.
 static void vcycle_coarse_f(Ctx& c, int l, hipStream_t s) {
   Level& L = c.lv[l];
   const int last = (int)c.lv.size() - 1;
   if (l == last) {
-    k_gemv_f<<<(int)(((long)L.n * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>(L.n, L.dense_invf, L.bf, L.xf);
+    k_gemv_f<<<(int)(((long)L.n * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>(L.n, L.dense_invf, L.bf, L.xsf);
     return;
   }
   Level& Nx = c.lv[l + 1];
-  k_scale_f<float><<<vgrid(L.n), kBlock, 0, s>>>(L.Mf, L.bf, L.xf, (float*)nullptr, L.n);
-  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.xf, (const float*)nullptr, L.tf, s);
+  spmv(3, L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.bf, (const float*)L.Mf, L.tf, s, L.xf);
   spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rxf, L.bf, L.tf, Nx.bf, s);
   vcycle_coarse_f(c, l + 1, s);
-  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Pxf, Nx.xf, (const float*)nullptr, L.xf, s);
-  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.xf, (const float*)nullptr, L.tf, s);
-  k_post_f<<<vgrid(L.n), kBlock, 0, s>>>(L.xf, L.Mf, L.bf, L.tf, L.n);
+  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Pxf, Nx.xsf, (const float*)nullptr, L.xf, s);
+  spmv_post(L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.xf, L.Mf, L.bf, L.xsf, s);
 }
 
 // Mixed-precision preconditioner: the whole V-cycle in FP32 (FP32 matrix-free fine applies,
 // FP32 coarse matrices and dense inverse); input r and output z in FP64, r.z in FP64.
+// The caller has cleared ptf.
 static int precond_f32(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
   const long N = c.N;
   Level& L0 = c.lv[0];
   Level& L1 = c.lv[1];
-  k_scale_f<double><<<vgrid(N), kBlock, 0, s>>>(L0.Mf, r, c.px0f, c.ptf, N);      // x = M0 r, t = 0
-  MM_TRY(launch_apply_f32(c, c.px0f, c.ptf, s));                                  // t = A x
+  MM_TRY(launch_apply_f32_scaled(c, r, L0.Mf, c.px0f, c.ptf, s));                 // x = M0 r, t = A x
   spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rxf, r, c.ptf, L1.bf, s);             // b1 = R0 (r - t)
   vcycle_coarse_f(c, 1, s);
-  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Pxf, L1.xf, (const float*)nullptr, c.px0f, s, c.ptf);
+  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Pxf, L1.xsf, (const float*)nullptr, c.px0f, s, c.ptf);
   MM_TRY(launch_apply_f32(c, c.px0f, c.ptf, s));                                  // t = A x
   k_post_dot_mixed<<<red_grid(N), kBlock, 0, s>>>(c.px0f, L0.Mf, r, c.ptf, z, N, c.partial, c.counter + 4, rz_out);
   c.n_vcycle += 1;
@@ -325,7 +388,7 @@ static int precond_f32(Ctx& c, const double* r, double* z, double* rz_out, hipSt
 }
 
 // z = V-cycle(r) with the matrix-free level 0; also writes r.z into out.
-static int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
+int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
   const long N = c.N;
   Level& L0 = c.lv[0];
   if (c.precision == 1 && c.lv.size() > 1 && L0.Mf != nullptr) return precond_f32(c, r, z, rz_out, s);
@@ -338,11 +401,11 @@ static int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream
     return 0;
   }
   Level& L1 = c.lv[1];
-  k_scale<<<vgrid(N), kBlock, 0, s>>>(L0.M, r, c.px0, c.pt, N);                  // x = M0 r, t = 0
-  MM_TRY(timed_apply(c, c.px0, c.pt, s));                                         // t = A x
+  // the caller has cleared pt
+  MM_TRY(timed_apply_scaled(c, r, L0.M, c.px0, c.pt, s));                          // x = M0 r, t = A x
   spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rx, r, c.pt, L1.b, s);                // b1 = R0 (r - t)
   vcycle_coarse(c, 1, s);
-  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Px, L1.x, (const double*)nullptr, c.px0, s, c.pt); // x += P0 e1, t = 0
+  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Px, L1.xs, (const double*)nullptr, c.px0, s, c.pt); // x += P0 e1, t = 0
   MM_TRY(timed_apply(c, c.px0, c.pt, s));                                         // t = A x
   k_post_dot<<<red_grid(N), kBlock, 0, s>>>(c.px0, L0.M, r, c.pt, z, N, c.partial, c.counter + 4, rz_out);
   c.n_apply += 2;
@@ -352,6 +415,8 @@ static int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream
 
 int precond_apply(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
   MM_CHECK(c.lv.size() >= 1 && c.lv[0].M != nullptr, "preconditioner not set up");
+  vfill(c.pt, 0.0, c.N, s);
+  if (c.ptf) MM_HIP(hipMemsetAsync(c.ptf, 0, c.N * sizeof(float), s));
   return precond(c, r, z, rz_out, s);
 }
 
@@ -367,7 +432,9 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
   vfill(c.pq, 0.0, N, s);
   MM_TRY(timed_apply(c, U, c.pq, s));
   c.n_apply++;
-  k_pcg_init<<<red_grid(N), kBlock, 0, s>>>(b, c.pq, c.pr, N, c.partial, c.counter + 5, sc + 3);
+  float* tf = c.precision == 1 ? c.ptf : nullptr;
+  double* t64 = c.precision == 1 ? nullptr : c.pt;
+  k_pcg_init<<<red_grid(N), kBlock, 0, s>>>(b, c.pq, c.pr, t64, tf, N, c.partial, c.counter + 5, sc + 3);
   MM_TRY(precond(c, c.pr, c.pz, sc + 0, s));
   k_pcg_p<<<vgrid(N), kBlock, 0, s>>>(sc, 0, 0, 1, c.pz, c.pp, c.pq, N);
   MM_HIP(hipMemcpyAsync(c.hscal + 3, sc + 3, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -381,7 +448,8 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
     MM_TRY(timed_apply(c, c.pp, c.pq, s));  // q was cleared by k_pcg_p
     c.n_apply++;
     vdot(c, c.pp, c.pq, N, sc + 2, s);
-    k_pcg_xr<<<red_grid(N), kBlock, 0, s>>>(sc, cur, 2, U, c.pr, c.pp, c.pq, N, c.partial, c.counter + 5, sc + 3);
+    k_pcg_xr<<<red_grid(N), kBlock, 0, s>>>(sc, cur, 2, U, c.pr, c.pp, c.pq, t64, tf, N, c.partial, c.counter + 5,
+                                             sc + 3);
     MM_HIP(hipMemcpyAsync(c.hscal + 3, sc + 3, sizeof(double), hipMemcpyDeviceToHost, s));
     MM_HIP(hipEventRecord(c.ev, s));
     // preconditioner and direction update run while the host waits for r.r

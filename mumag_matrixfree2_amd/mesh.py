@@ -186,3 +186,50 @@ def config_cube_mesh(L: float, n: int, layers: int, grading: float = 2.0, morton
         r = morton_reorder(mesh["knt"], mesh["ijk"], (mesh["boundary_mask"],))
         mesh = {"knt": r["knt"], "ijk": r["ijk"], "boundary_mask": r["extra"][0]}
     return mesh
+
+
+def polycrystal_cube(L: float = 200.0, n: int = 150, layers: int = 5, n_grains: int = 27, gb_width: float = 2.0,
+                     seed: int = 4, grading: float = 2.0, morton: bool = True) -> dict:
+    """Cube polycrystal with Voronoi grains and a grain-boundary phase, inside a graded airbox.
+
+    The role of the reference's neper-based polycrystal meshers (``src/mesh.py:1868-2044``),
+    on the structured grid: ``n_grains`` centres are drawn uniformly in the cube with
+    ``numpy.random.default_rng(seed)``; each magnet tet belongs to the grain of the centre nearest
+    to its centroid; tets whose centroid lies within ``gb_width`` of the bisector plane between its
+    nearest and second-nearest centre form the grain-boundary phase.
+
+    Material ids: 1..n_grains grains, n_grains + 1 grain boundary, n_grains + 2 air. Returns a
+    dict with ``knt``, ``ijk``, ``boundary_mask`` and ``centres``.
+    """
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-L / 2, L / 2, size=(n_grains, 3))
+    xs = graded_axis(L, n, layers, grading)
+    knt, tets = tensor_grid_tets(xs, xs, xs)
+    E = tets.shape[0]
+    mat = np.full(E, n_grains + 2, dtype=np.int32)
+    chunk = 2_000_000
+    for s in range(0, E, chunk):
+        t = tets[s:s + chunk].astype(np.int64)
+        cen = knt[t].mean(axis=1)
+        inside = np.all(np.abs(cen) < L / 2, axis=1)
+        d2 = ((cen[:, None, :] - centres[None, :, :]) ** 2).sum(axis=2)
+        order = np.argsort(d2, axis=1)[:, :2]
+        i1, i2 = order[:, 0], order[:, 1]
+        c1, c2 = centres[i1], centres[i2]
+        # distance from the centroid to the bisector plane of (c1, c2)
+        dist = (d2[np.arange(len(t)), i2] - d2[np.arange(len(t)), i1]) / (2.0 * np.linalg.norm(c2 - c1, axis=1))
+        m = np.where(dist < gb_width, n_grains + 1, i1 + 1).astype(np.int32)
+        mat[s:s + chunk] = np.where(inside, m, n_grains + 2)
+    ijk = np.hstack([tets, mat[:, None]])
+    bnd = outer_boundary_mask(tets, knt.shape[0])
+    out = {"knt": knt, "ijk": ijk, "boundary_mask": bnd, "centres": centres}
+    if morton:
+        r = morton_reorder(knt, ijk, (bnd,))
+        out.update(knt=r["knt"], ijk=r["ijk"], boundary_mask=r["extra"][0])
+    return out
+
+
+def random_easy_axes(n: int, seed: int) -> np.ndarray:
+    """Easy axes uniform on the unit sphere (normalised Gaussian vectors)."""
+    v = np.random.default_rng(seed).standard_normal((n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
