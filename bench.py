@@ -1,12 +1,24 @@
-"""Hysteresis benchmark: field-steps/s of the FP64 GPU path on the 2.06M-tet cube workload.
+"""Hysteresis benchmark: field-steps/s of the GPU path on the 2.06M-tet cube or the 24.6M-tet polycrystal.
 
-Workload (the plain FP64 GPU configuration): 80 nm NdFeB-like cube, 60^3 hex cells x 6 tets
-(1,296,000 magnet tets, 226,981 magnet nodes) inside a 5-layer graded airbox (2,058,000 tets,
-357,911 nodes), Morton-ordered; Js = 1.61 T, A = 7.7 pJ/m, K1 = 4.3 MJ/m^3, easy axis z;
-m0 = z + N(0, 0.01) (seed 0), renormalised; field +2 T -> -8 T in 41 steps along
-(sin 1 deg, 0, cos 1 deg); pcohen_hs, tau_f = 1e-10, PCG tol 1e-10 (line-search solves at
-min(cg_tol, 0.1 tau_f)); smoothed-aggregation AMG V-cycle with one pre- and one
-post-smoothing sweep.
+Default workload (``--workload cube2M``, the plain FP64 GPU configuration): 80 nm NdFeB-like
+cube, 60^3 hex cells x 6 tets (1,296,000 magnet tets, 226,981 magnet nodes) inside a 5-layer
+graded airbox (2,058,000 tets, 357,911 nodes), Morton-ordered; Js = 1.61 T, A = 7.7 pJ/m,
+K1 = 4.3 MJ/m^3, easy axis z; m0 = z + N(0, 0.01) (seed 0), renormalised; field +2 T -> -8 T
+in 41 steps along (sin 1 deg, 0, cos 1 deg); pcohen_hs, tau_f = 1e-10, PCG tol 1e-10
+(line-search solves at min(cg_tol, 0.1 tau_f)); smoothed-aggregation AMG V-cycle with one pre-
+and one post-smoothing sweep.
+
+``--workload grains64``: 64 independent grains (src of the set: ``mesh.grain_set``, default_rng(3)):
+total tets per grain log-uniform in [2e4, 4e5] (7,752,942 tets, 1,431,475 nodes in all), cube or
+sphere by coin flip, easy axis uniform on the sphere, each in its own 3-layer graded airbox with
+2 nm cells; 41 field values +2 T -> -8 T; one field step minimises every grain (host threads
+drive the grains' contexts concurrently, ``--threads``).
+
+``--workload poly25M``: 200 nm cube of 27 Voronoi grains (centres from default_rng(4)), 150^3
+hex cells x 6 tets (20.25M magnet tets) in a 5-layer graded airbox (24,576,000 tets, 4,173,281
+nodes); tets within 2 nm of a grain-boundary bisector plane form a grain-boundary phase with
+A = 1.5 pJ/m, K1 = 0; easy axes per grain uniform on the sphere (default_rng(40)); 81 field
+values +2 T -> -8 T (dB = -0.125 T); everything else as above.
 
 A step is one field step: a full energy minimisation at one field value, with the
 magnetisation and potential resident on the GPU. ``--warmup W`` field steps run first on
@@ -34,6 +46,10 @@ H_DIR = np.array([np.sin(np.deg2rad(1.0)), 0.0, np.cos(np.deg2rad(1.0))])
 HBM_PEAK_GBS = 8000.0
 
 
+A_GB_SI = 1.5e-12
+N_GRAINS = 27
+
+
 def materials():
     Kd = JS**2 / (2 * MU0)
     A = np.array([A_SI * 1e18 / Kd, 0.0])
@@ -41,6 +57,38 @@ def materials():
     Js = np.array([1.0, 0.0])
     k = np.array([[0.0, 0.0, 1.0], [0.0, 0.0, 1.0]])
     return A, K1, Js, k
+
+
+def poly_materials(n_grains=N_GRAINS):
+    """Material ids 1..n_grains grains, n_grains+1 grain boundary, n_grains+2 air (table index id-1)."""
+    from mumag_matrixfree2_amd.mesh import random_easy_axes
+
+    Kd = JS**2 / (2 * MU0)
+    G = n_grains + 2
+    A = np.zeros(G)
+    K1 = np.zeros(G)
+    Js = np.zeros(G)
+    k = np.zeros((G, 3))
+    k[:, 2] = 1.0
+    A[:n_grains], K1[:n_grains], Js[:n_grains] = A_SI * 1e18 / Kd, K1_SI / Kd, 1.0
+    k[:n_grains] = random_easy_axes(n_grains, 40)
+    A[n_grains], Js[n_grains] = A_GB_SI * 1e18 / Kd, 1.0
+    return A, K1, Js, k
+
+
+WORKLOADS = {
+    "cube2M": {"name": "cube80nm_60^3x6tets+5layer_airbox_2.06Mtet", "steps": 41, "dB": -0.25},
+    "grains64": {"name": "64grains_2e4-4e5tets_cube|sphere+3layer_airbox_7.75Mtet", "steps": 41, "dB": -0.25},
+    "poly25M": {"name": "poly200nm_27grains_150^3x6tets+5layer_airbox_24.6Mtet", "steps": 81, "dB": -0.125},
+}
+
+
+def make_workload(name):
+    from mumag_matrixfree2_amd.mesh import config_cube_mesh, polycrystal_cube
+
+    if name == "cube2M":
+        return config_cube_mesh(80.0, 60, 5), materials()
+    return polycrystal_cube(200.0, 150, 5, n_grains=N_GRAINS, gb_width=2.0, seed=4), poly_materials()
 
 
 def initial_m(N, seed=0, sigma=0.01):
@@ -114,16 +162,114 @@ def cpu_baseline(mesh, mats, steps_T, budget_s=60.0):
     return done, dt
 
 
+def run_grains(args, log):
+    """The 64-grain set: one context (mesh, AMG hierarchy, resident state) per grain; a field
+    step is ``minimize_batched`` over all grains."""
+    from mumag_matrixfree2_amd.context import MeshContext, make_params, minimize_batched
+    from mumag_matrixfree2_amd.hysteresis_loop import field_values, switching_step
+    from mumag_matrixfree2_amd.mesh import grain_set
+
+    wl = WORKLOADS["grains64"]
+    t0 = time.perf_counter()
+    grains = grain_set(64, seed=3)
+    A, K1, Js, _ = materials()
+    ctxs = []
+    for i, g in enumerate(grains):
+        k = np.array([g["easy_axis"], g["easy_axis"]])
+        c = MeshContext(g["knt"], g["ijk"][:, :4], g["ijk"][:, 4], A, K1, Js, k, boundary_mask=g["boundary_mask"])
+        c.setup_precond(args.precond, max_coarse=args.max_coarse)
+        c.set_precision(args.precision)
+        ctxs.append(c)
+        if i % 8 == 7:
+            log(f"[bench] {i + 1} grains set up, {time.perf_counter() - t0:.1f}s")
+    E_tot = sum(c.E for c in ctxs)
+    N_tot = sum(c.N for c in ctxs)
+    log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: {len(ctxs)} grains, N={N_tot} E={E_tot}")
+    p = make_params("pcohen_hs", tau_f=1e-10, eps_a=1e-12, cg_tol=1e-10, max_iter=2000)
+    B_vals = field_values(2.0 / JS, -8.0 / JS, wl["dB"] / JS, False)
+    steps = min(args.steps or wl["steps"], len(B_vals))
+    m0 = [initial_m(c.N) for c in ctxs]
+    vol = np.array([c.V_mag for c in ctxs])
+
+    def sweep(n_threads, timed):
+        for c, m in zip(ctxs, m0):
+            c.state_set(m, np.zeros(c.N))
+            if timed:
+                c.apply_timing(True)
+        J, its = [], 0
+        t_start = time.perf_counter()
+        for k in range(steps):
+            ts = time.perf_counter()
+            infos = minimize_batched(ctxs, p, B_vals[k] * H_DIR, n_threads=n_threads)
+            Jg = np.array([c.state_j_parallel(H_DIR) for c in ctxs])
+            J.append(float(vol @ Jg / vol.sum()))
+            its += sum(i["iters"] for i in infos)
+            log(f"[bench:grains t={n_threads}] step {k:2d} B={B_vals[k] * JS:+.3f} T J={J[-1] * JS:+.5f} T "
+                f"iters={sum(i['iters'] for i in infos)} max={max(i['iters'] for i in infos)} "
+                f"t={time.perf_counter() - ts:.3f}s")
+        return time.perf_counter() - t_start, np.array(J), its
+
+    for c, m in zip(ctxs, m0):  # warm-up on a separate state
+        c.state_set(m, np.zeros(c.N))
+    for k in range(args.warmup):
+        minimize_batched(ctxs, p, B_vals[k % len(B_vals)] * H_DIR, n_threads=args.threads)
+    elapsed, J, its = sweep(args.threads, timed=True)
+    timing = [c.apply_timing(False) for c in ctxs]
+    seq = None
+    if args.compare_sequential:
+        t1, J1, _ = sweep(1, timed=False)
+        seq = {"sequential_field_steps_per_s": steps / t1, "speedup_vs_sequential": t1 / elapsed,
+               "max_abs_J_diff": float(np.max(np.abs(J - J1)))}
+    algo = sum(apply_algorithmic_bytes(c) * n for c, (_, n) in zip(ctxs, timing))
+    t_apply = sum(ms for ms, _ in timing) * 1e-3
+    n_apply = sum(n for _, n in timing)
+    achieved = algo / max(t_apply, 1e-12) / 1e9
+    result = {
+        "metric": f"hysteresis field-steps/s ({args.precision}, {wl['name']})",
+        "value": steps / elapsed,
+        "unit": "field-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
+        "data": "synthetic (generated grain set, seeded initial magnetisation)",
+        "config": {"workload": f"{wl['name']}_{args.precision}_sweep+2..-8T_{len(B_vals)}steps", "grains": len(ctxs),
+                   "tets": E_tot, "nodes": N_tot, "method": "pcohen_hs", "threads": args.threads,
+                   "precond": f"SA-AMG V-cycle ({args.precond})", "pcg_tol": 1e-10, "tau_f": 1e-10},
+        "minimizer_iters": its,
+        "switching_step": switching_step(J),
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_apply<on_the_fly> over all grains (event time includes overlap with other grains)",
+            "avg_launch_us": t_apply / max(n_apply, 1) * 1e6, "launches": n_apply,
+        },
+    }
+    if seq is not None:
+        result["batched_vs_sequential"] = seq
+    print(json.dumps(result), flush=True)
+    for c in ctxs:
+        c.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=41)
+    ap.add_argument("--workload", default="cube2M", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precond", default="amgcl", choices=["amgcl", "amg"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--max-coarse", type=int, default=None)
+    ap.add_argument("--threads", type=int, default=8, help="host threads driving the grains (grains64)")
+    ap.add_argument("--compare-sequential", action="store_true",
+                    help="grains64: also time the same steps with one thread")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the hysteresis path runs on a single MI355X here "
@@ -133,23 +279,26 @@ def main():
 
     from mumag_matrixfree2_amd.context import MeshContext, make_params
     from mumag_matrixfree2_amd.hysteresis_loop import field_values, switching_step
-    from mumag_matrixfree2_amd.mesh import config_cube_mesh
 
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    if args.workload == "grains64":
+        return run_grains(args, log)
+    wl = WORKLOADS[args.workload]
     t0 = time.perf_counter()
-    mesh = config_cube_mesh(80.0, 60, 5)
-    mats = materials()
+    mesh, mats = make_workload(args.workload)
+    log(f"[bench] mesh {time.perf_counter() - t0:.1f}s")
     ctx = MeshContext(mesh["knt"], mesh["ijk"][:, :4], mesh["ijk"][:, 4], *mats,
                       boundary_mask=mesh["boundary_mask"])
-    ctx.setup_precond(args.precond, max_coarse=args.max_coarse)
+    log(f"[bench] context {time.perf_counter() - t0:.1f}s")
+    ctx.setup_precond(args.precond, max_coarse=args.max_coarse, log=log)
     ctx.set_precision(args.precision)
     log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: N={ctx.N} E={ctx.E} "
         f"AMG levels={[l['A'].shape[0] for l in ctx.amg_levels]}")
 
     p = make_params("pcohen_hs", tau_f=1e-10, eps_a=1e-12, cg_tol=1e-10, max_iter=2000)
-    B_vals = field_values(2.0 / JS, -8.0 / JS, -0.25 / JS, False)  # 41 values
+    B_vals = field_values(2.0 / JS, -8.0 / JS, wl["dB"] / JS, False)
     m0 = initial_m(ctx.N)
-    steps = min(args.steps, len(B_vals))
+    steps = min(args.steps or wl["steps"], len(B_vals))
 
     def sweep(precision, timed):
         ctx.set_precision(precision)
@@ -196,7 +345,7 @@ def main():
     achieved = algo / avg_apply_s / 1e9
     label = "FP64" if args.precision == "fp64" else "mixed FP32 V-cycle / FP64 PCG"
     result = {
-        "metric": f"hysteresis field-steps/s ({label}, 2.06M-tet cube + airbox)",
+        "metric": f"hysteresis field-steps/s ({label}, {wl['name']})",
         "value": steps / elapsed,
         "unit": "field-steps/s",
         "n_gpus": 1,
@@ -209,7 +358,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f64+f32",
         "data": "synthetic (generated mesh, seeded initial magnetisation)",
         "config": {
-            "workload": f"cube80nm_60^3x6tets+5layer_airbox_2.06Mtet_{args.precision}_sweep+2..-8T_41steps",
+            "workload": f"{wl['name']}_{args.precision}_sweep+2..-8T_{len(B_vals)}steps",
             "tets": ctx.E, "nodes": ctx.N, "method": "pcohen_hs", "precond": f"SA-AMG V-cycle ({args.precond})",
             "amg_levels": [int(l["A"].shape[0]) for l in ctx.amg_levels],
             "pcg_tol": 1e-10, "tau_f": 1e-10,
@@ -233,7 +382,7 @@ def main():
 
     if errors is not None:
         result["mixed_vs_fp64"] = errors
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == "cube2M":
         cpu_T = [2.0 - 0.25 * k for k in range(args.cpu_steps)]
         done, dt = cpu_baseline(mesh, mats, cpu_T)
         gpu_same = sum(rows[:done])

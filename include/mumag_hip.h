@@ -134,6 +134,20 @@ int mm_minimize(int64_t ctx, const mm_params* p, double* m_inout, double* U_inou
 /* J_parallel of the resident state. */
 int mm_state_j_parallel(int64_t ctx, const double* h, double* J);
 
+/* Batched field step over n independent contexts (the grains of a grain set, each its own mesh
+ * and airbox): context i minimises its resident state at field B[3i..3i+2] and fills out[i].
+ * Replaces a Python loop of minimize(m0, B_ext) calls over systems (src/minimizers.py:4053-4096);
+ * n_threads host threads (<= 0: min(n, 8)) each drive whole contexts on the contexts' own streams,
+ * so the small systems' kernels overlap on the GPU. Results are those of n sequential
+ * mm_minimize_resident calls. The contexts must be distinct. */
+int mm_minimize_batched(int n, const int64_t* ctxs, const mm_params* p, const double* B, int n_threads,
+                        mm_info* out);
+/* Batched solve_U over n independent contexts (src/poisson_solve.py:724-771 per system):
+ * m[i] (N_i,3) and U_inout[i] (N_i,) host arrays, per-system tolerance tol[i]; it[i] and
+ * rel_res[i] as in mm_solve_U. */
+int mm_solve_U_batched(int n, const int64_t* ctxs, const double* const* m, double* const* U_inout, const double* tol,
+                       int maxiter, int n_threads, int* it, double* rel_res);
+
 /* Host-side aggregation for the smoothed-aggregation AMG set-up (the role of
  * pyamg.smoothed_aggregation_solver's aggregation step, src/poisson_solve.py:596-598):
  * S is the strength graph in CSR (n rows); agg[i] receives the aggregate of node i or -1 for

@@ -88,6 +88,9 @@ SIGNATURES = {
     "mm_minimize_resident": [C.c_int64, C.POINTER(MMParams), f64p, C.POINTER(MMInfo)],
     "mm_minimize": [C.c_int64, C.POINTER(MMParams), f64p, f64p, f64p, C.POINTER(MMInfo)],
     "mm_state_j_parallel": [C.c_int64, f64p, f64p],
+    "mm_minimize_batched": [C.c_int, C.POINTER(C.c_int64), C.POINTER(MMParams), f64p, C.c_int, C.POINTER(MMInfo)],
+    "mm_solve_U_batched": [C.c_int, C.POINTER(C.c_int64), C.POINTER(f64p), C.POINTER(f64p), f64p, C.c_int, C.c_int,
+                           i32p, f64p],
     "mm_sa_aggregate": [C.c_int, i32p, i32p, i32p, i32p],
     "mm_apply_timing": [C.c_int64, C.c_int, f64p, i64p],
     "mm_counters": [C.c_int64, i64p],

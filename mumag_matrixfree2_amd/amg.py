@@ -94,7 +94,7 @@ def spectral_radius_DinvA(A: sp.csr_matrix) -> float:
 
 
 def build_hierarchy(A: sp.csr_matrix, smoother: str = "amgcl", max_levels: int = 10, max_coarse: int | None = None,
-                    theta=(0.1, 0.05), omega: float = 4.0 / 3.0) -> list[dict]:
+                    theta=(0.1, 0.05), omega: float = 4.0 / 3.0, log=None) -> list[dict]:
     """List of level dicts: ``A`` (CSR), ``M`` (smoother multiplier), ``P``, ``R`` (all but last),
     ``A_dense_inv`` (last). ``theta`` is one strength threshold or one per level (the last
     entry repeats). ``max_coarse`` defaults to N/80 clamped to [1000, 5000]: on the GPU a
@@ -124,6 +124,8 @@ def build_hierarchy(A: sp.csr_matrix, smoother: str = "amgcl", max_levels: int =
         levels.append({"A": A, "P": P, "R": R})
         A = (R @ A @ P).tocsr()
         A.sort_indices()
+        if log is not None:
+            log(f"[amg] level {len(levels)}: {A.shape[0]} rows, {A.nnz} nonzeros")
     levels.append({"A": A})
     for lv in levels:
         Al = lv["A"]

@@ -153,8 +153,9 @@ class MeshContext:
         return float(J[0])
 
     # ------------------------------------------------------------------ preconditioner
-    def setup_precond(self, precond_type="amgcl", **amg_kwargs):
-        """'jacobi', 'amgcl' (SPAI0 V-cycle) or 'amg' (damped-Jacobi V-cycle) on a host SA hierarchy."""
+    def setup_precond(self, precond_type="amgcl", log=None, **amg_kwargs):
+        """'jacobi', 'amgcl' (SPAI0 V-cycle) or 'amg' (damped-Jacobi V-cycle) on a host SA hierarchy.
+        ``log`` (a print-like callable) receives one line per set-up stage."""
         if precond_type in ("jacobi", "none"):
             call("mm_precond_jacobi", self.handle)
             self.precond_type = "jacobi"
@@ -162,7 +163,9 @@ class MeshContext:
         from .amg import build_hierarchy, fine_matrix, upload_hierarchy
 
         A = fine_matrix(self)
-        levels = build_hierarchy(A, smoother=precond_type, **amg_kwargs)
+        if log is not None:
+            log(f"[amg] fine matrix: {A.shape[0]} rows, {A.nnz} nonzeros")
+        levels = build_hierarchy(A, smoother=precond_type, log=log, **amg_kwargs)
         upload_hierarchy(self, levels, precond_type)
         self.precond_type = precond_type
         self.amg_levels = levels
@@ -206,16 +209,7 @@ class MeshContext:
         B = f64(B).reshape(3)
         info = _hip.MMInfo()
         call("mm_minimize_resident", self.handle, C.byref(params), ptr(B), C.byref(info))
-        return {
-            "iters": info.iters,
-            "evals": info.evals,
-            "demag_iters": info.demag_iters,
-            "preco_iters": info.preco_iters,
-            "converged": bool(info.converged),
-            "E": info.E,
-            "gnorm": info.gnorm,
-            "time": info.time,
-        }
+        return _info_dict(info)
 
     def state_j_parallel(self, h):
         h = f64(h).reshape(3)
@@ -239,6 +233,48 @@ class MeshContext:
         ms = C.c_double(0.0)
         call("mm_bench_apply", self.handle, GEOM_MODES[geom], int(reps), C.byref(ms))
         return float(ms.value)
+
+
+def _info_dict(info) -> dict:
+    return {
+        "iters": info.iters,
+        "evals": info.evals,
+        "demag_iters": info.demag_iters,
+        "preco_iters": info.preco_iters,
+        "converged": bool(info.converged),
+        "E": info.E,
+        "gnorm": info.gnorm,
+        "time": info.time,
+    }
+
+
+def minimize_batched(ctxs, params: "_hip.MMParams", B, n_threads: int = 0) -> list[dict]:
+    """One field step on each of several independent contexts (their resident states), driven
+    by ``n_threads`` host threads (0: up to 8); ``B`` is one field (3,) for all or one per
+    context (n, 3). Same results as calling ``minimize_resident`` on each context in turn."""
+    n = len(ctxs)
+    B = f64(np.broadcast_to(np.asarray(B, dtype=np.float64), (n, 3)))
+    handles = (C.c_int64 * n)(*[c.handle for c in ctxs])
+    infos = (_hip.MMInfo * n)()
+    call("mm_minimize_batched", n, handles, C.byref(params), ptr(B), int(n_threads), infos)
+    return [_info_dict(i) for i in infos]
+
+
+def solve_U_batched(ctxs, ms, U0s, tol, maxiter: int = 2000, n_threads: int = 0):
+    """solve_U on several independent contexts at once (per-system tolerance ``tol``, a scalar
+    or one per context). Returns lists (U, iterations, relative residual)."""
+    n = len(ctxs)
+    ms = [f64(m).reshape(-1) for m in ms]
+    Us = [f64(U).copy() for U in U0s]
+    tol = f64(np.broadcast_to(np.asarray(tol, dtype=np.float64), (n,)))
+    f64p = C.POINTER(C.c_double)
+    mp = (f64p * n)(*[m.ctypes.data_as(f64p) for m in ms])
+    Up = (f64p * n)(*[U.ctypes.data_as(f64p) for U in Us])
+    it = np.zeros(n, dtype=np.int32)
+    rr = np.zeros(n)
+    call("mm_solve_U_batched", n, (C.c_int64 * n)(*[c.handle for c in ctxs]), mp, Up, ptr(tol), int(maxiter),
+         int(n_threads), ptr(it), ptr(rr))
+    return Us, it.tolist(), rr.tolist()
 
 
 def make_params(method="pcohen_hs", *, max_iter=2000, tau_f=1e-8, eps_a=1e-12, tau0=1e-2, ls_eta1=0.1,

@@ -7,6 +7,8 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <atomic>
+#include <thread>
 
 #include "mm_device.h"
 #include "mm_solver.h"
@@ -793,6 +795,73 @@ int mm_minimize(int64_t h, const mm_params* p, double* m_inout, double* U_inout,
   MM_TRY(mm_state_set(h, m_inout, U_inout));
   MM_TRY(mm_minimize_resident(h, p, B, info));
   return mm_state_get(h, m_inout, U_inout);
+}
+
+}  // extern "C"
+
+// Runs job(i) for i in [0, n) on up to n_threads host threads (each takes the next index);
+// the first failing index's message becomes the caller's error.
+template <typename Job>
+static int run_batched(int n, const int64_t* ctxs, int n_threads, Job job) {
+  MM_CHECK(n >= 0 && (n == 0 || ctxs), "null context list");
+  for (int i = 0; i < n; ++i) {
+    MM_CHECK(get_ctx(ctxs[i]) != nullptr, "invalid context handle in batch");
+    for (int j = 0; j < i; ++j) MM_CHECK(ctxs[j] != ctxs[i], "a context appears twice in one batch");
+  }
+  const int T = std::max(1, std::min(n, n_threads > 0 ? n_threads : 8));
+  std::atomic<int> next{0};
+  std::mutex emu;
+  int first_bad = n, first_rc = 0;
+  std::string first_msg;
+  auto worker = [&]() {
+    for (int i = next++; i < n; i = next++) {
+      Ctx& c = *get_ctx(ctxs[i]);
+      int rc = 0;
+      if (hipSetDevice(c.device) != hipSuccess) {
+        set_error("hipSetDevice failed");
+        rc = -1;
+      } else {
+        rc = job(i);
+      }
+      if (rc != 0) {
+        std::lock_guard<std::mutex> lk(emu);
+        if (i < first_bad) {
+          first_bad = i;
+          first_rc = rc;
+          first_msg = g_err;
+        }
+      }
+    }
+  };
+  if (T == 1) {
+    worker();
+  } else {
+    std::vector<std::thread> pool;
+    pool.reserve(T);
+    for (int t = 0; t < T; ++t) pool.emplace_back(worker);
+    for (auto& th : pool) th.join();
+  }
+  if (first_bad < n) {
+    set_error("batch item " + std::to_string(first_bad) + ": " + first_msg);
+    return first_rc;
+  }
+  return 0;
+}
+
+extern "C" {
+
+int mm_minimize_batched(int n, const int64_t* ctxs, const mm_params* p, const double* B, int n_threads,
+                        mm_info* out) {
+  MM_CHECK(p && B && out, "null argument");
+  return run_batched(n, ctxs, n_threads, [&](int i) { return mm_minimize_resident(ctxs[i], p, B + 3 * i, out + i); });
+}
+
+int mm_solve_U_batched(int n, const int64_t* ctxs, const double* const* m, double* const* U_inout, const double* tol,
+                       int maxiter, int n_threads, int* it, double* rel_res) {
+  MM_CHECK(m && U_inout && tol && it && rel_res, "null argument");
+  return run_batched(n, ctxs, n_threads, [&](int i) {
+    return mm_solve_U(ctxs[i], m[i], U_inout[i], tol[i], maxiter, it + i, rel_res + i);
+  });
 }
 
 int mm_sa_aggregate(int n, const int32_t* Sp, const int32_t* Sj, int32_t* agg, int32_t* n_agg) {

@@ -105,7 +105,7 @@ def graded_airbox_cube(L: float, n: int, layers: int, grading: float = 2.0) -> d
     inside = np.all(np.abs(cen) < L / 2, axis=1)
     mat = np.where(inside, 1, 2).astype(np.int32)
     ijk = np.hstack([tets, mat[:, None]])
-    bnd = outer_boundary_mask(tets, knt.shape[0])
+    bnd = box_boundary_mask(knt)
     return {"knt": knt, "ijk": ijk, "boundary_mask": bnd}
 
 
@@ -118,16 +118,38 @@ def outer_boundary_mask(tets: np.ndarray, num_nodes: int) -> np.ndarray:
     t4 = np.asarray(tets)[:, :4].astype(np.int64)
     faces = t4[:, [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]]].reshape(-1, 3)
     keys = np.sort(faces, axis=1)
-    # pack three node ids (< 2^21 each) into one int64 key
-    if num_nodes >= (1 << 21):
-        _, inv, cnt = np.unique(keys, axis=0, return_inverse=True, return_counts=True)
-    else:
+    if num_nodes < (1 << 21):
+        # three node ids (< 2^21 each) packed into one int64 key
         packed = (keys[:, 0] << 42) | (keys[:, 1] << 21) | keys[:, 2]
         _, inv, cnt = np.unique(packed, return_inverse=True, return_counts=True)
-    surf = faces[cnt[inv.ravel()] == 1]
+        single = cnt[inv.ravel()] == 1
+    else:
+        # (k0, k1) packed, k2 as the minor sort key; a face is single if it equals neither neighbour
+        hi = keys[:, 0] * num_nodes + keys[:, 1]
+        order = np.lexsort((keys[:, 2], hi))
+        hs, ls = hi[order], keys[order, 2]
+        same = (hs[1:] == hs[:-1]) & (ls[1:] == ls[:-1])
+        dup = np.zeros(len(order), dtype=bool)
+        dup[1:] |= same
+        dup[:-1] |= same
+        single = np.empty(len(order), dtype=bool)
+        single[order] = ~dup
+    surf = faces[single]
     mask = np.ones(num_nodes, dtype=np.float64)
     mask[np.unique(surf)] = 0.0
     return mask
+
+
+def box_boundary_mask(knt: np.ndarray) -> np.ndarray:
+    """0.0 on nodes that lie on the faces of the axis-aligned bounding box, 1.0 elsewhere.
+
+    For a conforming tet mesh that fills its bounding box (the structured generators here) the
+    faces that belong to exactly one tet are exactly those on the box surface, so this equals
+    ``outer_boundary_mask`` without the face sort.
+    """
+    knt = np.asarray(knt, dtype=np.float64)
+    on = np.any((knt == knt.min(axis=0)) | (knt == knt.max(axis=0)), axis=1)
+    return np.where(on, 0.0, 1.0)
 
 
 def morton_encode(q: np.ndarray) -> np.ndarray:
@@ -212,16 +234,25 @@ def polycrystal_cube(L: float = 200.0, n: int = 150, layers: int = 5, n_grains: 
         t = tets[s:s + chunk].astype(np.int64)
         cen = knt[t].mean(axis=1)
         inside = np.all(np.abs(cen) < L / 2, axis=1)
-        d2 = ((cen[:, None, :] - centres[None, :, :]) ** 2).sum(axis=2)
-        order = np.argsort(d2, axis=1)[:, :2]
-        i1, i2 = order[:, 0], order[:, 1]
-        c1, c2 = centres[i1], centres[i2]
-        # distance from the centroid to the bisector plane of (c1, c2)
-        dist = (d2[np.arange(len(t)), i2] - d2[np.arange(len(t)), i1]) / (2.0 * np.linalg.norm(c2 - c1, axis=1))
+        # nearest and second-nearest centre, one pass per centre
+        d1 = np.full(len(t), np.inf)
+        d2 = np.full(len(t), np.inf)
+        i1 = np.zeros(len(t), dtype=np.int64)
+        i2 = np.zeros(len(t), dtype=np.int64)
+        for g in range(n_grains):
+            d = ((cen - centres[g]) ** 2).sum(axis=1)
+            closer = d < d1
+            second = ~closer & (d < d2)
+            d2 = np.where(closer, d1, np.where(second, d, d2))
+            i2 = np.where(closer, i1, np.where(second, g, i2))
+            d1 = np.where(closer, d, d1)
+            i1 = np.where(closer, g, i1)
+        # distance from the centroid to the bisector plane of the two nearest centres
+        dist = (d2 - d1) / (2.0 * np.linalg.norm(centres[i2] - centres[i1], axis=1))
         m = np.where(dist < gb_width, n_grains + 1, i1 + 1).astype(np.int32)
         mat[s:s + chunk] = np.where(inside, m, n_grains + 2)
     ijk = np.hstack([tets, mat[:, None]])
-    bnd = outer_boundary_mask(tets, knt.shape[0])
+    bnd = box_boundary_mask(knt)
     out = {"knt": knt, "ijk": ijk, "boundary_mask": bnd, "centres": centres}
     if morton:
         r = morton_reorder(knt, ijk, (bnd,))
@@ -233,3 +264,40 @@ def random_easy_axes(n: int, seed: int) -> np.ndarray:
     """Easy axes uniform on the unit sphere (normalised Gaussian vectors)."""
     v = np.random.default_rng(seed).standard_normal((n, 3))
     return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def grain_set(n_grains: int = 64, seed: int = 3, h: float = 2.0, layers: int = 3, t_min: float = 2e4,
+              t_max: float = 4e5, grading: float = 2.0, morton: bool = True) -> list[dict]:
+    """A set of independent grains, each a cube or a sphere inside its own graded airbox.
+
+    Drawn from ``numpy.random.default_rng(seed)`` in this order per grain: the total tet count
+    T (log-uniform in [t_min, t_max]), the shape (cube or sphere, a fair coin), the easy axis
+    (normalised Gaussian vector). The structured grid has ``n = round((T/6)^(1/3)) - 2*layers``
+    magnet cells per edge (at least 2) of size ``h``; a sphere grain keeps the tets whose
+    centroid lies within L/2 of the centre as magnet (material 1) and the rest as air
+    (material 2). Each dict holds ``knt``, ``ijk``, ``boundary_mask``, ``shape``, ``easy_axis``.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_grains):
+        T = float(np.exp(rng.uniform(np.log(t_min), np.log(t_max))))
+        shape = "sphere" if rng.uniform() < 0.5 else "cube"
+        k = rng.standard_normal(3)
+        k /= np.linalg.norm(k)
+        n = max(2, int(round((T / 6.0) ** (1.0 / 3.0))) - 2 * layers)
+        L = n * h
+        xs = graded_axis(L, n, layers, grading)
+        knt, tets = tensor_grid_tets(xs, xs, xs)
+        cen = knt[tets].mean(axis=1)
+        if shape == "cube":
+            inside = np.all(np.abs(cen) < L / 2, axis=1)
+        else:
+            inside = np.linalg.norm(cen, axis=1) < L / 2
+        ijk = np.hstack([tets, np.where(inside, 1, 2).astype(np.int32)[:, None]])
+        bnd = box_boundary_mask(knt)
+        g = {"knt": knt, "ijk": ijk, "boundary_mask": bnd, "shape": shape, "easy_axis": k}
+        if morton:
+            r = morton_reorder(knt, ijk, (bnd,))
+            g.update(knt=r["knt"], ijk=r["ijk"], boundary_mask=r["extra"][0])
+        out.append(g)
+    return out

@@ -1,0 +1,76 @@
+"""Host mesh generators: boundary masks, Morton reordering, the Voronoi polycrystal."""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def test_boundary_masks_agree():
+    """Packed-key and lexsort branches of the face count, and the bounding-box shortcut."""
+    from mumag_matrixfree2_amd.mesh import box_boundary_mask, graded_axis, outer_boundary_mask, tensor_grid_tets
+
+    xs = graded_axis(20.0, 6, 2, 2.0)
+    ys = graded_axis(10.0, 3, 1, 1.5)
+    knt, tets = tensor_grid_tets(xs, ys, xs)
+    N = knt.shape[0]
+    packed = outer_boundary_mask(tets, N)
+    lex = outer_boundary_mask(tets, 1 << 21)[:N]  # num_nodes >= 2^21 takes the lexsort branch
+    box = box_boundary_mask(knt)
+    assert np.array_equal(packed, lex)
+    assert np.array_equal(packed, box)
+    nb = (xs.size * ys.size * xs.size) - (xs.size - 2) * (ys.size - 2) * (xs.size - 2)
+    assert int((packed == 0).sum()) == nb
+
+
+def test_morton_reorder_is_a_relabelling():
+    from mumag_matrixfree2_amd.mesh import config_cube_mesh
+
+    a = config_cube_mesh(20.0, 5, 2, morton=False)
+    b = config_cube_mesh(20.0, 5, 2, morton=True)
+    assert a["knt"].shape == b["knt"].shape and a["ijk"].shape == b["ijk"].shape
+    # the same set of elements as coordinate quadruples, in some order
+    ka = np.sort(np.round(a["knt"][a["ijk"][:, :4]], 9).reshape(len(a["ijk"]), -1), axis=0)
+    kb = np.sort(np.round(b["knt"][b["ijk"][:, :4]], 9).reshape(len(b["ijk"]), -1), axis=0)
+    assert np.array_equal(ka, kb)
+    assert np.array_equal(np.sort(a["boundary_mask"]), np.sort(b["boundary_mask"]))
+
+
+def test_polycrystal_phases_by_brute_force():
+    """Grain = nearest centre, grain boundary = within gb_width of the bisector plane of the
+    two nearest centres, air = outside the cube; checked tet by tet with a full sort."""
+    from mumag_matrixfree2_amd.mesh import polycrystal_cube
+
+    L, ng, w = 30.0, 6, 1.5
+    m = polycrystal_cube(L, 8, 2, n_grains=ng, gb_width=w, seed=9, morton=False)
+    knt, ijk, C = m["knt"], m["ijk"], m["centres"]
+    cen = knt[ijk[:, :4]].mean(axis=1)
+    expect = np.empty(len(ijk), dtype=np.int64)
+    for e, c in enumerate(cen):
+        if np.any(np.abs(c) >= L / 2):
+            expect[e] = ng + 2
+            continue
+        d = np.linalg.norm(C - c, axis=1)
+        i1, i2 = np.argsort(d)[:2]
+        n = (C[i2] - C[i1]) / np.linalg.norm(C[i2] - C[i1])
+        dist = np.dot(c - 0.5 * (C[i1] + C[i2]), n)  # signed distance to the bisector, <= 0 on i1's side
+        expect[e] = ng + 1 if -dist < w else i1 + 1
+    assert np.array_equal(ijk[:, 4], expect)
+    assert set(np.unique(ijk[:, 4])) <= set(range(1, ng + 3))
+    assert np.all(m["boundary_mask"][ijk[ijk[:, 4] <= ng, :4]] == 1.0)  # magnet never touches the Dirichlet set
+
+
+def test_grain_set_draws():
+    from mumag_matrixfree2_amd.mesh import grain_set
+
+    a = grain_set(6, seed=3, t_min=2e3, t_max=2e4, morton=False)
+    b = grain_set(6, seed=3, t_min=2e3, t_max=2e4, morton=False)
+    for g, h in zip(a, b):
+        assert np.array_equal(g["ijk"], h["ijk"]) and g["shape"] == h["shape"]
+        assert abs(np.linalg.norm(g["easy_axis"]) - 1.0) < 1e-14
+        mag = g["ijk"][:, 4] == 1
+        cen = g["knt"][g["ijk"][:, :4]].mean(axis=1)
+        if g["shape"] == "sphere":
+            r = np.linalg.norm(cen, axis=1)
+            assert r[mag].max() < r[~mag].min()
+        assert 0 < mag.sum() < len(mag)
+    assert {g["shape"] for g in a} <= {"cube", "sphere"}
