@@ -246,6 +246,7 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   // nodes per element block (<= 256, the LDS staging capacity); MM_BLOCK_NODES overrides it
   int node_cap = kMaxBlockNodes;
   if (const char* env = std::getenv("MM_BLOCK_NODES")) node_cap = std::max(16, std::min(kMaxBlockNodes, std::atoi(env)));
+  if (const char* env = std::getenv("MM_ELEM_THREADS")) c.elem_threads = std::atoi(env);
   HostPart hp_all, hp_mag;
   build_partition(N, conn, mat0, order, is_bnd, hp_all, node_cap);
   std::vector<int> mag_elems;
@@ -346,6 +347,10 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   MM_TRY(dalloc(c, &c.dscal, kNumScal));
   MM_HIP(hipMemset(c.dscal, 0, kNumScal * sizeof(double)));
   MM_HIP(hipHostMalloc((void**)&c.hscal, kNumScal * sizeof(double), hipHostMallocDefault));
+  MM_TRY(dalloc(c, &c.part_pq, (size_t)8 * ((c.part_all.nblk + 7) / 8) + kRedGridMax));
+  MM_TRY(dalloc(c, &c.part_rr, 2 * (size_t)kRedGridMax));
+  MM_TRY(dalloc(c, &c.part_rz, (size_t)kRedGridMax));
+  MM_HIP(hipHostMalloc((void**)&c.hpart, 2 * kRedGridMax * sizeof(double), hipHostMallocDefault));
 
   // Poisson work vectors
   for (double** v : {&c.pb, &c.pr, &c.pz, &c.pp, &c.pq, &c.pt, &c.px0, &c.work_U, &c.diagA}) MM_TRY(dalloc(c, v, N));
@@ -360,6 +365,7 @@ static void destroy(Ctx* c) {
   (void)hipDeviceSynchronize();
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->hpart) (void)hipHostFree(c->hpart);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev) (void)hipEventDestroy(c->ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -630,7 +636,7 @@ int mm_precond_apply(int64_t h, const double* r, double* z) {
   GET_CTX(h);
   MM_CHECK(r && z, "null array");
   MM_HIP(hipMemcpyAsync(c.pr, r, c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
-  MM_TRY(precond_apply(c, c.pr, c.pz, c.dscal + 6, c.stream));
+  MM_TRY(precond_apply(c, c.pr, c.pz, c.part_rz, c.stream));
   MM_HIP(hipGetLastError());
   MM_HIP(hipMemcpyAsync(z, c.pz, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   MM_HIP(hipStreamSynchronize(c.stream));
@@ -953,13 +959,13 @@ int mm_bench_precond(int64_t h, int use_graph, int reps, double* ms) {
   hipEvent_t e0, e1;
   MM_HIP(hipEventCreate(&e0));
   MM_HIP(hipEventCreate(&e1));
-  MM_TRY(precond_apply(c, c.pr, c.pz, c.dscal + 6, c.stream));  // warm-up
+  MM_TRY(precond_apply(c, c.pr, c.pz, c.part_rz, c.stream));  // warm-up
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   if (use_graph) {
     MM_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     vfill(c.pt, 0.0, c.N, c.stream);
-    MM_TRY(precond(c, c.pr, c.pz, c.dscal + 6, c.stream));
+    MM_TRY(precond(c, c.pr, c.pz, c.part_rz, c.stream));
     MM_HIP(hipStreamEndCapture(c.stream, &graph));
     MM_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     MM_HIP(hipGraphLaunch(exec, c.stream));
@@ -970,7 +976,7 @@ int mm_bench_precond(int64_t h, int use_graph, int reps, double* ms) {
       MM_HIP(hipGraphLaunch(exec, c.stream));
     } else {
       vfill(c.pt, 0.0, c.N, c.stream);
-      MM_TRY(precond(c, c.pr, c.pz, c.dscal + 6, c.stream));
+      MM_TRY(precond(c, c.pr, c.pz, c.part_rz, c.stream));
     }
   }
   MM_HIP(hipEventRecord(e1, c.stream));

@@ -47,25 +47,51 @@ __device__ __forceinline__ void tet_grads(const double (*X)[kMaxBlockNodes], int
   vol = fabs(det) * (1.0 / 6.0);
 }
 
-// FP32 twin of tet_grads on block-relative FP32 coordinates (mixed-precision preconditioner).
-__device__ __forceinline__ void tet_grads_f(const float (*X)[kMaxBlockNodes], int l0, int l1, int l2, int l3,
-                                            float g[4][3], float& vol) {
-  const float ox = X[0][l0], oy = X[1][l0], oz = X[2][l0];
-  const float c1x = X[0][l1] - ox, c1y = X[1][l1] - oy, c1z = X[2][l1] - oz;
-  const float c2x = X[0][l2] - ox, c2y = X[1][l2] - oy, c2z = X[2][l2] - oz;
-  const float c3x = X[0][l3] - ox, c3y = X[1][l3] - oy, c3z = X[2][l3] - oz;
-  const float n1x = c2y * c3z - c2z * c3y, n1y = c2z * c3x - c2x * c3z, n1z = c2x * c3y - c2y * c3x;
-  const float n2x = c3y * c1z - c3z * c1y, n2y = c3z * c1x - c3x * c1z, n2z = c3x * c1y - c3y * c1x;
-  const float n3x = c1y * c2z - c1z * c2y, n3y = c1z * c2x - c1x * c2z, n3z = c1x * c2y - c1y * c2x;
-  const float det = c1x * n1x + c1y * n1y + c1z * n1z;
-  const float inv = 1.0f / det;
-  g[1][0] = n1x * inv; g[1][1] = n1y * inv; g[1][2] = n1z * inv;
-  g[2][0] = n2x * inv; g[2][1] = n2y * inv; g[2][2] = n2z * inv;
-  g[3][0] = n3x * inv; g[3][1] = n3y * inv; g[3][2] = n3z * inv;
-  g[0][0] = -(g[1][0] + g[2][0] + g[3][0]);
-  g[0][1] = -(g[1][1] + g[2][1] + g[3][1]);
-  g[0][2] = -(g[1][2] + g[2][2] + g[3][2]);
-  vol = fabsf(det) * (1.0f / 6.0f);
+// 1/a from the hardware estimate (v_rcp_f64) refined by two Newton-Raphson steps
+// r <- r + r (1 - a r); each step squares the relative error, so the result is within
+// about one ulp of 1/a for normal a (the element determinants here are O(1) in nm^3).
+__device__ __forceinline__ double rcp_nr(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = fma(-a, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-a, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Scaled corner normals of one tet: n_b = det * grad phi_b (b = 1..3) are the cross products
+// (c2 x c3), (c3 x c1), (c1 x c2) of the edge vectors c_b = x_b - x_0; returns det.
+// With them the element stiffness is Ve grad phi_a . grad phi_b = n_a . n_b / (6 |det|).
+template <typename T>
+__device__ __forceinline__ T tet_normals(const T (*X)[kMaxBlockNodes], int l0, int l1, int l2, int l3, T n[3][3]) {
+  const T ox = X[0][l0], oy = X[1][l0], oz = X[2][l0];
+  const T c1x = X[0][l1] - ox, c1y = X[1][l1] - oy, c1z = X[2][l1] - oz;
+  const T c2x = X[0][l2] - ox, c2y = X[1][l2] - oy, c2z = X[2][l2] - oz;
+  const T c3x = X[0][l3] - ox, c3y = X[1][l3] - oy, c3z = X[2][l3] - oz;
+  n[0][0] = c2y * c3z - c2z * c3y; n[0][1] = c2z * c3x - c2x * c3z; n[0][2] = c2x * c3y - c2y * c3x;
+  n[1][0] = c3y * c1z - c3z * c1y; n[1][1] = c3z * c1x - c3x * c1z; n[1][2] = c3x * c1y - c3y * c1x;
+  n[2][0] = c1y * c2z - c1z * c2y; n[2][1] = c1z * c2x - c1x * c2z; n[2][2] = c1x * c2y - c1y * c2x;
+  return c1x * n[0][0] + c1y * n[0][1] + c1z * n[0][2];
+}
+
+// Laplacian stiffness action of one tet on corner values x_0..x_3 (Ve grad phi_a . grad u):
+// w = sum_b (x_b - x_0) n_b = det * grad u, y_a = n_a . w / (6 |det|) for a = 1..3 and
+// y_0 = -(y_1 + y_2 + y_3) (the gradients sum to zero). Returns the element energy
+// sum_a x_a y_a = |w|^2 / (6 |det|).
+template <typename T>
+__device__ __forceinline__ T tet_laplace(const T n[3][3], T det, T d1, T d2, T d3, T y[4]) {
+  const T wx = d1 * n[0][0] + d2 * n[1][0] + d3 * n[2][0];
+  const T wy = d1 * n[0][1] + d2 * n[1][1] + d3 * n[2][1];
+  const T wz = d1 * n[0][2] + d2 * n[1][2] + d3 * n[2][2];
+  T s;
+  if constexpr (sizeof(T) == 8)
+    s = rcp_nr(6.0 * fabs(det));
+  else
+    s = __builtin_amdgcn_rcpf(6.0f * fabsf(det));
+  y[1] = (n[0][0] * wx + n[0][1] * wy + n[0][2] * wz) * s;
+  y[2] = (n[1][0] * wx + n[1][1] * wy + n[1][2] * wz) * s;
+  y[3] = (n[2][0] * wx + n[2][1] * wy + n[2][2] * wz) * s;
+  y[0] = -(y[1] + y[2] + y[3]);
+  return (wx * wx + wy * wy + wz * wz) * s;
 }
 
 __device__ __forceinline__ void unpack(uint32_t lc, int& l0, int& l1, int& l2, int& l3) {
@@ -175,9 +201,9 @@ __device__ bool reduce_last(double (&v)[K], double* __restrict__ partial, unsign
 
 // Block sum of K values written to partial[blockIdx.x * K + k] (no cross-block traffic);
 // sum_partials then reduces them in a fixed order.
-template <int K>
+template <int K, int NT = kBlock>
 __device__ void block_partial(double (&v)[K], double* __restrict__ partial) {
-  __shared__ double sh[K][kBlock / 64];
+  __shared__ double sh[K][NT / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = wave_red<false>(v[k]);
@@ -189,9 +215,28 @@ __device__ void block_partial(double (&v)[K], double* __restrict__ partial) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       double s = sh[k][0];
-      for (int w = 1; w < kBlock / 64; ++w) s += sh[k][w];
+      for (int w = 1; w < NT / 64; ++w) s += sh[k][w];
       partial[blockIdx.x * K + k] = s;
     }
+}
+
+// Sum of n partials written by an earlier kernel on the same stream (the kernel boundary
+// makes them visible). Every workgroup that calls it computes the same total in the same
+// fixed order, so the result is deterministic and identical across the grid; no fence or
+// counter atomic is involved. Valid in every thread.
+template <int NT>
+__device__ double sum_partials_all(const double* __restrict__ part, int n) {
+  __shared__ double sh[NT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double s = 0.0;
+  for (int i = tid; i < n; i += NT) s += part[i];
+  s = wave_red<false>(s);
+  if (lane == 0) sh[wv] = s;
+  __syncthreads();
+  double t = sh[0];
+#pragma unroll
+  for (int w = 1; w < NT / 64; ++w) t += sh[w];
+  return t;
 }
 
 // reduce_last + store of the totals to out[0..K-1]

@@ -126,6 +126,7 @@ struct Ctx {
   long long nodes_all = 0;            // node-list entries of the all-element partition
   int max_block_nodes = 0;
   int geom_mode = kGeomCoords;        // geometry source of the Poisson apply
+  int elem_threads = 256;             // workgroup size of the Poisson apply (64, 128, 256)
   std::vector<void*> allocs;          // every device allocation, freed at destroy
   MatTable hmt;
 
@@ -142,6 +143,11 @@ struct Ctx {
   unsigned* counter = nullptr;        // 16 last-block counters
   double* dscal = nullptr;            // kNumScal device scalars
   double* hscal = nullptr;            // pinned host mirror
+  // fence-free PCG reductions: per-workgroup partials consumed by the next kernel or the host
+  double* part_pq = nullptr;          // p.Ap partials of the Poisson apply (one per apply workgroup)
+  double* part_rr = nullptr;          // r.r (and b.b) partials, 2 x kRedGridMax
+  double* part_rz = nullptr;          // r.z partials, kRedGridMax
+  double* hpart = nullptr;            // pinned host copy of part_rr
 
   // Poisson / PCG vectors (N)
   double *pb = nullptr, *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *pt = nullptr, *px0 = nullptr;
@@ -173,11 +179,12 @@ struct Ctx {
   long long apply_timed = 0;
 };
 
-int timed_apply(Ctx& c, const double* x, double* y, hipStream_t s);
+int timed_apply(Ctx& c, const double* x, double* y, hipStream_t s, double* dot_part = nullptr);
 int resolve_apply_timing(Ctx& c);
 
 // element kernels (mm_kernels.hip)
-int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s);
+// dot_part != nullptr: also write per-workgroup partials of x . (A x) (grid_for(part_all) of them)
+int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s, double* dot_part = nullptr);
 int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s);
 // y = A (w .* r) with xo = w .* r written by the node owners (fused pre-smoothing); y zeroed first
 int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s);
@@ -198,8 +205,9 @@ int vdot(const Ctx& c, const double* a, const double* b, long n, double* out, hi
 int vmaxabs(const Ctx& c, const double* a, long n, double* out, hipStream_t s);
 
 // solver (mm_solver.hip)
-int precond_apply(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s);
-int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s);
+// z = M^-1 r; rz_part receives red_grid(N) partials of r.z
+int precond_apply(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s);
+int precond(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s);
 int solve_U_dev(Ctx& c, const double* m, double* U, double tol, int maxiter, int* it, double* rel_res);
 
 }  // namespace mm

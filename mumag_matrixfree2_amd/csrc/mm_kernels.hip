@@ -11,6 +11,8 @@
 //      a hardware FP64 atomic add (global_atomic_add_f64) for shared nodes.
 // Workgroup ids are remapped so that each XCD (ids i mod 8) runs a contiguous, spatially
 // compact range of blocks and keeps its node data in its own L2.
+#include <type_traits>
+
 #include "mm_device.h"
 
 namespace mm {
@@ -33,80 +35,119 @@ __device__ __forceinline__ void stored_grads(const TG* __restrict__ grad, const 
 // row written as y = x by its owner), reg added on the diagonal of interior rows.
 // With SCALED the input is x = w .* r (the level-0 pre-smoothing step of the V-cycle from a
 // zero guess): it is formed during the staging and each node's owner also writes it to xo.
-template <int GEOM, bool SCALED>
-__global__ __launch_bounds__(kBlock) void k_apply(Part p, Geo geo, const double* __restrict__ x,
-                                                  const double* __restrict__ w, double* __restrict__ xo,
-                                                  double* __restrict__ y, double reg) {
+// With DOT (plain apply only) the workgroup also writes its share of x . (A x) to
+// dot_part[blockIdx.x]: the element energies |w|^2 / (6 |det|) plus, per owned node, reg x^2
+// (interior) or x^2 (Dirichlet row), so x . A x needs no separate pass over x and y.
+// NT threads per workgroup (64, 128 or 256): the block's node flags stay in registers
+// between the staging and the write-back (kMaxBlockNodes / NT per thread).
+template <int GEOM, bool SCALED, bool DOT, int NT>
+__global__ __launch_bounds__(NT) void k_apply(Part p, Geo geo, const double* __restrict__ x,
+                                              const double* __restrict__ w, double* __restrict__ xo,
+                                              double* __restrict__ y, double reg, double* __restrict__ dot_part) {
   __shared__ double sx[kMaxBlockNodes];
   __shared__ double sy[kMaxBlockNodes];
   __shared__ double sX[GEOM == kGeomCoords ? 3 : 1][kMaxBlockNodes];
   const BlkInfo bi = blk_info(p);
-  if (!bi.valid) return;
+  if (!bi.valid) {
+    if (DOT && threadIdx.x == 0) dot_part[blockIdx.x] = 0.0;
+    return;
+  }
   const int tid = threadIdx.x;
-  // element words of the first kPre rounds are loaded before the staging barrier
-  constexpr int kPre = 4;
-  uint32_t lcv[kPre];
+  constexpr int kNode = kMaxBlockNodes / NT;  // node slots per thread
+  constexpr int kPre = 1024 / NT;             // element words loaded before the staging barrier
+  uint32_t lcv[kPre], fv[kNode];
 #pragma unroll
   for (int r = 0; r < kPre; ++r) {
-    const int e = bi.e0 + tid + r * kBlock;
+    const int e = bi.e0 + tid + r * NT;
     lcv[r] = e < bi.e1 ? p.lconn[e] : 0u;
   }
-  for (int i = tid; i < bi.nb; i += kBlock) {
-    const uint32_t f = p.nodes[bi.n0 + i];
-    const uint32_t id = f & kIdMask;
-    double xv = x[id];
-    if constexpr (SCALED) {
-      xv *= w[id];
-      if (f & kOwner) xo[id] = xv;
-    }
-    sx[i] = (f & kBoundary) ? 0.0 : xv;
-    sy[i] = 0.0;
-    if constexpr (GEOM == kGeomCoords) {
-      sX[0][i] = geo.knt[3 * id + 0];
-      sX[1][i] = geo.knt[3 * id + 1];
-      sX[2][i] = geo.knt[3 * id + 2];
+#pragma unroll
+  for (int r = 0; r < kNode; ++r) {
+    const int i = tid + r * NT;
+    fv[r] = i < bi.nb ? p.nodes[bi.n0 + i] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kNode; ++r) {
+    const int i = tid + r * NT;
+    if (i < bi.nb) {
+      const uint32_t f = fv[r];
+      const uint32_t id = f & kIdMask;
+      double xv = x[id];
+      if constexpr (SCALED) {
+        xv *= w[id];
+        if (f & kOwner) xo[id] = xv;
+      }
+      sx[i] = (f & kBoundary) ? 0.0 : xv;
+      sy[i] = 0.0;
+      if constexpr (GEOM == kGeomCoords) {
+        sX[0][i] = geo.knt[3 * id + 0];
+        sX[1][i] = geo.knt[3 * id + 1];
+        sX[2][i] = geo.knt[3 * id + 2];
+      }
     }
   }
   __syncthreads();
+  double acc = 0.0;
   auto element = [&](int e, uint32_t lc) {
     int l0, l1, l2, l3;
     unpack(lc, l0, l1, l2, l3);
-    double g[4][3], vol;
-    if constexpr (GEOM == kGeomCoords)
-      tet_grads(sX, l0, l1, l2, l3, g, vol);
-    else if constexpr (GEOM == kGeomStored64)
-      stored_grads(geo.grad, geo.Ve, p.Es, e, g, vol);
-    else
-      stored_grads(geo.grad32, geo.Ve32, p.Es, e, g, vol);
     const double x0 = sx[l0];
     const double d1 = sx[l1] - x0, d2 = sx[l2] - x0, d3 = sx[l3] - x0;
-    const double gx = (d1 * g[1][0] + d2 * g[2][0] + d3 * g[3][0]) * vol;
-    const double gy = (d1 * g[1][1] + d2 * g[2][1] + d3 * g[3][1]) * vol;
-    const double gz = (d1 * g[1][2] + d2 * g[2][2] + d3 * g[3][2]) * vol;
-    atomicAdd(&sy[l0], g[0][0] * gx + g[0][1] * gy + g[0][2] * gz);
-    atomicAdd(&sy[l1], g[1][0] * gx + g[1][1] * gy + g[1][2] * gz);
-    atomicAdd(&sy[l2], g[2][0] * gx + g[2][1] * gy + g[2][2] * gz);
-    atomicAdd(&sy[l3], g[3][0] * gx + g[3][1] * gy + g[3][2] * gz);
+    double ya[4];
+    if constexpr (GEOM == kGeomCoords) {
+      double n[3][3];
+      const double det = tet_normals(sX, l0, l1, l2, l3, n);
+      const double en = tet_laplace(n, det, d1, d2, d3, ya);
+      if constexpr (DOT) acc += en;
+    } else {
+      double g[4][3], vol;
+      if constexpr (GEOM == kGeomStored64)
+        stored_grads(geo.grad, geo.Ve, p.Es, e, g, vol);
+      else
+        stored_grads(geo.grad32, geo.Ve32, p.Es, e, g, vol);
+      const double gx = (d1 * g[1][0] + d2 * g[2][0] + d3 * g[3][0]) * vol;
+      const double gy = (d1 * g[1][1] + d2 * g[2][1] + d3 * g[3][1]) * vol;
+      const double gz = (d1 * g[1][2] + d2 * g[2][2] + d3 * g[3][2]) * vol;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) ya[a] = g[a][0] * gx + g[a][1] * gy + g[a][2] * gz;
+      if constexpr (DOT) acc += d1 * ya[1] + d2 * ya[2] + d3 * ya[3];
+    }
+    atomicAdd(&sy[l0], ya[0]);
+    atomicAdd(&sy[l1], ya[1]);
+    atomicAdd(&sy[l2], ya[2]);
+    atomicAdd(&sy[l3], ya[3]);
   };
 #pragma unroll
   for (int r = 0; r < kPre; ++r) {
-    const int e = bi.e0 + tid + r * kBlock;
+    const int e = bi.e0 + tid + r * NT;
     if (e < bi.e1) element(e, lcv[r]);
   }
-  for (int e = bi.e0 + tid + kPre * kBlock; e < bi.e1; e += kBlock) element(e, p.lconn[e]);
+  for (int e = bi.e0 + tid + kPre * NT; e < bi.e1; e += NT) element(e, p.lconn[e]);
   __syncthreads();
-  for (int i = tid; i < bi.nb; i += kBlock) {
-    const uint32_t f = p.nodes[bi.n0 + i];
+#pragma unroll
+  for (int r = 0; r < kNode; ++r) {
+    const int i = tid + r * NT;
+    if (i >= bi.nb) continue;
+    const uint32_t f = fv[r];
     if (f & kBoundary) {
       if (f & kOwner) {
         const uint32_t id = f & kIdMask;
-        y[id] = SCALED ? x[id] * w[id] : x[id];
+        const double xb = SCALED ? x[id] * w[id] : x[id];
+        y[id] = xb;
+        if constexpr (DOT) acc += xb * xb;
       }
       continue;
     }
     double v = sy[i];
-    if (f & kOwner) v += reg * sx[i];
+    if (f & kOwner) {
+      v += reg * sx[i];
+      if constexpr (DOT) acc += reg * sx[i] * sx[i];
+    }
     node_out(y, f, v);
+  }
+  if constexpr (DOT) {
+    double v1[1] = {acc};
+    block_partial<1, NT>(v1, dot_part);
   }
 }
 
@@ -116,66 +157,76 @@ __global__ __launch_bounds__(kBlock) void k_apply(Part p, Geo geo, const double*
 // FP32. The LDS accumulator stays FP64: on gfx950 ds_add_f32 runs at a fraction of the rate of
 // ds_add_f64 (measured: 53 us vs 16 us per apply on the 2.06M-tet cube). Used inside the
 // mixed-precision V-cycle only.
-template <bool SCALED>
-__global__ __launch_bounds__(kBlock) void k_apply_f32(Part p, Geo geo, const float* __restrict__ x,
-                                                      const double* __restrict__ r, const float* __restrict__ w,
-                                                      float* __restrict__ xo, float* __restrict__ y, float reg) {
+template <bool SCALED, int NT>
+__global__ __launch_bounds__(NT) void k_apply_f32(Part p, Geo geo, const float* __restrict__ x,
+                                                  const double* __restrict__ r, const float* __restrict__ w,
+                                                  float* __restrict__ xo, float* __restrict__ y, float reg) {
   __shared__ float sx[kMaxBlockNodes];
   __shared__ double sy[kMaxBlockNodes];
   __shared__ float sX[3][kMaxBlockNodes];
   const BlkInfo bi = blk_info(p);
   if (!bi.valid) return;
   const int tid = threadIdx.x;
-  constexpr int kPre = 4;
-  uint32_t lcv[kPre];
+  constexpr int kNode = kMaxBlockNodes / NT;
+  constexpr int kPre = 1024 / NT;
+  uint32_t lcv[kPre], fv[kNode];
 #pragma unroll
-  for (int r = 0; r < kPre; ++r) {
-    const int e = bi.e0 + tid + r * kBlock;
-    lcv[r] = e < bi.e1 ? p.lconn[e] : 0u;
+  for (int q = 0; q < kPre; ++q) {
+    const int e = bi.e0 + tid + q * NT;
+    lcv[q] = e < bi.e1 ? p.lconn[e] : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < kNode; ++q) {
+    const int i = tid + q * NT;
+    fv[q] = i < bi.nb ? p.nodes[bi.n0 + i] : 0u;
   }
   const uint32_t id0 = p.nodes[bi.n0] & kIdMask;
   const double ox = geo.knt[3 * id0], oy = geo.knt[3 * id0 + 1], oz = geo.knt[3 * id0 + 2];
-  for (int i = tid; i < bi.nb; i += kBlock) {
-    const uint32_t f = p.nodes[bi.n0 + i];
-    const uint32_t id = f & kIdMask;
-    float xv;
-    if constexpr (SCALED) {
-      xv = (float)(w[id] * r[id]);
-      if (f & kOwner) xo[id] = xv;
-    } else {
-      xv = x[id];
+#pragma unroll
+  for (int q = 0; q < kNode; ++q) {
+    const int i = tid + q * NT;
+    if (i < bi.nb) {
+      const uint32_t f = fv[q];
+      const uint32_t id = f & kIdMask;
+      float xv;
+      if constexpr (SCALED) {
+        xv = (float)(w[id] * r[id]);
+        if (f & kOwner) xo[id] = xv;
+      } else {
+        xv = x[id];
+      }
+      sx[i] = (f & kBoundary) ? 0.0f : xv;
+      sy[i] = 0.0;
+      sX[0][i] = (float)(geo.knt[3 * id + 0] - ox);
+      sX[1][i] = (float)(geo.knt[3 * id + 1] - oy);
+      sX[2][i] = (float)(geo.knt[3 * id + 2] - oz);
     }
-    sx[i] = (f & kBoundary) ? 0.0f : xv;
-    sy[i] = 0.0;
-    sX[0][i] = (float)(geo.knt[3 * id + 0] - ox);
-    sX[1][i] = (float)(geo.knt[3 * id + 1] - oy);
-    sX[2][i] = (float)(geo.knt[3 * id + 2] - oz);
   }
   __syncthreads();
   auto element = [&](uint32_t lc) {
     int l0, l1, l2, l3;
     unpack(lc, l0, l1, l2, l3);
-    float g[4][3], vol;
-    tet_grads_f(sX, l0, l1, l2, l3, g, vol);
+    float n[3][3], ya[4];
+    const float det = tet_normals(sX, l0, l1, l2, l3, n);
     const float x0 = sx[l0];
-    const float d1 = sx[l1] - x0, d2 = sx[l2] - x0, d3 = sx[l3] - x0;
-    const float gx = (d1 * g[1][0] + d2 * g[2][0] + d3 * g[3][0]) * vol;
-    const float gy = (d1 * g[1][1] + d2 * g[2][1] + d3 * g[3][1]) * vol;
-    const float gz = (d1 * g[1][2] + d2 * g[2][2] + d3 * g[3][2]) * vol;
-    atomicAdd(&sy[l0], (double)(g[0][0] * gx + g[0][1] * gy + g[0][2] * gz));
-    atomicAdd(&sy[l1], (double)(g[1][0] * gx + g[1][1] * gy + g[1][2] * gz));
-    atomicAdd(&sy[l2], (double)(g[2][0] * gx + g[2][1] * gy + g[2][2] * gz));
-    atomicAdd(&sy[l3], (double)(g[3][0] * gx + g[3][1] * gy + g[3][2] * gz));
+    (void)tet_laplace(n, det, sx[l1] - x0, sx[l2] - x0, sx[l3] - x0, ya);
+    atomicAdd(&sy[l0], (double)ya[0]);
+    atomicAdd(&sy[l1], (double)ya[1]);
+    atomicAdd(&sy[l2], (double)ya[2]);
+    atomicAdd(&sy[l3], (double)ya[3]);
   };
 #pragma unroll
-  for (int r = 0; r < kPre; ++r) {
-    const int e = bi.e0 + tid + r * kBlock;
-    if (e < bi.e1) element(lcv[r]);
+  for (int q = 0; q < kPre; ++q) {
+    const int e = bi.e0 + tid + q * NT;
+    if (e < bi.e1) element(lcv[q]);
   }
-  for (int e = bi.e0 + tid + kPre * kBlock; e < bi.e1; e += kBlock) element(p.lconn[e]);
+  for (int e = bi.e0 + tid + kPre * NT; e < bi.e1; e += NT) element(p.lconn[e]);
   __syncthreads();
-  for (int i = tid; i < bi.nb; i += kBlock) {
-    const uint32_t f = p.nodes[bi.n0 + i];
+#pragma unroll
+  for (int q = 0; q < kNode; ++q) {
+    const int i = tid + q * NT;
+    if (i >= bi.nb) continue;
+    const uint32_t f = fv[q];
     const uint32_t id = f & kIdMask;
     if (f & kBoundary) {
       if (f & kOwner) y[id] = SCALED ? (float)(w[id] * r[id]) : x[id];
@@ -524,41 +575,65 @@ __global__ __launch_bounds__(kBlock) void k_jpar(Part p, Geo geo, const double* 
 // ------------------------------------------------------------------------ launchers
 static inline int grid_for(const Part& p) { return 8 * ((p.nblk + 7) / 8); }
 
-int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s) {
+// Calls f(std::integral_constant<int, NT>) for the context's workgroup size NT (64, 128, 256).
+template <typename F>
+static void with_nt(int nt, F&& f) {
+  switch (nt) {
+    case 64: f(std::integral_constant<int, 64>{}); break;
+    case 128: f(std::integral_constant<int, 128>{}); break;
+    default: f(std::integral_constant<int, 256>{}); break;
+  }
+}
+
+int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s, double* dot_part) {
   const Part& p = c.part_all;
   if (p.nblk == 0) return 0;
   const int grid = grid_for(p);
-  switch (geom) {
-    case kGeomCoords:
-      k_apply<kGeomCoords, false><<<grid, kBlock, 0, s>>>(p, c.geo, x, nullptr, nullptr, y, c.reg);
-      break;
-    case kGeomStored64:
-      k_apply<kGeomStored64, false><<<grid, kBlock, 0, s>>>(p, c.geo, x, nullptr, nullptr, y, c.reg);
-      break;
-    default:
-      k_apply<kGeomStored32, false><<<grid, kBlock, 0, s>>>(p, c.geo, x, nullptr, nullptr, y, c.reg);
-  }
+  MM_CHECK(dot_part == nullptr || geom == kGeomCoords, "x.Ax partials need the on-the-fly apply");
+  with_nt(c.elem_threads, [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    if (dot_part)
+      k_apply<kGeomCoords, false, true, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, y, c.reg, dot_part);
+    else if (geom == kGeomCoords)
+      k_apply<kGeomCoords, false, false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, y, c.reg, nullptr);
+    else if (geom == kGeomStored64)
+      k_apply<kGeomStored64, false, false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, y, c.reg, nullptr);
+    else
+      k_apply<kGeomStored32, false, false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, y, c.reg, nullptr);
+  });
   return 0;
 }
 
 int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s) {
   const Part& p = c.part_all;
   if (p.nblk == 0) return 0;
-  k_apply<kGeomCoords, true><<<grid_for(p), kBlock, 0, s>>>(p, c.geo, r, w, xo, y, c.reg);
+  const int grid = grid_for(p);
+  with_nt(c.elem_threads, [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    k_apply<kGeomCoords, true, false, NT><<<grid, NT, 0, s>>>(p, c.geo, r, w, xo, y, c.reg, nullptr);
+  });
   return 0;
 }
 
 int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s) {
   const Part& p = c.part_all;
   if (p.nblk == 0) return 0;
-  k_apply_f32<false><<<grid_for(p), kBlock, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, (float)c.reg);
+  const int grid = grid_for(p);
+  with_nt(c.elem_threads, [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    k_apply_f32<false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, (float)c.reg);
+  });
   return 0;
 }
 
 int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float* xo, float* y, hipStream_t s) {
   const Part& p = c.part_all;
   if (p.nblk == 0) return 0;
-  k_apply_f32<true><<<grid_for(p), kBlock, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg);
+  const int grid = grid_for(p);
+  with_nt(c.elem_threads, [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    k_apply_f32<true, NT><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg);
+  });
   return 0;
 }
 

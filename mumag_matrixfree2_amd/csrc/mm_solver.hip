@@ -2,10 +2,12 @@
 // for the demagnetising potential (``make_pcg_solve`` + ``make_jax_amgcl_vcycle`` /
 // ``make_jax_amg_vcycle`` semantics, src/poisson_solve.py:422-478, src/amg_utils.py:491-623).
 //
-// All PCG scalars live in device memory: alpha and beta are formed inside the update
-// kernels from the dot products written by the previous kernel, so the host never waits
-// inside an iteration except to read r.r for the stopping test, and that read overlaps
-// the V-cycle of the same iteration.
+// All PCG scalars live in device memory and the reductions need no fences: a producing
+// kernel writes one partial per workgroup, and the consuming kernel (every one of its
+// workgroups, in the same fixed order) sums them in its prologue before forming alpha or
+// beta. p.Ap comes out of the Poisson apply itself (per-block element energies). The host
+// never waits inside an iteration except to sum the r.r partials for the stopping test, and
+// that copy overlaps the V-cycle of the same iteration.
 #include <cmath>
 
 #include "mm_device.h"
@@ -37,6 +39,14 @@ __global__ __launch_bounds__(kBlock) void k_maxabs(const double* __restrict__ a,
   reduce_store<1, true>(v, partial, counter, out);
 }
 
+// per-workgroup partials of a.b (no cross-workgroup traffic)
+__global__ __launch_bounds__(kBlock) void k_dot_part(const double* __restrict__ a, const double* __restrict__ b, long n,
+                                                     double* __restrict__ part) {
+  double v[1] = {0.0};
+  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) v[0] += a[i] * b[i];
+  block_partial<1>(v, part);
+}
+
 static inline int vgrid(long n) {
   long g = (n + kBlock - 1) / kBlock;
   if (g > 8192) g = 8192;
@@ -61,12 +71,11 @@ int vmaxabs(const Ctx& c, const double* a, long n, double* out, hipStream_t s) {
 }
 
 // ----------------------------------------------------------------------------- PCG kernels
-// r = b - A x0 (A x0 in q); z = M0 r handled by the preconditioner; here: r, r.r and b.b.
+// r = b - A x0 (A x0 in q); partials of r.r and b.b (2 per workgroup) for the host.
 // t / tf (the preconditioner's first apply output) are cleared on the way.
 __global__ __launch_bounds__(kBlock) void k_pcg_init(const double* __restrict__ b, const double* __restrict__ q,
                                                      double* __restrict__ r, double* __restrict__ t,
-                                                     float* __restrict__ tf, long n, double* __restrict__ partial,
-                                                     unsigned* __restrict__ counter, double* __restrict__ out) {
+                                                     float* __restrict__ tf, long n, double* __restrict__ part_rr) {
   double v[2] = {0.0, 0.0};
   for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
     const double ri = b[i] - q[i];
@@ -76,17 +85,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(const double* __restrict__ 
     v[0] += ri * ri;
     v[1] += b[i] * b[i];
   }
-  reduce_store<2, false>(v, partial, counter, out);  // out[0] = r.r, out[1] = b.b
+  block_partial<2>(v, part_rr);
 }
 
-// alpha = rz / (p.q + 1e-30); x += alpha p; r -= alpha q; out = r.r; t / tf cleared
-__global__ __launch_bounds__(kBlock) void k_pcg_xr(const double* __restrict__ sc, int rz_slot, int pq_slot,
+// alpha = rz / (p.q + 1e-30) with p.q summed from n_pq partials; x += alpha p; r -= alpha q;
+// partials of r.r; t / tf cleared
+__global__ __launch_bounds__(kBlock) void k_pcg_xr(const double* __restrict__ sc, int rz_slot,
+                                                   const double* __restrict__ part_pq, int n_pq,
                                                    double* __restrict__ x, double* __restrict__ r,
                                                    const double* __restrict__ p, const double* __restrict__ q,
                                                    double* __restrict__ t, float* __restrict__ tf, long n,
-                                                   double* __restrict__ partial, unsigned* __restrict__ counter,
-                                                   double* __restrict__ out) {
-  const double alpha = sc[rz_slot] / (sc[pq_slot] + 1e-30);
+                                                   double* __restrict__ part_rr) {
+  const double alpha = sc[rz_slot] / (sum_partials_all<kBlock>(part_pq, n_pq) + 1e-30);
   double v[1] = {0.0};
   for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
     if (t) t[i] = 0.0;
@@ -96,14 +106,19 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xr(const double* __restrict__ sc
     r[i] = ri;
     v[0] += ri * ri;
   }
-  reduce_store<1, false>(v, partial, counter, out);
+  block_partial<1>(v, part_rr);
 }
 
-// beta = rz_new / (rz_old + 1e-30); p = z + beta p  (first == 1: p = z); q = 0 for the next apply
-__global__ void k_pcg_p(const double* __restrict__ sc, int new_slot, int old_slot, int first,
-                        const double* __restrict__ z, double* __restrict__ p, double* __restrict__ q, long n) {
-  const double beta = first ? 0.0 : sc[new_slot] / (sc[old_slot] + 1e-30);
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+// rz_new = sum of n_rz partials, stored to sc[new_slot] by workgroup 0;
+// beta = rz_new / (sc[old_slot] + 1e-30); p = z + beta p (first: p = z); q = 0 for the next apply
+__global__ __launch_bounds__(kBlock) void k_pcg_p(double* __restrict__ sc, const double* __restrict__ part_rz,
+                                                  int n_rz, int new_slot, int old_slot, int first,
+                                                  const double* __restrict__ z, double* __restrict__ p,
+                                                  double* __restrict__ q, long n) {
+  const double rz = sum_partials_all<kBlock>(part_rz, n_rz);
+  const double beta = first ? 0.0 : rz / (sc[old_slot] + 1e-30);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[new_slot] = rz;
+  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
     p[i] = first ? z[i] : z[i] + beta * p[i];
     q[i] = 0.0;
   }
@@ -118,18 +133,17 @@ __global__ void k_scale(const double* __restrict__ M, const double* __restrict__
   }
 }
 
-// z = x + M .* (b - t), and out = r.z with r == b (level-0 post-smoothing fused with the dot)
+// z = x + M .* (b - t), and partials of r.z with r == b (level-0 post-smoothing fused with the dot)
 __global__ __launch_bounds__(kBlock) void k_post_dot(const double* __restrict__ x, const double* __restrict__ M,
                                                      const double* __restrict__ b, const double* __restrict__ t,
-                                                     double* __restrict__ z, long n, double* __restrict__ partial,
-                                                     unsigned* __restrict__ counter, double* __restrict__ out) {
+                                                     double* __restrict__ z, long n, double* __restrict__ part) {
   double v[1] = {0.0};
   for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
     const double zi = x[i] + M[i] * (b[i] - t[i]);
     z[i] = zi;
     v[0] += b[i] * zi;
   }
-  reduce_store<1, false>(v, partial, counter, out);
+  block_partial<1>(v, part);
 }
 
 // x += M .* (b - t)
@@ -278,19 +292,17 @@ __global__ __launch_bounds__(kBlock) void k_gemv_f(int n, const float* __restric
   if (lane == 0) y[row] = s;
 }
 
-// z = x + M (r - t) in FP64 from FP32 x, M, t and FP64 r; out = r.z (FP64)
+// z = x + M (r - t) in FP64 from FP32 x, M, t and FP64 r; partials of r.z (FP64)
 __global__ __launch_bounds__(kBlock) void k_post_dot_mixed(const float* __restrict__ x, const float* __restrict__ M,
                                                            const double* __restrict__ r, const float* __restrict__ t,
-                                                           double* __restrict__ z, long n,
-                                                           double* __restrict__ partial,
-                                                           unsigned* __restrict__ counter, double* __restrict__ out) {
+                                                           double* __restrict__ z, long n, double* __restrict__ part) {
   double v[1] = {0.0};
   for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
     const double zi = (double)x[i] + (double)M[i] * (r[i] - (double)t[i]);
     z[i] = zi;
     v[0] += r[i] * zi;
   }
-  reduce_store<1, false>(v, partial, counter, out);
+  block_partial<1>(v, part);
 }
 
 // --------------------------------------------------------------------- apply timing
@@ -307,15 +319,15 @@ int resolve_apply_timing(Ctx& c) {
   return 0;
 }
 
-int timed_apply(Ctx& c, const double* x, double* y, hipStream_t s) {
-  if (!c.time_apply) return launch_apply(c, c.geom_mode, x, y, s);
+int timed_apply(Ctx& c, const double* x, double* y, hipStream_t s, double* dot_part) {
+  if (!c.time_apply) return launch_apply(c, c.geom_mode, x, y, s, dot_part);
   if (c.ev_pool.empty()) {
     c.ev_pool.resize(512);
     for (auto& e : c.ev_pool) MM_HIP(hipEventCreate(&e));
   }
   if (2 * c.ev_used >= (int)c.ev_pool.size()) MM_TRY(resolve_apply_timing(c));
   MM_HIP(hipEventRecord(c.ev_pool[2 * c.ev_used], s));
-  MM_TRY(launch_apply(c, c.geom_mode, x, y, s));
+  MM_TRY(launch_apply(c, c.geom_mode, x, y, s, dot_part));
   MM_HIP(hipEventRecord(c.ev_pool[2 * c.ev_used + 1], s));
   ++c.ev_used;
   return 0;
@@ -373,7 +385,7 @@ static void vcycle_coarse_f(Ctx& c, int l, hipStream_t s) {
 // Mixed-precision preconditioner: the whole V-cycle in FP32 (FP32 matrix-free fine applies,
 // FP32 coarse matrices and dense inverse); input r and output z in FP64, r.z in FP64.
 // The caller has cleared ptf.
-static int precond_f32(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
+static int precond_f32(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s) {
   const long N = c.N;
   Level& L0 = c.lv[0];
   Level& L1 = c.lv[1];
@@ -382,22 +394,22 @@ static int precond_f32(Ctx& c, const double* r, double* z, double* rz_out, hipSt
   vcycle_coarse_f(c, 1, s);
   spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Pxf, L1.xsf, (const float*)nullptr, c.px0f, s, c.ptf);
   MM_TRY(launch_apply_f32(c, c.px0f, c.ptf, s));                                  // t = A x
-  k_post_dot_mixed<<<red_grid(N), kBlock, 0, s>>>(c.px0f, L0.Mf, r, c.ptf, z, N, c.partial, c.counter + 4, rz_out);
+  k_post_dot_mixed<<<red_grid(N), kBlock, 0, s>>>(c.px0f, L0.Mf, r, c.ptf, z, N, rz_part);
   c.n_vcycle += 1;
   return 0;
 }
 
-// z = V-cycle(r) with the matrix-free level 0; also writes r.z into out.
-int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
+// z = V-cycle(r) with the matrix-free level 0; also writes red_grid(N) partials of r.z.
+int precond(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s) {
   const long N = c.N;
   Level& L0 = c.lv[0];
-  if (c.precision == 1 && c.lv.size() > 1 && L0.Mf != nullptr) return precond_f32(c, r, z, rz_out, s);
+  if (c.precision == 1 && c.lv.size() > 1 && L0.Mf != nullptr) return precond_f32(c, r, z, rz_part, s);
   if (c.lv.size() <= 1) {
     if (L0.dense_inv)  // single-level hierarchy: exact dense solve
       k_gemv<<<(int)((N * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>((int)N, L0.dense_inv, r, z);
     else  // Jacobi on diag(A)
       k_scale<<<vgrid(N), kBlock, 0, s>>>(L0.M, r, z, nullptr, N);
-    vdot(c, r, z, N, rz_out, s);
+    k_dot_part<<<red_grid(N), kBlock, 0, s>>>(r, z, N, rz_part);
     return 0;
   }
   Level& L1 = c.lv[1];
@@ -407,58 +419,70 @@ int precond(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
   vcycle_coarse(c, 1, s);
   spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Px, L1.xs, (const double*)nullptr, c.px0, s, c.pt); // x += P0 e1, t = 0
   MM_TRY(timed_apply(c, c.px0, c.pt, s));                                         // t = A x
-  k_post_dot<<<red_grid(N), kBlock, 0, s>>>(c.px0, L0.M, r, c.pt, z, N, c.partial, c.counter + 4, rz_out);
+  k_post_dot<<<red_grid(N), kBlock, 0, s>>>(c.px0, L0.M, r, c.pt, z, N, rz_part);
   c.n_apply += 2;
   c.n_vcycle += 1;
   return 0;
 }
 
-int precond_apply(Ctx& c, const double* r, double* z, double* rz_out, hipStream_t s) {
+int precond_apply(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s) {
   MM_CHECK(c.lv.size() >= 1 && c.lv[0].M != nullptr, "preconditioner not set up");
   vfill(c.pt, 0.0, c.N, s);
   if (c.ptf) MM_HIP(hipMemsetAsync(c.ptf, 0, c.N * sizeof(float), s));
-  return precond(c, r, z, rz_out, s);
+  return precond(c, r, z, rz_part, s);
 }
 
 // Device-resident PCG: U (in/out, device) solves A U = b (device), stop when
-// it >= maxiter or r.r <= tol^2 b.b. Returns iterations and r.r / b.b via the host mirror.
+// it >= maxiter or r.r <= tol^2 b.b (make_pcg_solve, src/poisson_solve.py:422-478).
+// Returns iterations, r.r and b.b; r.r is summed on the host from the per-workgroup
+// partials in a fixed order.
 int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it_out, double* r2_out,
             double* bb_out) {
   hipStream_t s = c.stream;
   const long N = c.N;
   MM_CHECK(c.lv.size() >= 1 && c.lv[0].M != nullptr, "preconditioner not set up (call mm_amg_upload or mm_precond_jacobi)");
-  double* sc = c.dscal;
-  // slots: 0/1 rz ping-pong, 2 p.q, 3 r.r, 4 b.b
+  double* sc = c.dscal;  // slots 0/1: r.z ping-pong
+  const int gv = red_grid(N);
+  // p.Ap from the apply's per-block element energies (on-the-fly geometry), else a dot pass
+  const bool fused = c.geom_mode == kGeomCoords;
+  const int n_pq = fused ? 8 * ((c.part_all.nblk + 7) / 8) : gv;
   vfill(c.pq, 0.0, N, s);
   MM_TRY(timed_apply(c, U, c.pq, s));
   c.n_apply++;
   float* tf = c.precision == 1 ? c.ptf : nullptr;
   double* t64 = c.precision == 1 ? nullptr : c.pt;
-  k_pcg_init<<<red_grid(N), kBlock, 0, s>>>(b, c.pq, c.pr, t64, tf, N, c.partial, c.counter + 5, sc + 3);
-  MM_TRY(precond(c, c.pr, c.pz, sc + 0, s));
-  k_pcg_p<<<vgrid(N), kBlock, 0, s>>>(sc, 0, 0, 1, c.pz, c.pp, c.pq, N);
-  MM_HIP(hipMemcpyAsync(c.hscal + 3, sc + 3, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  k_pcg_init<<<gv, kBlock, 0, s>>>(b, c.pq, c.pr, t64, tf, N, c.part_rr);
+  MM_HIP(hipMemcpyAsync(c.hpart, c.part_rr, 2 * gv * sizeof(double), hipMemcpyDeviceToHost, s));
+  MM_TRY(precond(c, c.pr, c.pz, c.part_rz, s));
+  k_pcg_p<<<gv, kBlock, 0, s>>>(sc, c.part_rz, gv, 0, 0, 1, c.pz, c.pp, c.pq, N);
   MM_HIP(hipStreamSynchronize(s));
-  const double bb = c.hscal[4];
+  double r2 = 0.0, bb = 0.0;
+  for (int i = 0; i < gv; ++i) {
+    r2 += c.hpart[2 * i];
+    bb += c.hpart[2 * i + 1];
+  }
   const double tol2 = tol * tol * bb;
-  double r2 = c.hscal[3];
   int it = 0;
   int cur = 0;  // slot holding the current r.z
   while (it < maxiter && r2 > tol2) {
-    MM_TRY(timed_apply(c, c.pp, c.pq, s));  // q was cleared by k_pcg_p
+    if (fused) {
+      MM_TRY(timed_apply(c, c.pp, c.pq, s, c.part_pq));  // q was cleared by k_pcg_p
+    } else {
+      MM_TRY(timed_apply(c, c.pp, c.pq, s));
+      k_dot_part<<<gv, kBlock, 0, s>>>(c.pp, c.pq, N, c.part_pq);
+    }
     c.n_apply++;
-    vdot(c, c.pp, c.pq, N, sc + 2, s);
-    k_pcg_xr<<<red_grid(N), kBlock, 0, s>>>(sc, cur, 2, U, c.pr, c.pp, c.pq, t64, tf, N, c.partial, c.counter + 5,
-                                             sc + 3);
-    MM_HIP(hipMemcpyAsync(c.hscal + 3, sc + 3, sizeof(double), hipMemcpyDeviceToHost, s));
+    k_pcg_xr<<<gv, kBlock, 0, s>>>(sc, cur, c.part_pq, n_pq, U, c.pr, c.pp, c.pq, t64, tf, N, c.part_rr);
+    MM_HIP(hipMemcpyAsync(c.hpart, c.part_rr, gv * sizeof(double), hipMemcpyDeviceToHost, s));
     MM_HIP(hipEventRecord(c.ev, s));
     // preconditioner and direction update run while the host waits for r.r
-    MM_TRY(precond(c, c.pr, c.pz, sc + (1 - cur), s));
-    k_pcg_p<<<vgrid(N), kBlock, 0, s>>>(sc, 1 - cur, cur, 0, c.pz, c.pp, c.pq, N);
+    MM_TRY(precond(c, c.pr, c.pz, c.part_rz, s));
+    k_pcg_p<<<gv, kBlock, 0, s>>>(sc, c.part_rz, gv, 1 - cur, cur, 0, c.pz, c.pp, c.pq, N);
     cur = 1 - cur;
     ++it;
     MM_HIP(hipEventSynchronize(c.ev));
-    r2 = c.hscal[3];
+    r2 = 0.0;
+    for (int i = 0; i < gv; ++i) r2 += c.hpart[i];
   }
   MM_HIP(hipStreamSynchronize(s));
   MM_TRY(resolve_apply_timing(c));
