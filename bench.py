@@ -100,9 +100,12 @@ def initial_m(N, seed=0, sigma=0.01):
 
 
 def apply_algorithmic_bytes(ctx) -> int:
-    """Bytes one on-the-fly Poisson apply must move if every stream is touched once:
-    packed corner indices (4 B/tet), node lists (4 B/entry), node coordinates (24 B/node),
-    x (8 B/node) and y (8 B/node)."""
+    """Bytes one on-the-fly Poisson apply must move if every stream is touched once: node
+    coordinates (24 B/node), x (8 B/node), y (8 B/node), the blocks' node lists (4 B/entry)
+    and the work items: per edge-star cluster 3 words (12 B: 8 slot indices, tet and slot
+    masks), or per tet 4 packed corner indices (4 B)."""
+    if ctx.apply_shape["clusters"]:
+        return 12 * ctx.n_clusters + 4 * ctx.cluster_node_list_entries + 40 * ctx.N
     return 4 * ctx.E + 4 * ctx.node_list_entries + 40 * ctx.N
 
 
@@ -165,6 +168,7 @@ def cpu_baseline(mesh, mats, steps_T, budget_s=60.0):
 def run_grains(args, log):
     """The 64-grain set: one context (mesh, AMG hierarchy, resident state) per grain; a field
     step is ``minimize_batched`` over all grains."""
+    from mumag_matrixfree2_amd.autotune import tune_apply
     from mumag_matrixfree2_amd.context import MeshContext, make_params, minimize_batched
     from mumag_matrixfree2_amd.hysteresis_loop import field_values, switching_step
     from mumag_matrixfree2_amd.mesh import grain_set
@@ -179,6 +183,7 @@ def run_grains(args, log):
         c = MeshContext(g["knt"], g["ijk"][:, :4], g["ijk"][:, 4], A, K1, Js, k, boundary_mask=g["boundary_mask"])
         c.setup_precond(args.precond, max_coarse=args.max_coarse)
         c.set_precision(args.precision)
+        tune_apply(c)
         ctxs.append(c)
         if i % 8 == 7:
             log(f"[bench] {i + 1} grains set up, {time.perf_counter() - t0:.1f}s")
@@ -292,6 +297,10 @@ def main():
     log(f"[bench] context {time.perf_counter() - t0:.1f}s")
     ctx.setup_precond(args.precond, max_coarse=args.max_coarse, log=log)
     ctx.set_precision(args.precision)
+    from mumag_matrixfree2_amd.autotune import tune_apply
+
+    shape = tune_apply(ctx)
+    log(f"[bench] apply shape {shape}")
     log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: N={ctx.N} E={ctx.E} "
         f"AMG levels={[l['A'].shape[0] for l in ctx.amg_levels]}")
 
@@ -361,6 +370,8 @@ def main():
             "workload": f"{wl['name']}_{args.precision}_sweep+2..-8T_{len(B_vals)}steps",
             "tets": ctx.E, "nodes": ctx.N, "method": "pcohen_hs", "precond": f"SA-AMG V-cycle ({args.precond})",
             "amg_levels": [int(l["A"].shape[0]) for l in ctx.amg_levels],
+            "apply_shape": {"unit": "edge-star cluster" if shape["clusters"] else "element",
+                            "threads": shape["threads"], "block_nodes": ctx.block_nodes},
             "pcg_tol": 1e-10, "tau_f": 1e-10,
         },
         "pcg_iters_per_s": pcg_its / elapsed,

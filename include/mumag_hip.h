@@ -56,14 +56,16 @@ int mm_device_info(char* buf, int len);
  * k_easy (G,3); bnd_mask (N,) 1.0 interior / 0.0 Dirichlet, or NULL (no Dirichlet nodes);
  * reg: Poisson diagonal regularisation (src/amg_utils.py:73).
  * flags bit 0..1: geometry source of the Poisson apply (0 on-the-fly from coordinates,
- * 1 stored FP64 gradients, 2 stored FP32 gradients). */
+ * 1 stored FP64 gradients, 2 stored FP32 gradients); bits 8..16: distinct nodes per element
+ * block, 16..256 (0: 256). */
 int mm_ctx_create(int N, int E, const double* knt, const int32_t* conn, const int32_t* mat_id, int G,
                   const double* A_red, const double* K1_red, const double* Js_red, const double* k_easy,
                   const double* bnd_mask, double reg, int flags, int64_t* out_ctx);
 int mm_ctx_destroy(int64_t ctx);
 
-/* info[0..7] = V_mag, N, E, element blocks (all), element blocks (magnetic),
- * largest node list of a block, magnetic elements, node-list entries of all blocks */
+/* info[0..11] = V_mag, N, E, element blocks (all), element blocks (magnetic),
+ * largest node list of a block, magnetic elements, node-list entries of all blocks,
+ * cluster blocks, node-list entries of the cluster blocks, edge-star clusters, block node cap */
 int mm_ctx_info(int64_t ctx, double* info);
 
 /* Oriented connectivity (E,4), Ve (E,) and shape-function gradients of corners 1..3 (E,3,3)
@@ -84,6 +86,11 @@ int mm_set_precision(int64_t ctx, int precision);
 
 /* Select the geometry source of the Poisson apply (see flags of mm_ctx_create). */
 int mm_set_geometry_mode(int64_t ctx, int mode);
+
+/* Work shape of the on-the-fly Poisson apply: clusters != 0 runs it over edge-star clusters
+ * (see mm_edge_clusters), 0 per element; threads = workgroup size (64, 128 or 256). Results
+ * agree to rounding; the choice is a tuning knob with no reference counterpart. */
+int mm_set_apply_shape(int64_t ctx, int clusters, int threads);
 
 /* y = A x: assembled-mode apply_A (src/poisson_solve.py:166-171) computed matrix-free. */
 int mm_poisson_apply(int64_t ctx, const double* x, double* y);
@@ -153,6 +160,15 @@ int mm_solve_U_batched(int n, const int64_t* ctxs, const double* const* m, doubl
  * S is the strength graph in CSR (n rows); agg[i] receives the aggregate of node i or -1 for
  * an isolated node; n_agg the number of aggregates. No device work. */
 int mm_sa_aggregate(int n, const int32_t* Sp, const int32_t* Sj, int32_t* agg, int32_t* n_agg);
+
+/* Host-side grouping of the tets into edge-star clusters, the unit of work of the Poisson
+ * apply: up to 6 tets sharing one edge on 8 node slots [E0, E1, R0..R5], tet j = (E0, E1, Rj,
+ * R(j+1) mod 6) when bit j of tets[i] is set; unused slots are -1. nodes8 (at least E x 8)
+ * and tets (at least E) receive the clusters, n_clusters their count. No device work; no
+ * reference counterpart (the reference applies the operator per element or as CSR,
+ * src/poisson_solve.py:166-171,210-252). */
+int mm_edge_clusters(int N, int E, const double* knt, const int32_t* conn, int32_t* nodes8, uint32_t* tets,
+                     int32_t* n_clusters);
 
 /* Live timing of the Poisson apply kernel inside the PCG: HIP events on the context's stream
  * bracket every launch. Returns the accumulated milliseconds and launch count since the last

@@ -69,6 +69,7 @@ SIGNATURES = {
     "mm_ctx_get_nodal": [C.c_int64, f64p, f64p, f64p],
     "mm_set_geometry_mode": [C.c_int64, C.c_int],
     "mm_set_precision": [C.c_int64, C.c_int],
+    "mm_set_apply_shape": [C.c_int64, C.c_int, C.c_int],
     "mm_poisson_apply": [C.c_int64, f64p, f64p],
     "mm_poisson_rhs": [C.c_int64, f64p, f64p],
     "mm_poisson_diag": [C.c_int64, f64p],
@@ -92,6 +93,7 @@ SIGNATURES = {
     "mm_solve_U_batched": [C.c_int, C.POINTER(C.c_int64), C.POINTER(f64p), C.POINTER(f64p), f64p, C.c_int, C.c_int,
                            i32p, f64p],
     "mm_sa_aggregate": [C.c_int, i32p, i32p, i32p, i32p],
+    "mm_edge_clusters": [C.c_int, C.c_int, f64p, i32p, i32p, C.POINTER(C.c_uint32), i32p],
     "mm_apply_timing": [C.c_int64, C.c_int, f64p, i64p],
     "mm_counters": [C.c_int64, i64p],
     "mm_bench_apply": [C.c_int64, C.c_int, C.c_int, f64p],

@@ -31,10 +31,11 @@ class MeshContext:
         boundary_mask: (N,) 1.0 interior / 0.0 Dirichlet, or None.
         reg: Poisson diagonal regularisation.
         geom: geometry source of the Poisson apply ('on_the_fly', 'stored_f64', 'stored_f32').
+        block_nodes: distinct nodes per element block (16..256), the LDS staging size.
     """
 
     def __init__(self, knt, conn, mat_id, A_red, K1_red, Js_red, k_easy, boundary_mask=None, reg=1e-12,
-                 geom="on_the_fly"):
+                 geom="on_the_fly", block_nodes: int = 256):
         self.knt = f64(knt).reshape(-1, 3)
         conn = i32(np.asarray(conn)[:, :4])
         mat_id = i32(mat_id)
@@ -54,10 +55,13 @@ class MeshContext:
         self.reg = float(reg)
         self.Js_red = Js_red
         h = C.c_int64(0)
+        if not 16 <= int(block_nodes) <= 256:
+            raise ValueError("block_nodes must be in [16, 256]")
+        flags = GEOM_MODES[geom] | (int(block_nodes) % 256) << 8
         call("mm_ctx_create", self.N, self.E, ptr(self.knt), ptr(conn), ptr(mat_id), G, ptr(A_red), ptr(K1_red),
-             ptr(Js_red), ptr(k_easy), ptr(bnd), self.reg, GEOM_MODES[geom], C.byref(h))
+             ptr(Js_red), ptr(k_easy), ptr(bnd), self.reg, flags, C.byref(h))
         self.handle = h.value
-        info = np.zeros(8)
+        info = np.zeros(12)
         call("mm_ctx_info", self.handle, ptr(info))
         self.V_mag = float(info[0])
         self.n_blocks_all = int(info[3])
@@ -65,6 +69,11 @@ class MeshContext:
         self.max_block_nodes = int(info[5])
         self.n_mag_elems = int(info[6])
         self.node_list_entries = int(info[7])
+        self.n_blocks_cl = int(info[8])
+        self.cluster_node_list_entries = int(info[9])
+        self.n_clusters = int(info[10])
+        self.block_nodes = int(info[11])
+        self.apply_shape = {"clusters": True, "threads": 256}
         self.precond_type = None
         self.amg_levels = None
         self.precision = "fp64"
@@ -108,6 +117,12 @@ class MeshContext:
         """'fp64' or 'mixed' (FP32 V-cycle under an FP64 PCG)."""
         call("mm_set_precision", self.handle, {"fp64": 0, "mixed": 1}[precision])
         self.precision = precision
+
+    def set_apply_shape(self, clusters: bool = True, threads: int = 256):
+        """Work shape of the on-the-fly Poisson apply: edge-star clusters or single elements per
+        thread, and the workgroup size (64, 128, 256)."""
+        call("mm_set_apply_shape", self.handle, int(bool(clusters)), int(threads))
+        self.apply_shape = {"clusters": bool(clusters), "threads": int(threads)}
 
     def set_geometry_mode(self, geom: str):
         call("mm_set_geometry_mode", self.handle, GEOM_MODES[geom])

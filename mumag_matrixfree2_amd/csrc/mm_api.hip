@@ -162,6 +162,235 @@ static void build_partition(int N, const std::vector<int>& conn, const std::vect
     if (touches[f & kIdMask] == 1) f |= kPrivate;
 }
 
+// ------------------------------------------------------------------ edge-star clusters
+// The Poisson apply reads every corner's coordinates and value from LDS; per tet that is
+// 16 doubles plus 4 LDS atomics, and LDS bandwidth is what bounds the per-tet kernel.
+// Tets that share an edge share its two nodes, and neighbours around the edge share a
+// third, so the apply processes clusters: up to 6 tets on 8 node slots [E0, E1, R0..R5],
+// tet j having corners (E0, E1, Rj, R(j+1) mod 6). One thread loads the 8 slots once,
+// keeps their sums in registers and issues 8 LDS atomics for the whole cluster.
+// Tets are grouped by their longest edge (ties: the smaller node pair); on a six-tet hex
+// split that edge is the hex diagonal shared by all six tets, so a cluster is a whole hex.
+// Around the edge, tets that share a face are chained so consecutive tets share a slot.
+struct HostCluster {
+  int node[8];      // global node ids, -1 for an unused slot
+  uint32_t tets;    // bit j: tet j present
+};
+
+static void build_clusters(const double* knt, int E, const std::vector<int>& conn, std::vector<HostCluster>& out) {
+  std::vector<std::pair<uint64_t, int>> key(E);
+  for (int e = 0; e < E; ++e) {
+    const int* v = &conn[4 * (size_t)e];
+    double best = -1.0;
+    uint64_t bk = 0;
+    for (int a = 0; a < 4; ++a)
+      for (int b = a + 1; b < 4; ++b) {
+        double l2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          const double d = knt[3 * (size_t)v[a] + k] - knt[3 * (size_t)v[b] + k];
+          l2 += d * d;
+        }
+        const uint64_t lo = (uint64_t)std::min(v[a], v[b]), hi = (uint64_t)std::max(v[a], v[b]);
+        const uint64_t k2 = (lo << 32) | hi;
+        if (l2 > best || (l2 == best && k2 < bk)) {
+          best = l2;
+          bk = k2;
+        }
+      }
+    key[e] = {bk, e};
+  }
+  std::sort(key.begin(), key.end());
+  out.clear();
+  out.reserve(E / 4 + 16);
+  std::vector<int> T, opp0, opp1;
+  std::vector<char> used;
+  std::vector<std::vector<int>> chains;
+  for (size_t g0 = 0; g0 < key.size();) {
+    size_t g1 = g0;
+    while (g1 < key.size() && key[g1].first == key[g0].first) ++g1;
+    const int a = (int)(key[g0].first >> 32), b = (int)(key[g0].first & 0xffffffffu);
+    const int n = (int)(g1 - g0);
+    T.resize(n);
+    opp0.resize(n);
+    opp1.resize(n);
+    for (int i = 0; i < n; ++i) {
+      const int e = key[g0 + i].second;
+      T[i] = e;
+      int o[2], k = 0;
+      for (int q = 0; q < 4; ++q) {
+        const int v = conn[4 * (size_t)e + q];
+        if (v != a && v != b && k < 2) o[k++] = v;
+      }
+      opp0[i] = o[0];
+      opp1[i] = o[1];
+    }
+    // chains of face-sharing tets around the edge: ring-node sequences (tets = nodes - 1)
+    auto degree = [&](int v) {
+      int d = 0;
+      for (int i = 0; i < n; ++i) d += (opp0[i] == v) + (opp1[i] == v);
+      return d;
+    };
+    used.assign(n, 0);
+    chains.clear();
+    bool closed6 = false;
+    for (int left = n; left > 0;) {
+      int start = -1, snode = -1;
+      for (int i = 0; i < n && start < 0; ++i)
+        if (!used[i]) {
+          if (degree(opp0[i]) == 1) start = i, snode = opp0[i];
+          else if (degree(opp1[i]) == 1) start = i, snode = opp1[i];
+        }
+      if (start < 0)
+        for (int i = 0; i < n && start < 0; ++i)
+          if (!used[i]) start = i, snode = opp0[i];
+      std::vector<int> ch{snode};
+      int t = start, cur = snode;
+      while (t >= 0) {
+        used[t] = 1;
+        --left;
+        cur = (opp0[t] == cur) ? opp1[t] : opp0[t];
+        ch.push_back(cur);
+        t = -1;
+        for (int i = 0; i < n; ++i)
+          if (!used[i] && (opp0[i] == cur || opp1[i] == cur)) {
+            t = i;
+            break;
+          }
+      }
+      if (ch.size() == 7 && ch.front() == ch.back() && n == 6) closed6 = true;
+      chains.push_back(std::move(ch));
+    }
+    if (closed6) {
+      HostCluster hc;
+      hc.node[0] = a;
+      hc.node[1] = b;
+      for (int s2 = 0; s2 < 6; ++s2) hc.node[2 + s2] = chains[0][s2];
+      hc.tets = 63u;
+      out.push_back(hc);
+    } else {
+      // pieces of at most 5 tets (6 ring nodes), packed onto the 6 ring slots
+      HostCluster hc;
+      int used_slots = 0;
+      auto flush = [&]() {
+        if (used_slots == 0) return;
+        hc.node[0] = a;
+        hc.node[1] = b;
+        for (int s2 = used_slots; s2 < 6; ++s2) hc.node[2 + s2] = -1;
+        out.push_back(hc);
+        used_slots = 0;
+        hc.tets = 0;
+      };
+      hc.tets = 0;
+      for (const auto& ch : chains)
+        for (size_t p0 = 0; p0 + 1 < ch.size(); p0 += 5) {
+          const size_t p1 = std::min(ch.size(), p0 + 6);  // nodes [p0, p1)
+          const int len = (int)(p1 - p0);
+          if (used_slots + len > 6) flush();
+          for (int q = 0; q < len; ++q) hc.node[2 + used_slots + q] = ch[p0 + q];
+          for (int q = 0; q + 1 < len; ++q) hc.tets |= 1u << (used_slots + q);
+          used_slots += len;
+        }
+      flush();
+    }
+    g0 = g1;
+  }
+}
+
+// Greedy cut of the clusters (ordered by smallest, then largest node id) into blocks touching
+// at most node_cap distinct nodes, with the flags of build_partition. Per cluster: w0/w1 hold
+// the block-local indices of slots 0-3 / 4-7 (uint8 each), wm bits 0-5 the tets and bits 8-15
+// the used slots.
+struct HostCPart {
+  std::vector<int> cptr, nptr;
+  std::vector<uint32_t> nodes, w0, w1, wm;
+  int max_nodes = 0;
+};
+
+static void build_cluster_partition(int N, std::vector<HostCluster>& cl, const std::vector<char>& is_bnd, HostCPart& hp,
+                                    int node_cap) {
+  std::vector<std::pair<uint64_t, int>> key(cl.size());
+  for (size_t i = 0; i < cl.size(); ++i) {
+    int lo = INT32_MAX, hi = -1;
+    for (int q = 0; q < 8; ++q)
+      if (cl[i].node[q] >= 0) {
+        lo = std::min(lo, cl[i].node[q]);
+        hi = std::max(hi, cl[i].node[q]);
+      }
+    key[i] = {((uint64_t)lo << 32) | (uint64_t)hi, (int)i};
+  }
+  std::stable_sort(key.begin(), key.end());
+  std::vector<int> stamp(N, -1), local(N, 0), touches(N, 0);
+  std::vector<char> owned(N, 0);
+  std::vector<int> blk_nodes;
+  hp.cptr.assign(1, 0);
+  hp.nptr.assign(1, 0);
+  int blk = 0;
+  size_t c0 = 0;
+  std::vector<int> members;
+  auto close_block = [&]() {
+    const int nb = (int)blk_nodes.size();
+    std::vector<int> order(nb);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return blk_nodes[x] < blk_nodes[y]; });
+    std::vector<int> remap(nb);
+    for (int i = 0; i < nb; ++i) remap[order[i]] = i;
+    for (int ci : members) {
+      const HostCluster& h = cl[ci];
+      uint32_t w[2] = {0u, 0u}, m = h.tets;
+      for (int q = 0; q < 8; ++q)
+        if (h.node[q] >= 0) {
+          w[q >> 2] |= (uint32_t)remap[local[h.node[q]]] << (8 * (q & 3));
+          m |= 1u << (8 + q);
+        }
+      hp.w0.push_back(w[0]);
+      hp.w1.push_back(w[1]);
+      hp.wm.push_back(m);
+    }
+    for (int i = 0; i < nb; ++i) {
+      const int v = blk_nodes[order[i]];
+      uint32_t f = (uint32_t)v;
+      if (!owned[v]) {
+        owned[v] = 1;
+        f |= kOwner;
+      }
+      if (is_bnd[v]) f |= kBoundary;
+      touches[v]++;
+      hp.nodes.push_back(f);
+    }
+    hp.max_nodes = std::max(hp.max_nodes, nb);
+    hp.cptr.push_back((int)hp.w0.size());
+    hp.nptr.push_back((int)hp.nodes.size());
+    blk_nodes.clear();
+    members.clear();
+    ++blk;
+  };
+  for (size_t k = 0; k < key.size(); ++k) {
+    const HostCluster& h = cl[key[k].second];
+    int fresh = 0;
+    for (int q = 0; q < 8; ++q) {
+      const int v = h.node[q];
+      if (v < 0) continue;
+      bool dup = false;
+      for (int q2 = 0; q2 < q; ++q2) dup |= h.node[q2] == v;
+      if (!dup && stamp[v] != blk) ++fresh;
+    }
+    if ((int)blk_nodes.size() + fresh > node_cap || (int)members.size() >= kMaxBlockElems) close_block();
+    for (int q = 0; q < 8; ++q) {
+      const int v = h.node[q];
+      if (v >= 0 && stamp[v] != blk) {
+        stamp[v] = blk;
+        local[v] = (int)blk_nodes.size();
+        blk_nodes.push_back(v);
+      }
+    }
+    members.push_back(key[k].second);
+  }
+  if (!blk_nodes.empty()) close_block();
+  for (auto& f : hp.nodes)
+    if (touches[f & kIdMask] == 1) f |= kPrivate;
+  (void)c0;
+}
+
 static int upload_part(Ctx& c, const HostPart& hp, int E, Part& p) {
   p.nblk = (int)hp.eptr.size() - 1;
   p.E = E;
@@ -194,6 +423,7 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   c.G = G;
   c.reg = reg;
   c.geom_mode = flags & 3;
+  c.node_cap = (flags >> 8) & 511 ? (flags >> 8) & 511 : kMaxBlockNodes;
   MM_CHECK(c.geom_mode <= 2, "invalid geometry mode");
   MM_HIP(hipGetDevice(&c.device));
   MM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -243,10 +473,10 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
     for (int k = 0; k < 3; ++k) c.hmt.k[g][k] = k_easy[3 * g + k];
   }
 
-  // nodes per element block (<= 256, the LDS staging capacity); MM_BLOCK_NODES overrides it
-  int node_cap = kMaxBlockNodes;
-  if (const char* env = std::getenv("MM_BLOCK_NODES")) node_cap = std::max(16, std::min(kMaxBlockNodes, std::atoi(env)));
-  if (const char* env = std::getenv("MM_ELEM_THREADS")) c.elem_threads = std::atoi(env);
+  // distinct nodes per element block: flags bits 8..16 (0: 256, the LDS staging capacity)
+  int node_cap = (flags >> 8) & 511;
+  if (node_cap == 0) node_cap = kMaxBlockNodes;
+  MM_CHECK(node_cap >= 16 && node_cap <= kMaxBlockNodes, "block node cap (flags bits 8..16) must be in [16, 256]");
   HostPart hp_all, hp_mag;
   build_partition(N, conn, mat0, order, is_bnd, hp_all, node_cap);
   std::vector<int> mag_elems;
@@ -259,6 +489,30 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   build_partition(N, conn, mat0, mag_elems, no_bnd, hp_mag, node_cap);
 
   MM_TRY(upload_part(c, hp_all, E, c.part_all));
+  {
+    std::vector<HostCluster> cl;
+    build_clusters(knt, E, conn, cl);
+    HostCPart hc;
+    build_cluster_partition(N, cl, is_bnd, hc, node_cap);
+    Part& q = c.part_cl;
+    q.nblk = (int)hc.cptr.size() - 1;
+    q.E = (int)hc.w0.size();
+    int *cp, *np;
+    uint32_t *nodes, *w0, *w1, *wm;
+    MM_TRY(upload(c, &cp, hc.cptr.data(), hc.cptr.size()));
+    MM_TRY(upload(c, &np, hc.nptr.data(), hc.nptr.size()));
+    MM_TRY(upload(c, &nodes, hc.nodes.data(), hc.nodes.size()));
+    MM_TRY(upload(c, &w0, hc.w0.data(), hc.w0.size()));
+    MM_TRY(upload(c, &w1, hc.w1.data(), hc.w1.size()));
+    MM_TRY(upload(c, &wm, hc.wm.data(), hc.wm.size()));
+    q.eptr = cp;
+    q.nptr = np;
+    q.nodes = nodes;
+    q.lconn = w0;
+    q.lconn2 = w1;
+    q.cmask = wm;
+    c.nodes_cl = (long long)hc.nodes.size();
+  }
   c.nodes_all = (long long)hp_all.nodes.size();
   MM_TRY(upload_part(c, hp_mag, (int)mag_elems.size(), c.part_mag));
   c.max_block_nodes = std::max(hp_all.max_nodes, hp_mag.max_nodes);
@@ -347,7 +601,7 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   MM_TRY(dalloc(c, &c.dscal, kNumScal));
   MM_HIP(hipMemset(c.dscal, 0, kNumScal * sizeof(double)));
   MM_HIP(hipHostMalloc((void**)&c.hscal, kNumScal * sizeof(double), hipHostMallocDefault));
-  MM_TRY(dalloc(c, &c.part_pq, (size_t)8 * ((c.part_all.nblk + 7) / 8) + kRedGridMax));
+  MM_TRY(dalloc(c, &c.part_pq, (size_t)8 * ((std::max(c.part_all.nblk, c.part_cl.nblk) + 7) / 8) + kRedGridMax));
   MM_TRY(dalloc(c, &c.part_rr, 2 * (size_t)kRedGridMax));
   MM_TRY(dalloc(c, &c.part_rz, (size_t)kRedGridMax));
   MM_HIP(hipHostMalloc((void**)&c.hpart, 2 * kRedGridMax * sizeof(double), hipHostMallocDefault));
@@ -463,6 +717,10 @@ int mm_ctx_info(int64_t h, double* info) {
   info[5] = c.max_block_nodes;
   info[6] = c.n_mag;
   info[7] = c.nodes_all;
+  info[8] = c.part_cl.nblk;
+  info[9] = c.nodes_cl;
+  info[10] = c.part_cl.E;
+  info[11] = c.node_cap;
   return 0;
 }
 
@@ -500,6 +758,14 @@ int mm_set_geometry_mode(int64_t h, int mode) {
   GET_CTX(h);
   MM_CHECK(mode >= 0 && mode <= 2, "geometry mode must be 0, 1 or 2");
   c.geom_mode = mode;
+  return 0;
+}
+
+int mm_set_apply_shape(int64_t h, int clusters, int threads) {
+  GET_CTX(h);
+  MM_CHECK(threads == 64 || threads == 128 || threads == 256, "threads must be 64, 128 or 256");
+  c.apply_clusters = clusters != 0;
+  c.elem_threads = threads;
   return 0;
 }
 
@@ -868,6 +1134,22 @@ int mm_solve_U_batched(int n, const int64_t* ctxs, const double* const* m, doubl
   return run_batched(n, ctxs, n_threads, [&](int i) {
     return mm_solve_U(ctxs[i], m[i], U_inout[i], tol[i], maxiter, it + i, rel_res + i);
   });
+}
+
+int mm_edge_clusters(int N, int E, const double* knt, const int32_t* conn, int32_t* nodes8, uint32_t* tets,
+                     int32_t* n_clusters) {
+  MM_CHECK(N > 0 && E > 0 && knt && conn && nodes8 && tets && n_clusters, "bad arguments");
+  std::vector<int> cv(conn, conn + 4 * (size_t)E);
+  for (int v : cv) MM_CHECK(v >= 0 && v < N, "connectivity index out of range");
+  std::vector<HostCluster> cl;
+  build_clusters(knt, E, cv, cl);
+  MM_CHECK(cl.size() <= (size_t)E, "more clusters than tets");
+  for (size_t i = 0; i < cl.size(); ++i) {
+    for (int q = 0; q < 8; ++q) nodes8[8 * i + q] = cl[i].node[q];
+    tets[i] = cl[i].tets;
+  }
+  *n_clusters = (int32_t)cl.size();
+  return 0;
 }
 
 int mm_sa_aggregate(int n, const int32_t* Sp, const int32_t* Sj, int32_t* agg, int32_t* n_agg) {

@@ -79,6 +79,10 @@ struct Part {
   const uint32_t* nodes = nullptr;    // node ids | flags
   const uint32_t* lconn = nullptr;    // E packed local corners
   const uint8_t* mat = nullptr;       // E material index (0-based)
+  // edge-star cluster partition only: eptr are cluster offsets, lconn / lconn2 the local
+  // indices of slots 0-3 / 4-7 and cmask bits 0-5 the tets, bits 8-15 the used slots
+  const uint32_t* lconn2 = nullptr;
+  const uint32_t* cmask = nullptr;
 };
 
 struct Geo {
@@ -121,12 +125,16 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr;
   Part part_all, part_mag;
+  Part part_cl;                       // edge-star clusters of all elements (Poisson apply)
+  long long nodes_cl = 0;             // node-list entries of the cluster partition
+  bool apply_clusters = true;         // Poisson apply over clusters (else per element)
   Geo geo;
   int n_mag = 0;                      // magnetic elements
   long long nodes_all = 0;            // node-list entries of the all-element partition
   int max_block_nodes = 0;
   int geom_mode = kGeomCoords;        // geometry source of the Poisson apply
   int elem_threads = 256;             // workgroup size of the Poisson apply (64, 128, 256)
+  int node_cap = kMaxBlockNodes;      // distinct nodes per element block
   std::vector<void*> allocs;          // every device allocation, freed at destroy
   MatTable hmt;
 

@@ -241,6 +241,170 @@ __global__ __launch_bounds__(NT) void k_apply_f32(Part p, Geo geo, const float* 
   }
 }
 
+// ------------------------------------------- Poisson apply over edge-star clusters (K1)
+// Same operator as k_apply, one cluster per thread: up to 6 tets sharing the edge
+// (slot 0, slot 1), tet j with corners (slot 0, slot 1, slot 2+j, slot 2+(j+1)%6). The 8
+// slots' coordinates and values are read from LDS once, the edge vector c1 = x1 - x0 and
+// the cross products c1 x (x_s - x0) are shared by the tets, and the slot sums leave
+// through 8 LDS atomics. T = double: the FP64 operator (x, y FP64, DOT as in k_apply);
+// T = float: the FP32 operator of the mixed-precision V-cycle (block-relative FP32
+// coordinates, FP64 LDS accumulation). SCALED: input w .* r from the FP64 residual r.
+template <typename T, bool SCALED, bool DOT, int NT>
+__global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __restrict__ x,
+                                                 const double* __restrict__ r, const T* __restrict__ w,
+                                                 T* __restrict__ xo, T* __restrict__ y, T reg,
+                                                 double* __restrict__ dot_part) {
+  __shared__ T sx[kMaxBlockNodes];
+  __shared__ double sy[kMaxBlockNodes];
+  __shared__ T sX[3][kMaxBlockNodes];
+  const BlkInfo bi = blk_info(p);
+  if (!bi.valid) {
+    if (DOT && threadIdx.x == 0) dot_part[blockIdx.x] = 0.0;
+    return;
+  }
+  const int tid = threadIdx.x;
+  constexpr int kNode = kMaxBlockNodes / NT;
+  uint32_t fv[kNode];
+  const int c0 = bi.e0 + tid;
+  uint32_t cw0 = 0u, cw1 = 0u, cwm = 0u;
+  if (c0 < bi.e1) {
+    cw0 = p.lconn[c0];
+    cw1 = p.lconn2[c0];
+    cwm = p.cmask[c0];
+  }
+#pragma unroll
+  for (int q = 0; q < kNode; ++q) {
+    const int i = tid + q * NT;
+    fv[q] = i < bi.nb ? p.nodes[bi.n0 + i] : 0u;
+  }
+  double ox = 0.0, oy = 0.0, oz = 0.0;
+  if constexpr (sizeof(T) == 4) {
+    const uint32_t id0 = p.nodes[bi.n0] & kIdMask;
+    ox = geo.knt[3 * id0];
+    oy = geo.knt[3 * id0 + 1];
+    oz = geo.knt[3 * id0 + 2];
+  }
+#pragma unroll
+  for (int q = 0; q < kNode; ++q) {
+    const int i = tid + q * NT;
+    if (i < bi.nb) {
+      const uint32_t f = fv[q];
+      const uint32_t id = f & kIdMask;
+      T xv;
+      if constexpr (SCALED) {
+        xv = (T)(w[id] * r[id]);
+        if (f & kOwner) xo[id] = xv;
+      } else {
+        xv = x[id];
+      }
+      sx[i] = (f & kBoundary) ? (T)0 : xv;
+      sy[i] = 0.0;
+      sX[0][i] = (T)(geo.knt[3 * id + 0] - ox);
+      sX[1][i] = (T)(geo.knt[3 * id + 1] - oy);
+      sX[2][i] = (T)(geo.knt[3 * id + 2] - oz);
+    }
+  }
+  __syncthreads();
+  double acc_e = 0.0;
+  auto cluster = [&](uint32_t w0, uint32_t w1, uint32_t wm) {
+    int li[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      li[q] = (w0 >> (8 * q)) & 255u;
+      li[4 + q] = (w1 >> (8 * q)) & 255u;
+    }
+    T X[8][3], v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (wm & (1u << (8 + q))) {
+        X[q][0] = sX[0][li[q]];
+        X[q][1] = sX[1][li[q]];
+        X[q][2] = sX[2][li[q]];
+        v[q] = sx[li[q]];
+      } else {
+        X[q][0] = X[q][1] = X[q][2] = v[q] = (T)0;
+      }
+    }
+    // edge vectors from slot 0, value differences, shared cross products m_s = c1 x c_s
+    T c[8][3], dv[8], m[8][3];
+#pragma unroll
+    for (int q = 1; q < 8; ++q) {
+      c[q][0] = X[q][0] - X[0][0];
+      c[q][1] = X[q][1] - X[0][1];
+      c[q][2] = X[q][2] - X[0][2];
+      dv[q] = v[q] - v[0];
+    }
+#pragma unroll
+    for (int q = 2; q < 8; ++q) {
+      m[q][0] = c[1][1] * c[q][2] - c[1][2] * c[q][1];
+      m[q][1] = c[1][2] * c[q][0] - c[1][0] * c[q][2];
+      m[q][2] = c[1][0] * c[q][1] - c[1][1] * c[q][0];
+    }
+    T acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = (T)0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      if (wm & (1u << j)) {
+        const int s2 = 2 + j, s3 = 2 + (j + 1) % 6;
+        T n[3][3];
+        // corners (0, 1, s2, s3): n_1 = c_s2 x c_s3, n_2 = c_s3 x c_1 = -m_s3, n_3 = c_1 x c_s2 = m_s2
+        n[0][0] = c[s2][1] * c[s3][2] - c[s2][2] * c[s3][1];
+        n[0][1] = c[s2][2] * c[s3][0] - c[s2][0] * c[s3][2];
+        n[0][2] = c[s2][0] * c[s3][1] - c[s2][1] * c[s3][0];
+        n[1][0] = -m[s3][0];
+        n[1][1] = -m[s3][1];
+        n[1][2] = -m[s3][2];
+        n[2][0] = m[s2][0];
+        n[2][1] = m[s2][1];
+        n[2][2] = m[s2][2];
+        const T det = c[1][0] * n[0][0] + c[1][1] * n[0][1] + c[1][2] * n[0][2];
+        T ya[4];
+        const T en = tet_laplace(n, det, dv[1], dv[s2], dv[s3], ya);
+        if constexpr (DOT) acc_e += (double)en;
+        acc[0] += ya[0];
+        acc[1] += ya[1];
+        acc[s2] += ya[2];
+        acc[s3] += ya[3];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (wm & (1u << (8 + q))) atomicAdd(&sy[li[q]], (double)acc[q]);
+  };
+  if (c0 < bi.e1) cluster(cw0, cw1, cwm);
+  for (int cc = c0 + NT; cc < bi.e1; cc += NT) cluster(p.lconn[cc], p.lconn2[cc], p.cmask[cc]);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kNode; ++q) {
+    const int i = tid + q * NT;
+    if (i >= bi.nb) continue;
+    const uint32_t f = fv[q];
+    const uint32_t id = f & kIdMask;
+    if (f & kBoundary) {
+      if (f & kOwner) {
+        const T xb = SCALED ? (T)(w[id] * r[id]) : x[id];
+        y[id] = xb;
+        if constexpr (DOT) acc_e += (double)xb * (double)xb;
+      }
+      continue;
+    }
+    double v = sy[i];
+    if (f & kOwner) {
+      v += (double)reg * (double)sx[i];
+      if constexpr (DOT) acc_e += (double)reg * (double)sx[i] * (double)sx[i];
+    }
+    if (f & kPrivate)
+      y[id] = (T)v;
+    else
+      unsafeAtomicAdd(&y[id], (T)v);
+  }
+  if constexpr (DOT) {
+    double v1[1] = {acc_e};
+    block_partial<1, NT>(v1, dot_part);
+  }
+}
+
 // ------------------------------------------------------------- demag right-hand side (K2)
 // b_a = sum_e (Js_e Ve / 4) grad phi_a . (m_0 + m_1 + m_2 + m_3) over magnetic elements.
 __global__ __launch_bounds__(kBlock) void k_rhs(Part p, Geo geo, const double* __restrict__ m,
@@ -586,10 +750,25 @@ static void with_nt(int nt, F&& f) {
 }
 
 int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s, double* dot_part) {
+  MM_CHECK(dot_part == nullptr || geom == kGeomCoords, "x.Ax partials need the on-the-fly apply");
+  if (geom == kGeomCoords && c.apply_clusters) {
+    const Part& p = c.part_cl;
+    if (p.nblk == 0) return 0;
+    const int grid = grid_for(p);
+    with_nt(c.elem_threads, [&](auto nt) {
+      constexpr int NT = decltype(nt)::value;
+      if (dot_part)
+        k_apply_cl<double, false, true, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, c.reg,
+                                                                 dot_part);
+      else
+        k_apply_cl<double, false, false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, c.reg,
+                                                                  nullptr);
+    });
+    return 0;
+  }
   const Part& p = c.part_all;
   if (p.nblk == 0) return 0;
   const int grid = grid_for(p);
-  MM_CHECK(dot_part == nullptr || geom == kGeomCoords, "x.Ax partials need the on-the-fly apply");
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
     if (dot_part)
@@ -605,34 +784,44 @@ int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t
 }
 
 int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s) {
-  const Part& p = c.part_all;
+  const Part& p = c.apply_clusters ? c.part_cl : c.part_all;
   if (p.nblk == 0) return 0;
   const int grid = grid_for(p);
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
-    k_apply<kGeomCoords, true, false, NT><<<grid, NT, 0, s>>>(p, c.geo, r, w, xo, y, c.reg, nullptr);
+    if (c.apply_clusters)
+      k_apply_cl<double, true, false, NT><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, c.reg, nullptr);
+    else
+      k_apply<kGeomCoords, true, false, NT><<<grid, NT, 0, s>>>(p, c.geo, r, w, xo, y, c.reg, nullptr);
   });
   return 0;
 }
 
 int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s) {
-  const Part& p = c.part_all;
+  const Part& p = c.apply_clusters ? c.part_cl : c.part_all;
   if (p.nblk == 0) return 0;
   const int grid = grid_for(p);
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
-    k_apply_f32<false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, (float)c.reg);
+    if (c.apply_clusters)
+      k_apply_cl<float, false, false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, (float)c.reg,
+                                                              nullptr);
+    else
+      k_apply_f32<false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, (float)c.reg);
   });
   return 0;
 }
 
 int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float* xo, float* y, hipStream_t s) {
-  const Part& p = c.part_all;
+  const Part& p = c.apply_clusters ? c.part_cl : c.part_all;
   if (p.nblk == 0) return 0;
   const int grid = grid_for(p);
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
-    k_apply_f32<true, NT><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg);
+    if (c.apply_clusters)
+      k_apply_cl<float, true, false, NT><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg, nullptr);
+    else
+      k_apply_f32<true, NT><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg);
   });
   return 0;
 }
