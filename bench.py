@@ -127,10 +127,10 @@ def measured_traffic():
     return None
 
 
-def cpu_baseline(mesh, mats, steps_T, budget_s=60.0):
-    """The oracle (NumPy/SciPy restatement, kind 'port') on the same mesh, hierarchy and method,
-    timed on the host for the first field steps of the sweep."""
-    sys.path.insert(0, str(ROOT / "tests"))
+def cpu_baseline(mesh, mats, max_iter):
+    """The oracle (NumPy/SciPy restatement, kind 'port') on the same mesh, hierarchy and method:
+    the first ``max_iter`` minimiser iterations of field step 0 (+2 T), timed on the host.
+    Returns (iterations, seconds, PCG iterations)."""
     from mumag_matrixfree2_amd.amg import build_hierarchy
     from oracle.fem import Problem, spai0_diagonal
     from oracle.minimize import Minimizer
@@ -152,17 +152,9 @@ def cpu_baseline(mesh, mats, steps_T, budget_s=60.0):
         return solve_U(p, m, x0, tol, precond="amg", levels=levels, kind="amgcl")
 
     mn = Minimizer(p, sU, method="pcohen_hs", cg_tol=1e-10)
-    m = initial_m(p.N)
-    U = np.zeros(p.N)
-    done = 0
     t0 = time.perf_counter()
-    for BT in steps_T:
-        m, U, info = mn.minimize(m, BT / JS * H_DIR, U0=U, tau_f=1e-10)
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return done, dt
+    _, _, info = mn.minimize(initial_m(p.N), 2.0 / JS * H_DIR, U0=np.zeros(p.N), tau_f=1e-10, max_iter=max_iter)
+    return int(info["iters"]), time.perf_counter() - t0, int(info.get("demag_iters", 0))
 
 
 def run_grains(args, log):
@@ -251,7 +243,7 @@ def run_grains(args, log):
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "kernel": "k_apply<on_the_fly> over all grains (event time includes overlap with other grains)",
-            "avg_launch_us": t_apply / max(n_apply, 1) * 1e6, "launches": n_apply,
+            "avg_launch_us": t_apply / max(n_apply, 1) * 1e6, "timed_launches": n_apply,
         },
     }
     if seq is not None:
@@ -269,9 +261,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precond", default="amgcl", choices=["amgcl", "amg"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-iters", type=int, default=3,
+                    help="minimiser iterations of field step 0 timed for the CPU baseline")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--max-coarse", type=int, default=None)
+    ap.add_argument("--apply-shape", default="auto",
+                    help="'auto' (autotuned at set-up) or unit:threads, e.g. cluster:256, element:128")
     ap.add_argument("--threads", type=int, default=8, help="host threads driving the grains (grains64)")
     ap.add_argument("--compare-sequential", action="store_true",
                     help="grains64: also time the same steps with one thread")
@@ -299,7 +294,12 @@ def main():
     ctx.set_precision(args.precision)
     from mumag_matrixfree2_amd.autotune import tune_apply
 
-    shape = tune_apply(ctx)
+    if args.apply_shape == "auto":
+        shape = tune_apply(ctx)
+    else:
+        unit, nt = args.apply_shape.split(":")
+        ctx.set_apply_shape(unit == "cluster", int(nt))
+        shape = {"clusters": unit == "cluster", "threads": int(nt)}
     log(f"[bench] apply shape {shape}")
     log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: N={ctx.N} E={ctx.E} "
         f"AMG levels={[l['A'].shape[0] for l in ctx.amg_levels]}")
@@ -314,7 +314,7 @@ def main():
         ctx.state_set(m0, np.zeros(ctx.N))
         if timed:
             ctx.apply_timing(True)
-        J, E, rows = [], [], []
+        J, E, rows, iters = [], [], [], []
         t_start = time.perf_counter()
         for k in range(steps):
             ts = time.perf_counter()
@@ -322,10 +322,11 @@ def main():
             J.append(ctx.state_j_parallel(H_DIR))
             E.append(info["E"])
             rows.append(time.perf_counter() - ts)
+            iters.append(info["iters"])
             log(f"[bench:{precision}] step {k:2d} B={B_vals[k] * JS:+.3f} T J={J[-1] * JS:+.5f} T "
                 f"E={info['E']:+.10e} it={info['iters']} nf={info['evals']} icg={info['demag_iters']} "
                 f"t={rows[-1]:.3f}s")
-        return time.perf_counter() - t_start, np.array(J), np.array(E), rows
+        return time.perf_counter() - t_start, np.array(J), np.array(E), rows, iters
 
     # warm-up on a separate state
     ctx.set_precision(args.precision)
@@ -334,14 +335,14 @@ def main():
         ctx.minimize_resident(p, B_vals[k % len(B_vals)] * H_DIR)
 
     c0 = ctx.counters()
-    elapsed, J, Es, rows = sweep(args.precision, timed=True)
+    elapsed, J, Es, rows, iters = sweep(args.precision, timed=True)
     apply_ms, apply_n = ctx.apply_timing(False)
     c1 = ctx.counters()
     pcg_its = c1["pcg_iters"] - c0["pcg_iters"]
     errors = None
     if args.precision == "mixed":
         # reference FP64 sweep in the same run: per-step relative energy error and switching step
-        t64, J64, E64, _ = sweep("fp64", timed=False)
+        t64, J64, E64, _, _ = sweep("fp64", timed=False)
         errors = {
             "max_rel_energy_err_vs_fp64": float(np.max(np.abs(Es - E64) / np.maximum(np.abs(E64), 1.0))),
             "switching_step_fp64": switching_step(J64),
@@ -385,7 +386,7 @@ def main():
             "traffic": measured_traffic(),
             "kernel": "k_apply<on_the_fly> (matrix-free Poisson apply)",
             "avg_launch_us": avg_apply_s * 1e6,
-            "launches": apply_n,
+            "timed_launches": apply_n, "launches": c1["apply"] - c0["apply"],
             "algorithmic_bytes_per_launch": algo,
             "reference_layout_GBps": reference_layout_bytes(ctx) / avg_apply_s / 1e9,
         },
@@ -394,14 +395,14 @@ def main():
     if errors is not None:
         result["mixed_vs_fp64"] = errors
     if not args.no_cpu_baseline and args.workload == "cube2M":
-        cpu_T = [2.0 - 0.25 * k for k in range(args.cpu_steps)]
-        done, dt = cpu_baseline(mesh, mats, cpu_T)
-        gpu_same = sum(rows[:done])
+        it_cpu, dt, icg_cpu = cpu_baseline(mesh, mats, args.cpu_iters)
+        it_per_step = sum(iters) / len(iters)
         result["cpu_baseline"] = {
-            "value": done / dt, "unit": "field-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (NumPy/SciPy, single thread) pcohen_hs on the same mesh and AMG hierarchy, "
-                      f"first {done} field steps (+2.00 T ...), {dt:.1f}s; GPU on the same steps: "
-                      f"{done / max(gpu_same, 1e-12):.2f} field-steps/s",
+            "value": it_cpu / dt / it_per_step, "unit": "field-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (NumPy/SciPy, one thread) pcohen_hs on the same mesh and AMG hierarchy: the first "
+                      f"{it_cpu} minimiser iterations of field step 0 (+2 T; {icg_cpu} PCG iterations) took "
+                      f"{dt:.1f} s; scaled by the {it_per_step:.2f} minimiser iterations per field step of the "
+                      f"GPU sweep (GPU: {iters[0]} iterations of step 0 in {rows[0] * 1e3:.1f} ms)",
         }
     print(json.dumps(result), flush=True)
     ctx.close()

@@ -171,8 +171,9 @@ int mm_edge_clusters(int N, int E, const double* knt, const int32_t* conn, int32
                      int32_t* n_clusters);
 
 /* Live timing of the Poisson apply kernel inside the PCG: HIP events on the context's stream
- * bracket every launch. Returns the accumulated milliseconds and launch count since the last
- * call, then resets them and enables (enable != 0) or disables the timing. */
+ * bracket every 8th launch. Returns the accumulated milliseconds and count of the timed
+ * launches since the last call, then resets them and enables (enable != 0) or disables the
+ * timing. */
 int mm_apply_timing(int64_t ctx, int enable, double* total_ms, int64_t* count);
 
 /* Counters of kernel launches since creation: applies, rhs, energy, local, V-cycles, PCG iterations. */

@@ -604,7 +604,10 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   MM_TRY(dalloc(c, &c.part_pq, (size_t)8 * ((std::max(c.part_all.nblk, c.part_cl.nblk) + 7) / 8) + kRedGridMax));
   MM_TRY(dalloc(c, &c.part_rr, 2 * (size_t)kRedGridMax));
   MM_TRY(dalloc(c, &c.part_rz, (size_t)kRedGridMax));
-  MM_HIP(hipHostMalloc((void**)&c.hpart, 2 * kRedGridMax * sizeof(double), hipHostMallocDefault));
+  // r.r partials go straight from the PCG kernels into host-coherent pinned memory: the host
+  // reads them after the iteration's event, with no copy on the stream
+  MM_HIP(hipHostMalloc((void**)&c.hpart, 2 * kRedGridMax * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+  MM_HIP(hipHostGetDevicePointer((void**)&c.hpart_dev, c.hpart, 0));
 
   // Poisson work vectors
   for (double** v : {&c.pb, &c.pr, &c.pz, &c.pp, &c.pq, &c.pt, &c.px0, &c.work_U, &c.diagA}) MM_TRY(dalloc(c, v, N));
@@ -831,7 +834,29 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
     MM_TRY(upload(c, &L.Ri, Ri, Rp[n_next]));
     MM_TRY(upload(c, &L.Rx, Rx, Rp[n_next]));
   }
-  if (dense_inv) MM_TRY(upload(c, &L.dense_inv, dense_inv, (size_t)n * n));
+  std::vector<double> tiles64;
+  if (dense_inv) {
+    // lower-triangle tiles of the (symmetric) inverse; padding rows / columns are zero
+    const int T = (n + 63) / 64;
+    L.tiles = T;
+    const size_t nt = (size_t)T * (T + 1) / 2;
+    tiles64.assign(nt * 4096, 0.0);
+    for (int I = 0; I < T; ++I)
+      for (int J = 0; J <= I; ++J) {
+        double* tile = &tiles64[((size_t)I * (I + 1) / 2 + J) * 4096];
+        for (int r = 0; r < 64; ++r) {
+          const int row = I * 64 + r;
+          if (row >= n) break;
+          for (int q = 0; q < 64; ++q) {
+            const int col = J * 64 + q;
+            if (col >= n) break;
+            tile[r * 64 + q] = dense_inv[(size_t)row * n + col];
+          }
+        }
+      }
+    MM_TRY(upload(c, &L.dense_inv, tiles64.data(), tiles64.size()));
+    MM_TRY(dalloc(c, &L.symv_part, (size_t)T * T * 64));
+  }
   // FP32 copies for the mixed-precision V-cycle
   auto to_f32 = [](const double* a, size_t k) { return std::vector<float>(a, a + k); };
   MM_TRY(upload(c, &L.Mf, to_f32(M, n).data(), n));
@@ -840,7 +865,7 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
     MM_TRY(upload(c, &L.Pxf, to_f32(Px, Pp[n]).data(), Pp[n]));
     MM_TRY(upload(c, &L.Rxf, to_f32(Rx, Rp[n_next]).data(), Rp[n_next]));
   }
-  if (dense_inv) MM_TRY(upload(c, &L.dense_invf, to_f32(dense_inv, (size_t)n * n).data(), (size_t)n * n));
+  if (dense_inv) MM_TRY(upload(c, &L.dense_invf_t, to_f32(tiles64.data(), tiles64.size()).data(), tiles64.size()));
   // SpMV lanes per row: power of two near a quarter of the average row length, in [4, 64]
   auto lanes = [](long nnz, long rows) {
     const double avg = rows > 0 ? (double)nnz / (double)rows : 0.0;
@@ -1228,6 +1253,7 @@ int mm_apply_timing(int64_t h, int enable, double* total_ms, int64_t* count) {
   if (count) *count = c.apply_timed;
   c.apply_ms = 0.0;
   c.apply_timed = 0;
+  c.apply_seen = 0;
   c.time_apply = enable != 0;
   return 0;
 }

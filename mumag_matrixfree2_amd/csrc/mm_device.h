@@ -121,6 +121,57 @@ __device__ __forceinline__ double wave_red(double v) {
   return v;
 }
 
+// Compensated (Neumaier) accumulation: s + c carries the sum with the rounding error of every
+// addition kept in c, so sums of large terms that cancel (the energy contributions cross zero
+// during a sweep) keep their low-order digits.
+struct CSum {
+  double s = 0.0, c = 0.0;
+  __device__ __forceinline__ void add(double x) {
+    const double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ void add(const CSum& o) {
+    add(o.s);
+    c += o.c;
+  }
+};
+
+// Wave-level compensated reduction (lane 0 holds the total).
+__device__ __forceinline__ CSum wave_red_cs(CSum v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    CSum u;
+    u.s = __shfl_down(v.s, o, 64);
+    u.c = __shfl_down(v.c, o, 64);
+    v.add(u);
+  }
+  return v;
+}
+
+// Block compensated sum written as (s, c) to partial[2 * blockIdx.x + {0, 1}].
+__device__ void block_partial_cs(CSum v, double* __restrict__ partial) {
+  __shared__ double sh[2][kBlock / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  v = wave_red_cs(v);
+  if (lane == 0) {
+    sh[0][wv] = v.s;
+    sh[1][wv] = v.c;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    CSum t;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      CSum u;
+      u.s = sh[0][w];
+      u.c = sh[1][w];
+      t.add(u);
+    }
+    partial[2 * blockIdx.x] = t.s;
+    partial[2 * blockIdx.x + 1] = t.c;
+  }
+}
+
 // Returns true in exactly one workgroup: the last one to arrive. Every workgroup of the
 // grid must call it. The counter is reset by the last workgroup.
 __device__ __forceinline__ bool last_block(unsigned* counter) {

@@ -110,11 +110,17 @@ struct Level {
   int *Rp = nullptr, *Ri = nullptr;
   double* Rx = nullptr;
   double* M = nullptr;                // smoother multiplier: x += M * (b - A x)
-  double* dense_inv = nullptr;        // coarsest level
+  // coarsest level: the dense inverse as the lower-triangle 64 x 64 tiles of a symmetric matrix
+  // (tile (I, J), J <= I, at index I (I + 1) / 2 + J, row-major inside), FP64 and FP32, and the
+  // per-tile-pair partial products of the symmetric multiply
+  int tiles = 0;                      // tile rows: ceil(n / 64)
+  double* dense_inv = nullptr;
+  float* dense_invf_t = nullptr;
+  double* symv_part = nullptr;        // tiles x tiles x 64
   int lanesA = 8, lanesP = 8, lanesR = 8;  // SpMV lanes per row
   double *b = nullptr, *x = nullptr, *t = nullptr, *xs = nullptr;  // xs: post-smoothed result
   // FP32 copies for the mixed-precision V-cycle
-  float *Axf = nullptr, *Pxf = nullptr, *Rxf = nullptr, *Mf = nullptr, *dense_invf = nullptr;
+  float *Axf = nullptr, *Pxf = nullptr, *Rxf = nullptr, *Mf = nullptr;
   float *bf = nullptr, *xf = nullptr, *tf = nullptr, *xsf = nullptr;
 };
 
@@ -155,7 +161,8 @@ struct Ctx {
   double* part_pq = nullptr;          // p.Ap partials of the Poisson apply (one per apply workgroup)
   double* part_rr = nullptr;          // r.r (and b.b) partials, 2 x kRedGridMax
   double* part_rz = nullptr;          // r.z partials, kRedGridMax
-  double* hpart = nullptr;            // pinned host copy of part_rr
+  double* hpart = nullptr;            // host-coherent pinned r.r (and b.b) partials of the PCG
+  double* hpart_dev = nullptr;        // its device address
 
   // Poisson / PCG vectors (N)
   double *pb = nullptr, *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *pt = nullptr, *px0 = nullptr;
@@ -185,6 +192,7 @@ struct Ctx {
   int ev_used = 0;
   double apply_ms = 0.0;
   long long apply_timed = 0;
+  long long apply_seen = 0;           // applies issued while timing is on (every 8th is timed)
 };
 
 int timed_apply(Ctx& c, const double* x, double* y, hipStream_t s, double* dot_part = nullptr);

@@ -448,8 +448,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs(Part p, Geo geo, const double* _
 // ------------------------------------------------------------ energy + gradient (K7)
 // Per magnetic element: exchange 2A Ve grad phi_a . grad m, uniaxial anisotropy
 // -(2 K1 Ve/20) (sum_b m_b.k + m_a.k) k, demag (2 Js Ve/4) grad U on all four corners.
-// Owners add the Zeeman term -2 M_i B. Energy E = 1/2 sum m.(g_quad) + sum m.g_z,
-// reduced per block and finalised by the last block. Output g and E are divided by V_mag.
+// Owners add the Zeeman term -2 M_i B. Energy E = 1/2 sum m.(g_quad) + sum m.g_z with
+// compensated summation from the element terms up to the total (per-block (sum, error)
+// partials, reduced by k_sum_partials_cs). Output g and E are divided by V_mag.
 __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double* __restrict__ m,
                                                    const double* __restrict__ U, double Bx, double By, double Bz,
                                                    double inv_V, double* __restrict__ g, double* __restrict__ partial,
@@ -460,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
   __shared__ double sg[3][kMaxBlockNodes];
   const BlkInfo bi = blk_info(p);
   const int tid = threadIdx.x;
-  double acc[1] = {0.0};
+  CSum acc;
   for (int i = tid; i < bi.nb; i += kBlock) {
     const uint32_t id = p.nodes[bi.n0 + i] & kIdMask;
 #pragma unroll
@@ -517,7 +518,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
 #pragma unroll
       for (int c = 0; c < 3; ++c) atomicAdd(&sg[c][l[a]], c3[c]);
     }
-    acc[0] += 0.5 * e_el;
+    acc.add(0.5 * e_el);
   }
   __syncthreads();
   for (int i = tid; i < bi.nb; i += kBlock) {
@@ -530,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
       v0 += z0;
       v1 += z1;
       v2 += z2;
-      acc[0] += sm[0][i] * z0 + sm[1][i] * z1 + sm[2][i] * z2;
+      acc.add(sm[0][i] * z0 + sm[1][i] * z1 + sm[2][i] * z2);
     }
     if (f & kPrivate) {
       g[3 * id + 0] = v0 * inv_V;
@@ -542,8 +543,9 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
       unsafeAtomicAdd(&g[3 * id + 2], v2 * inv_V);
     }
   }
-  acc[0] *= inv_V;
-  block_partial<1>(acc, partial);
+  acc.s *= inv_V;
+  acc.c *= inv_V;
+  block_partial_cs(acc, partial);
 }
 
 // out[k] = sum_b partial[b * K + k], one workgroup, fixed order
@@ -570,6 +572,36 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restric
       for (int w = 1; w < kBlock / 64; ++w) s += sh[k][w];
       out[k] = s;
     }
+}
+
+// out[0] = compensated sum of n (s, c) partial pairs, one workgroup, fixed order
+__global__ __launch_bounds__(kBlock) void k_sum_partials_cs(const double* __restrict__ partial, int n,
+                                                            double* __restrict__ out) {
+  __shared__ double sh[2][kBlock / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  CSum v;
+  for (int i = tid; i < n; i += kBlock) {
+    CSum u;
+    u.s = partial[2 * i];
+    u.c = partial[2 * i + 1];
+    v.add(u);
+  }
+  v = wave_red_cs(v);
+  if (lane == 0) {
+    sh[0][wv] = v.s;
+    sh[1][wv] = v.c;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    CSum t;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      CSum u;
+      u.s = sh[0][w];
+      u.c = sh[1][w];
+      t.add(u);
+    }
+    out[0] = t.s + t.c;
+  }
 }
 
 // --------------------------------------------------- local Hessian action C v (K8)
@@ -840,7 +872,7 @@ int launch_energy(const Ctx& c, const double* m, const double* U, const double B
   const int grid = grid_for(p);
   k_energy<<<grid, kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, c.partial, c.counter + 1,
                                    E_out);
-  k_sum_partials<1><<<1, kBlock, 0, s>>>(c.partial, grid, E_out);
+  k_sum_partials_cs<<<1, kBlock, 0, s>>>(c.partial, grid, E_out);
   return 0;
 }
 

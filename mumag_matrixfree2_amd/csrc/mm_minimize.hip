@@ -86,12 +86,20 @@ __global__ __launch_bounds__(kBlock) void k_tangents(const double* __restrict__ 
 }
 
 // ------------------------------------------------------------- Steihaug-PCG (solve_Py_g)
-// y = 0; r = g_ext; z = Mp r; p = z; rho = r.z
+// The loop state (rho, Q, iteration, done, active, target) lives in two banks of device
+// slots: iteration k reads bank k % 2 and its last kernel writes bank (k + 1) % 2, so no
+// kernel overwrites a slot that other workgroups of the same launch still read. Dot
+// products are per-workgroup partials summed by the consuming kernel (fixed order, no
+// fences): p.q in part_a, r.z in part_b.
+enum PcBank { B_RHO = 0, B_Q = 1, B_IT = 2, B_DONE = 3, B_ACT = 4, B_TGT = 5 };
+constexpr int kPcBank0 = 224;  // banks at slots 224..229 and 232..237
+__host__ __device__ constexpr int pc_bank(int k) { return kPcBank0 + 8 * (k & 1); }
+
+// y = 0; r = g_ext; z = Mp r; p = z; partials of rho = r.z
 __global__ __launch_bounds__(kBlock) void k_pc_init(const double* __restrict__ g_ext, const double* __restrict__ Mp,
                                                     double* __restrict__ y, double* __restrict__ r,
                                                     double* __restrict__ z, double* __restrict__ p,
-                                                    double* __restrict__ cv, long N, double* __restrict__ partial,
-                                                    unsigned* __restrict__ counter, double* __restrict__ out) {
+                                                    double* __restrict__ cv, long N, double* __restrict__ part) {
   double v[1] = {0.0};
   NODE_LOOP(i, N) {
     const double s = Mp[i];
@@ -106,33 +114,39 @@ __global__ __launch_bounds__(kBlock) void k_pc_init(const double* __restrict__ g
       v[0] += ri * ri * s;
     }
   }
-  reduce_store<1, false>(v, partial, counter, out);
+  block_partial<1>(v, part);
 }
 
-__device__ __forceinline__ double pc_active(const double* sc, int max_iter) {
-  const double rho = sc[S_RHO];
-  const bool act = (sc[S_IT] < (double)max_iter) && (rho > sc[S_TGT]) && (rho > 1e-25) && (rho < 1e20) &&
-                   (sc[S_DONE] == 0.0);
+__device__ __forceinline__ double pc_active(const double* bank, int max_iter) {
+  const double rho = bank[B_RHO];
+  const bool act = (bank[B_IT] < (double)max_iter) && (rho > bank[B_TGT]) && (rho > 1e-25) && (rho < 1e20) &&
+                   (bank[B_DONE] == 0.0);
   return act ? 1.0 : 0.0;
 }
 
-// forcing-sequence tolerance and loop state (src/minimizers.py:968-972, 445-450)
-__global__ void k_pc_state_init(double* __restrict__ sc, int pc_auto, double eta, double alpha_f, int max_iter) {
+// bank 0: rho = sum of the k_pc_init partials, forcing-sequence tolerance (src/minimizers.py:968-972,
+// 445-450), Q = 0, iteration 0, active flag. One workgroup.
+__global__ __launch_bounds__(kBlock) void k_pc_state_init(double* __restrict__ sc, const double* __restrict__ part,
+                                                          int n_part, int pc_auto, double eta, double alpha_f,
+                                                          int max_iter) {
+  const double rho = sum_partials_all<kBlock>(part, n_part);
+  if (threadIdx.x != 0) return;
+  double* bk = sc + pc_bank(0);
   const double tol = pc_auto ? fmin(eta, pow(sc[S_GN], alpha_f)) : 0.0;
-  sc[S_TGT] = tol * tol * sc[S_RHO];
-  sc[S_Q] = 0.0;
-  sc[S_IT] = 0.0;
-  sc[S_DONE] = 0.0;
-  sc[S_ACT] = pc_active(sc, max_iter);
+  bk[B_RHO] = rho;
+  bk[B_TGT] = tol * tol * rho;
+  bk[B_Q] = 0.0;
+  bk[B_IT] = 0.0;
+  bk[B_DONE] = 0.0;
+  bk[B_ACT] = pc_active(bk, max_iter);
 }
 
-// q = P p = Cv - (m.Cv) m - (m.g) p + reg p; p.q -> alpha, dq, negative curvature / stagnation
-__global__ __launch_bounds__(kBlock) void k_pc_ap(double* __restrict__ sc, const double* __restrict__ m,
+// q = P p = Cv - (m.Cv) m - (m.g) p + reg p; partials of p.q
+__global__ __launch_bounds__(kBlock) void k_pc_ap(const double* __restrict__ bank, const double* __restrict__ m,
                                                   const double* __restrict__ g, const double* __restrict__ p,
                                                   const double* __restrict__ cv, double* __restrict__ q, double reg,
-                                                  double nu, long N, double* __restrict__ partial,
-                                                  unsigned* __restrict__ counter) {
-  if (sc[S_ACT] == 0.0) return;
+                                                  long N, double* __restrict__ part) {
+  if (bank[B_ACT] == 0.0) return;
   double v[1] = {0.0};
   NODE_LOOP(i, N) {
     const double* mi = m + 3 * i;
@@ -146,34 +160,37 @@ __global__ __launch_bounds__(kBlock) void k_pc_ap(double* __restrict__ sc, const
       v[0] += pk * qk;
     }
   }
-  if (reduce_last<1, false>(v, partial, counter) && threadIdx.x == 0) {
-    const double pAp = v[0], rho = sc[S_RHO];
-    const double alpha = rho / (pAp + 1e-30);
-    const double dq = 0.5 * alpha * rho;
-    const bool stag = (sc[S_IT] > 0.0) && (dq <= nu * sc[S_Q]);
+  block_partial<1>(v, part);
+}
+
+// p.q from the partials -> alpha, dq, negative curvature / stagnation (workgroup 0 stores them);
+// y += alpha p; r -= alpha q (unless done now); z = Mp r; partials of rho_n = r.z
+__global__ __launch_bounds__(kBlock) void k_pc_yr(double* __restrict__ sc, const double* __restrict__ bank,
+                                                  const double* __restrict__ part_a, int n_part, double nu,
+                                                  double* __restrict__ y, double* __restrict__ r,
+                                                  double* __restrict__ z, const double* __restrict__ p,
+                                                  const double* __restrict__ q, const double* __restrict__ Mp, long N,
+                                                  double* __restrict__ part_b) {
+  if (bank[B_ACT] == 0.0) return;
+  const double pAp = sum_partials_all<kBlock>(part_a, n_part);
+  const double rho = bank[B_RHO];
+  const double alpha = rho / (pAp + 1e-30);
+  const double dq = 0.5 * alpha * rho;
+  const bool stag = (bank[B_IT] > 0.0) && (dq <= nu * bank[B_Q]);
+  const bool dn = pAp <= 0.0 || stag;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc[S_PAP] = pAp;
     sc[S_ALPHA] = alpha;
     sc[S_DQ] = dq;
-    sc[S_DN] = (pAp <= 0.0 || stag) ? 1.0 : 0.0;
+    sc[S_DN] = dn ? 1.0 : 0.0;
   }
-}
-
-// y += alpha p; r -= alpha q (unless done now); z = Mp r; rho_n = r.z
-__global__ __launch_bounds__(kBlock) void k_pc_yr(double* __restrict__ sc, double* __restrict__ y,
-                                                  double* __restrict__ r, double* __restrict__ z,
-                                                  const double* __restrict__ p, const double* __restrict__ q,
-                                                  const double* __restrict__ Mp, long N, double* __restrict__ partial,
-                                                  unsigned* __restrict__ counter) {
-  if (sc[S_ACT] == 0.0) return;
-  const bool upd = sc[S_DN] == 0.0;
-  const double alpha = sc[S_ALPHA];
   double v[1] = {0.0};
   NODE_LOOP(i, N) {
     const double s = Mp[i];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       double rk = r[3 * i + k];
-      if (upd) {
+      if (!dn) {
         y[3 * i + k] += alpha * p[3 * i + k];
         rk -= alpha * q[3 * i + k];
         r[3 * i + k] = rk;
@@ -183,16 +200,22 @@ __global__ __launch_bounds__(kBlock) void k_pc_yr(double* __restrict__ sc, doubl
       v[0] += rk * zk;
     }
   }
-  if (reduce_last<1, false>(v, partial, counter) && threadIdx.x == 0) sc[S_RHON] = v[0];
+  block_partial<1>(v, part_b);
 }
 
-// p = z + beta p (unless done now); cv = 0 for the next iteration; loop-state update
-__global__ __launch_bounds__(kBlock) void k_pc_p(double* __restrict__ sc, const double* __restrict__ z,
-                                                 double* __restrict__ p, double* __restrict__ cv, int max_iter, long N,
-                                                 unsigned* __restrict__ counter) {
-  if (sc[S_ACT] == 0.0) return;
+// rho_n from the partials; p = z + beta p (unless done now); cv = 0 for the next iteration;
+// workgroup 0 writes the next bank (a copy of this one when the loop is no longer active)
+__global__ __launch_bounds__(kBlock) void k_pc_p(const double* __restrict__ sc, const double* __restrict__ bank,
+                                                 double* __restrict__ next, const double* __restrict__ part_b,
+                                                 int n_part, const double* __restrict__ z, double* __restrict__ p,
+                                                 double* __restrict__ cv, int max_iter, long N) {
+  if (bank[B_ACT] == 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x < 6) next[threadIdx.x] = bank[threadIdx.x];
+    return;
+  }
+  const double rhon = sum_partials_all<kBlock>(part_b, n_part);
   const bool upd = sc[S_DN] == 0.0;
-  const double beta = sc[S_RHON] / (sc[S_RHO] + 1e-30);
+  const double beta = rhon / (bank[B_RHO] + 1e-30);
   NODE_LOOP(i, N) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -200,20 +223,20 @@ __global__ __launch_bounds__(kBlock) void k_pc_p(double* __restrict__ sc, const 
       cv[3 * i + k] = 0.0;
     }
   }
-  if (last_block(counter) && threadIdx.x == 0) {
-    sc[S_RHO] = sc[S_RHON];
-    sc[S_Q] += sc[S_DQ];
-    sc[S_IT] += 1.0;
-    sc[S_DONE] = sc[S_DN];
-    sc[S_ACT] = pc_active(sc, max_iter);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    next[B_RHO] = rhon;
+    next[B_Q] = bank[B_Q] + sc[S_DQ];
+    next[B_IT] = bank[B_IT] + 1.0;
+    next[B_DONE] = sc[S_DN];
+    next[B_TGT] = bank[B_TGT];
+    next[B_ACT] = pc_active(next, max_iter);
   }
 }
 
-// y.y, zf.zf (zf = Mp g_ext), y.g_ext
-__global__ __launch_bounds__(kBlock) void k_pc_norms(double* __restrict__ sc, const double* __restrict__ y,
-                                                     const double* __restrict__ g, const double* __restrict__ Mp,
-                                                     long N, double* __restrict__ partial,
-                                                     unsigned* __restrict__ counter) {
+// partials of y.y, zf.zf (zf = Mp g_ext), y.g_ext (3 per workgroup)
+__global__ __launch_bounds__(kBlock) void k_pc_norms(const double* __restrict__ y, const double* __restrict__ g,
+                                                     const double* __restrict__ Mp, long N,
+                                                     double* __restrict__ part) {
   double v[3] = {0.0, 0.0, 0.0};
   NODE_LOOP(i, N) {
     const double s = Mp[i];
@@ -225,21 +248,49 @@ __global__ __launch_bounds__(kBlock) void k_pc_norms(double* __restrict__ sc, co
       v[2] += yk * gk;
     }
   }
-  reduce_store<3, false>(v, partial, counter, sc + S_YY);
+  block_partial<3>(v, part);
 }
 
-// clip |y| <= 10 |zf|; fall back to zf when y is not a descent direction (src/minimizers.py:478-486)
-__global__ void k_pc_finish(double* __restrict__ sc, double* __restrict__ y, const double* __restrict__ g,
-                            const double* __restrict__ Mp, long N) {
-  const double yn = sqrt(sc[S_YY]), zn = sqrt(sc[S_ZZ]);
+// clip |y| <= 10 |zf|; fall back to zf when y is not a descent direction (src/minimizers.py:478-486);
+// the three norms are summed from the k_pc_norms partials (stride 3)
+__global__ __launch_bounds__(kBlock) void k_pc_finish(double* __restrict__ sc, const double* __restrict__ bank,
+                                                      const double* __restrict__ part, int n_part,
+                                                      double* __restrict__ y, const double* __restrict__ g,
+                                                      const double* __restrict__ Mp, long N) {
+  __shared__ double tot[3];
+  {
+    // the three strided sums, each in the fixed order of sum_partials_all
+    const int tid = threadIdx.x;
+    for (int k = 0; k < 3; ++k) {
+      double sk = 0.0;
+      for (int i = tid; i < n_part; i += kBlock) sk += part[3 * i + k];
+      sk = wave_red<false>(sk);
+      __shared__ double sh[kBlock / 64];
+      if ((tid & 63) == 0) sh[tid >> 6] = sk;
+      __syncthreads();
+      if (tid == 0) {
+        double t = sh[0];
+        for (int w = 1; w < kBlock / 64; ++w) t += sh[w];
+        tot[k] = t;
+      }
+      __syncthreads();
+    }
+  }
+  const double yy = tot[0], zz = tot[1], yg = tot[2];
+  const double yn = sqrt(yy), zn = sqrt(zz);
   const double scale = (yn > 10.0 * zn) ? (10.0 * zn / (yn + 1e-30)) : 1.0;
-  const bool fallback = !(scale * sc[S_YG] > 1e-12);
+  const bool fallback = !(scale * yg > 1e-12);
   NODE_LOOP(i, N) {
     const double s = Mp[i];
 #pragma unroll
     for (int k = 0; k < 3; ++k) y[3 * i + k] = fallback ? g[3 * i + k] * s : y[3 * i + k] * scale;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_PCIT] = sc[S_IT];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_YY] = yy;
+    sc[S_ZZ] = zz;
+    sc[S_YG] = yg;
+    sc[S_PCIT] = bank[B_IT];
+  }
 }
 
 // ----------------------------------------------------------------- direction kernels
@@ -470,20 +521,22 @@ static int solve_P(Ctx& c, const MinParams& p) {
   const long N = c.N;
   hipStream_t s = c.stream;
   double* sc = c.dscal;
-  unsigned* cnt = c.counter + 8;
-  k_pc_init<<<red_grid(N), kBlock, 0, s>>>(c.g_ext, c.inv_M_prec, c.y, c.r3, c.z3, c.p3, c.cv3, N, c.partial, cnt,
-                                            sc + S_RHO);
-  k_pc_state_init<<<1, 1, 0, s>>>(sc, p.pc_auto, p.pc_force_eta, p.pc_force_alpha, p.pc_iters);
+  const int gr = red_grid(N);
+  double* part_a = c.part_rz;  // p.q partials (the PCG is not running meanwhile)
+  double* part_b = c.part_rr;  // r.z partials
+  k_pc_init<<<gr, kBlock, 0, s>>>(c.g_ext, c.inv_M_prec, c.y, c.r3, c.z3, c.p3, c.cv3, N, part_b);
+  k_pc_state_init<<<1, kBlock, 0, s>>>(sc, part_b, gr, p.pc_auto, p.pc_force_eta, p.pc_force_alpha, p.pc_iters);
   for (int k = 0; k < p.pc_iters; ++k) {
-    MM_TRY(launch_local(c, c.p3, c.cv3, sc + S_ACT, s));
+    double* bank = sc + pc_bank(k);
+    MM_TRY(launch_local(c, c.p3, c.cv3, bank + B_ACT, s));
     c.n_local++;
-    k_pc_ap<<<red_grid(N), kBlock, 0, s>>>(sc, c.m, c.g_raw, c.p3, c.cv3, c.q3, p.pc_reg, p.pc_stagnation_nu, N,
-                                            c.partial, cnt);
-    k_pc_yr<<<red_grid(N), kBlock, 0, s>>>(sc, c.y, c.r3, c.z3, c.p3, c.q3, c.inv_M_prec, N, c.partial, cnt);
-    k_pc_p<<<red_grid(N), kBlock, 0, s>>>(sc, c.z3, c.p3, c.cv3, p.pc_iters, N, cnt);
+    k_pc_ap<<<gr, kBlock, 0, s>>>(bank, c.m, c.g_raw, c.p3, c.cv3, c.q3, p.pc_reg, N, part_a);
+    k_pc_yr<<<gr, kBlock, 0, s>>>(sc, bank, part_a, gr, p.pc_stagnation_nu, c.y, c.r3, c.z3, c.p3, c.q3,
+                                   c.inv_M_prec, N, part_b);
+    k_pc_p<<<gr, kBlock, 0, s>>>(sc, bank, sc + pc_bank(k + 1), part_b, gr, c.z3, c.p3, c.cv3, p.pc_iters, N);
   }
-  k_pc_norms<<<red_grid(N), kBlock, 0, s>>>(sc, c.y, c.g_ext, c.inv_M_prec, N, c.partial, cnt);
-  k_pc_finish<<<vg(N), kBlock, 0, s>>>(sc, c.y, c.g_ext, c.inv_M_prec, N);
+  k_pc_norms<<<gr, kBlock, 0, s>>>(c.y, c.g_ext, c.inv_M_prec, N, c.partial);
+  k_pc_finish<<<gr, kBlock, 0, s>>>(sc, sc + pc_bank(p.pc_iters), c.partial, gr, c.y, c.g_ext, c.inv_M_prec, N);
   return 0;
 }
 

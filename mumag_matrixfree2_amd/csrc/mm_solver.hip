@@ -202,24 +202,82 @@ __global__ __launch_bounds__(kBlock) void k_spmv_post(int nrows, const int* __re
   if (row < nrows && lane == 0) xs[row] = x[row] + M[row] * (b[row] - s);
 }
 
-// y = Ainv b, dense row-major n x n: one wave per row, 4 independent loads per lane in flight
-__global__ __launch_bounds__(kBlock) void k_gemv(int n, const double* __restrict__ Ainv, const double* __restrict__ b,
-                                                 double* __restrict__ y) {
-  const int lane = threadIdx.x & 63;
-  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  if (row >= n) return;
-  const double* a = Ainv + (long)row * n;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int j = lane;
-  for (; j + 192 < n; j += 256) {
-    s0 += a[j] * b[j];
-    s1 += a[j + 64] * b[j + 64];
-    s2 += a[j + 128] * b[j + 128];
-    s3 += a[j + 192] * b[j + 192];
+// y = A b for the symmetric coarsest-level inverse stored as its lower-triangle 64 x 64 tiles
+// (each tile read once, half the bytes of the full matrix). k_symv_tiles: one workgroup per
+// tile (I, J); it writes the row products A_IJ b_J to part[I][J] and, off the diagonal, the
+// column products A_IJ^T b_I to part[J][I]. k_symv_reduce sums part[I][0..T-1] in order, so
+// the result is deterministic.
+template <typename T>
+__global__ __launch_bounds__(256) void k_symv_tiles(int n, int T_, const T* __restrict__ tiles,
+                                                    const T* __restrict__ b, double* __restrict__ part) {
+  __shared__ double sbI[64], sbJ[64];
+  __shared__ double sQ[4][64];
+  const int t = blockIdx.x;
+  int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  while (I * (I + 1) / 2 > t) --I;
+  const int J = t - I * (I + 1) / 2;
+  const int tid = threadIdx.x;
+  if (tid < 64) {
+    const int i = I * 64 + tid, j = J * 64 + tid;
+    sbI[tid] = i < n ? (double)b[i] : 0.0;
+    sbJ[tid] = j < n ? (double)b[j] : 0.0;
   }
-  for (; j < n; j += 64) s0 += a[j] * b[j];
-  double s = wave_red<false>((s0 + s1) + (s2 + s3));
-  if (lane == 0) y[row] = s;
+  __syncthreads();
+  // thread (R, Q) holds the 4 x 4 sub-block rows 4R..4R+3, columns 4Q..4Q+3 (R = tid / 16,
+  // Q = tid % 16): row sums reduce over Q (lane bits 0..3), column sums over R (lane bits 4, 5,
+  // then the 4 waves in LDS)
+  const int R = tid >> 4, Q = tid & 15;
+  const T* a = tiles + (size_t)t * 4096 + (4 * R) * 64 + 4 * Q;
+  double av[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) av[i][k] = (double)a[i * 64 + k];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double u = av[i][0] * sbJ[4 * Q] + av[i][1] * sbJ[4 * Q + 1] + av[i][2] * sbJ[4 * Q + 2] + av[i][3] * sbJ[4 * Q + 3];
+    u += __shfl_xor(u, 1, 64);
+    u += __shfl_xor(u, 2, 64);
+    u += __shfl_xor(u, 4, 64);
+    u += __shfl_xor(u, 8, 64);
+    if (Q == 0) part[((size_t)I * T_ + J) * 64 + 4 * R + i] = u;
+  }
+  if (I == J) return;  // uniform per workgroup
+  const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double v = av[0][k] * sbI[4 * R] + av[1][k] * sbI[4 * R + 1] + av[2][k] * sbI[4 * R + 2] + av[3][k] * sbI[4 * R + 3];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (lane < 16) sQ[wv][4 * Q + k] = v;
+  }
+  __syncthreads();
+  if (tid < 64) part[((size_t)J * T_ + I) * 64 + tid] = (sQ[0][tid] + sQ[1][tid]) + (sQ[2][tid] + sQ[3][tid]);
+}
+
+// y[64 I + r] = sum over J of part[I][J][r]: one workgroup per tile row I; thread (r, g) sums the
+// J = g mod 4 terms, the four group sums are combined in a fixed order
+template <typename T>
+__global__ __launch_bounds__(256) void k_symv_reduce(int n, int T_, const double* __restrict__ part,
+                                                     T* __restrict__ y) {
+  __shared__ double sh[4][64];
+  const int I = blockIdx.x, r = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const double* p = part + (size_t)I * T_ * 64 + r;
+  double s = 0.0;
+  for (int J = g; J < T_; J += 4) s += p[(size_t)J * 64];
+  sh[g][r] = s;
+  __syncthreads();
+  const int i = I * 64 + threadIdx.x;
+  if (threadIdx.x < 64 && i < n) y[i] = (T)((sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]));
+}
+
+// y = dense_inv b on level L (FP64 or FP32 tiles)
+template <typename T>
+static void symv(const Level& L, const T* tiles, const T* b, T* y, hipStream_t s) {
+  const int T_ = L.tiles;
+  k_symv_tiles<T><<<T_ * (T_ + 1) / 2, 256, 0, s>>>(L.n, T_, tiles, b, L.symv_part);
+  k_symv_reduce<T><<<T_, 256, 0, s>>>(L.n, T_, L.symv_part, y);
 }
 
 template <int MODE, typename TA, typename TX, typename TX2, typename TY>
@@ -280,18 +338,6 @@ __global__ void k_post_f(float* __restrict__ x, const float* __restrict__ M, con
     x[i] += M[i] * (b[i] - t[i]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_gemv_f(int n, const float* __restrict__ Ainv, const float* __restrict__ b,
-                                                   float* __restrict__ y) {
-  const int lane = threadIdx.x & 63;
-  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  if (row >= n) return;
-  float s = 0.0f;
-  for (int j = lane; j < n; j += 64) s += Ainv[(long)row * n + j] * b[j];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-  if (lane == 0) y[row] = s;
-}
-
 // z = x + M (r - t) in FP64 from FP32 x, M, t and FP64 r; partials of r.z (FP64)
 __global__ __launch_bounds__(kBlock) void k_post_dot_mixed(const float* __restrict__ x, const float* __restrict__ M,
                                                            const double* __restrict__ r, const float* __restrict__ t,
@@ -319,8 +365,12 @@ int resolve_apply_timing(Ctx& c) {
   return 0;
 }
 
+// Every kApplySample-th apply inside the PCG is bracketed by an event pair: enough samples for
+// the average launch time, without an event pair on every launch of the timed sweep.
+constexpr long long kApplySample = 8;
+
 int timed_apply(Ctx& c, const double* x, double* y, hipStream_t s, double* dot_part) {
-  if (!c.time_apply) return launch_apply(c, c.geom_mode, x, y, s, dot_part);
+  if (!c.time_apply || (c.apply_seen++ % kApplySample) != 0) return launch_apply(c, c.geom_mode, x, y, s, dot_part);
   if (c.ev_pool.empty()) {
     c.ev_pool.resize(512);
     for (auto& e : c.ev_pool) MM_HIP(hipEventCreate(&e));
@@ -334,7 +384,7 @@ int timed_apply(Ctx& c, const double* x, double* y, hipStream_t s, double* dot_p
 }
 
 static int timed_apply_scaled(Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s) {
-  if (!c.time_apply) return launch_apply_scaled(c, r, w, xo, y, s);
+  if (!c.time_apply || (c.apply_seen++ % kApplySample) != 0) return launch_apply_scaled(c, r, w, xo, y, s);
   if (c.ev_pool.empty()) {
     c.ev_pool.resize(512);
     for (auto& e : c.ev_pool) MM_HIP(hipEventCreate(&e));
@@ -355,7 +405,7 @@ static void vcycle_coarse(Ctx& c, int l, hipStream_t s) {
   Level& L = c.lv[l];
   const int last = (int)c.lv.size() - 1;
   if (l == last) {
-    k_gemv<<<(int)(((long)L.n * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>(L.n, L.dense_inv, L.b, L.xs);
+    symv<double>(L, L.dense_inv, L.b, L.xs, s);
     return;
   }
   Level& Nx = c.lv[l + 1];
@@ -371,7 +421,7 @@ static void vcycle_coarse_f(Ctx& c, int l, hipStream_t s) {
   Level& L = c.lv[l];
   const int last = (int)c.lv.size() - 1;
   if (l == last) {
-    k_gemv_f<<<(int)(((long)L.n * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>(L.n, L.dense_invf, L.bf, L.xsf);
+    symv<float>(L, L.dense_invf_t, L.bf, L.xsf, s);
     return;
   }
   Level& Nx = c.lv[l + 1];
@@ -406,7 +456,7 @@ int precond(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s) 
   if (c.precision == 1 && c.lv.size() > 1 && L0.Mf != nullptr) return precond_f32(c, r, z, rz_part, s);
   if (c.lv.size() <= 1) {
     if (L0.dense_inv)  // single-level hierarchy: exact dense solve
-      k_gemv<<<(int)((N * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>((int)N, L0.dense_inv, r, z);
+      symv<double>(L0, L0.dense_inv, r, z, s);
     else  // Jacobi on diag(A)
       k_scale<<<vgrid(N), kBlock, 0, s>>>(L0.M, r, z, nullptr, N);
     k_dot_part<<<red_grid(N), kBlock, 0, s>>>(r, z, N, rz_part);
@@ -435,7 +485,7 @@ int precond_apply(Ctx& c, const double* r, double* z, double* rz_part, hipStream
 // Device-resident PCG: U (in/out, device) solves A U = b (device), stop when
 // it >= maxiter or r.r <= tol^2 b.b (make_pcg_solve, src/poisson_solve.py:422-478).
 // Returns iterations, r.r and b.b; r.r is summed on the host from the per-workgroup
-// partials in a fixed order.
+// partials (written by the kernels into host-coherent memory) in a fixed order.
 int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it_out, double* r2_out,
             double* bb_out) {
   hipStream_t s = c.stream;
@@ -451,8 +501,7 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
   c.n_apply++;
   float* tf = c.precision == 1 ? c.ptf : nullptr;
   double* t64 = c.precision == 1 ? nullptr : c.pt;
-  k_pcg_init<<<gv, kBlock, 0, s>>>(b, c.pq, c.pr, t64, tf, N, c.part_rr);
-  MM_HIP(hipMemcpyAsync(c.hpart, c.part_rr, 2 * gv * sizeof(double), hipMemcpyDeviceToHost, s));
+  k_pcg_init<<<gv, kBlock, 0, s>>>(b, c.pq, c.pr, t64, tf, N, c.hpart_dev);
   MM_TRY(precond(c, c.pr, c.pz, c.part_rz, s));
   k_pcg_p<<<gv, kBlock, 0, s>>>(sc, c.part_rz, gv, 0, 0, 1, c.pz, c.pp, c.pq, N);
   MM_HIP(hipStreamSynchronize(s));
@@ -472,8 +521,7 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
       k_dot_part<<<gv, kBlock, 0, s>>>(c.pp, c.pq, N, c.part_pq);
     }
     c.n_apply++;
-    k_pcg_xr<<<gv, kBlock, 0, s>>>(sc, cur, c.part_pq, n_pq, U, c.pr, c.pp, c.pq, t64, tf, N, c.part_rr);
-    MM_HIP(hipMemcpyAsync(c.hpart, c.part_rr, gv * sizeof(double), hipMemcpyDeviceToHost, s));
+    k_pcg_xr<<<gv, kBlock, 0, s>>>(sc, cur, c.part_pq, n_pq, U, c.pr, c.pp, c.pq, t64, tf, N, c.hpart_dev);
     MM_HIP(hipEventRecord(c.ev, s));
     // preconditioner and direction update run while the host waits for r.r
     MM_TRY(precond(c, c.pr, c.pz, c.part_rz, s));
