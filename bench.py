@@ -117,8 +117,8 @@ def reference_layout_bytes(ctx) -> int:
 
 def measured_traffic():
     """HBM bytes per k_apply launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    (profiles/k_apply_cube2M_fp64.md), or None if absent."""
-    f = ROOT / "profiles" / "k_apply_cube2M_fp64.md"
+    (profiles/k_apply_cl_cube2M_fp64.md), or None if absent."""
+    f = ROOT / "profiles" / "k_apply_cl_cube2M_fp64.md"
     if not f.exists():
         return None
     for line in f.read_text().splitlines():

@@ -54,6 +54,8 @@ class MeshContext:
         self.boundary_mask = bnd
         self.reg = float(reg)
         self.Js_red = Js_red
+        self._conn = conn
+        self._mat_id = mat_id
         h = C.c_int64(0)
         if not 16 <= int(block_nodes) <= 256:
             raise ValueError("block_nodes must be in [16, 256]")
@@ -98,6 +100,13 @@ class MeshContext:
         grad = np.zeros((self.E, 3, 3))
         call("mm_ctx_get_geometry", self.handle, ptr(conn), ptr(Ve), ptr(grad))
         return conn, Ve, grad
+
+    def element_table(self):
+        """(conn (E,4), Ve (E,), magnetic (E,) bool) in the caller's element order, on the host:
+        Ve = |det J| / 6 from the coordinates, magnetic = the element's material has Js > 0."""
+        x = self.knt[self._conn]
+        d = np.linalg.det(x[:, 1:, :] - x[:, :1, :])
+        return self._conn, np.abs(d) / 6.0, self.Js_red[self._mat_id - 1] > 0
 
     def nodal(self):
         """(M_nodal, inv_M_rel, inv_M_prec), each (N,)."""

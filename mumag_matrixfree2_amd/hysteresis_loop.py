@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import time
 from dataclasses import dataclass, field
+from pathlib import Path
 
 import numpy as np
 
@@ -75,12 +76,25 @@ def switching_step(J_par: np.ndarray) -> int:
     return int(idx[0]) if idx.size else -1
 
 
+def volume_averaged_m(m: np.ndarray, conn: np.ndarray, Ve: np.ndarray, magnetic: np.ndarray) -> np.ndarray:
+    """Volume average of the element-mean magnetisation over magnetic elements (Js > 0)
+    (``src/hysteresis_loop.py:193-222``)."""
+    me = np.asarray(m)[np.asarray(conn)].mean(axis=1)
+    w = np.where(magnetic, Ve, 0.0)
+    return (w[:, None] * me).sum(axis=0) / (w.sum() + 1e-30)
+
+
 def run_hysteresis_loop(ctx: MeshContext, m0: np.ndarray, params: LoopParams, *, steps: int | None = None,
-                        on_step=None) -> dict:
+                        on_step=None, out_dir=None, name: str = "hysteresis", snapshot_every: int = 0) -> dict:
     """Sweep the field along ``params.h_dir`` and minimise at each value.
 
     Returns a dict with ``history`` rows (B_ext [Js_ref units], J_par [Js_ref units], E, gnorm,
     iters, evals, demag_iters, preco_iters, seconds) and the final ``m``, ``U``.
+
+    With ``out_dir``, the result files of ``src/hysteresis_loop.py:455-657`` are written there:
+    ``hysteresis.csv`` (one row per step), ``<name>.mh`` (B [T], J_par [T], mx, my, mz,
+    E [J/m^3]) and, every ``snapshot_every`` steps, a ``state_cfg*.vtu`` with m and U. The
+    host then downloads m once per step for the volume-averaged (mx, my, mz).
     """
     h = np.asarray(params.h_dir, dtype=np.float64)
     h = h / (np.linalg.norm(h) + 1e-30)
@@ -99,7 +113,15 @@ def run_hysteresis_loop(ctx: MeshContext, m0: np.ndarray, params: LoopParams, *,
     B_vals = field_values(params.B_start, params.B_end, params.dB, params.loop)
     if steps is not None:
         B_vals = B_vals[:steps]
-    hist = []
+    hist, mh = [], []
+    if out_dir is not None:
+        from . import io_utils
+
+        out = Path(out_dir)
+        out.mkdir(parents=True, exist_ok=True)
+        csv_path = out / "hysteresis.csv"
+        io_utils.write_hysteresis_header(csv_path)
+        conn, Ve, mag = ctx.element_table()
     t_all = time.perf_counter()
     for k, Bmag in enumerate(B_vals):
         t0 = time.perf_counter()
@@ -109,6 +131,15 @@ def run_hysteresis_loop(ctx: MeshContext, m0: np.ndarray, params: LoopParams, *,
         row = [Bmag, J, info["E"], info["gnorm"], info["iters"], info["evals"], info["demag_iters"],
                info["preco_iters"], dt]
         hist.append(row)
+        if out_dir is not None:
+            B_T, J_T = Bmag * params.Js_ref, J * params.Js_ref
+            m_k, U_k = ctx.state_get()
+            mh.append([B_T, J_T, *volume_averaged_m(m_k, conn, Ve, mag),
+                       info["E"] * params.Js_ref**2 / (2.0 * 4e-7 * np.pi)])
+            io_utils.append_hysteresis_row(csv_path, k, B_T, J_T, info["E"], info["gnorm"])
+            if snapshot_every > 0 and k % snapshot_every == 0:
+                io_utils.write_vtu_tetra(out / f"state_cfg{k:05d}_B{B_T:+.4e}T.vtu", ctx.knt, conn,
+                                         point_data={"m": m_k.astype(np.float32), "U": U_k.astype(np.float32)})
         if params.verbose:
             print(f"step {k:05d} B={Bmag * params.Js_ref:+.6e} T J_par={J * params.Js_ref:+.6e} T "
                   f"E={info['E']:.6e} t={dt:.3f}s it={info['iters']} nf={info['evals']} "
@@ -119,5 +150,7 @@ def run_hysteresis_loop(ctx: MeshContext, m0: np.ndarray, params: LoopParams, *,
             break
     m, U = ctx.state_get()
     hist = np.asarray(hist)
+    if out_dir is not None and mh:
+        io_utils.write_mh(Path(out_dir) / name, np.asarray(mh))
     return {"history": hist, "m": m, "U": U, "seconds": time.perf_counter() - t_all,
             "switching_step": switching_step(hist[:, 1]) if len(hist) else -1}

@@ -84,3 +84,30 @@ def test_sweep_switching_step_matches_oracle(small_pair):
     assert switching_step(J_gpu) == switching_step(J_ref)
     assert switching_step(J_ref) > 0
     assert np.max(np.abs(J_gpu - J_ref)) < 1e-3
+
+
+def test_sweep_writes_result_files(small_pair, tmp_path):
+    """hysteresis.csv, .mh and a .vtu snapshot from a 3-step sweep: the CSV J_par and E are the
+    sweep's, the .mh energy is E Js_ref^2 / (2 mu0) and (mx, my, mz) is the magnetic-volume
+    average of m, which for Js = 1 everywhere in the magnet equals J along each axis."""
+    from mumag_matrixfree2_amd import io_utils
+    from mumag_matrixfree2_amd.hysteresis_loop import LoopParams, run_hysteresis_loop
+
+    ctx, p, sU = small_pair
+    Js_ref = 1.61
+    h = np.array([0.017452, 0.0, 0.999848])
+    params = LoopParams(h_dir=h, B_start=2.0 / Js_ref, B_end=-8.0 / Js_ref, dB=-0.5 / Js_ref, tau_f=1e-10,
+                        cg_tol=1e-10, method="pcohen_hs", Js_ref=Js_ref)
+    res = run_hysteresis_loop(ctx, tilted_m0(ctx.N, seed=0, sigma=0.01), params, steps=3, out_dir=tmp_path,
+                              name="cube", snapshot_every=2)
+    header, csv = io_utils.read_hysteresis_csv(tmp_path / "hysteresis.csv")
+    hist = res["history"]
+    np.testing.assert_allclose(csv[:, 2], hist[:, 1] * Js_ref, rtol=1e-9)
+    np.testing.assert_allclose(csv[:, 3], hist[:, 2], rtol=1e-9)
+    mh = np.loadtxt(tmp_path / "cube.mh")
+    assert mh.shape == (3, 6)
+    np.testing.assert_allclose(mh[:, 5], hist[:, 2] * Js_ref**2 / (2 * 4e-7 * np.pi), rtol=1e-9)
+    m_fin = res["m"]
+    mx_ref = [ctx.j_parallel(m_fin, e) for e in np.eye(3)]
+    np.testing.assert_allclose(mh[-1, 2:5], mx_ref, atol=1e-12)
+    assert sorted(f.name.split("_B")[0] for f in tmp_path.glob("*.vtu")) == ["state_cfg00000", "state_cfg00002"]
