@@ -153,6 +153,8 @@ __global__ void k_post(double* __restrict__ x, const double* __restrict__ M, con
     x[i] += M[i] * (b[i] - t[i]);
 }
 
+constexpr int kUnroll = 4;  // entries per lane whose loads are in flight together
+
 // CSR SpMV with LANES lanes per row, accumulation in the matrix value type TA.
 // MODE 0: y = A x; 1: y += A x (and z = 0 on the same rows when z != nullptr); 2: y = A (x - x2);
 // 3: y = A (x2 .* x) and z = x2 .* x on the same rows (pre-smoothing from a zero guess, fused)
@@ -165,10 +167,24 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrows, const int* __restric
   TA s = 0;
   if (row < nrows) {
     const int j0 = Ap[row], j1 = Ap[row + 1];
-    for (int j = j0 + lane; j < j1; j += LANES) {
-      const int c = Ai[j];
-      const TA xv = (MODE == 2) ? (TA)((TA)x[c] - (TA)x2[c]) : (MODE == 3) ? (TA)((TA)x2[c] * (TA)x[c]) : (TA)x[c];
-      s += Ax[j] * xv;
+    // kUnroll strided entries per lane: their index and value loads are all issued before the
+    // dependent gathers of x, so a lane waits for one round trip per kUnroll entries
+    for (int jb = j0 + lane; jb < j1; jb += LANES * kUnroll) {
+      int cu[kUnroll];
+      TA au[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int j = jb + u * LANES;
+        cu[u] = j < j1 ? Ai[j] : -1;
+        au[u] = j < j1 ? Ax[j] : (TA)0;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        if (cu[u] < 0) continue;
+        const int c = cu[u];
+        const TA xv = (MODE == 2) ? (TA)((TA)x[c] - (TA)x2[c]) : (MODE == 3) ? (TA)((TA)x2[c] * (TA)x[c]) : (TA)x[c];
+        s += au[u] * xv;
+      }
     }
   }
 #pragma unroll
@@ -195,7 +211,19 @@ __global__ __launch_bounds__(kBlock) void k_spmv_post(int nrows, const int* __re
   T s = 0;
   if (row < nrows) {
     const int j0 = Ap[row], j1 = Ap[row + 1];
-    for (int j = j0 + lane; j < j1; j += LANES) s += Ax[j] * x[Ai[j]];
+    for (int jb = j0 + lane; jb < j1; jb += LANES * kUnroll) {
+      int cu[kUnroll];
+      T au[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int j = jb + u * LANES;
+        cu[u] = j < j1 ? Ai[j] : -1;
+        au[u] = j < j1 ? Ax[j] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u)
+        if (cu[u] >= 0) s += au[u] * x[cu[u]];
+    }
   }
 #pragma unroll
   for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LANES);
