@@ -104,7 +104,7 @@ def apply_algorithmic_bytes(ctx) -> int:
     coordinates (24 B/node), x (8 B/node), y (8 B/node), the blocks' node lists (4 B/entry)
     and the work items: per edge-star cluster 3 words (12 B: 8 slot indices, tet and slot
     masks), or per tet 4 packed corner indices (4 B)."""
-    if ctx.apply_shape["clusters"]:
+    if ctx.apply_shape["unit"] in ("cluster", "persistent"):
         return 12 * ctx.n_clusters + 4 * ctx.cluster_node_list_entries + 40 * ctx.N
     return 4 * ctx.E + 4 * ctx.node_list_entries + 40 * ctx.N
 
@@ -266,7 +266,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--max-coarse", type=int, default=None)
     ap.add_argument("--apply-shape", default="auto",
-                    help="'auto' (autotuned at set-up) or unit:threads, e.g. cluster:256, element:128")
+                    help="'auto' (autotuned at set-up) or unit:param, e.g. cluster:256, element:128, persistent:2")
     ap.add_argument("--threads", type=int, default=8, help="host threads driving the grains (grains64)")
     ap.add_argument("--compare-sequential", action="store_true",
                     help="grains64: also time the same steps with one thread")
@@ -297,9 +297,9 @@ def main():
     if args.apply_shape == "auto":
         shape = tune_apply(ctx)
     else:
-        unit, nt = args.apply_shape.split(":")
-        ctx.set_apply_shape(unit == "cluster", int(nt))
-        shape = {"clusters": unit == "cluster", "threads": int(nt)}
+        unit, param = args.apply_shape.split(":")
+        ctx.set_apply_shape(unit, int(param))
+        shape = {"unit": unit, "param": int(param)}
     log(f"[bench] apply shape {shape}")
     log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: N={ctx.N} E={ctx.E} "
         f"AMG levels={[l['A'].shape[0] for l in ctx.amg_levels]}")
@@ -371,8 +371,7 @@ def main():
             "workload": f"{wl['name']}_{args.precision}_sweep+2..-8T_{len(B_vals)}steps",
             "tets": ctx.E, "nodes": ctx.N, "method": "pcohen_hs", "precond": f"SA-AMG V-cycle ({args.precond})",
             "amg_levels": [int(l["A"].shape[0]) for l in ctx.amg_levels],
-            "apply_shape": {"unit": "edge-star cluster" if shape["clusters"] else "element",
-                            "threads": shape["threads"], "block_nodes": ctx.block_nodes},
+            "apply_shape": {"unit": shape["unit"], "param": shape["param"], "block_nodes": ctx.block_nodes},
             "pcg_tol": 1e-10, "tau_f": 1e-10,
         },
         "pcg_iters_per_s": pcg_its / elapsed,

@@ -87,10 +87,12 @@ int mm_set_precision(int64_t ctx, int precision);
 /* Select the geometry source of the Poisson apply (see flags of mm_ctx_create). */
 int mm_set_geometry_mode(int64_t ctx, int mode);
 
-/* Work shape of the on-the-fly Poisson apply: clusters != 0 runs it over edge-star clusters
- * (see mm_edge_clusters), 0 per element; threads = workgroup size (64, 128 or 256). Results
- * agree to rounding; the choice is a tuning knob with no reference counterpart. */
-int mm_set_apply_shape(int64_t ctx, int clusters, int threads);
+/* Work shape of the on-the-fly Poisson apply: unit 0 = one tet per thread, 1 = one edge-star
+ * cluster per thread (see mm_edge_clusters), with param = workgroup size (64, 128 or 256);
+ * unit 2 = clusters on a persistent grid of param (1..4) workgroups per CU that prefetches
+ * its next block while computing the current one (FP64 applies; the FP32 V-cycle keeps unit 1).
+ * Results agree to rounding; the choice is a tuning knob with no reference counterpart. */
+int mm_set_apply_shape(int64_t ctx, int unit, int param);
 
 /* y = A x: assembled-mode apply_A (src/poisson_solve.py:166-171) computed matrix-free. */
 int mm_poisson_apply(int64_t ctx, const double* x, double* y);

@@ -4,8 +4,10 @@ The apply (``k_apply`` / ``k_apply_cl``) has two knobs that change its speed but
 result beyond rounding:
 
 * the unit of work of a thread: one edge-star cluster (up to 6 tets sharing an edge, whose
-  8 node slots are read from LDS once) or one tet;
-* the workgroup size (64, 128 or 256 threads per element block).
+  8 node slots are read from LDS once) or one tet, and for clusters whether a persistent grid
+  walks the blocks with the next block's staging loads in flight;
+* the workgroup size (64, 128 or 256 threads per element block), or for the persistent grid
+  the workgroups per CU.
 
 A third knob, the number of distinct nodes per element block (the LDS staging size, hence
 the elements per block), is fixed when a context is created; ``tune_block_nodes`` times
@@ -18,25 +20,22 @@ apply is a fixed JAX segment-sum or a CSR SpMV (``src/poisson_solve.py:166-171,2
 
 from __future__ import annotations
 
-SHAPES = ((True, 256), (True, 128), (True, 64), (False, 256), (False, 128), (False, 64))
-
-
-def _label(clusters: bool, threads: int) -> str:
-    return f"{'cluster' if clusters else 'element'}x{threads}"
+SHAPES = (("persistent", 2), ("persistent", 3), ("cluster", 256), ("cluster", 128), ("cluster", 64),
+          ("element", 256), ("element", 128), ("element", 64))
 
 
 def tune_apply(ctx, shapes=SHAPES, reps: int = 40) -> dict:
-    """Time every (clusters, threads) shape on ``ctx``, keep the fastest set on the context and
-    return ``{"clusters", "threads", "us_per_apply": {label: microseconds}}``."""
+    """Time every (unit, param) shape on ``ctx``, keep the fastest set on the context and return
+    ``{"unit", "param", "us_per_apply": {"unit:param": microseconds}}``."""
     times = {}
-    for clusters, threads in shapes:
-        ctx.set_apply_shape(clusters, threads)
+    for unit, param in shapes:
+        ctx.set_apply_shape(unit, param)
         ctx.bench_apply(reps=3)
-        times[(clusters, threads)] = ctx.bench_apply(reps=reps) * 1e3
+        times[(unit, param)] = ctx.bench_apply(reps=reps) * 1e3
     best = min(times, key=times.get)
     ctx.set_apply_shape(*best)
-    return {"clusters": best[0], "threads": best[1],
-            "us_per_apply": {_label(*k): round(v, 3) for k, v in times.items()}}
+    return {"unit": best[0], "param": best[1],
+            "us_per_apply": {f"{u}:{q}": round(v, 3) for (u, q), v in times.items()}}
 
 
 def tune_block_nodes(make_ctx, caps=(128, 192, 256), reps: int = 40):

@@ -17,6 +17,7 @@ from ._hip import call, f64, i32, ptr
 
 GEOM_MODES = {"on_the_fly": 0, "stored_f64": 1, "stored_f32": 2, "on_the_fly_f32": 3}
 METHODS = {"pcohen_hs": 0, "lbfgs": 1}
+APPLY_UNITS = {"element": 0, "cluster": 1, "persistent": 2}
 
 
 class MeshContext:
@@ -75,7 +76,7 @@ class MeshContext:
         self.cluster_node_list_entries = int(info[9])
         self.n_clusters = int(info[10])
         self.block_nodes = int(info[11])
-        self.apply_shape = {"clusters": True, "threads": 256}
+        self.apply_shape = {"unit": "cluster", "param": 256}
         self.precond_type = None
         self.amg_levels = None
         self.precision = "fp64"
@@ -127,11 +128,13 @@ class MeshContext:
         call("mm_set_precision", self.handle, {"fp64": 0, "mixed": 1}[precision])
         self.precision = precision
 
-    def set_apply_shape(self, clusters: bool = True, threads: int = 256):
-        """Work shape of the on-the-fly Poisson apply: edge-star clusters or single elements per
-        thread, and the workgroup size (64, 128, 256)."""
-        call("mm_set_apply_shape", self.handle, int(bool(clusters)), int(threads))
-        self.apply_shape = {"clusters": bool(clusters), "threads": int(threads)}
+    def set_apply_shape(self, unit: str = "cluster", param: int = 256):
+        """Work shape of the on-the-fly Poisson apply: ``unit`` 'element' (one tet per thread) or
+        'cluster' (one edge-star cluster per thread) with ``param`` = workgroup size (64, 128,
+        256), or 'persistent' (clusters on a persistent grid that prefetches its next block) with
+        ``param`` = workgroups per CU (1..4)."""
+        call("mm_set_apply_shape", self.handle, APPLY_UNITS[unit], int(param))
+        self.apply_shape = {"unit": unit, "param": int(param)}
 
     def set_geometry_mode(self, geom: str):
         call("mm_set_geometry_mode", self.handle, GEOM_MODES[geom])

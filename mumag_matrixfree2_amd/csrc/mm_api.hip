@@ -426,6 +426,7 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   c.node_cap = (flags >> 8) & 511 ? (flags >> 8) & 511 : kMaxBlockNodes;
   MM_CHECK(c.geom_mode <= 2, "invalid geometry mode");
   MM_HIP(hipGetDevice(&c.device));
+  MM_HIP(hipDeviceGetAttribute(&c.num_cu, hipDeviceAttributeMultiprocessorCount, c.device));
   MM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   MM_HIP(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming));
 
@@ -764,11 +765,20 @@ int mm_set_geometry_mode(int64_t h, int mode) {
   return 0;
 }
 
-int mm_set_apply_shape(int64_t h, int clusters, int threads) {
+int mm_set_apply_shape(int64_t h, int unit, int param) {
   GET_CTX(h);
-  MM_CHECK(threads == 64 || threads == 128 || threads == 256, "threads must be 64, 128 or 256");
-  c.apply_clusters = clusters != 0;
-  c.elem_threads = threads;
+  MM_CHECK(unit >= 0 && unit <= 2, "unit must be 0 (element), 1 (cluster) or 2 (persistent cluster)");
+  if (unit == 2) {
+    MM_CHECK(param >= 1 && param <= 4, "persistent cluster apply: 1..4 workgroups per CU");
+    c.apply_clusters = true;
+    c.apply_persist = param;
+    c.elem_threads = kMaxBlockNodes;
+    return 0;
+  }
+  MM_CHECK(param == 64 || param == 128 || param == 256, "threads must be 64, 128 or 256");
+  c.apply_clusters = unit == 1;
+  c.apply_persist = 0;
+  c.elem_threads = param;
   return 0;
 }
 

@@ -134,6 +134,8 @@ struct Ctx {
   Part part_cl;                       // edge-star clusters of all elements (Poisson apply)
   long long nodes_cl = 0;             // node-list entries of the cluster partition
   bool apply_clusters = true;         // Poisson apply over clusters (else per element)
+  int apply_persist = 0;              // > 0: FP64 cluster apply as a persistent grid, this many workgroups per CU
+  int num_cu = 256;                   // compute units of the device
   Geo geo;
   int n_mag = 0;                      // magnetic elements
   long long nodes_all = 0;            // node-list entries of the all-element partition
@@ -199,8 +201,10 @@ int timed_apply(Ctx& c, const double* x, double* y, hipStream_t s, double* dot_p
 int resolve_apply_timing(Ctx& c);
 
 // element kernels (mm_kernels.hip)
-// dot_part != nullptr: also write per-workgroup partials of x . (A x) (grid_for(part_all) of them)
+// dot_part != nullptr: also write per-block partials of x . (A x) (apply_dot_count(c) of them)
 int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s, double* dot_part = nullptr);
+// number of x . (A x) partials the on-the-fly apply writes
+int apply_dot_count(const Ctx& c);
 int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s);
 // y = A (w .* r) with xo = w .* r written by the node owners (fused pre-smoothing); y zeroed first
 int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s);

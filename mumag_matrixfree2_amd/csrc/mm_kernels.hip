@@ -11,6 +11,7 @@
 //      a hardware FP64 atomic add (global_atomic_add_f64) for shared nodes.
 // Workgroup ids are remapped so that each XCD (ids i mod 8) runs a contiguous, spatially
 // compact range of blocks and keeps its node data in its own L2.
+#include <algorithm>
 #include <type_traits>
 
 #include "mm_device.h"
@@ -242,13 +243,84 @@ __global__ __launch_bounds__(NT) void k_apply_f32(Part p, Geo geo, const float* 
 }
 
 // ------------------------------------------- Poisson apply over edge-star clusters (K1)
-// Same operator as k_apply, one cluster per thread: up to 6 tets sharing the edge
-// (slot 0, slot 1), tet j with corners (slot 0, slot 1, slot 2+j, slot 2+(j+1)%6). The 8
-// slots' coordinates and values are read from LDS once, the edge vector c1 = x1 - x0 and
-// the cross products c1 x (x_s - x0) are shared by the tets, and the slot sums leave
-// through 8 LDS atomics. T = double: the FP64 operator (x, y FP64, DOT as in k_apply);
-// T = float: the FP32 operator of the mixed-precision V-cycle (block-relative FP32
-// coordinates, FP64 LDS accumulation). SCALED: input w .* r from the FP64 residual r.
+// One cluster per thread: up to 6 tets sharing the edge (slot 0, slot 1), tet j with corners
+// (slot 0, slot 1, slot 2+j, slot 2+(j+1)%6). The 8 slots' coordinates and values are read
+// from LDS once, the edge vector c1 = x1 - x0 and the cross products c1 x (x_s - x0) are
+// shared by the tets, and the slot sums leave through 8 LDS atomics. Adds the tets' energies
+// sum_a x_a y_a to acc_e when DOT.
+template <typename T, bool DOT>
+__device__ __forceinline__ void cluster_apply(uint32_t w0, uint32_t w1, uint32_t wm, const T (*sX)[kMaxBlockNodes],
+                                              const T* sx, double* sy, double& acc_e) {
+  int li[8];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    li[q] = (w0 >> (8 * q)) & 255u;
+    li[4 + q] = (w1 >> (8 * q)) & 255u;
+  }
+  T X[8][3], v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    if (wm & (1u << (8 + q))) {
+      X[q][0] = sX[0][li[q]];
+      X[q][1] = sX[1][li[q]];
+      X[q][2] = sX[2][li[q]];
+      v[q] = sx[li[q]];
+    } else {
+      X[q][0] = X[q][1] = X[q][2] = v[q] = (T)0;
+    }
+  }
+  // edge vectors from slot 0, value differences, shared cross products m_s = c1 x c_s
+  T c[8][3], dv[8], m[8][3];
+#pragma unroll
+  for (int q = 1; q < 8; ++q) {
+    c[q][0] = X[q][0] - X[0][0];
+    c[q][1] = X[q][1] - X[0][1];
+    c[q][2] = X[q][2] - X[0][2];
+    dv[q] = v[q] - v[0];
+  }
+#pragma unroll
+  for (int q = 2; q < 8; ++q) {
+    m[q][0] = c[1][1] * c[q][2] - c[1][2] * c[q][1];
+    m[q][1] = c[1][2] * c[q][0] - c[1][0] * c[q][2];
+    m[q][2] = c[1][0] * c[q][1] - c[1][1] * c[q][0];
+  }
+  T acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = (T)0;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (wm & (1u << j)) {
+      const int s2 = 2 + j, s3 = 2 + (j + 1) % 6;
+      T n[3][3];
+      // corners (0, 1, s2, s3): n_1 = c_s2 x c_s3, n_2 = c_s3 x c_1 = -m_s3, n_3 = c_1 x c_s2 = m_s2
+      n[0][0] = c[s2][1] * c[s3][2] - c[s2][2] * c[s3][1];
+      n[0][1] = c[s2][2] * c[s3][0] - c[s2][0] * c[s3][2];
+      n[0][2] = c[s2][0] * c[s3][1] - c[s2][1] * c[s3][0];
+      n[1][0] = -m[s3][0];
+      n[1][1] = -m[s3][1];
+      n[1][2] = -m[s3][2];
+      n[2][0] = m[s2][0];
+      n[2][1] = m[s2][1];
+      n[2][2] = m[s2][2];
+      const T det = c[1][0] * n[0][0] + c[1][1] * n[0][1] + c[1][2] * n[0][2];
+      T ya[4];
+      const T en = tet_laplace(n, det, dv[1], dv[s2], dv[s3], ya);
+      if constexpr (DOT) acc_e += (double)en;
+      acc[0] += ya[0];
+      acc[1] += ya[1];
+      acc[s2] += ya[2];
+      acc[s3] += ya[3];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (wm & (1u << (8 + q))) atomicAdd(&sy[li[q]], (double)acc[q]);
+}
+
+// Same operator as k_apply, one workgroup per block of clusters. T = double: the FP64
+// operator (x, y FP64, DOT as in k_apply); T = float: the FP32 operator of the
+// mixed-precision V-cycle (block-relative FP32 coordinates, FP64 LDS accumulation).
+// SCALED: input w .* r from the FP64 residual r.
 template <typename T, bool SCALED, bool DOT, int NT>
 __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __restrict__ x,
                                                  const double* __restrict__ r, const T* __restrict__ w,
@@ -306,74 +378,9 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
   }
   __syncthreads();
   double acc_e = 0.0;
-  auto cluster = [&](uint32_t w0, uint32_t w1, uint32_t wm) {
-    int li[8];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      li[q] = (w0 >> (8 * q)) & 255u;
-      li[4 + q] = (w1 >> (8 * q)) & 255u;
-    }
-    T X[8][3], v[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      if (wm & (1u << (8 + q))) {
-        X[q][0] = sX[0][li[q]];
-        X[q][1] = sX[1][li[q]];
-        X[q][2] = sX[2][li[q]];
-        v[q] = sx[li[q]];
-      } else {
-        X[q][0] = X[q][1] = X[q][2] = v[q] = (T)0;
-      }
-    }
-    // edge vectors from slot 0, value differences, shared cross products m_s = c1 x c_s
-    T c[8][3], dv[8], m[8][3];
-#pragma unroll
-    for (int q = 1; q < 8; ++q) {
-      c[q][0] = X[q][0] - X[0][0];
-      c[q][1] = X[q][1] - X[0][1];
-      c[q][2] = X[q][2] - X[0][2];
-      dv[q] = v[q] - v[0];
-    }
-#pragma unroll
-    for (int q = 2; q < 8; ++q) {
-      m[q][0] = c[1][1] * c[q][2] - c[1][2] * c[q][1];
-      m[q][1] = c[1][2] * c[q][0] - c[1][0] * c[q][2];
-      m[q][2] = c[1][0] * c[q][1] - c[1][1] * c[q][0];
-    }
-    T acc[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = (T)0;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      if (wm & (1u << j)) {
-        const int s2 = 2 + j, s3 = 2 + (j + 1) % 6;
-        T n[3][3];
-        // corners (0, 1, s2, s3): n_1 = c_s2 x c_s3, n_2 = c_s3 x c_1 = -m_s3, n_3 = c_1 x c_s2 = m_s2
-        n[0][0] = c[s2][1] * c[s3][2] - c[s2][2] * c[s3][1];
-        n[0][1] = c[s2][2] * c[s3][0] - c[s2][0] * c[s3][2];
-        n[0][2] = c[s2][0] * c[s3][1] - c[s2][1] * c[s3][0];
-        n[1][0] = -m[s3][0];
-        n[1][1] = -m[s3][1];
-        n[1][2] = -m[s3][2];
-        n[2][0] = m[s2][0];
-        n[2][1] = m[s2][1];
-        n[2][2] = m[s2][2];
-        const T det = c[1][0] * n[0][0] + c[1][1] * n[0][1] + c[1][2] * n[0][2];
-        T ya[4];
-        const T en = tet_laplace(n, det, dv[1], dv[s2], dv[s3], ya);
-        if constexpr (DOT) acc_e += (double)en;
-        acc[0] += ya[0];
-        acc[1] += ya[1];
-        acc[s2] += ya[2];
-        acc[s3] += ya[3];
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      if (wm & (1u << (8 + q))) atomicAdd(&sy[li[q]], (double)acc[q]);
-  };
-  if (c0 < bi.e1) cluster(cw0, cw1, cwm);
-  for (int cc = c0 + NT; cc < bi.e1; cc += NT) cluster(p.lconn[cc], p.lconn2[cc], p.cmask[cc]);
+  if (c0 < bi.e1) cluster_apply<T, DOT>(cw0, cw1, cwm, sX, sx, sy, acc_e);
+  for (int cc = c0 + NT; cc < bi.e1; cc += NT)
+    cluster_apply<T, DOT>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kNode; ++q) {
@@ -402,6 +409,112 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
   if constexpr (DOT) {
     double v1[1] = {acc_e};
     block_partial<1, NT>(v1, dot_part);
+  }
+}
+
+// Persistent form of k_apply_cl (FP64): a grid of a few workgroups per CU walks the blocks of
+// its XCD's range, and while it computes the clusters of one block it already has the node
+// flags, values and coordinates of its next block in flight (register double buffering), so
+// the global-load latency of the staging overlaps the LDS work of the previous block. One
+// node per thread (NT == kMaxBlockNodes). DOT partials go to dot_part[block id].
+template <bool SCALED, bool DOT>
+__global__ __launch_bounds__(kMaxBlockNodes) void k_apply_clp(Part p, Geo geo, const double* __restrict__ x,
+                                                              const double* __restrict__ r,
+                                                              const double* __restrict__ w, double* __restrict__ xo,
+                                                              double* __restrict__ y, double reg,
+                                                              double* __restrict__ dot_part) {
+  constexpr int NT = kMaxBlockNodes;
+  __shared__ double sx[kMaxBlockNodes];
+  __shared__ double sy[kMaxBlockNodes];
+  __shared__ double sX[3][kMaxBlockNodes];
+  __shared__ double sred[NT / 64];
+  const int tid = threadIdx.x;
+  const int nxcd = (int)(gridDim.x >> 3);          // workgroups per XCD (grid is a multiple of 8)
+  const int chunk = (p.nblk + 7) >> 3;
+  const int b_end = min(p.nblk, ((int)(blockIdx.x & 7) + 1) * chunk);
+  int b = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+  // staged registers of the current block. The loads below are unconditional (indices are
+  // clamped into the block, out-of-range lanes discard their values): a load under a branch
+  // makes the compiler wait for every outstanding load at the join.
+  uint32_t f = 0u;
+  double xv = 0.0, X0 = 0.0, X1 = 0.0, X2 = 0.0;
+  int e0 = 0, e1 = 0, n0 = 0, nb = 0;
+  auto fetch_meta = [&](int bb) {
+    e0 = p.eptr[bb];
+    e1 = p.eptr[bb + 1];
+    n0 = p.nptr[bb];
+    nb = p.nptr[bb + 1] - n0;
+  };
+  auto fetch_node = [&]() { f = p.nodes[n0 + min(tid, nb - 1)]; };
+  auto fetch_vals = [&]() {
+    const uint32_t id = f & kIdMask;
+    xv = SCALED ? r[id] * w[id] : x[id];
+    X0 = geo.knt[3 * id];
+    X1 = geo.knt[3 * id + 1];
+    X2 = geo.knt[3 * id + 2];
+  };
+  if (b < b_end) {
+    fetch_meta(b);
+    fetch_node();
+    fetch_vals();
+  }
+  for (; b < b_end; b += nxcd) {
+    // stage the current block
+    const int ce0 = e0, ce1 = e1, cnb = nb;
+    const uint32_t cf = tid < cnb ? f : 0u;
+    if (tid < cnb) {
+      if (SCALED && (cf & kOwner)) xo[cf & kIdMask] = xv;
+      sx[tid] = (cf & kBoundary) ? 0.0 : xv;
+      sy[tid] = 0.0;
+      sX[0][tid] = X0;
+      sX[1][tid] = X1;
+      sX[2][tid] = X2;
+    }
+    const double cxv = xv;
+    // this thread's first cluster words are loaded before any prefetch is issued: vector loads
+    // complete in order, so a later wait for them would also wait for the prefetch
+    const int c0 = ce0 + tid;
+    const int cl = min(c0, ce1 - 1);
+    const uint32_t cw0 = p.lconn[cl], cw1 = p.lconn2[cl], cwm = c0 < ce1 ? p.cmask[cl] : 0u;
+    __syncthreads();
+    // next block (this one again after the last): metadata and node flags in flight during the
+    // cluster work, its node values and coordinates during the write-back
+    const int bn = b + nxcd < b_end ? b + nxcd : b;
+    fetch_meta(bn);
+    fetch_node();
+    double acc_e = 0.0;
+    cluster_apply<double, DOT>(cw0, cw1, cwm, sX, sx, sy, acc_e);  // cwm == 0: no tets, no slots
+    for (int cc = c0 + NT; cc < ce1; cc += NT)
+      cluster_apply<double, DOT>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
+    fetch_vals();
+    __syncthreads();
+    if (tid < cnb) {
+      const uint32_t id = cf & kIdMask;
+      if (cf & kBoundary) {
+        if (cf & kOwner) {
+          y[id] = cxv;
+          if constexpr (DOT) acc_e += cxv * cxv;
+        }
+      } else {
+        double v = sy[tid];
+        if (cf & kOwner) {
+          v += reg * sx[tid];
+          if constexpr (DOT) acc_e += reg * sx[tid] * sx[tid];
+        }
+        node_out(y, cf, v);
+      }
+    }
+    if constexpr (DOT) {
+      const double t = wave_red<false>(acc_e);
+      if ((tid & 63) == 0) sred[tid >> 6] = t;
+      __syncthreads();
+      if (tid == 0) {
+        double s = sred[0];
+        for (int q = 1; q < NT / 64; ++q) s += sred[q];
+        dot_part[b] = s;
+      }
+    }
+    __syncthreads();  // LDS is restaged for the next block
   }
 }
 
@@ -781,8 +894,30 @@ static void with_nt(int nt, F&& f) {
   }
 }
 
+// persistent grid: apply_persist workgroups per CU, a multiple of 8 (one share per XCD)
+static int persist_grid(const Ctx& c) {
+  const int g = 8 * std::max(1, (c.num_cu * c.apply_persist) / 8);
+  return std::min(g, 8 * ((c.part_cl.nblk + 7) / 8));
+}
+
+int apply_dot_count(const Ctx& c) {
+  if (c.apply_clusters && c.apply_persist > 0) return c.part_cl.nblk;
+  return 8 * (((c.apply_clusters ? c.part_cl : c.part_all).nblk + 7) / 8);
+}
+
 int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s, double* dot_part) {
   MM_CHECK(dot_part == nullptr || geom == kGeomCoords, "x.Ax partials need the on-the-fly apply");
+  if (geom == kGeomCoords && c.apply_clusters && c.apply_persist > 0) {
+    const Part& p = c.part_cl;
+    if (p.nblk == 0) return 0;
+    if (dot_part)
+      k_apply_clp<false, true><<<persist_grid(c), kMaxBlockNodes, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
+                                                                        c.reg, dot_part);
+    else
+      k_apply_clp<false, false><<<persist_grid(c), kMaxBlockNodes, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
+                                                                         c.reg, nullptr);
+    return 0;
+  }
   if (geom == kGeomCoords && c.apply_clusters) {
     const Part& p = c.part_cl;
     if (p.nblk == 0) return 0;
@@ -818,6 +953,11 @@ int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t
 int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s) {
   const Part& p = c.apply_clusters ? c.part_cl : c.part_all;
   if (p.nblk == 0) return 0;
+  if (c.apply_clusters && c.apply_persist > 0) {
+    k_apply_clp<true, false><<<persist_grid(c), kMaxBlockNodes, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, c.reg,
+                                                                      nullptr);
+    return 0;
+  }
   const int grid = grid_for(p);
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;

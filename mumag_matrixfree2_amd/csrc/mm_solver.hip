@@ -523,7 +523,7 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
   const int gv = red_grid(N);
   // p.Ap from the apply's per-block element energies (on-the-fly geometry), else a dot pass
   const bool fused = c.geom_mode == kGeomCoords;
-  const int n_pq = fused ? 8 * (((c.apply_clusters ? c.part_cl : c.part_all).nblk + 7) / 8) : gv;
+  const int n_pq = fused ? apply_dot_count(c) : gv;
   vfill(c.pq, 0.0, N, s);
   MM_TRY(timed_apply(c, U, c.pq, s));
   c.n_apply++;

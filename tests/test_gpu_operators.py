@@ -52,6 +52,40 @@ def test_poisson_apply(ctx_c0, oracle_c0, geom):
     assert rel(y, oracle_c0.A @ x) <= TOL
 
 
+@pytest.mark.parametrize("shape", ["persistent:1", "persistent:3", "cluster:256", "cluster:64", "element:128",
+                                   "element:64"])
+def test_poisson_apply_every_work_shape(ctx_c0, oracle_c0, shape):
+    """Every apply shape the autotuner can pick gives the oracle's A x, in one apply and inside
+    a PCG solve (the persistent grid writes its x.Ax partials per block)."""
+    from oracle.solvers import solve_U as oracle_solve_U
+
+    unit, param = shape.split(":")
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(ctx_c0.N)
+    ctx_c0.set_apply_shape(unit, int(param))
+    try:
+        assert rel(ctx_c0.apply_A(x), oracle_c0.A @ x) <= TOL
+        if ctx_c0.precond_type is None:
+            ctx_c0.setup_precond("jacobi")
+        m = tilted_m0(ctx_c0.N, seed=3)
+        U, it, rr = ctx_c0.solve_U(m, np.zeros(ctx_c0.N), 1e-10)
+        U_ref, _, _ = oracle_solve_U(oracle_c0, m, np.zeros(ctx_c0.N), 1e-12, precond="direct")
+        assert rr <= 1e-10 and rel(U, U_ref) <= 1e-7
+    finally:
+        ctx_c0.set_apply_shape("cluster", 256)
+
+
+def test_autotuner_keeps_the_fastest_shape(ctx_c0):
+    from mumag_matrixfree2_amd.autotune import tune_apply
+
+    res = tune_apply(ctx_c0, reps=5)
+    times = res["us_per_apply"]
+    assert len(times) == 8 and all(t > 0 for t in times.values())
+    assert times[f"{res['unit']}:{res['param']}"] == min(times.values())
+    assert ctx_c0.apply_shape == {"unit": res["unit"], "param": res["param"]}
+    ctx_c0.set_apply_shape("cluster", 256)
+
+
 def test_poisson_apply_fp32_geometry(ctx_c0, oracle_c0):
     rng = np.random.default_rng(2)
     x = rng.standard_normal(ctx_c0.N)
