@@ -341,11 +341,28 @@ def main():
     pcg_its = c1["pcg_iters"] - c0["pcg_iters"]
     errors = None
     if args.precision == "mixed":
-        # reference FP64 sweep in the same run: per-step relative energy error and switching step
+        # reference FP64 sweep in the same run: per-step relative energy error (energies in units
+        # of Kd V_mag, floored at 1), switching step, and the final states' potential, gradient
+        # and magnetisation; plus the mixed and FP64 solvers on the same magnetisation
+        B_last = B_vals[steps - 1] * H_DIR
+        m_mx, U_mx = ctx.state_get()
+        _, g_mx = ctx.energy_grad(m_mx, U_mx, B_last)
         t64, J64, E64, _, _ = sweep("fp64", timed=False)
+        m64, U64 = ctx.state_get()
+        _, g64 = ctx.energy_grad(m64, U64, B_last)
+        U_same64, _, _ = ctx.solve_U(m64, np.zeros(ctx.N), 1e-10)
+        ctx.set_precision("mixed")
+        U_samemx, _, _ = ctx.solve_U(m64, np.zeros(ctx.N), 1e-10)
+        ctx.set_precision(args.precision)
+
+        def rel(a, b):
+            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
         errors = {
             "max_rel_energy_err_vs_fp64": float(np.max(np.abs(Es - E64) / np.maximum(np.abs(E64), 1.0))),
             "switching_step_fp64": switching_step(J64),
+            "final_state_rel_err": {"U": rel(U_mx, U64), "grad": rel(g_mx, g64), "m": rel(m_mx, m64)},
+            "same_m_potential_rel_err": rel(U_samemx, U_same64),
             "fp64_field_steps_per_s_same_run": steps / t64,
             "speedup_vs_fp64": t64 / elapsed,
         }
