@@ -63,6 +63,17 @@ int mm_ctx_create(int N, int E, const double* knt, const int32_t* conn, const in
                   const double* bnd_mask, double reg, int flags, int64_t* out_ctx);
 int mm_ctx_destroy(int64_t ctx);
 
+/* Context over several independent systems (segments) at once, e.g. the grains of a grain set
+ * concatenated into one mesh: segment g owns nodes seg_node_off[g] .. seg_node_off[g+1]-1
+ * (seg_node_off[0] = 0, seg_node_off[nseg] = N) and every element lies in one segment. Element
+ * blocks never span two segments, so mm_solve_U_segmented can solve all segments in one launch
+ * per kernel with per-segment PCG scalars. Other arguments as mm_ctx_create. No reference
+ * counterpart: the reference solves one system per solve_U call (src/poisson_solve.py:748-771). */
+int mm_ctx_create_segmented(int N, int E, const double* knt, const int32_t* conn, const int32_t* mat_id, int G,
+                            const double* A_red, const double* K1_red, const double* Js_red, const double* k_easy,
+                            const double* bnd_mask, double reg, int flags, int nseg, const int32_t* seg_node_off,
+                            int64_t* out_ctx);
+
 /* info[0..11] = V_mag, N, E, element blocks (all), element blocks (magnetic),
  * largest node list of a block, magnetic elements, node-list entries of all blocks,
  * cluster blocks, node-list entries of the cluster blocks, edge-star clusters, block node cap */
@@ -123,6 +134,13 @@ int mm_pcg_solve(int64_t ctx, const double* b, double* x_inout, double tol, int 
 
 /* solve_U (src/poisson_solve.py:748-771): U_inout holds x0 on entry. */
 int mm_solve_U(int64_t ctx, const double* m, double* U_inout, double tol, int maxiter, int* it, double* rel_res);
+
+/* solve_U (src/poisson_solve.py:748-771) for every segment of a segmented context at once:
+ * one launch per kernel for all segments, while each segment keeps its own alpha, beta and
+ * stopping test (r_g.r_g <= tol[g]^2 b_g.b_g); it[g] and rel_res[g] per segment, as
+ * mm_solve_U would return for the segment alone. Needs the FP64 AMG V-cycle. */
+int mm_solve_U_segmented(int64_t ctx, const double* m, double* U_inout, const double* tol, int maxiter, int* it,
+                         double* rel_res);
 
 /* energy_and_grad (src/energy_kernels.py:340-459): E scalar, g (N,3). */
 int mm_energy_grad(int64_t ctx, const double* m, const double* U, const double* B, double* E, double* g);

@@ -64,6 +64,8 @@ SIGNATURES = {
     "mm_ctx_create": [C.c_int, C.c_int, f64p, i32p, i32p, C.c_int, f64p, f64p, f64p, f64p, f64p, C.c_double,
                       C.c_int, i64p],
     "mm_ctx_destroy": [C.c_int64],
+    "mm_ctx_create_segmented": [C.c_int, C.c_int, f64p, i32p, i32p, C.c_int, f64p, f64p, f64p, f64p, f64p, C.c_double,
+                                C.c_int, C.c_int, i32p, i64p],
     "mm_ctx_info": [C.c_int64, f64p],
     "mm_ctx_get_geometry": [C.c_int64, i32p, f64p, f64p],
     "mm_ctx_get_nodal": [C.c_int64, f64p, f64p, f64p],
@@ -81,6 +83,7 @@ SIGNATURES = {
     "mm_ctx_get_exchange_diag": [C.c_int64, f64p],
     "mm_pcg_solve": [C.c_int64, f64p, f64p, C.c_double, C.c_int, i32p, f64p],
     "mm_solve_U": [C.c_int64, f64p, f64p, C.c_double, C.c_int, i32p, f64p],
+    "mm_solve_U_segmented": [C.c_int64, f64p, f64p, f64p, C.c_int, i32p, f64p],
     "mm_energy_grad": [C.c_int64, f64p, f64p, f64p, f64p, f64p],
     "mm_local_grad": [C.c_int64, f64p, f64p],
     "mm_j_parallel": [C.c_int64, f64p, f64p, f64p],

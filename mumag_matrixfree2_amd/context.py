@@ -33,10 +33,12 @@ class MeshContext:
         reg: Poisson diagonal regularisation.
         geom: geometry source of the Poisson apply ('on_the_fly', 'stored_f64', 'stored_f32').
         block_nodes: distinct nodes per element block (16..256), the LDS staging size.
+        segments: optional node offsets (nseg + 1,) of independent systems sharing the mesh
+            (``mm_ctx_create_segmented``); enables ``solve_U_segmented``.
     """
 
     def __init__(self, knt, conn, mat_id, A_red, K1_red, Js_red, k_easy, boundary_mask=None, reg=1e-12,
-                 geom="on_the_fly", block_nodes: int = 256):
+                 geom="on_the_fly", block_nodes: int = 256, segments=None):
         self.knt = f64(knt).reshape(-1, 3)
         conn = i32(np.asarray(conn)[:, :4])
         mat_id = i32(mat_id)
@@ -61,8 +63,14 @@ class MeshContext:
         if not 16 <= int(block_nodes) <= 256:
             raise ValueError("block_nodes must be in [16, 256]")
         flags = GEOM_MODES[geom] | (int(block_nodes) % 256) << 8
-        call("mm_ctx_create", self.N, self.E, ptr(self.knt), ptr(conn), ptr(mat_id), G, ptr(A_red), ptr(K1_red),
-             ptr(Js_red), ptr(k_easy), ptr(bnd), self.reg, flags, C.byref(h))
+        self.segments = None if segments is None else i32(segments)
+        if self.segments is None:
+            call("mm_ctx_create", self.N, self.E, ptr(self.knt), ptr(conn), ptr(mat_id), G, ptr(A_red), ptr(K1_red),
+                 ptr(Js_red), ptr(k_easy), ptr(bnd), self.reg, flags, C.byref(h))
+        else:
+            call("mm_ctx_create_segmented", self.N, self.E, ptr(self.knt), ptr(conn), ptr(mat_id), G, ptr(A_red),
+                 ptr(K1_red), ptr(Js_red), ptr(k_easy), ptr(bnd), self.reg, flags, self.segments.size - 1,
+                 ptr(self.segments), C.byref(h))
         self.handle = h.value
         info = np.zeros(12)
         call("mm_ctx_info", self.handle, ptr(info))
@@ -220,6 +228,21 @@ class MeshContext:
         rr = C.c_double(0.0)
         call("mm_solve_U", self.handle, ptr(m), ptr(U), float(tol), int(maxiter), C.byref(it), C.byref(rr))
         return U, int(it.value), float(rr.value)
+
+    def solve_U_segmented(self, m, x0, tol, maxiter=2000):
+        """solve_U for every segment at once (one launch per kernel for all segments, per-segment
+        PCG scalars): (U, iterations per segment, relative residual per segment). ``tol`` is a
+        scalar or one value per segment."""
+        if self.segments is None:
+            raise RuntimeError("solve_U_segmented needs a context created with segments")
+        nseg = self.segments.size - 1
+        m = f64(m).reshape(-1)
+        U = f64(np.array(x0, dtype=np.float64, copy=True))
+        tol = f64(np.broadcast_to(np.asarray(tol, dtype=np.float64), (nseg,)))
+        it = np.zeros(nseg, dtype=np.int32)
+        rr = np.zeros(nseg)
+        call("mm_solve_U_segmented", self.handle, ptr(m), ptr(U), ptr(tol), int(maxiter), ptr(it), ptr(rr))
+        return U, it, rr
 
     # ------------------------------------------------------------------ minimiser
     def state_set(self, m, U=None):

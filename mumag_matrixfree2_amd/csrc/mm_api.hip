@@ -59,7 +59,8 @@ struct HostPart {
 // block touching a node (owner), nodes touched by one block only (private) and Dirichlet nodes.
 static void build_partition(int N, const std::vector<int>& conn, const std::vector<int>& mat0,
                             const std::vector<int>& elems, const std::vector<char>& is_bnd, HostPart& hp,
-                            int node_cap) {
+                            int node_cap, const std::vector<int>& seg_of) {
+  int blk_seg = -1;
   std::vector<int> stamp(N, -1), local(N, 0), touches(N, 0);
   std::vector<char> owned(N, 0);
   hp.eptr.assign(1, 0);
@@ -141,8 +142,11 @@ static void build_partition(int N, const std::vector<int>& conn, const std::vect
       for (int b2 = 0; b2 < a; ++b2) dup |= conn[4 * e + b2] == v;
       if (!dup && stamp[v] != blk) ++fresh;
     }
-    if ((int)blk_nodes.size() + fresh > node_cap || (int)(hp.lconn.size() - blk_e0) >= kMaxBlockElems)
+    const int sg = seg_of[conn[4 * e]];
+    if (!blk_nodes.empty() && ((int)blk_nodes.size() + fresh > node_cap ||
+                               (int)(hp.lconn.size() - blk_e0) >= kMaxBlockElems || sg != blk_seg))
       close_block();
+    blk_seg = sg;
     uint32_t lc = 0;
     for (int a = 0; a < 4; ++a) {
       const int v = conn[4 * e + a];
@@ -307,7 +311,8 @@ struct HostCPart {
 };
 
 static void build_cluster_partition(int N, std::vector<HostCluster>& cl, const std::vector<char>& is_bnd, HostCPart& hp,
-                                    int node_cap) {
+                                    int node_cap, const std::vector<int>& seg_of) {
+  int blk_seg = -1;
   std::vector<std::pair<uint64_t, int>> key(cl.size());
   for (size_t i = 0; i < cl.size(); ++i) {
     int lo = INT32_MAX, hi = -1;
@@ -374,7 +379,11 @@ static void build_cluster_partition(int N, std::vector<HostCluster>& cl, const s
       for (int q2 = 0; q2 < q; ++q2) dup |= h.node[q2] == v;
       if (!dup && stamp[v] != blk) ++fresh;
     }
-    if ((int)blk_nodes.size() + fresh > node_cap || (int)members.size() >= kMaxBlockElems) close_block();
+    const int sg = seg_of[h.node[0]];
+    if (!blk_nodes.empty() &&
+        ((int)blk_nodes.size() + fresh > node_cap || (int)members.size() >= kMaxBlockElems || sg != blk_seg))
+      close_block();
+    blk_seg = sg;
     for (int q = 0; q < 8; ++q) {
       const int v = h.node[q];
       if (v >= 0 && stamp[v] != blk) {
@@ -413,10 +422,11 @@ static int upload_part(Ctx& c, const HostPart& hp, int E, Part& p) {
 
 static int create(int N, int E, const double* knt, const int32_t* conn_in, const int32_t* mat_id, int G,
                   const double* A_red, const double* K1_red, const double* Js_red, const double* k_easy,
-                  const double* bnd_mask, double reg, int flags, Ctx& c) {
+                  const double* bnd_mask, double reg, int flags, Ctx& c, int nseg = 0,
+                  const int32_t* seg_node_off = nullptr) {
   MM_CHECK(N > 0 && E > 0, "N and E must be positive");
   MM_CHECK(N < (int)kIdMask, "too many nodes");
-  MM_CHECK(G >= 1 && G <= kMaxMat, "material count must be in [1, 64]");
+  MM_CHECK(G >= 1 && G <= kMaxMat, "material count must be in [1, 255]");
   MM_CHECK(knt && conn_in && mat_id && A_red && K1_red && Js_red && k_easy, "null input array");
   c.N = N;
   c.E = E;
@@ -429,6 +439,24 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   MM_HIP(hipDeviceGetAttribute(&c.num_cu, hipDeviceAttributeMultiprocessorCount, c.device));
   MM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   MM_HIP(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming));
+
+  // segments: contiguous node ranges of independent systems (one range when none are given)
+  std::vector<int> seg_of(N, 0);
+  c.h_seg_node_off.assign({0, N});
+  if (nseg > 0) {
+    MM_CHECK(seg_node_off && seg_node_off[0] == 0 && seg_node_off[nseg] == N, "segment offsets must run from 0 to N");
+    c.h_seg_node_off.assign(seg_node_off, seg_node_off + nseg + 1);
+    for (int g = 0; g < nseg; ++g) {
+      MM_CHECK(seg_node_off[g + 1] > seg_node_off[g], "segments must be non-empty and increasing");
+      for (int v = seg_node_off[g]; v < seg_node_off[g + 1]; ++v) seg_of[v] = g;
+    }
+    for (int e = 0; e < E; ++e)
+      for (int q = 1; q < 4; ++q)
+        MM_CHECK(conn_in[4 * (size_t)e + q] >= 0 && conn_in[4 * (size_t)e + q] < N &&
+                     seg_of[conn_in[4 * (size_t)e + q]] == seg_of[conn_in[4 * (size_t)e]],
+                 "an element couples two segments");
+  }
+  c.nseg = (int)c.h_seg_node_off.size() - 1;
 
   // orientation fix (src/loop.py:70-85): swap corners 1 and 2 where det J < 0
   std::vector<int> conn(conn_in, conn_in + 4 * (size_t)E);
@@ -479,7 +507,7 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   if (node_cap == 0) node_cap = kMaxBlockNodes;
   MM_CHECK(node_cap >= 16 && node_cap <= kMaxBlockNodes, "block node cap (flags bits 8..16) must be in [16, 256]");
   HostPart hp_all, hp_mag;
-  build_partition(N, conn, mat0, order, is_bnd, hp_all, node_cap);
+  build_partition(N, conn, mat0, order, is_bnd, hp_all, node_cap, seg_of);
   std::vector<int> mag_elems;
   mag_elems.reserve(E);
   for (int k = 0; k < E; ++k) {
@@ -487,14 +515,30 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
     if (A_red[g] != 0.0 || K1_red[g] != 0.0 || Js_red[g] != 0.0) mag_elems.push_back(order[k]);
   }
   const std::vector<char> no_bnd(N, 0);
-  build_partition(N, conn, mat0, mag_elems, no_bnd, hp_mag, node_cap);
+  build_partition(N, conn, mat0, mag_elems, no_bnd, hp_mag, node_cap, seg_of);
 
   MM_TRY(upload_part(c, hp_all, E, c.part_all));
+  // block offsets per segment; blocks follow the element order (smallest node id first), so a
+  // segment's blocks are contiguous
+  bool seg_sorted = true;
+  auto seg_blocks = [&](const std::vector<int>& nptr, const std::vector<uint32_t>& nodes) {
+    std::vector<int> off(c.nseg + 1, 0);
+    int prev = 0;
+    for (size_t b = 0; b + 1 < nptr.size(); ++b) {
+      const int sg = seg_of[nodes[nptr[b]] & kIdMask];
+      seg_sorted &= sg >= prev;
+      prev = sg;
+      off[sg + 1]++;
+    }
+    for (int g = 0; g < c.nseg; ++g) off[g + 1] += off[g];
+    return off;
+  };
+  c.h_seg_blk_all = seg_blocks(hp_all.nptr, hp_all.nodes);
   {
     std::vector<HostCluster> cl;
     build_clusters(knt, E, conn, cl);
     HostCPart hc;
-    build_cluster_partition(N, cl, is_bnd, hc, node_cap);
+    build_cluster_partition(N, cl, is_bnd, hc, node_cap, seg_of);
     Part& q = c.part_cl;
     q.nblk = (int)hc.cptr.size() - 1;
     q.E = (int)hc.w0.size();
@@ -513,7 +557,9 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
     q.lconn2 = w1;
     q.cmask = wm;
     c.nodes_cl = (long long)hc.nodes.size();
+    c.h_seg_blk_cl = seg_blocks(hc.nptr, hc.nodes);
   }
+  MM_CHECK(seg_sorted, "element blocks are not ordered by segment");
   c.nodes_all = (long long)hp_all.nodes.size();
   MM_TRY(upload_part(c, hp_mag, (int)mag_elems.size(), c.part_mag));
   c.max_block_nodes = std::max(hp_all.max_nodes, hp_mag.max_nodes);
@@ -624,6 +670,7 @@ static void destroy(Ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hpart) (void)hipHostFree(c->hpart);
+  if (c->hseg) (void)hipHostFree(c->hseg);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev) (void)hipEventDestroy(c->ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -687,6 +734,27 @@ int mm_ctx_create(int N, int E, const double* knt, const int32_t* conn, const in
   }
   Ctx* c = new Ctx();
   const int r = create(N, E, knt, conn, mat_id, G, A_red, K1_red, Js_red, k_easy, bnd_mask, reg, flags, *c);
+  if (r != 0) {
+    destroy(c);
+    return r;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_ctx.push_back(c);
+  *out_ctx = (int64_t)g_ctx.size();
+  return 0;
+}
+
+int mm_ctx_create_segmented(int N, int E, const double* knt, const int32_t* conn, const int32_t* mat_id, int G,
+                            const double* A_red, const double* K1_red, const double* Js_red, const double* k_easy,
+                            const double* bnd_mask, double reg, int flags, int nseg, const int32_t* seg_node_off,
+                            int64_t* out_ctx) {
+  if (!out_ctx || nseg < 1 || !seg_node_off) {
+    set_error("mm_ctx_create_segmented: out_ctx, nseg >= 1 and seg_node_off are required");
+    return -1;
+  }
+  Ctx* c = new Ctx();
+  const int r = create(N, E, knt, conn, mat_id, G, A_red, K1_red, Js_red, k_easy, bnd_mask, reg, flags, *c, nseg,
+                       seg_node_off);
   if (r != 0) {
     destroy(c);
     return r;
@@ -974,6 +1042,21 @@ int mm_solve_U(int64_t h, const double* m, double* U_inout, double tol, int maxi
   MM_HIP(hipStreamSynchronize(c.stream));
   if (it) *it = iters;
   if (rel_res) *rel_res = rr;
+  return 0;
+}
+
+int mm_solve_U_segmented(int64_t h, const double* m, double* U_inout, const double* tol, int maxiter, int* it,
+                         double* rel_res) {
+  GET_CTX(h);
+  MM_CHECK(m && U_inout && tol && it && rel_res, "null argument");
+  MM_CHECK(maxiter >= 0, "maxiter must be >= 0");
+  MM_TRY(alloc_min_state(c));
+  MM_HIP(hipMemcpyAsync(c.m_t, m, 3 * c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  MM_HIP(hipMemcpyAsync(c.work_U, U_inout, c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  MM_TRY(solve_U_seg(c, c.m_t, c.work_U, tol, maxiter, it, rel_res));
+  MM_HIP(hipGetLastError());
+  MM_HIP(hipMemcpyAsync(U_inout, c.work_U, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
   return 0;
 }
 

@@ -12,6 +12,7 @@ __device__ __forceinline__ BlkInfo blk_info(const Part& p) {
   BlkInfo bi;
   const int chunk = (p.nblk + 7) >> 3;
   const int b = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+  bi.b = b;
   bi.valid = b < p.nblk;
   if (bi.valid) {
     bi.e0 = p.eptr[b];
@@ -288,6 +289,21 @@ __device__ double sum_partials_all(const double* __restrict__ part, int n) {
 #pragma unroll
   for (int w = 1; w < NT / 64; ++w) t += sh[w];
   return t;
+}
+
+// Block sum of v written by thread 0 to *out (no cross-block traffic).
+template <int NT>
+__device__ void block_sum_to(double v, double* __restrict__ out) {
+  __shared__ double sh[NT / 64];
+  const int tid = threadIdx.x;
+  v = wave_red<false>(v);
+  if ((tid & 63) == 0) sh[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0) {
+    double s = sh[0];
+    for (int w = 1; w < NT / 64; ++w) s += sh[w];
+    *out = s;
+  }
 }
 
 // reduce_last + store of the totals to out[0..K-1]

@@ -31,7 +31,7 @@ constexpr uint32_t kOwner = 1u << 31;    // this block adds reg*x / writes the b
 constexpr uint32_t kPrivate = 1u << 30;  // no other block touches the node: plain store
 constexpr uint32_t kBoundary = 1u << 29; // Dirichlet node (outer surface)
 constexpr uint32_t kIdMask = (1u << 29) - 1;
-constexpr int kMaxMat = 64;
+constexpr int kMaxMat = 255;               // materials (uint8 index per element)
 constexpr int kRedBlocks = 2048;         // max workgroups of a reducing kernel
 constexpr int kNumScal = 256;            // device scalar slots
 
@@ -96,6 +96,7 @@ struct Geo {
 };
 
 struct BlkInfo {
+  int b;                              // logical block id (after the XCD remap)
   int e0, e1, n0, nb;
   bool valid;
 };
@@ -185,6 +186,23 @@ struct Ctx {
   double *S = nullptr, *Yh = nullptr;  // mem x 3N
   bool have_state = false;
 
+  // independent systems (segments) sharing the context: contiguous node ranges; no element
+  // couples two segments and no element block spans two
+  int nseg = 1;
+  std::vector<int> h_seg_node_off;    // nseg + 1
+  std::vector<int> h_seg_blk_all;     // nseg + 1 block offsets of part_all
+  std::vector<int> h_seg_blk_cl;      // nseg + 1 block offsets of part_cl
+  // segmented PCG work decomposition (set up on first use): workgroup w handles nodes of
+  // segment wg_seg[w] only, kSegChunk at a time
+  int seg_nwg = 0;
+  int *d_seg_node_off = nullptr, *d_seg_blk_all = nullptr, *d_seg_blk_cl = nullptr;
+  int *d_seg_wg_off = nullptr, *d_wg_seg = nullptr, *d_seg_active = nullptr;
+  double* d_seg_rz = nullptr;         // 2 x nseg r.z ping-pong
+  double* seg_part_rz = nullptr;      // seg_nwg r.z partials
+  double *hseg = nullptr, *hseg_dev = nullptr;  // host-coherent r.r / b.b partials, 2 x seg_nwg
+  std::vector<int> h_seg_wg_off, h_seg_active;
+  bool seg_running = false;           // the preconditioner ends with per-segment r.z partials
+
   long long n_apply = 0, n_rhs = 0, n_energy = 0, n_local = 0, n_vcycle = 0, n_pcg_it = 0;
 
   // live timing of the Poisson apply kernel: event pairs recorded on `stream` around each
@@ -229,5 +247,8 @@ int vmaxabs(const Ctx& c, const double* a, long n, double* out, hipStream_t s);
 int precond_apply(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s);
 int precond(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s);
 int solve_U_dev(Ctx& c, const double* m, double* U, double tol, int maxiter, int* it, double* rel_res);
+// solve_U for every segment of the context at once (one launch per kernel for all segments,
+// per-segment PCG scalars and stopping test): tol, it, rel_res have one entry per segment
+int solve_U_seg(Ctx& c, const double* m, double* U, const double* tol, int maxiter, int* it, double* rel_res);
 
 }  // namespace mm

@@ -37,7 +37,7 @@ __device__ __forceinline__ void stored_grads(const TG* __restrict__ grad, const 
 // With SCALED the input is x = w .* r (the level-0 pre-smoothing step of the V-cycle from a
 // zero guess): it is formed during the staging and each node's owner also writes it to xo.
 // With DOT (plain apply only) the workgroup also writes its share of x . (A x) to
-// dot_part[blockIdx.x]: the element energies |w|^2 / (6 |det|) plus, per owned node, reg x^2
+// dot_part[logical block id]: the element energies |w|^2 / (6 |det|) plus, per owned node, reg x^2
 // (interior) or x^2 (Dirichlet row), so x . A x needs no separate pass over x and y.
 // NT threads per workgroup (64, 128 or 256): the block's node flags stay in registers
 // between the staging and the write-back (kMaxBlockNodes / NT per thread).
@@ -49,10 +49,7 @@ __global__ __launch_bounds__(NT) void k_apply(Part p, Geo geo, const double* __r
   __shared__ double sy[kMaxBlockNodes];
   __shared__ double sX[GEOM == kGeomCoords ? 3 : 1][kMaxBlockNodes];
   const BlkInfo bi = blk_info(p);
-  if (!bi.valid) {
-    if (DOT && threadIdx.x == 0) dot_part[blockIdx.x] = 0.0;
-    return;
-  }
+  if (!bi.valid) return;
   const int tid = threadIdx.x;
   constexpr int kNode = kMaxBlockNodes / NT;  // node slots per thread
   constexpr int kPre = 1024 / NT;             // element words loaded before the staging barrier
@@ -146,10 +143,7 @@ __global__ __launch_bounds__(NT) void k_apply(Part p, Geo geo, const double* __r
     }
     node_out(y, f, v);
   }
-  if constexpr (DOT) {
-    double v1[1] = {acc};
-    block_partial<1, NT>(v1, dot_part);
-  }
+  if constexpr (DOT) block_sum_to<NT>(acc, dot_part + bi.b);
 }
 
 // ------------------------------------------------ Poisson apply, FP32 (preconditioner)
@@ -330,10 +324,7 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
   __shared__ double sy[kMaxBlockNodes];
   __shared__ T sX[3][kMaxBlockNodes];
   const BlkInfo bi = blk_info(p);
-  if (!bi.valid) {
-    if (DOT && threadIdx.x == 0) dot_part[blockIdx.x] = 0.0;
-    return;
-  }
+  if (!bi.valid) return;
   const int tid = threadIdx.x;
   constexpr int kNode = kMaxBlockNodes / NT;
   uint32_t fv[kNode];
@@ -406,10 +397,7 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
     else
       unsafeAtomicAdd(&y[id], (T)v);
   }
-  if constexpr (DOT) {
-    double v1[1] = {acc_e};
-    block_partial<1, NT>(v1, dot_part);
-  }
+  if constexpr (DOT) block_sum_to<NT>(acc_e, dot_part + bi.b);
 }
 
 // Persistent form of k_apply_cl (FP64): a grid of a few workgroups per CU walks the blocks of
@@ -900,10 +888,7 @@ static int persist_grid(const Ctx& c) {
   return std::min(g, 8 * ((c.part_cl.nblk + 7) / 8));
 }
 
-int apply_dot_count(const Ctx& c) {
-  if (c.apply_clusters && c.apply_persist > 0) return c.part_cl.nblk;
-  return 8 * (((c.apply_clusters ? c.part_cl : c.part_all).nblk + 7) / 8);
-}
+int apply_dot_count(const Ctx& c) { return (c.apply_clusters ? c.part_cl : c.part_all).nblk; }
 
 int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s, double* dot_part) {
   MM_CHECK(dot_part == nullptr || geom == kGeomCoords, "x.Ax partials need the on-the-fly apply");

@@ -9,6 +9,7 @@
 // never waits inside an iteration except to sum the r.r partials for the stopping test, and
 // that copy overlaps the V-cycle of the same iteration.
 #include <cmath>
+#include <vector>
 
 #include "mm_device.h"
 #include "mm_solver.h"
@@ -425,6 +426,251 @@ static int timed_apply_scaled(Ctx& c, const double* r, const double* w, double* 
   return 0;
 }
 
+// ---------------------------------------------------------------- segmented PCG kernels
+// Several independent systems (segments: contiguous node ranges, e.g. the grains of a grain set)
+// share one context and are solved together: every kernel launch covers all segments, and each
+// segment keeps its own alpha, beta and stopping test. Workgroup w works on nodes of segment
+// wg_seg[w] only (kSegChunk per workgroup), so its partial sums belong to one segment; a
+// consumer sums the partials of its own segment in a fixed order.
+constexpr int kSegChunk = 4 * kBlock;
+
+struct SegView {
+  const int* node_off;  // nseg + 1
+  const int* blk_off;   // nseg + 1 block offsets of the apply's partition (x.Ax partials)
+  const int* wg_off;    // nseg + 1
+  const int* wg_seg;    // per workgroup
+  const int* active;    // per segment: still iterating
+  double* rz;           // 2 x nseg
+};
+
+__device__ __forceinline__ void seg_range(const SegView& S, int& g, long& lo, long& hi) {
+  const int w = blockIdx.x;
+  g = S.wg_seg[w];
+  lo = S.node_off[g] + (long)(w - S.wg_off[g]) * kSegChunk;
+  hi = min(lo + (long)kSegChunk, (long)S.node_off[g + 1]);
+}
+
+// r = b - q, q = 0, t = 0; partials of r.r and b.b (2 per workgroup) for the host
+__global__ __launch_bounds__(kBlock) void k_seg_init(SegView S, const double* __restrict__ b, double* __restrict__ q,
+                                                     double* __restrict__ r, double* __restrict__ t,
+                                                     double* __restrict__ part_rr) {
+  int g;
+  long lo, hi;
+  seg_range(S, g, lo, hi);
+  double v[2] = {0.0, 0.0};
+  for (long i = lo + threadIdx.x; i < hi; i += kBlock) {
+    const double ri = b[i] - q[i];
+    q[i] = 0.0;
+    t[i] = 0.0;
+    r[i] = ri;
+    v[0] += ri * ri;
+    v[1] += b[i] * b[i];
+  }
+  block_partial<2>(v, part_rr);
+}
+
+// z = x + M .* (b - t) (level-0 post-smoothing), partials of r.z with r == b
+__global__ __launch_bounds__(kBlock) void k_seg_post_dot(SegView S, const double* __restrict__ x,
+                                                         const double* __restrict__ M, const double* __restrict__ b,
+                                                         const double* __restrict__ t, double* __restrict__ z,
+                                                         double* __restrict__ part) {
+  int g;
+  long lo, hi;
+  seg_range(S, g, lo, hi);
+  double v[1] = {0.0};
+  for (long i = lo + threadIdx.x; i < hi; i += kBlock) {
+    const double zi = x[i] + M[i] * (b[i] - t[i]);
+    z[i] = zi;
+    v[0] += b[i] * zi;
+  }
+  block_partial<1>(v, part);
+}
+
+// rz_g = the segment's r.z; beta_g = rz_g / rz[old]; p = z + beta p on active segments (first:
+// p = z); q = 0 everywhere; the segment's first workgroup stores rz_g to rz[new]
+__global__ __launch_bounds__(kBlock) void k_seg_p(SegView S, int first, int slot_new, int slot_old,
+                                                  const double* __restrict__ part_rz, const double* __restrict__ z,
+                                                  double* __restrict__ p, double* __restrict__ q) {
+  int g;
+  long lo, hi;
+  seg_range(S, g, lo, hi);
+  const double rz = sum_partials_all<kBlock>(part_rz + S.wg_off[g], S.wg_off[g + 1] - S.wg_off[g]);
+  const double beta = first ? 0.0 : rz / (S.rz[2 * g + slot_old] + 1e-30);
+  const bool act = S.active[g] != 0;
+  if ((int)blockIdx.x == S.wg_off[g] && threadIdx.x == 0) S.rz[2 * g + slot_new] = rz;
+  for (long i = lo + threadIdx.x; i < hi; i += kBlock) {
+    if (act) p[i] = first ? z[i] : z[i] + beta * p[i];
+    q[i] = 0.0;
+  }
+}
+
+// alpha_g = rz[slot] / p.Ap (the segment's block partials); x += alpha p, r -= alpha q on active
+// segments; t = 0; partials of r.r for the host
+__global__ __launch_bounds__(kBlock) void k_seg_xr(SegView S, int slot, const double* __restrict__ dot_part,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   const double* __restrict__ p, const double* __restrict__ q,
+                                                   double* __restrict__ t, double* __restrict__ part_rr) {
+  int g;
+  long lo, hi;
+  seg_range(S, g, lo, hi);
+  const double pq = sum_partials_all<kBlock>(dot_part + S.blk_off[g], S.blk_off[g + 1] - S.blk_off[g]);
+  const double alpha = S.rz[2 * g + slot] / (pq + 1e-30);
+  const bool act = S.active[g] != 0;
+  double v[1] = {0.0};
+  for (long i = lo + threadIdx.x; i < hi; i += kBlock) {
+    t[i] = 0.0;
+    double ri = r[i];
+    if (act) {
+      x[i] += alpha * p[i];
+      ri -= alpha * q[i];
+      r[i] = ri;
+    }
+    v[0] += ri * ri;
+  }
+  block_partial<1>(v, part_rr);
+}
+
+static SegView seg_view(const Ctx& c) {
+  SegView S;
+  S.node_off = c.d_seg_node_off;
+  S.blk_off = c.apply_clusters ? c.d_seg_blk_cl : c.d_seg_blk_all;
+  S.wg_off = c.d_seg_wg_off;
+  S.wg_seg = c.d_wg_seg;
+  S.active = c.d_seg_active;
+  S.rz = c.d_seg_rz;
+  return S;
+}
+
+void launch_seg_post_dot(const Ctx& c, const double* x, const double* M, const double* b, const double* t, double* z,
+                         hipStream_t s) {
+  k_seg_post_dot<<<c.seg_nwg, kBlock, 0, s>>>(seg_view(c), x, M, b, t, z, c.seg_part_rz);
+}
+
+// device tables of the segmented decomposition (once per context)
+static int ensure_seg(Ctx& c) {
+  if (c.seg_nwg > 0) return 0;
+  std::vector<int> wg_off(c.nseg + 1, 0), wg_seg;
+  for (int g = 0; g < c.nseg; ++g) {
+    const int n = c.h_seg_node_off[g + 1] - c.h_seg_node_off[g];
+    const int k = (n + kSegChunk - 1) / kSegChunk;
+    wg_off[g + 1] = wg_off[g] + k;
+    wg_seg.insert(wg_seg.end(), k, g);
+  }
+  c.seg_nwg = wg_off[c.nseg];
+  c.h_seg_wg_off = wg_off;
+  auto up = [&](int** d, const std::vector<int>& h) -> int {
+    MM_HIP(hipMalloc((void**)d, h.size() * sizeof(int)));
+    c.allocs.push_back(*d);
+    MM_HIP(hipMemcpy(*d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+    return 0;
+  };
+  MM_TRY(up(&c.d_seg_node_off, c.h_seg_node_off));
+  MM_TRY(up(&c.d_seg_blk_all, c.h_seg_blk_all));
+  MM_TRY(up(&c.d_seg_blk_cl, c.h_seg_blk_cl));
+  MM_TRY(up(&c.d_seg_wg_off, wg_off));
+  MM_TRY(up(&c.d_wg_seg, wg_seg));
+  MM_TRY(up(&c.d_seg_active, std::vector<int>(c.nseg, 1)));
+  MM_HIP(hipMalloc((void**)&c.d_seg_rz, 2 * (size_t)c.nseg * sizeof(double)));
+  c.allocs.push_back(c.d_seg_rz);
+  MM_HIP(hipMalloc((void**)&c.seg_part_rz, (size_t)c.seg_nwg * sizeof(double)));
+  c.allocs.push_back(c.seg_part_rz);
+  MM_HIP(hipHostMalloc((void**)&c.hseg, 2 * (size_t)c.seg_nwg * sizeof(double),
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  MM_HIP(hipHostGetDevicePointer((void**)&c.hseg_dev, c.hseg, 0));
+  return 0;
+}
+
+// PCG for every segment at once (make_pcg_solve per system, src/poisson_solve.py:422-478):
+// stop segment g when r_g.r_g <= tol_g^2 b_g.b_g or after maxiter iterations.
+static int pcg_seg(Ctx& c, const double* b, double* U, const double* tol, int maxiter, int* it_out, double* r2_out,
+                   double* bb_out) {
+  hipStream_t s = c.stream;
+  const long N = c.N;
+  const int G = c.nseg, W = c.seg_nwg;
+  const std::vector<int>& wo = c.h_seg_wg_off;
+  double* sc = c.dscal;
+  (void)sc;
+  c.h_seg_active.assign(G, 1);
+  MM_HIP(hipMemcpyAsync(c.d_seg_active, c.h_seg_active.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
+  const SegView S = seg_view(c);
+  vfill(c.pq, 0.0, N, s);
+  MM_TRY(launch_apply(c, c.geom_mode, U, c.pq, s));
+  c.n_apply++;
+  k_seg_init<<<W, kBlock, 0, s>>>(S, b, c.pq, c.pr, c.pt, c.hseg_dev);
+  c.seg_running = true;
+  int rc = precond(c, c.pr, c.pz, c.part_rz, s);
+  c.seg_running = false;
+  MM_TRY(rc);
+  k_seg_p<<<W, kBlock, 0, s>>>(S, 1, 0, 0, c.seg_part_rz, c.pz, c.pp, c.pq);
+  MM_HIP(hipStreamSynchronize(s));
+  std::vector<double> r2(G, 0.0), bb(G, 0.0), tol2(G);
+  for (int g = 0; g < G; ++g) {
+    for (int w = wo[g]; w < wo[g + 1]; ++w) {
+      r2[g] += c.hseg[2 * w];
+      bb[g] += c.hseg[2 * w + 1];
+    }
+    tol2[g] = tol[g] * tol[g] * bb[g];
+    it_out[g] = 0;
+  }
+  int live = 0;
+  for (int g = 0; g < G; ++g) {
+    c.h_seg_active[g] = r2[g] > tol2[g] && maxiter > 0;
+    live += c.h_seg_active[g];
+  }
+  MM_HIP(hipMemcpyAsync(c.d_seg_active, c.h_seg_active.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
+  int it = 0, cur = 0;
+  while (live > 0 && it < maxiter) {
+    MM_TRY(launch_apply(c, c.geom_mode, c.pp, c.pq, s, c.part_pq));  // q was cleared by k_seg_p
+    c.n_apply++;
+    k_seg_xr<<<W, kBlock, 0, s>>>(S, cur, c.part_pq, U, c.pr, c.pp, c.pq, c.pt, c.hseg_dev);
+    MM_HIP(hipEventRecord(c.ev, s));
+    c.seg_running = true;
+    rc = precond(c, c.pr, c.pz, c.part_rz, s);
+    c.seg_running = false;
+    MM_TRY(rc);
+    k_seg_p<<<W, kBlock, 0, s>>>(S, 0, 1 - cur, cur, c.seg_part_rz, c.pz, c.pp, c.pq);
+    cur = 1 - cur;
+    ++it;
+    MM_HIP(hipEventSynchronize(c.ev));
+    bool changed = false;
+    for (int g = 0; g < G; ++g) {
+      r2[g] = 0.0;
+      for (int w = wo[g]; w < wo[g + 1]; ++w) r2[g] += c.hseg[w];
+      if (c.h_seg_active[g]) {
+        it_out[g] = it;
+        if (r2[g] <= tol2[g] || it >= maxiter) {
+          c.h_seg_active[g] = 0;
+          --live;
+          changed = true;
+        }
+      }
+    }
+    if (changed)
+      MM_HIP(hipMemcpyAsync(c.d_seg_active, c.h_seg_active.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
+  }
+  MM_HIP(hipStreamSynchronize(s));
+  c.n_pcg_it += it;
+  for (int g = 0; g < G; ++g) {
+    r2_out[g] = r2[g];
+    bb_out[g] = bb[g];
+  }
+  return 0;
+}
+
+int solve_U_seg(Ctx& c, const double* m, double* U, const double* tol, int maxiter, int* it, double* rel_res) {
+  MM_CHECK(c.lv.size() > 1 && c.lv[0].M != nullptr && c.precision == 0 && c.geom_mode == kGeomCoords,
+           "segmented solves need the FP64 AMG V-cycle and the on-the-fly apply");
+  MM_TRY(ensure_seg(c));
+  hipStream_t s = c.stream;
+  vfill(c.pb, 0.0, c.N, s);
+  MM_TRY(launch_rhs(c, m, c.pb, s));
+  c.n_rhs++;
+  std::vector<double> r2(c.nseg), bb(c.nseg);
+  MM_TRY(pcg_seg(c, c.pb, U, tol, maxiter, it, r2.data(), bb.data()));
+  for (int g = 0; g < c.nseg; ++g) rel_res[g] = std::sqrt(r2[g] / (bb[g] + 1e-30));
+  return 0;
+}
+
 // ----------------------------------------------------------------------------- V-cycle
 // Coarse part of the V-cycle on levels l >= 1; the correction is returned in lv[l].xs.
 // Pre-smoothing from a zero guess is fused into the first SpMV (t = A (M b), x = M b) and the
@@ -497,7 +743,10 @@ int precond(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s) 
   vcycle_coarse(c, 1, s);
   spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Px, L1.xs, (const double*)nullptr, c.px0, s, c.pt); // x += P0 e1, t = 0
   MM_TRY(timed_apply(c, c.px0, c.pt, s));                                         // t = A x
-  k_post_dot<<<red_grid(N), kBlock, 0, s>>>(c.px0, L0.M, r, c.pt, z, N, rz_part);
+  if (c.seg_running)
+    launch_seg_post_dot(c, c.px0, L0.M, r, c.pt, z, s);
+  else
+    k_post_dot<<<red_grid(N), kBlock, 0, s>>>(c.px0, L0.M, r, c.pt, z, N, rz_part);
   c.n_apply += 2;
   c.n_vcycle += 1;
   return 0;
