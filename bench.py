@@ -11,8 +11,9 @@ and one post-smoothing sweep.
 ``--workload grains64``: 64 independent grains (src of the set: ``mesh.grain_set``, default_rng(3)):
 total tets per grain log-uniform in [2e4, 4e5] (7,752,942 tets, 1,431,475 nodes in all), cube or
 sphere by coin flip, easy axis uniform on the sphere, each in its own 3-layer graded airbox with
-2 nm cells; 41 field values +2 T -> -8 T; one field step minimises every grain (host threads
-drive the grains' contexts concurrently, ``--threads``).
+2 nm cells; 41 field values +2 T -> -8 T; one field step minimises every grain: by default all
+grains in one segmented context (one launch per kernel for all grains, per-grain scalars);
+``--grains-form threads``: one context per grain, host threads driving them (``--threads``).
 
 ``--workload poly25M``: 200 nm cube of 27 Voronoi grains (centres from default_rng(4)), 150^3
 hex cells x 6 tets (20.25M magnet tets) in a 5-layer graded airbox (24,576,000 tets, 4,173,281
@@ -157,11 +158,62 @@ def cpu_baseline(mesh, mats, max_iter):
     return int(info["iters"]), time.perf_counter() - t0, int(info.get("demag_iters", 0))
 
 
+def segmented_poisson(grains, ctxs, args, log, union=None, reps=3, tol=1e-10):
+    """Poisson solves of all grains: one segmented union context (every kernel launch covers
+    every grain, per-grain PCG scalars) against the per-grain contexts driven by host threads.
+    Both solve the same magnetisations from zero to ``tol``; FP64 V-cycle on both sides."""
+    from mumag_matrixfree2_amd.context import solve_U_batched
+    from mumag_matrixfree2_amd.grainset import grain_set_context, split
+
+    t0 = time.perf_counter()
+    u = union
+    if u is None:
+        A, K1, Js, _ = materials()
+        tables = [(A, K1, Js, np.array([g["easy_axis"], g["easy_axis"]])) for g in grains]
+        u = grain_set_context(grains, tables)
+        u.setup_precond(args.precond, max_coarse=args.max_coarse)
+        log(f"[bench:segmented] union context N={u.N} E={u.E}, AMG levels "
+            f"{[l['A'].shape[0] for l in u.amg_levels]}, {time.perf_counter() - t0:.1f}s")
+    precisions = [c.precision for c in ctxs] + [u.precision]
+    for c in ctxs + [u]:
+        c.set_precision("fp64")
+    ms = [initial_m(c.N) for c in ctxs]
+    zeros = [np.zeros(c.N) for c in ctxs]
+    m_u, z_u = np.vstack(ms), np.zeros(u.N)
+    solve_U_batched(ctxs, ms, zeros, tol, n_threads=args.threads)
+    u.solve_U_segmented(m_u, z_u, tol)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Us, its_t, _ = solve_U_batched(ctxs, ms, zeros, tol, n_threads=args.threads)
+    t_thr = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        U_u, its_s, rr_s = u.solve_U_segmented(m_u, z_u, tol)
+    t_seg = (time.perf_counter() - t0) / reps
+    for c, prec in zip(ctxs + [u], precisions):
+        c.set_precision(prec)
+    diff = max(float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)) for a, b in zip(split(u, U_u), Us))
+    out = {"grains": len(ctxs), "tol": tol, "threads": args.threads,
+           "threaded_ms_per_all_grain_solve": t_thr * 1e3, "segmented_ms_per_all_grain_solve": t_seg * 1e3,
+           "speedup": t_thr / t_seg, "pcg_iters_max_threaded": int(np.max(its_t)),
+           "pcg_iters_max_segmented": int(np.max(its_s)), "max_rel_res_segmented": float(np.max(rr_s)),
+           "max_rel_U_diff": diff}
+    log(f"[bench:segmented] {out}")
+    if union is None:
+        u.close()
+    return out
+
+
 def run_grains(args, log):
-    """The 64-grain set: one context (mesh, AMG hierarchy, resident state) per grain; a field
-    step is ``minimize_batched`` over all grains."""
+    """The 64-grain set. Form "segmented" (default): the grains concatenated into one segmented
+    context (one AMG hierarchy on the union, per-grain scalars), a field step is
+    ``minimize_segmented``, one launch per kernel for all grains. Form "threads": one context
+    per grain, a field step is ``minimize_batched`` with host threads driving the contexts.
+    Unless --no-compare-forms, the other form runs the same sweep and the report gives its rate
+    and the largest per-grain differences."""
     from mumag_matrixfree2_amd.autotune import tune_apply
     from mumag_matrixfree2_amd.context import MeshContext, make_params, minimize_batched
+    from mumag_matrixfree2_amd.grainset import grain_set_context
     from mumag_matrixfree2_amd.hysteresis_loop import field_values, switching_step
     from mumag_matrixfree2_amd.mesh import grain_set
 
@@ -169,60 +221,105 @@ def run_grains(args, log):
     t0 = time.perf_counter()
     grains = grain_set(64, seed=3)
     A, K1, Js, _ = materials()
-    ctxs = []
-    for i, g in enumerate(grains):
-        k = np.array([g["easy_axis"], g["easy_axis"]])
-        c = MeshContext(g["knt"], g["ijk"][:, :4], g["ijk"][:, 4], A, K1, Js, k, boundary_mask=g["boundary_mask"])
-        c.setup_precond(args.precond, max_coarse=args.max_coarse)
-        c.set_precision(args.precision)
-        tune_apply(c)
-        ctxs.append(c)
-        if i % 8 == 7:
-            log(f"[bench] {i + 1} grains set up, {time.perf_counter() - t0:.1f}s")
-    E_tot = sum(c.E for c in ctxs)
-    N_tot = sum(c.N for c in ctxs)
-    log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: {len(ctxs)} grains, N={N_tot} E={E_tot}")
+    tables = [(A, K1, Js, np.array([g["easy_axis"], g["easy_axis"]])) for g in grains]
+    m0 = [initial_m(g["knt"].shape[0]) for g in grains]
     p = make_params("pcohen_hs", tau_f=1e-10, eps_a=1e-12, cg_tol=1e-10, max_iter=2000)
     B_vals = field_values(2.0 / JS, -8.0 / JS, wl["dB"] / JS, False)
     steps = min(args.steps or wl["steps"], len(B_vals))
-    m0 = [initial_m(c.N) for c in ctxs]
-    vol = np.array([c.V_mag for c in ctxs])
+    forms = [args.grains_form] + ([] if args.no_compare_forms else
+                                  [f for f in ("segmented", "threads") if f != args.grains_form])
 
-    def sweep(n_threads, timed):
-        for c, m in zip(ctxs, m0):
-            c.state_set(m, np.zeros(c.N))
+    ctxs, union = [], None
+    if "threads" in forms or not args.no_segmented:
+        for i, (g, t) in enumerate(zip(grains, tables)):
+            c = MeshContext(g["knt"], g["ijk"][:, :4], g["ijk"][:, 4], *t, boundary_mask=g["boundary_mask"])
+            c.setup_precond(args.precond, max_coarse=args.max_coarse)
+            c.set_precision(args.precision)
+            tune_apply(c)
+            ctxs.append(c)
+        log(f"[bench] {len(ctxs)} grain contexts set up, {time.perf_counter() - t0:.1f}s")
+    if "segmented" in forms:
+        union = grain_set_context(grains, tables)
+        union.setup_precond(args.precond, max_coarse=args.max_coarse)
+        union.set_precision(args.precision)
+        shape = tune_apply(union)
+        log(f"[bench] union context N={union.N} E={union.E} AMG levels "
+            f"{[l['A'].shape[0] for l in union.amg_levels]} apply {shape['unit']}:{shape['param']}, "
+            f"{time.perf_counter() - t0:.1f}s")
+    E_tot = sum(g["ijk"].shape[0] for g in grains)
+    N_tot = sum(g["knt"].shape[0] for g in grains)
+
+    def sweep(form, timed):
+        """(seconds, J per step, per-grain energies per step, minimiser iterations, apply timing)"""
+        if form == "segmented":
+            union.state_set(np.vstack(m0), np.zeros(union.N))
+            vol = union.segment_volumes()
             if timed:
-                c.apply_timing(True)
-        J, its = [], 0
+                union.apply_timing(True)
+        else:
+            for c, m in zip(ctxs, m0):
+                c.state_set(m, np.zeros(c.N))
+                if timed:
+                    c.apply_timing(True)
+            vol = np.array([c.V_mag for c in ctxs])
+        J, Es, its = [], [], 0
         t_start = time.perf_counter()
         for k in range(steps):
             ts = time.perf_counter()
-            infos = minimize_batched(ctxs, p, B_vals[k] * H_DIR, n_threads=n_threads)
-            Jg = np.array([c.state_j_parallel(H_DIR) for c in ctxs])
+            B = B_vals[k] * H_DIR
+            if form == "segmented":
+                infos = union.minimize_segmented(p, B)
+                Jg = union.state_j_parallel_segmented(H_DIR)
+            else:
+                infos = minimize_batched(ctxs, p, B, n_threads=args.threads)
+                Jg = np.array([c.state_j_parallel(H_DIR) for c in ctxs])
             J.append(float(vol @ Jg / vol.sum()))
+            Es.append([i["E"] for i in infos])
             its += sum(i["iters"] for i in infos)
-            log(f"[bench:grains t={n_threads}] step {k:2d} B={B_vals[k] * JS:+.3f} T J={J[-1] * JS:+.5f} T "
+            log(f"[bench:grains {form}] step {k:2d} B={B_vals[k] * JS:+.3f} T J={J[-1] * JS:+.5f} T "
                 f"iters={sum(i['iters'] for i in infos)} max={max(i['iters'] for i in infos)} "
                 f"t={time.perf_counter() - ts:.3f}s")
-        return time.perf_counter() - t_start, np.array(J), its
+        el = time.perf_counter() - t_start
+        timing = None
+        if timed:
+            timing = [union.apply_timing(False)] if form == "segmented" else [c.apply_timing(False) for c in ctxs]
+        return el, np.array(J), np.array(Es), its, timing
 
-    for c, m in zip(ctxs, m0):  # warm-up on a separate state
-        c.state_set(m, np.zeros(c.N))
-    for k in range(args.warmup):
-        minimize_batched(ctxs, p, B_vals[k % len(B_vals)] * H_DIR, n_threads=args.threads)
-    elapsed, J, its = sweep(args.threads, timed=True)
-    timing = [c.apply_timing(False) for c in ctxs]
-    seq = None
-    if args.compare_sequential:
-        t1, J1, _ = sweep(1, timed=False)
-        seq = {"sequential_field_steps_per_s": steps / t1, "speedup_vs_sequential": t1 / elapsed,
-               "max_abs_J_diff": float(np.max(np.abs(J - J1)))}
-    algo = sum(apply_algorithmic_bytes(c) * n for c, (_, n) in zip(ctxs, timing))
-    t_apply = sum(ms for ms, _ in timing) * 1e-3
-    n_apply = sum(n for _, n in timing)
+    seg = None if args.no_segmented or not ctxs else segmented_poisson(grains, ctxs, args, log, union)
+    # warm-up on a separate state
+    if args.warmup:
+        if args.grains_form == "segmented":
+            union.state_set(np.vstack(m0), np.zeros(union.N))
+            for k in range(args.warmup):
+                union.minimize_segmented(p, B_vals[k % len(B_vals)] * H_DIR)
+        else:
+            for c, m in zip(ctxs, m0):
+                c.state_set(m, np.zeros(c.N))
+            for k in range(args.warmup):
+                minimize_batched(ctxs, p, B_vals[k % len(B_vals)] * H_DIR, n_threads=args.threads)
+    elapsed, J, Es, its, timing = sweep(args.grains_form, timed=True)
+    other = None
+    if len(forms) > 1:
+        t1, J1, Es1, its1, _ = sweep(forms[1], timed=False)
+        other = {"form": forms[1], "field_steps_per_s": steps / t1, "speedup_of_headline_form": t1 / elapsed,
+                 "minimizer_iters": its1, "max_abs_J_diff": float(np.max(np.abs(J - J1))),
+                 "max_rel_grain_energy_diff": float(np.max(np.abs(Es - Es1) / np.maximum(np.abs(Es1), 1.0))),
+                 "switching_step": switching_step(J1)}
+    if args.grains_form == "segmented":
+        # the sweep's applies cover shrinking segment subsets; the roofline is taken on
+        # back-to-back applies over every grain (HIP events on the context's stream)
+        reps = 100
+        t_apply = union.bench_apply(reps=reps) * 1e-3 * reps
+        n_apply = reps
+        algo = apply_algorithmic_bytes(union) * reps
+    else:
+        algo = sum(apply_algorithmic_bytes(c) * n for c, (_, n) in zip(ctxs, timing))
+        t_apply = sum(ms for ms, _ in timing) * 1e-3
+        n_apply = sum(n for _, n in timing)
     achieved = algo / max(t_apply, 1e-12) / 1e9
+    label = "FP64" if args.precision == "fp64" else "mixed FP32 V-cycle / FP64 PCG"
     result = {
-        "metric": f"hysteresis field-steps/s ({args.precision}, {wl['name']})",
+        "metric": f"hysteresis field-steps/s ({label}, {wl['name']})",
         "value": steps / elapsed,
         "unit": "field-steps/s",
         "n_gpus": 1,
@@ -234,23 +331,30 @@ def run_grains(args, log):
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64+f32",
         "data": "synthetic (generated grain set, seeded initial magnetisation)",
-        "config": {"workload": f"{wl['name']}_{args.precision}_sweep+2..-8T_{len(B_vals)}steps", "grains": len(ctxs),
-                   "tets": E_tot, "nodes": N_tot, "method": "pcohen_hs", "threads": args.threads,
+        "config": {"workload": f"{wl['name']}_{args.precision}_sweep+2..-8T_{len(B_vals)}steps", "grains": len(grains),
+                   "tets": E_tot, "nodes": N_tot, "method": "pcohen_hs", "form": args.grains_form,
+                   "threads": args.threads if args.grains_form == "threads" else 1,
                    "precond": f"SA-AMG V-cycle ({args.precond})", "pcg_tol": 1e-10, "tau_f": 1e-10},
         "minimizer_iters": its,
         "switching_step": switching_step(J),
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_apply<on_the_fly> over all grains (event time includes overlap with other grains)",
+            "kernel": ("k_apply_cl over all grains in one launch (back-to-back applies after the sweep)"
+                       if args.grains_form == "segmented" else
+                       "k_apply over all grains (event time includes overlap with other grains)"),
             "avg_launch_us": t_apply / max(n_apply, 1) * 1e6, "timed_launches": n_apply,
         },
     }
-    if seq is not None:
-        result["batched_vs_sequential"] = seq
+    if other is not None:
+        result["other_form"] = other
+    if seg is not None:
+        result["segmented_poisson"] = seg
     print(json.dumps(result), flush=True)
     for c in ctxs:
         c.close()
+    if union is not None:
+        union.close()
 
 
 def main():
@@ -268,8 +372,12 @@ def main():
     ap.add_argument("--apply-shape", default="auto",
                     help="'auto' (autotuned at set-up) or unit:param, e.g. cluster:256, element:128, persistent:2")
     ap.add_argument("--threads", type=int, default=8, help="host threads driving the grains (grains64)")
-    ap.add_argument("--compare-sequential", action="store_true",
-                    help="grains64: also time the same steps with one thread")
+    ap.add_argument("--grains-form", default="segmented", choices=["segmented", "threads"],
+                    help="grains64: one segmented context for all grains, or one context per grain on host threads")
+    ap.add_argument("--no-compare-forms", action="store_true",
+                    help="grains64: do not run the sweep in the other form")
+    ap.add_argument("--no-segmented", action="store_true",
+                    help="grains64: skip the segmented vs threaded Poisson solve comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the hysteresis path runs on a single MI355X here "

@@ -66,8 +66,11 @@ int mm_ctx_destroy(int64_t ctx);
 /* Context over several independent systems (segments) at once, e.g. the grains of a grain set
  * concatenated into one mesh: segment g owns nodes seg_node_off[g] .. seg_node_off[g+1]-1
  * (seg_node_off[0] = 0, seg_node_off[nseg] = N) and every element lies in one segment. Element
- * blocks never span two segments, so mm_solve_U_segmented can solve all segments in one launch
- * per kernel with per-segment PCG scalars. Other arguments as mm_ctx_create. No reference
+ * blocks never span two segments, so mm_solve_U_segmented and mm_minimize_segmented run all
+ * segments in one launch per kernel with per-segment scalars. Each segment keeps its own
+ * magnetic volume V_mag (energies, gradients and the minimiser's scalings are per segment), so
+ * a segment behaves as a context of its own mesh; mm_energy_grad on the whole context returns
+ * the sum of the segments' energies. Other arguments as mm_ctx_create. No reference
  * counterpart: the reference solves one system per solve_U call (src/poisson_solve.py:748-771). */
 int mm_ctx_create_segmented(int N, int E, const double* knt, const int32_t* conn, const int32_t* mat_id, int G,
                             const double* A_red, const double* K1_red, const double* Js_red, const double* k_easy,
@@ -138,7 +141,7 @@ int mm_solve_U(int64_t ctx, const double* m, double* U_inout, double tol, int ma
 /* solve_U (src/poisson_solve.py:748-771) for every segment of a segmented context at once:
  * one launch per kernel for all segments, while each segment keeps its own alpha, beta and
  * stopping test (r_g.r_g <= tol[g]^2 b_g.b_g); it[g] and rel_res[g] per segment, as
- * mm_solve_U would return for the segment alone. Needs the FP64 AMG V-cycle. */
+ * mm_solve_U would return for the segment alone. Needs the AMG V-cycle (FP64 or mixed). */
 int mm_solve_U_segmented(int64_t ctx, const double* m, double* U_inout, const double* tol, int maxiter, int* it,
                          double* rel_res);
 
@@ -160,6 +163,19 @@ int mm_minimize_resident(int64_t ctx, const mm_params* p, const double* B, mm_in
 int mm_minimize(int64_t ctx, const mm_params* p, double* m_inout, double* U_inout, const double* B, mm_info* info);
 /* J_parallel of the resident state. */
 int mm_state_j_parallel(int64_t ctx, const double* h, double* J);
+
+/* Field step of every segment of a segmented context at once (pcohen_hs only): the resident
+ * state of segment g is minimised at field B as mm_minimize_resident would minimise it in a
+ * context of its own, with its own line search, Steihaug loop and convergence test, while all
+ * segments share each kernel launch; info has nseg entries (time: the whole call). Segments
+ * that converge keep their state while the others iterate. Needs the AMG V-cycle (FP64 or mixed).
+ * mm_minimize_resident rejects segmented contexts. No reference counterpart (the reference
+ * minimises one system per call, src/minimizers.py:4053-4096). */
+int mm_minimize_segmented(int64_t ctx, const mm_params* p, const double* B, mm_info* info);
+/* J_parallel of each segment's resident state (nseg values). */
+int mm_state_j_parallel_segmented(int64_t ctx, const double* h, double* J);
+/* Magnetic volume V_mag of each segment (nseg values). */
+int mm_seg_volumes(int64_t ctx, double* V_mag);
 
 /* Batched field step over n independent contexts (the grains of a grain set, each its own mesh
  * and airbox): context i minimises its resident state at field B[3i..3i+2] and fills out[i].

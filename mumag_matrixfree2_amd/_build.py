@@ -9,7 +9,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libmumag_hip.so"
-SOURCES = ["mm_api.hip", "mm_kernels.hip", "mm_solver.hip", "mm_minimize.hip"]
+SOURCES = ["mm_api.hip", "mm_kernels.hip", "mm_solver.hip", "mm_minimize.hip", "mm_segmin.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -73,12 +73,22 @@ def strength(A: sp.csr_matrix, theta: float = 0.0) -> sp.csr_matrix:
 
 
 def aggregate(S: sp.csr_matrix) -> tuple[np.ndarray, int]:
+    """Aggregate ids per node (-1: unaggregated), numbered in the order of each aggregate's
+    first node, so the coarse unknowns follow the node order: the coarse rows of a set of
+    independent systems on contiguous node ranges (a segmented context) are contiguous too."""
     Sp = i32(S.indptr)
     Sj = i32(S.indices)
     agg = np.empty(S.shape[0], dtype=np.int32)
     na = C.c_int32(0)
     call("mm_sa_aggregate", S.shape[0], ptr(Sp), ptr(Sj), ptr(agg), C.byref(na))
-    return agg, int(na.value)
+    na = int(na.value)
+    rows = np.flatnonzero(agg >= 0)
+    first = np.full(na, S.shape[0], dtype=np.int64)
+    np.minimum.at(first, agg[rows], rows)
+    new_id = np.empty(na, dtype=np.int32)
+    new_id[np.argsort(first, kind="stable")] = np.arange(na, dtype=np.int32)
+    agg[rows] = new_id[agg[rows]]
+    return agg, na
 
 
 def spectral_radius_DinvA(A: sp.csr_matrix) -> float:

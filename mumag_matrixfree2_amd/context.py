@@ -267,6 +267,29 @@ class MeshContext:
         call("mm_state_j_parallel", self.handle, ptr(h), ptr(J))
         return float(J[0])
 
+    def minimize_segmented(self, params: "_hip.MMParams", B) -> list[dict]:
+        """Field step of every segment at once (pcohen_hs): one info dict per segment, as
+        ``minimize_resident`` would return for the segment in a context of its own."""
+        if self.segments is None:
+            raise RuntimeError("minimize_segmented needs a context created with segments")
+        B = f64(B).reshape(3)
+        infos = (_hip.MMInfo * (self.segments.size - 1))()
+        call("mm_minimize_segmented", self.handle, C.byref(params), ptr(B), infos)
+        return [_info_dict(i) for i in infos]
+
+    def state_j_parallel_segmented(self, h) -> np.ndarray:
+        """J_parallel of each segment's resident state."""
+        h = f64(h).reshape(3)
+        J = np.zeros(self.segments.size - 1)
+        call("mm_state_j_parallel_segmented", self.handle, ptr(h), ptr(J))
+        return J
+
+    def segment_volumes(self) -> np.ndarray:
+        """Magnetic volume V_mag of each segment."""
+        V = np.zeros(1 if self.segments is None else self.segments.size - 1)
+        call("mm_seg_volumes", self.handle, ptr(V))
+        return V
+
     def counters(self):
         out = np.zeros(6, dtype=np.int64)
         call("mm_counters", self.handle, ptr(out))

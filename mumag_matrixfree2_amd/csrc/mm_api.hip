@@ -34,6 +34,9 @@ static int dalloc(Ctx& c, T** p, size_t n) {
   void* q = nullptr;
   MM_HIP(hipMalloc(&q, n * sizeof(T)));
   c.allocs.push_back(q);
+  // zeroed once: a segmented solve leaves the entries of segments it does not iterate
+  // untouched, and they must hold finite values (the coarsest level multiplies whole vectors)
+  MM_HIP(hipMemset(q, 0, n * sizeof(T)));
   *p = (T*)q;
   return 0;
 }
@@ -559,6 +562,7 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
     c.nodes_cl = (long long)hc.nodes.size();
     c.h_seg_blk_cl = seg_blocks(hc.nptr, hc.nodes);
   }
+  c.h_seg_blk_mag = seg_blocks(hp_mag.nptr, hp_mag.nodes);
   MM_CHECK(seg_sorted, "element blocks are not ordered by segment");
   c.nodes_all = (long long)hp_all.nodes.size();
   MM_TRY(upload_part(c, hp_mag, (int)mag_elems.size(), c.part_mag));
@@ -602,10 +606,20 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
 
   // nodal set-up: V_mag, M_nodal, exchange diagonal (src/loop.py:862-865,929-939;
   // src/energy_kernels.py:492-566; src/hysteresis_loop.py:340-353)
+  // (a context of several independent systems keeps one V_mag per segment)
   double V_mag = 0.0;
+  c.h_seg_vmag.assign(c.nseg, 0.0);
   for (int k = 0; k < E; ++k)
-    if (Js_red[mat0[order[k]]] > 0) V_mag += c.h_Ve[k];
+    if (Js_red[mat0[order[k]]] > 0) {
+      V_mag += c.h_Ve[k];
+      c.h_seg_vmag[seg_of[c.h_conn[4 * (size_t)k]]] += c.h_Ve[k];
+    }
   c.V_mag = V_mag == 0.0 ? 1.0 : V_mag;
+  for (double& v : c.h_seg_vmag)
+    if (v == 0.0) v = 1.0;
+  std::vector<double> vnode(N, c.V_mag);
+  if (c.nseg > 1)
+    for (int i = 0; i < N; ++i) vnode[i] = c.h_seg_vmag[seg_of[i]];
   c.h_M_nodal.assign(N, 0.0);
   std::vector<double> dex(N, 0.0);
   for (int k = 0; k < E; ++k) {
@@ -630,8 +644,8 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   c.h_exdiag.resize(N);
   for (int i = 0; i < N; ++i) {
     const double M = c.h_M_nodal[i];
-    c.h_inv_M_rel[i] = M > 1e-20 ? c.V_mag / M : 0.0;
-    const double dd = dex[i] / c.V_mag;
+    c.h_inv_M_rel[i] = M > 1e-20 ? vnode[i] / M : 0.0;
+    const double dd = dex[i] / vnode[i];
     c.h_exdiag[i] = dd;
     c.h_inv_M_prec[i] = dd > 1e-20 ? 1.0 / dd : 1.0;
   }
@@ -639,6 +653,14 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   MM_TRY(upload(c, &c.inv_M_rel, c.h_inv_M_rel.data(), N));
   MM_TRY(upload(c, &c.inv_M_prec, c.h_inv_M_prec.data(), N));
   c.geo.M_nodal = c.M_nodal;
+  if (c.nseg > 1) {
+    for (double& v : vnode) v = 1.0 / v;
+    MM_TRY(upload(c, &c.inv_V_node, vnode.data(), N));
+    c.geo.inv_V_node = c.inv_V_node;
+    int* d = nullptr;
+    MM_TRY(upload(c, &d, c.h_seg_blk_mag.data(), c.h_seg_blk_mag.size()));
+    c.d_seg_blk_mag = d;
+  }
 
   // reductions and scalars
   const int max_grid = std::max({kRedBlocks, 8 * ((c.part_all.nblk + 7) / 8), 8 * ((c.part_mag.nblk + 7) / 8)});
@@ -956,6 +978,31 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
     L.lanesP = lanes(Pp[n], n);
     L.lanesR = lanes(Rp[n_next], n_next);
   }
+  // rows of each segment on this level (contiguous when the coarse unknowns follow the node
+  // order, see amg.py), and from P those of the next level
+  if ((int)c.amg_seg_off.size() <= l + 1) c.amg_seg_off.resize(l + 2);
+  if (l == 0) c.amg_seg_off[0] = c.nseg > 1 ? c.h_seg_node_off : std::vector<int>();
+  L.seg_off = c.amg_seg_off[l];
+  c.amg_seg_off[l + 1].clear();
+  if (n_next > 0 && (int)L.seg_off.size() == c.nseg + 1) {
+    std::vector<int> nx(c.nseg + 1, 0);
+    bool ok = true;
+    int expect = 0;
+    for (int g = 0; g < c.nseg && ok; ++g) {
+      int lo = INT32_MAX, hi = -1;
+      for (int r = L.seg_off[g]; r < L.seg_off[g + 1]; ++r)
+        for (int k = Pp[r]; k < Pp[r + 1]; ++k) {
+          lo = std::min(lo, Pi[k]);
+          hi = std::max(hi, Pi[k]);
+        }
+      ok = hi >= lo && lo == expect;  // non-empty and right after the previous segment
+      nx[g] = lo;
+      expect = hi + 1;
+    }
+    ok = ok && expect == n_next;
+    nx[c.nseg] = n_next;
+    if (ok) c.amg_seg_off[l + 1] = nx;
+  }
   if (l > 0) {
     MM_TRY(dalloc(c, &L.b, n));
     MM_TRY(dalloc(c, &L.x, n));
@@ -972,6 +1019,7 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
 
 int mm_amg_commit(int64_t h, int kind, int nlev) {
   GET_CTX(h);
+  c.blist_key.clear();  // segment subsets recompute the row runs of the new levels
   MM_CHECK(nlev >= 1 && (int)c.lv.size() >= nlev, "levels missing");
   c.lv.resize(nlev);
   for (int l = 0; l < nlev; ++l) {
@@ -1166,6 +1214,7 @@ int mm_minimize_resident(int64_t h, const mm_params* p, const double* B, mm_info
   MM_CHECK(p && B && info, "null argument");
   MM_CHECK(c.have_state, "no resident state (call mm_state_set)");
   MM_CHECK(p->method == 0 || p->method == 1, "method must be 0 (pcohen_hs) or 1 (lbfgs)");
+  MM_CHECK(c.nseg == 1, "the context holds several independent systems: use mm_minimize_segmented");
   MinParams q;
   to_minparams(p, q);
   MinInfo mi;
@@ -1178,6 +1227,47 @@ int mm_minimize_resident(int64_t h, const mm_params* p, const double* B, mm_info
   info->E = mi.E;
   info->gnorm = mi.gnorm;
   info->time = mi.time;
+  return 0;
+}
+
+int mm_minimize_segmented(int64_t h, const mm_params* p, const double* B, mm_info* info) {
+  GET_CTX(h);
+  MM_CHECK(p && B && info, "null argument");
+  MM_CHECK(c.have_state, "no resident state (call mm_state_set)");
+  MM_CHECK(c.nseg > 1, "mm_minimize_segmented needs a context created with mm_ctx_create_segmented");
+  MM_CHECK(c.lv.size() > 1 && c.lv[0].M != nullptr && c.geom_mode == kGeomCoords,
+           "segmented minimisation needs the AMG V-cycle and the on-the-fly apply");
+  MinParams q;
+  to_minparams(p, q);
+  std::vector<MinInfo> mi(c.nseg);
+  MM_TRY(minimize_segmented(c, q, B, mi.data()));
+  for (int g = 0; g < c.nseg; ++g) {
+    info[g].iters = mi[g].iters;
+    info[g].evals = mi[g].evals;
+    info[g].demag_iters = mi[g].demag_iters;
+    info[g].preco_iters = mi[g].preco_iters;
+    info[g].converged = mi[g].converged;
+    info[g].E = mi[g].E;
+    info[g].gnorm = mi[g].gnorm;
+    info[g].time = mi[g].time;
+  }
+  return 0;
+}
+
+int mm_state_j_parallel_segmented(int64_t h, const double* hdir, double* J) {
+  GET_CTX(h);
+  MM_CHECK(hdir && J, "null argument");
+  MM_CHECK(c.have_state, "no resident state");
+  MM_CHECK(c.nseg > 1, "mm_state_j_parallel_segmented needs a segmented context");
+  const double hn = std::sqrt(hdir[0] * hdir[0] + hdir[1] * hdir[1] + hdir[2] * hdir[2]) + 1e-30;
+  const double hh[3] = {hdir[0] / hn, hdir[1] / hn, hdir[2] / hn};
+  return j_parallel_segmented(c, c.m, hh, J);
+}
+
+int mm_seg_volumes(int64_t h, double* V_mag) {
+  GET_CTX(h);
+  MM_CHECK(V_mag, "null argument");
+  for (int g = 0; g < c.nseg; ++g) V_mag[g] = c.h_seg_vmag[g];
   return 0;
 }
 

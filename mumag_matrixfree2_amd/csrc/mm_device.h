@@ -7,13 +7,17 @@
 namespace mm {
 
 // Workgroup -> element block. Workgroups are dispatched round-robin over the 8 XCDs
-// (id mod 8), so XCD j gets the contiguous block range [j*chunk, (j+1)*chunk).
+// (id mod 8), so XCD j gets the contiguous position range [j*chunk, (j+1)*chunk) of the
+// block sequence (every block, or the subset blist). bi.b is the block id, -1 for workgroups
+// past the end (they write no per-block partial).
 __device__ __forceinline__ BlkInfo blk_info(const Part& p) {
   BlkInfo bi;
-  const int chunk = (p.nblk + 7) >> 3;
-  const int b = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+  const int n = p.blist ? p.nlist : p.nblk;
+  const int chunk = (n + 7) >> 3;
+  const int i = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+  bi.valid = i < n;
+  const int b = bi.valid ? (p.blist ? p.blist[i] : i) : -1;
   bi.b = b;
-  bi.valid = b < p.nblk;
   if (bi.valid) {
     bi.e0 = p.eptr[b];
     bi.e1 = p.eptr[b + 1];
@@ -150,8 +154,8 @@ __device__ __forceinline__ CSum wave_red_cs(CSum v) {
   return v;
 }
 
-// Block compensated sum written as (s, c) to partial[2 * blockIdx.x + {0, 1}].
-__device__ void block_partial_cs(CSum v, double* __restrict__ partial) {
+// Block compensated sum written as (s, c) to partial[2 * idx + {0, 1}].
+__device__ void block_partial_cs(CSum v, double* __restrict__ partial, int idx) {
   __shared__ double sh[2][kBlock / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   v = wave_red_cs(v);
@@ -168,8 +172,8 @@ __device__ void block_partial_cs(CSum v, double* __restrict__ partial) {
       u.c = sh[1][w];
       t.add(u);
     }
-    partial[2 * blockIdx.x] = t.s;
-    partial[2 * blockIdx.x + 1] = t.c;
+    partial[2 * idx] = t.s;
+    partial[2 * idx + 1] = t.c;
   }
 }
 
@@ -251,14 +255,15 @@ __device__ bool reduce_last(double (&v)[K], double* __restrict__ partial, unsign
   return true;
 }
 
-// Block sum of K values written to partial[blockIdx.x * K + k] (no cross-block traffic);
-// sum_partials then reduces them in a fixed order.
-template <int K, int NT = kBlock>
-__device__ void block_partial(double (&v)[K], double* __restrict__ partial) {
+// Block sum (or maximum) of K values written to partial[idx * K + k], idx = blockIdx.x by
+// default (no cross-block traffic); sum_partials then reduces them in a fixed order.
+template <int K, int NT = kBlock, bool MAX = false>
+__device__ void block_partial(double (&v)[K], double* __restrict__ partial, int idx = -1) {
   __shared__ double sh[K][NT / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (idx < 0) idx = blockIdx.x;
 #pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = wave_red<false>(v[k]);
+  for (int k = 0; k < K; ++k) v[k] = wave_red<MAX>(v[k]);
   if (lane == 0)
 #pragma unroll
     for (int k = 0; k < K; ++k) sh[k][wv] = v[k];
@@ -267,8 +272,8 @@ __device__ void block_partial(double (&v)[K], double* __restrict__ partial) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       double s = sh[k][0];
-      for (int w = 1; w < NT / 64; ++w) s += sh[k][w];
-      partial[blockIdx.x * K + k] = s;
+      for (int w = 1; w < NT / 64; ++w) s = MAX ? fmax(s, sh[k][w]) : s + sh[k][w];
+      partial[idx * K + k] = s;
     }
 }
 

@@ -20,6 +20,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace mm {
@@ -83,6 +84,20 @@ struct Part {
   // indices of slots 0-3 / 4-7 and cmask bits 0-5 the tets, bits 8-15 the used slots
   const uint32_t* lconn2 = nullptr;
   const uint32_t* cmask = nullptr;
+  // optional subset of the blocks (the blocks of some segments of a segmented context):
+  // workgroup position i < nlist works on block blist[i]; null: every block
+  const int* blist = nullptr;
+  int nlist = 0;
+};
+
+// Workgroup tables of a segmented vector kernel over a subset of the segments: workgroup w
+// covers chunk wg_chunk[w] (kSegChunk nodes) of segment wg_seg[w]; the workgroups of segment g
+// are wg_off[g] .. wg_off[g + 1] - 1 (empty when g is not in the subset).
+struct WgTables {
+  int n = 0;                          // workgroups
+  int *seg = nullptr, *chunk = nullptr, *off = nullptr;  // device
+  std::vector<int> h_off;             // host copy of off
+  std::vector<int> key;               // segment mask the tables were built for
 };
 
 struct Geo {
@@ -93,6 +108,9 @@ struct Geo {
   const float* Ve32 = nullptr;
   const MatTable* mt = nullptr;
   const double* M_nodal = nullptr;    // N
+  // N: 1 / V_mag of the node's segment (contexts of several independent systems); null: the
+  // scalar 1 / V_mag passed to the kernel
+  const double* inv_V_node = nullptr;
 };
 
 struct BlkInfo {
@@ -100,6 +118,8 @@ struct BlkInfo {
   int e0, e1, n0, nb;
   bool valid;
 };
+
+using Runs = std::vector<std::pair<int, int>>;
 
 // One AMG level; level 0 uses the matrix-free apply (A arrays unused there).
 struct Level {
@@ -123,6 +143,11 @@ struct Level {
   // FP32 copies for the mixed-precision V-cycle
   float *Axf = nullptr, *Pxf = nullptr, *Rxf = nullptr, *Mf = nullptr;
   float *bf = nullptr, *xf = nullptr, *tf = nullptr, *xsf = nullptr;
+  // segmented contexts: the rows of segment g are seg_off[g] .. seg_off[g + 1] - 1 (empty when
+  // the level's unknowns are not ordered by segment); runs: the row ranges of the segments a
+  // segmented solve still iterates (empty: every row)
+  std::vector<int> seg_off;
+  Runs runs;
 };
 
 struct Ctx {
@@ -173,6 +198,7 @@ struct Ctx {
   // AMG
   int amg_kind = 0;                   // 0: Jacobi on diag(A) only, 1: SPAI0 V-cycle, 2: damped-Jacobi V-cycle
   std::vector<Level> lv;
+  std::vector<std::vector<int>> amg_seg_off;  // per level: rows of each segment (set-up)
   double* diagA = nullptr;            // N
   int precision = 0;                  // 0: FP64 V-cycle, 1: FP32 V-cycle (mixed precision)
   float *px0f = nullptr, *ptf = nullptr;  // level-0 FP32 work vectors
@@ -192,16 +218,28 @@ struct Ctx {
   std::vector<int> h_seg_node_off;    // nseg + 1
   std::vector<int> h_seg_blk_all;     // nseg + 1 block offsets of part_all
   std::vector<int> h_seg_blk_cl;      // nseg + 1 block offsets of part_cl
+  std::vector<int> h_seg_blk_mag;     // nseg + 1 block offsets of part_mag
+  std::vector<double> h_seg_vmag;     // magnetic volume per segment
   // segmented PCG work decomposition (set up on first use): workgroup w handles nodes of
   // segment wg_seg[w] only, kSegChunk at a time
   int seg_nwg = 0;
   int *d_seg_node_off = nullptr, *d_seg_blk_all = nullptr, *d_seg_blk_cl = nullptr;
-  int *d_seg_wg_off = nullptr, *d_wg_seg = nullptr, *d_seg_active = nullptr;
+  int* d_seg_active = nullptr;
   double* d_seg_rz = nullptr;         // 2 x nseg r.z ping-pong
   double* seg_part_rz = nullptr;      // seg_nwg r.z partials
   double *hseg = nullptr, *hseg_dev = nullptr;  // host-coherent r.r / b.b partials, 2 x seg_nwg
   std::vector<int> h_seg_wg_off, h_seg_active;
   bool seg_running = false;           // the preconditioner ends with per-segment r.z partials
+  int* d_seg_blk_mag = nullptr;
+  // block subsets of the three partitions (set_block_lists) and the segment mask they hold
+  int *d_blist_cl = nullptr, *d_blist_all = nullptr, *d_blist_mag = nullptr;
+  std::vector<int> h_blist_cl, h_blist_all, h_blist_mag, blist_key;
+  WgTables wg_pcg, wg_min;            // vector-kernel subsets of the segmented PCG / minimiser
+  double* inv_V_node = nullptr;       // N (segmented contexts only)
+  // segmented minimiser: per-segment scalar table, host-set inputs, partials
+  double* sm_tab = nullptr;           // nseg x kSegSlots
+  double* sm_in = nullptr;            // nseg x kSegIn
+  double *sm_pa = nullptr, *sm_pb = nullptr;  // seg_nwg x 4 partials each
 
   long long n_apply = 0, n_rhs = 0, n_energy = 0, n_local = 0, n_vcycle = 0, n_pcg_it = 0;
 
@@ -250,5 +288,19 @@ int solve_U_dev(Ctx& c, const double* m, double* U, double tol, int maxiter, int
 // solve_U for every segment of the context at once (one launch per kernel for all segments,
 // per-segment PCG scalars and stopping test): tol, it, rel_res have one entry per segment
 int solve_U_seg(Ctx& c, const double* m, double* U, const double* tol, int maxiter, int* it, double* rel_res);
+// device-resident form: b = rhs(m) already in c.pb is not assumed; only segments with want[g] != 0
+// iterate (the others keep U); it / rel_res per segment
+int solve_U_seg_dev(Ctx& c, const double* m, double* U, const double* tol, const int* want, int maxiter, int* it,
+                    double* rel_res);
+// segmented decomposition tables (once per context)
+int ensure_seg(Ctx& c);
+// restrict the element kernels to the blocks of the segments with mask[g] != 0 (null: all)
+int set_block_lists(Ctx& c, const int* mask);
+// workgroup tables of the segments with mask[g] != 0 (null: all)
+int build_wg_tables(Ctx& c, const int* mask, WgTables& T);
+// per-segment sums of the magnetic-partition block partials: out[g * stride + k] for k < K
+// (part holds K values per logical block; compensated pairs when CS)
+int launch_seg_block_sums(const Ctx& c, const double* part, int K, bool cs, double* out, int stride, int slot,
+                          hipStream_t s);
 
 }  // namespace mm

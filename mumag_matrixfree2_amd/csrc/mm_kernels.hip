@@ -143,7 +143,8 @@ __global__ __launch_bounds__(NT) void k_apply(Part p, Geo geo, const double* __r
     }
     node_out(y, f, v);
   }
-  if constexpr (DOT) block_sum_to<NT>(acc, dot_part + bi.b);
+  if constexpr (DOT)
+    if (bi.b >= 0) block_sum_to<NT>(acc, dot_part + bi.b);
 }
 
 // ------------------------------------------------ Poisson apply, FP32 (preconditioner)
@@ -397,7 +398,8 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
     else
       unsafeAtomicAdd(&y[id], (T)v);
   }
-  if constexpr (DOT) block_sum_to<NT>(acc_e, dot_part + bi.b);
+  if constexpr (DOT)
+    if (bi.b >= 0) block_sum_to<NT>(acc_e, dot_part + bi.b);
 }
 
 // Persistent form of k_apply_cl (FP64): a grid of a few workgroups per CU walks the blocks of
@@ -418,8 +420,10 @@ __global__ __launch_bounds__(kMaxBlockNodes) void k_apply_clp(Part p, Geo geo, c
   __shared__ double sred[NT / 64];
   const int tid = threadIdx.x;
   const int nxcd = (int)(gridDim.x >> 3);          // workgroups per XCD (grid is a multiple of 8)
-  const int chunk = (p.nblk + 7) >> 3;
-  const int b_end = min(p.nblk, ((int)(blockIdx.x & 7) + 1) * chunk);
+  // b walks positions of the block sequence (every block, or the subset blist)
+  const int nseq = p.blist ? p.nlist : p.nblk;
+  const int chunk = (nseq + 7) >> 3;
+  const int b_end = min(nseq, ((int)(blockIdx.x & 7) + 1) * chunk);
   int b = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
   // staged registers of the current block. The loads below are unconditional (indices are
   // clamped into the block, out-of-range lanes discard their values): a load under a branch
@@ -427,7 +431,8 @@ __global__ __launch_bounds__(kMaxBlockNodes) void k_apply_clp(Part p, Geo geo, c
   uint32_t f = 0u;
   double xv = 0.0, X0 = 0.0, X1 = 0.0, X2 = 0.0;
   int e0 = 0, e1 = 0, n0 = 0, nb = 0;
-  auto fetch_meta = [&](int bb) {
+  auto fetch_meta = [&](int pos) {
+    const int bb = p.blist ? p.blist[pos] : pos;
     e0 = p.eptr[bb];
     e1 = p.eptr[bb + 1];
     n0 = p.nptr[bb];
@@ -499,7 +504,7 @@ __global__ __launch_bounds__(kMaxBlockNodes) void k_apply_clp(Part p, Geo geo, c
       if (tid == 0) {
         double s = sred[0];
         for (int q = 1; q < NT / 64; ++q) s += sred[q];
-        dot_part[b] = s;
+        dot_part[p.blist ? p.blist[b] : b] = s;
       }
     }
     __syncthreads();  // LDS is restaged for the next block
@@ -562,6 +567,8 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
   __shared__ double sg[3][kMaxBlockNodes];
   const BlkInfo bi = blk_info(p);
   const int tid = threadIdx.x;
+  // a block lies inside one segment: one 1 / V_mag for all its nodes
+  const double iv = (geo.inv_V_node && bi.nb > 0) ? geo.inv_V_node[p.nodes[bi.n0] & kIdMask] : inv_V;
   CSum acc;
   for (int i = tid; i < bi.nb; i += kBlock) {
     const uint32_t id = p.nodes[bi.n0 + i] & kIdMask;
@@ -635,18 +642,18 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
       acc.add(sm[0][i] * z0 + sm[1][i] * z1 + sm[2][i] * z2);
     }
     if (f & kPrivate) {
-      g[3 * id + 0] = v0 * inv_V;
-      g[3 * id + 1] = v1 * inv_V;
-      g[3 * id + 2] = v2 * inv_V;
+      g[3 * id + 0] = v0 * iv;
+      g[3 * id + 1] = v1 * iv;
+      g[3 * id + 2] = v2 * iv;
     } else {
-      unsafeAtomicAdd(&g[3 * id + 0], v0 * inv_V);
-      unsafeAtomicAdd(&g[3 * id + 1], v1 * inv_V);
-      unsafeAtomicAdd(&g[3 * id + 2], v2 * inv_V);
+      unsafeAtomicAdd(&g[3 * id + 0], v0 * iv);
+      unsafeAtomicAdd(&g[3 * id + 1], v1 * iv);
+      unsafeAtomicAdd(&g[3 * id + 2], v2 * iv);
     }
   }
-  acc.s *= inv_V;
-  acc.c *= inv_V;
-  block_partial_cs(acc, partial);
+  acc.s *= iv;
+  acc.c *= iv;
+  if (bi.b >= 0) block_partial_cs(acc, partial, bi.b);  // by block id: a segment's blocks are contiguous
 }
 
 // out[k] = sum_b partial[b * K + k], one workgroup, fixed order
@@ -705,6 +712,65 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials_cs(const double* __rest
   }
 }
 
+// Per-segment sums of block partials written by logical block: workgroup g reduces blocks
+// blk_off[g] .. blk_off[g + 1] (a segment's blocks are contiguous) in a fixed order into
+// out[g * stride + slot + k], K values per block; CS: (s, c) pairs with compensation (K = 1).
+template <int K, bool CS>
+__global__ __launch_bounds__(kBlock) void k_seg_block_sums(const int* __restrict__ blk_off,
+                                                           const double* __restrict__ part, double* __restrict__ out,
+                                                           int stride, int slot) {
+  const int g = blockIdx.x;
+  const int b0 = blk_off[g], b1 = blk_off[g + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if constexpr (CS) {
+    __shared__ double sh[2][kBlock / 64];
+    CSum v;
+    for (int i = b0 + tid; i < b1; i += kBlock) {
+      CSum u;
+      u.s = part[2 * i];
+      u.c = part[2 * i + 1];
+      v.add(u);
+    }
+    v = wave_red_cs(v);
+    if (lane == 0) {
+      sh[0][wv] = v.s;
+      sh[1][wv] = v.c;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      CSum t;
+      for (int w = 0; w < kBlock / 64; ++w) {
+        CSum u;
+        u.s = sh[0][w];
+        u.c = sh[1][w];
+        t.add(u);
+      }
+      out[(long)g * stride + slot] = t.s + t.c;
+    }
+  } else {
+    __shared__ double sh[K][kBlock / 64];
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = 0.0;
+    for (int i = b0 + tid; i < b1; i += kBlock)
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] += part[(long)i * K + k];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      v[k] = wave_red<false>(v[k]);
+      if (lane == 0) sh[k][wv] = v[k];
+    }
+    __syncthreads();
+    if (tid == 0)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double t = sh[k][0];
+        for (int w = 1; w < kBlock / 64; ++w) t += sh[k][w];
+        out[(long)g * stride + slot + k] = t;
+      }
+  }
+}
+
 // --------------------------------------------------- local Hessian action C v (K8)
 // C v = (K_ex + K_an) v / V_mag: exchange and anisotropy parts of K7 without demag/Zeeman.
 __global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const double* __restrict__ vin,
@@ -717,6 +783,7 @@ __global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const double*
   const BlkInfo bi = blk_info(p);
   if (!bi.valid) return;
   const int tid = threadIdx.x;
+  const double iv = (geo.inv_V_node && bi.nb > 0) ? geo.inv_V_node[p.nodes[bi.n0] & kIdMask] : inv_V;
   for (int i = tid; i < bi.nb; i += kBlock) {
     const uint32_t id = p.nodes[bi.n0 + i] & kIdMask;
 #pragma unroll
@@ -764,10 +831,10 @@ __global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const double*
     const uint32_t id = f & kIdMask;
     if (f & kPrivate) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) out[3 * id + k] = sg[k][i] * inv_V;
+      for (int k = 0; k < 3; ++k) out[3 * id + k] = sg[k][i] * iv;
     } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&out[3 * id + k], sg[k][i] * inv_V);
+      for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&out[3 * id + k], sg[k][i] * iv);
     }
   }
 }
@@ -866,11 +933,12 @@ __global__ __launch_bounds__(kBlock) void k_jpar(Part p, Geo geo, const double* 
     acc[0] += vol * Js * 0.25 * (sh[l[0]] + sh[l[1]] + sh[l[2]] + sh[l[3]]);
     acc[1] += vol;
   }
-  block_partial<2>(acc, partial);
+  if (bi.b >= 0) block_partial<2>(acc, partial, bi.b);
 }
 
 // ------------------------------------------------------------------------ launchers
-static inline int grid_for(const Part& p) { return 8 * ((p.nblk + 7) / 8); }
+static inline int part_len(const Part& p) { return p.blist ? p.nlist : p.nblk; }
+static inline int grid_for(const Part& p) { return 8 * ((part_len(p) + 7) / 8); }
 
 // Calls f(std::integral_constant<int, NT>) for the context's workgroup size NT (64, 128, 256).
 template <typename F>
@@ -885,7 +953,7 @@ static void with_nt(int nt, F&& f) {
 // persistent grid: apply_persist workgroups per CU, a multiple of 8 (one share per XCD)
 static int persist_grid(const Ctx& c) {
   const int g = 8 * std::max(1, (c.num_cu * c.apply_persist) / 8);
-  return std::min(g, 8 * ((c.part_cl.nblk + 7) / 8));
+  return std::min(g, 8 * ((part_len(c.part_cl) + 7) / 8));
 }
 
 int apply_dot_count(const Ctx& c) { return (c.apply_clusters ? c.part_cl : c.part_all).nblk; }
@@ -894,7 +962,7 @@ int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t
   MM_CHECK(dot_part == nullptr || geom == kGeomCoords, "x.Ax partials need the on-the-fly apply");
   if (geom == kGeomCoords && c.apply_clusters && c.apply_persist > 0) {
     const Part& p = c.part_cl;
-    if (p.nblk == 0) return 0;
+    if (part_len(p) == 0) return 0;
     if (dot_part)
       k_apply_clp<false, true><<<persist_grid(c), kMaxBlockNodes, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
                                                                         c.reg, dot_part);
@@ -905,7 +973,7 @@ int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t
   }
   if (geom == kGeomCoords && c.apply_clusters) {
     const Part& p = c.part_cl;
-    if (p.nblk == 0) return 0;
+    if (part_len(p) == 0) return 0;
     const int grid = grid_for(p);
     with_nt(c.elem_threads, [&](auto nt) {
       constexpr int NT = decltype(nt)::value;
@@ -919,7 +987,7 @@ int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t
     return 0;
   }
   const Part& p = c.part_all;
-  if (p.nblk == 0) return 0;
+  if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
@@ -937,7 +1005,7 @@ int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t
 
 int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s) {
   const Part& p = c.apply_clusters ? c.part_cl : c.part_all;
-  if (p.nblk == 0) return 0;
+  if (part_len(p) == 0) return 0;
   if (c.apply_clusters && c.apply_persist > 0) {
     k_apply_clp<true, false><<<persist_grid(c), kMaxBlockNodes, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, c.reg,
                                                                       nullptr);
@@ -956,7 +1024,7 @@ int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* 
 
 int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s) {
   const Part& p = c.apply_clusters ? c.part_cl : c.part_all;
-  if (p.nblk == 0) return 0;
+  if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
@@ -971,7 +1039,7 @@ int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s) {
 
 int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float* xo, float* y, hipStream_t s) {
   const Part& p = c.apply_clusters ? c.part_cl : c.part_all;
-  if (p.nblk == 0) return 0;
+  if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
@@ -985,7 +1053,7 @@ int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float
 
 int launch_rhs(const Ctx& c, const double* m, double* b, hipStream_t s) {
   const Part& p = c.part_mag;
-  if (p.nblk == 0) return 0;
+  if (part_len(p) == 0) return 0;
   k_rhs<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, m, b);
   return 0;
 }
@@ -993,18 +1061,30 @@ int launch_rhs(const Ctx& c, const double* m, double* b, hipStream_t s) {
 int launch_energy(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,
                   hipStream_t s) {
   const Part& p = c.part_mag;
-  if (p.nblk == 0) return 0;
+  if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
   k_energy<<<grid, kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, c.partial, c.counter + 1,
                                    E_out);
-  k_sum_partials_cs<<<1, kBlock, 0, s>>>(c.partial, grid, E_out);
+  k_sum_partials_cs<<<1, kBlock, 0, s>>>(c.partial, p.nblk, E_out);  // partials by block id
   return 0;
 }
 
 int launch_local(const Ctx& c, const double* v, double* out, const double* gate, hipStream_t s) {
   const Part& p = c.part_mag;
-  if (p.nblk == 0) return 0;
+  if (part_len(p) == 0) return 0;
   k_local<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, v, 1.0 / c.V_mag, out, gate);
+  return 0;
+}
+
+int launch_seg_block_sums(const Ctx& c, const double* part, int K, bool cs, double* out, int stride, int slot,
+                          hipStream_t s) {
+  if (c.d_seg_blk_mag == nullptr) return -1;
+  if (cs)
+    k_seg_block_sums<1, true><<<c.nseg, kBlock, 0, s>>>(c.d_seg_blk_mag, part, out, stride, slot);
+  else if (K == 1)
+    k_seg_block_sums<1, false><<<c.nseg, kBlock, 0, s>>>(c.d_seg_blk_mag, part, out, stride, slot);
+  else
+    k_seg_block_sums<2, false><<<c.nseg, kBlock, 0, s>>>(c.d_seg_blk_mag, part, out, stride, slot);
   return 0;
 }
 
@@ -1019,17 +1099,17 @@ int launch_geometry(const Ctx& c, const int* conn_dev, hipStream_t s) {
 
 int launch_diag(const Ctx& c, double* d, hipStream_t s) {
   const Part& p = c.part_all;
-  if (p.nblk == 0) return 0;
+  if (part_len(p) == 0) return 0;
   k_diag<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, c.reg, d);
   return 0;
 }
 
 int launch_jpar(const Ctx& c, const double* m, const double h[3], double* out, hipStream_t s) {
   const Part& p = c.part_mag;
-  if (p.nblk == 0) return 0;
+  if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
   k_jpar<<<grid, kBlock, 0, s>>>(p, c.geo, m, h[0], h[1], h[2], c.partial, c.counter + 2, out);
-  k_sum_partials<2><<<1, kBlock, 0, s>>>(c.partial, grid, out);
+  k_sum_partials<2><<<1, kBlock, 0, s>>>(c.partial, p.nblk, out);  // partials by block id
   return 0;
 }
 

@@ -30,5 +30,9 @@ struct MinInfo {
 };
 
 int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& info);
+// pcohen_hs field step of every segment of a segmented context at once; info has nseg entries
+int minimize_segmented(Ctx& c, const MinParams& p, const double B[3], MinInfo* info);
+// per-segment volume-averaged J along the unit vector h (nseg values)
+int j_parallel_segmented(Ctx& c, const double* m, const double h[3], double* out);
 
 }  // namespace mm

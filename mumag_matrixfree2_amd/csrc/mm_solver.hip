@@ -11,6 +11,8 @@
 #include <cmath>
 #include <vector>
 
+#include <functional>
+
 #include "mm_device.h"
 #include "mm_solver.h"
 
@@ -160,11 +162,12 @@ constexpr int kUnroll = 4;  // entries per lane whose loads are in flight togeth
 // MODE 0: y = A x; 1: y += A x (and z = 0 on the same rows when z != nullptr); 2: y = A (x - x2);
 // 3: y = A (x2 .* x) and z = x2 .* x on the same rows (pre-smoothing from a zero guess, fused)
 template <int MODE, int LANES, typename TA, typename TX, typename TX2, typename TY>
-__global__ __launch_bounds__(kBlock) void k_spmv(int nrows, const int* __restrict__ Ap, const int* __restrict__ Ai,
-                                                 const TA* __restrict__ Ax, const TX* __restrict__ x,
-                                                 const TX2* __restrict__ x2, TY* __restrict__ y, TY* __restrict__ z) {
+__global__ __launch_bounds__(kBlock) void k_spmv(int row0, int nrows, const int* __restrict__ Ap,
+                                                 const int* __restrict__ Ai, const TA* __restrict__ Ax,
+                                                 const TX* __restrict__ x, const TX2* __restrict__ x2,
+                                                 TY* __restrict__ y, TY* __restrict__ z) {
   const int lane = threadIdx.x & (LANES - 1);
-  const int row = (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
+  const int row = row0 + (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
   TA s = 0;
   if (row < nrows) {
     const int j0 = Ap[row], j1 = Ap[row + 1];
@@ -203,12 +206,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrows, const int* __restric
 
 // Post-smoothing step as one SpMV: xs = x + M .* (b - A x), written to a separate vector.
 template <int LANES, typename T>
-__global__ __launch_bounds__(kBlock) void k_spmv_post(int nrows, const int* __restrict__ Ap,
+__global__ __launch_bounds__(kBlock) void k_spmv_post(int row0, int nrows, const int* __restrict__ Ap,
                                                       const int* __restrict__ Ai, const T* __restrict__ Ax,
                                                       const T* __restrict__ x, const T* __restrict__ M,
                                                       const T* __restrict__ b, T* __restrict__ xs) {
   const int lane = threadIdx.x & (LANES - 1);
-  const int row = (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
+  const int row = row0 + (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
   T s = 0;
   if (row < nrows) {
     const int j0 = Ap[row], j1 = Ap[row + 1];
@@ -310,45 +313,58 @@ static void symv(const Level& L, const T* tiles, const T* b, T* y, hipStream_t s
 }
 
 template <int MODE, typename TA, typename TX, typename TX2, typename TY>
-static void spmv_lanes(int lanes, int nrows, const int* Ap, const int* Ai, const TA* Ax, const TX* x, const TX2* x2,
-                       TY* y, TY* z, hipStream_t s) {
-  const int grid = (int)(((long)nrows * lanes + kBlock - 1) / kBlock);
+static void spmv_lanes(int lanes, int row0, int row1, const int* Ap, const int* Ai, const TA* Ax, const TX* x,
+                       const TX2* x2, TY* y, TY* z, hipStream_t s) {
+  const int grid = (int)(((long)(row1 - row0) * lanes + kBlock - 1) / kBlock);
   switch (lanes) {
-    case 4: k_spmv<MODE, 4, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
-    case 8: k_spmv<MODE, 8, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
-    case 16: k_spmv<MODE, 16, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
-    case 32: k_spmv<MODE, 32, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
-    default: k_spmv<MODE, 64, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, x2, y, z); break;
+    case 4: k_spmv<MODE, 4, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z); break;
+    case 8: k_spmv<MODE, 8, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z); break;
+    case 16: k_spmv<MODE, 16, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z); break;
+    case 32: k_spmv<MODE, 32, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z); break;
+    default: k_spmv<MODE, 64, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z); break;
   }
+}
+
+// Row ranges of one launch: every row of the level (runs null or empty), or the rows of the
+// segments a segmented solve still iterates (set_block_lists)
+static inline void for_runs(const Runs* runs, int nrows, const std::function<void(int, int)>& f) {
+  if (runs == nullptr || runs->empty()) {
+    if (nrows > 0) f(0, nrows);
+    return;
+  }
+  for (const auto& r : *runs)
+    if (r.second > r.first) f(r.first, r.second);
 }
 
 // y = A x (mode 0), y += A x and z = 0 (mode 1), y = A (x - x2) (mode 2)
 template <typename TA, typename TX, typename TX2, typename TY>
 static void spmv(int mode, int lanes, int nrows, const int* Ap, const int* Ai, const TA* Ax, const TX* x,
-                 const TX2* x2, TY* y, hipStream_t s, TY* z = nullptr) {
-  if (nrows <= 0) return;
-  if (mode == 0)
-    spmv_lanes<0>(lanes, nrows, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, s);
-  else if (mode == 1)
-    spmv_lanes<1>(lanes, nrows, Ap, Ai, Ax, x, x2, y, z, s);
-  else if (mode == 2)
-    spmv_lanes<2>(lanes, nrows, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, s);
-  else
-    spmv_lanes<3>(lanes, nrows, Ap, Ai, Ax, x, x2, y, z, s);
+                 const TX2* x2, TY* y, hipStream_t s, TY* z = nullptr, const Runs* runs = nullptr) {
+  for_runs(runs, nrows, [&](int r0, int r1) {
+    if (mode == 0)
+      spmv_lanes<0>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, s);
+    else if (mode == 1)
+      spmv_lanes<1>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, z, s);
+    else if (mode == 2)
+      spmv_lanes<2>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, s);
+    else
+      spmv_lanes<3>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, z, s);
+  });
 }
 
 template <typename T>
 static void spmv_post(int lanes, int nrows, const int* Ap, const int* Ai, const T* Ax, const T* x, const T* M,
-                      const T* b, T* xs, hipStream_t s) {
-  if (nrows <= 0) return;
-  const int grid = (int)(((long)nrows * lanes + kBlock - 1) / kBlock);
-  switch (lanes) {
-    case 4: k_spmv_post<4, T><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, M, b, xs); break;
-    case 8: k_spmv_post<8, T><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, M, b, xs); break;
-    case 16: k_spmv_post<16, T><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, M, b, xs); break;
-    case 32: k_spmv_post<32, T><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, M, b, xs); break;
-    default: k_spmv_post<64, T><<<grid, kBlock, 0, s>>>(nrows, Ap, Ai, Ax, x, M, b, xs); break;
-  }
+                      const T* b, T* xs, hipStream_t s, const Runs* runs = nullptr) {
+  for_runs(runs, nrows, [&](int r0, int r1) {
+    const int grid = (int)(((long)(r1 - r0) * lanes + kBlock - 1) / kBlock);
+    switch (lanes) {
+      case 4: k_spmv_post<4, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      case 8: k_spmv_post<8, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      case 16: k_spmv_post<16, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      case 32: k_spmv_post<32, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      default: k_spmv_post<64, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+    }
+  });
 }
 
 // ------------------------------------------------------------ FP32 V-cycle helpers
@@ -428,17 +444,20 @@ static int timed_apply_scaled(Ctx& c, const double* r, const double* w, double* 
 
 // ---------------------------------------------------------------- segmented PCG kernels
 // Several independent systems (segments: contiguous node ranges, e.g. the grains of a grain set)
-// share one context and are solved together: every kernel launch covers all segments, and each
-// segment keeps its own alpha, beta and stopping test. Workgroup w works on nodes of segment
-// wg_seg[w] only (kSegChunk per workgroup), so its partial sums belong to one segment; a
-// consumer sums the partials of its own segment in a fixed order.
+// share one context and are solved together: each kernel launch covers the segments that are
+// still iterating, and each segment keeps its own alpha, beta and stopping test. Workgroup w
+// works on chunk wg_chunk[w] (kSegChunk nodes) of segment wg_seg[w] only, so its partial sums
+// belong to one segment; a consumer sums the partials of its own segment in a fixed order.
+// The element kernels run over the blocks of the iterating segments (set_block_lists); as
+// segments converge the launches shrink to the others.
 constexpr int kSegChunk = 4 * kBlock;
 
 struct SegView {
   const int* node_off;  // nseg + 1
   const int* blk_off;   // nseg + 1 block offsets of the apply's partition (x.Ax partials)
-  const int* wg_off;    // nseg + 1
+  const int* wg_off;    // nseg + 1 (workgroups of each segment in this launch)
   const int* wg_seg;    // per workgroup
+  const int* wg_chunk;  // per workgroup
   const int* active;    // per segment: still iterating
   double* rz;           // 2 x nseg
 };
@@ -446,14 +465,15 @@ struct SegView {
 __device__ __forceinline__ void seg_range(const SegView& S, int& g, long& lo, long& hi) {
   const int w = blockIdx.x;
   g = S.wg_seg[w];
-  lo = S.node_off[g] + (long)(w - S.wg_off[g]) * kSegChunk;
+  lo = S.node_off[g] + (long)S.wg_chunk[w] * kSegChunk;
   hi = min(lo + (long)kSegChunk, (long)S.node_off[g + 1]);
 }
 
-// r = b - q, q = 0, t = 0; partials of r.r and b.b (2 per workgroup) for the host
+// r = b - q, q = 0, t = 0 (the FP64 or the FP32 level-0 work vector); partials of r.r and b.b
+// (2 per workgroup) for the host
 __global__ __launch_bounds__(kBlock) void k_seg_init(SegView S, const double* __restrict__ b, double* __restrict__ q,
                                                      double* __restrict__ r, double* __restrict__ t,
-                                                     double* __restrict__ part_rr) {
+                                                     float* __restrict__ tf, double* __restrict__ part_rr) {
   int g;
   long lo, hi;
   seg_range(S, g, lo, hi);
@@ -461,7 +481,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_init(SegView S, const double* __
   for (long i = lo + threadIdx.x; i < hi; i += kBlock) {
     const double ri = b[i] - q[i];
     q[i] = 0.0;
-    t[i] = 0.0;
+    if (t) t[i] = 0.0;
+    if (tf) tf[i] = 0.0f;
     r[i] = ri;
     v[0] += ri * ri;
     v[1] += b[i] * b[i];
@@ -480,6 +501,24 @@ __global__ __launch_bounds__(kBlock) void k_seg_post_dot(SegView S, const double
   double v[1] = {0.0};
   for (long i = lo + threadIdx.x; i < hi; i += kBlock) {
     const double zi = x[i] + M[i] * (b[i] - t[i]);
+    z[i] = zi;
+    v[0] += b[i] * zi;
+  }
+  block_partial<1>(v, part);
+}
+
+// mixed-precision form of k_seg_post_dot: z = x + M .* (b - t) from the FP32 level-0 vectors
+__global__ __launch_bounds__(kBlock) void k_seg_post_dot_mixed(SegView S, const float* __restrict__ x,
+                                                               const float* __restrict__ M,
+                                                               const double* __restrict__ b,
+                                                               const float* __restrict__ t, double* __restrict__ z,
+                                                               double* __restrict__ part) {
+  int g;
+  long lo, hi;
+  seg_range(S, g, lo, hi);
+  double v[1] = {0.0};
+  for (long i = lo + threadIdx.x; i < hi; i += kBlock) {
+    const double zi = (double)x[i] + (double)M[i] * (b[i] - (double)t[i]);
     z[i] = zi;
     v[0] += b[i] * zi;
   }
@@ -509,7 +548,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_p(SegView S, int first, int slot
 __global__ __launch_bounds__(kBlock) void k_seg_xr(SegView S, int slot, const double* __restrict__ dot_part,
                                                    double* __restrict__ x, double* __restrict__ r,
                                                    const double* __restrict__ p, const double* __restrict__ q,
-                                                   double* __restrict__ t, double* __restrict__ part_rr) {
+                                                   double* __restrict__ t, float* __restrict__ tf,
+                                                   double* __restrict__ part_rr) {
   int g;
   long lo, hi;
   seg_range(S, g, lo, hi);
@@ -518,7 +558,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_xr(SegView S, int slot, const do
   const bool act = S.active[g] != 0;
   double v[1] = {0.0};
   for (long i = lo + threadIdx.x; i < hi; i += kBlock) {
-    t[i] = 0.0;
+    if (t) t[i] = 0.0;
+    if (tf) tf[i] = 0.0f;
     double ri = r[i];
     if (act) {
       x[i] += alpha * p[i];
@@ -534,8 +575,9 @@ static SegView seg_view(const Ctx& c) {
   SegView S;
   S.node_off = c.d_seg_node_off;
   S.blk_off = c.apply_clusters ? c.d_seg_blk_cl : c.d_seg_blk_all;
-  S.wg_off = c.d_seg_wg_off;
-  S.wg_seg = c.d_wg_seg;
+  S.wg_off = c.wg_pcg.off;
+  S.wg_seg = c.wg_pcg.seg;
+  S.wg_chunk = c.wg_pcg.chunk;
   S.active = c.d_seg_active;
   S.rz = c.d_seg_rz;
   return S;
@@ -543,18 +585,23 @@ static SegView seg_view(const Ctx& c) {
 
 void launch_seg_post_dot(const Ctx& c, const double* x, const double* M, const double* b, const double* t, double* z,
                          hipStream_t s) {
-  k_seg_post_dot<<<c.seg_nwg, kBlock, 0, s>>>(seg_view(c), x, M, b, t, z, c.seg_part_rz);
+  if (c.wg_pcg.n > 0)
+    k_seg_post_dot<<<c.wg_pcg.n, kBlock, 0, s>>>(seg_view(c), x, M, b, t, z, c.seg_part_rz);
+}
+
+static void launch_seg_post_dot_mixed(const Ctx& c, const float* x, const float* M, const double* b, const float* t,
+                                      double* z, hipStream_t s) {
+  if (c.wg_pcg.n > 0)
+    k_seg_post_dot_mixed<<<c.wg_pcg.n, kBlock, 0, s>>>(seg_view(c), x, M, b, t, z, c.seg_part_rz);
 }
 
 // device tables of the segmented decomposition (once per context)
-static int ensure_seg(Ctx& c) {
+int ensure_seg(Ctx& c) {
   if (c.seg_nwg > 0) return 0;
-  std::vector<int> wg_off(c.nseg + 1, 0), wg_seg;
+  std::vector<int> wg_off(c.nseg + 1, 0);
   for (int g = 0; g < c.nseg; ++g) {
     const int n = c.h_seg_node_off[g + 1] - c.h_seg_node_off[g];
-    const int k = (n + kSegChunk - 1) / kSegChunk;
-    wg_off[g + 1] = wg_off[g] + k;
-    wg_seg.insert(wg_seg.end(), k, g);
+    wg_off[g + 1] = wg_off[g] + (n + kSegChunk - 1) / kSegChunk;
   }
   c.seg_nwg = wg_off[c.nseg];
   c.h_seg_wg_off = wg_off;
@@ -567,9 +614,18 @@ static int ensure_seg(Ctx& c) {
   MM_TRY(up(&c.d_seg_node_off, c.h_seg_node_off));
   MM_TRY(up(&c.d_seg_blk_all, c.h_seg_blk_all));
   MM_TRY(up(&c.d_seg_blk_cl, c.h_seg_blk_cl));
-  MM_TRY(up(&c.d_seg_wg_off, wg_off));
-  MM_TRY(up(&c.d_wg_seg, wg_seg));
   MM_TRY(up(&c.d_seg_active, std::vector<int>(c.nseg, 1)));
+  for (WgTables* T : {&c.wg_pcg, &c.wg_min}) {
+    MM_TRY(up(&T->seg, std::vector<int>(c.seg_nwg, 0)));
+    MM_TRY(up(&T->chunk, std::vector<int>(c.seg_nwg, 0)));
+    MM_TRY(up(&T->off, std::vector<int>(c.nseg + 1, 0)));
+    T->key.clear();
+  }
+  // block lists hold at most every block of their partition
+  MM_TRY(up(&c.d_blist_cl, std::vector<int>(std::max(c.part_cl.nblk, 1), 0)));
+  MM_TRY(up(&c.d_blist_all, std::vector<int>(std::max(c.part_all.nblk, 1), 0)));
+  MM_TRY(up(&c.d_blist_mag, std::vector<int>(std::max(c.part_mag.nblk, 1), 0)));
+  c.blist_key.clear();
   MM_HIP(hipMalloc((void**)&c.d_seg_rz, 2 * (size_t)c.nseg * sizeof(double)));
   c.allocs.push_back(c.d_seg_rz);
   MM_HIP(hipMalloc((void**)&c.seg_part_rz, (size_t)c.seg_nwg * sizeof(double)));
@@ -580,49 +636,148 @@ static int ensure_seg(Ctx& c) {
   return 0;
 }
 
+static std::vector<int> full_mask(const Ctx& c, const int* mask) {
+  std::vector<int> k(c.nseg, 1);
+  if (mask)
+    for (int g = 0; g < c.nseg; ++g) k[g] = mask[g] != 0;
+  return k;
+}
+
+int build_wg_tables(Ctx& c, const int* mask, WgTables& T) {
+  MM_TRY(ensure_seg(c));
+  std::vector<int> key = full_mask(c, mask);
+  if (key == T.key) return 0;
+  std::vector<int> seg, chunk, off(c.nseg + 1, 0);
+  seg.reserve(c.seg_nwg);
+  chunk.reserve(c.seg_nwg);
+  for (int g = 0; g < c.nseg; ++g) {
+    const int k = key[g] ? c.h_seg_wg_off[g + 1] - c.h_seg_wg_off[g] : 0;
+    for (int j = 0; j < k; ++j) {
+      seg.push_back(g);
+      chunk.push_back(j);
+    }
+    off[g + 1] = off[g] + k;
+  }
+  T.n = off[c.nseg];
+  // stream-ordered after the kernels that read the previous tables
+  if (T.n > 0) {
+    MM_HIP(hipMemcpyAsync(T.seg, seg.data(), T.n * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    MM_HIP(hipMemcpyAsync(T.chunk, chunk.data(), T.n * sizeof(int), hipMemcpyHostToDevice, c.stream));
+  }
+  MM_HIP(hipMemcpyAsync(T.off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+  T.h_off = std::move(off);
+  T.key = std::move(key);
+  return 0;
+}
+
+int set_block_lists(Ctx& c, const int* mask) {
+  MM_TRY(ensure_seg(c));
+  std::vector<int> key = full_mask(c, mask);
+  if (key == c.blist_key) return 0;
+  bool all = true;
+  for (int v : key) all &= v != 0;
+  auto fill = [&](Part& p, const std::vector<int>& off, std::vector<int>& h, int* d) -> int {
+    if (all) {
+      p.blist = nullptr;
+      p.nlist = 0;
+      return 0;
+    }
+    h.clear();
+    for (int g = 0; g < c.nseg; ++g)
+      if (key[g])
+        for (int b = off[g]; b < off[g + 1]; ++b) h.push_back(b);
+    if (!h.empty()) MM_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    p.blist = d;
+    p.nlist = (int)h.size();
+    return 0;
+  };
+  MM_TRY(fill(c.part_cl, c.h_seg_blk_cl, c.h_blist_cl, c.d_blist_cl));
+  MM_TRY(fill(c.part_all, c.h_seg_blk_all, c.h_blist_all, c.d_blist_all));
+  MM_TRY(fill(c.part_mag, c.h_seg_blk_mag, c.h_blist_mag, c.d_blist_mag));
+  // AMG rows of the same segments: runs of consecutive segments (at most kMaxRuns launches per
+  // SpMV, else the span from the first to the last); every row where a level's unknowns are not
+  // ordered by segment
+  constexpr int kMaxRuns = 8;
+  for (Level& L : c.lv) {
+    L.runs.clear();
+    if (all || (int)L.seg_off.size() != c.nseg + 1) continue;
+    for (int g = 0; g < c.nseg;) {
+      if (!key[g]) {
+        ++g;
+        continue;
+      }
+      int h = g;
+      while (h + 1 < c.nseg && key[h + 1]) ++h;
+      L.runs.emplace_back(L.seg_off[g], L.seg_off[h + 1]);
+      g = h + 1;
+    }
+    if ((int)L.runs.size() > kMaxRuns) L.runs = {{L.runs.front().first, L.runs.back().second}};
+    if (L.runs.empty()) L.runs.emplace_back(0, 0);  // no segment: launch nothing
+  }
+  c.blist_key = std::move(key);
+  return 0;
+}
+
 // PCG for every segment at once (make_pcg_solve per system, src/poisson_solve.py:422-478):
-// stop segment g when r_g.r_g <= tol_g^2 b_g.b_g or after maxiter iterations.
-static int pcg_seg(Ctx& c, const double* b, double* U, const double* tol, int maxiter, int* it_out, double* r2_out,
-                   double* bb_out) {
+// stop segment g when r_g.r_g <= tol_g^2 b_g.b_g or after maxiter iterations. Only segments
+// with want[g] != 0 (all when want is null) take part; the others keep U and report r2 = b.b = 0.
+static int pcg_seg(Ctx& c, const double* b, double* U, const double* tol, const int* want, int maxiter, int* it_out,
+                   double* r2_out, double* bb_out) {
   hipStream_t s = c.stream;
   const long N = c.N;
-  const int G = c.nseg, W = c.seg_nwg;
-  const std::vector<int>& wo = c.h_seg_wg_off;
-  double* sc = c.dscal;
-  (void)sc;
-  c.h_seg_active.assign(G, 1);
+  const int G = c.nseg;
+  c.h_seg_active = full_mask(c, want);
   MM_HIP(hipMemcpyAsync(c.d_seg_active, c.h_seg_active.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
+  MM_TRY(build_wg_tables(c, c.h_seg_active.data(), c.wg_pcg));
+  MM_TRY(set_block_lists(c, c.h_seg_active.data()));
+  std::vector<double> r2(G, 0.0), bb(G, 0.0), tol2(G, 0.0);
+  for (int g = 0; g < G; ++g) it_out[g] = 0;
+  int W = c.wg_pcg.n;
+  if (W == 0) {
+    for (int g = 0; g < G; ++g) r2_out[g] = bb_out[g] = 0.0;
+    return set_block_lists(c, nullptr);
+  }
   const SegView S = seg_view(c);
   vfill(c.pq, 0.0, N, s);
-  MM_TRY(launch_apply(c, c.geom_mode, U, c.pq, s));
+  MM_TRY(timed_apply(c, U, c.pq, s));
   c.n_apply++;
-  k_seg_init<<<W, kBlock, 0, s>>>(S, b, c.pq, c.pr, c.pt, c.hseg_dev);
+  float* tf = c.precision == 1 ? c.ptf : nullptr;
+  double* t64 = c.precision == 1 ? nullptr : c.pt;
+  k_seg_init<<<W, kBlock, 0, s>>>(S, b, c.pq, c.pr, t64, tf, c.hseg_dev);
   c.seg_running = true;
   int rc = precond(c, c.pr, c.pz, c.part_rz, s);
   c.seg_running = false;
   MM_TRY(rc);
   k_seg_p<<<W, kBlock, 0, s>>>(S, 1, 0, 0, c.seg_part_rz, c.pz, c.pp, c.pq);
   MM_HIP(hipStreamSynchronize(s));
-  std::vector<double> r2(G, 0.0), bb(G, 0.0), tol2(G);
-  for (int g = 0; g < G; ++g) {
-    for (int w = wo[g]; w < wo[g + 1]; ++w) {
-      r2[g] += c.hseg[2 * w];
-      bb[g] += c.hseg[2 * w + 1];
+  {
+    const std::vector<int>& wo = c.wg_pcg.h_off;
+    for (int g = 0; g < G; ++g) {
+      for (int w = wo[g]; w < wo[g + 1]; ++w) {
+        r2[g] += c.hseg[2 * w];
+        bb[g] += c.hseg[2 * w + 1];
+      }
+      tol2[g] = tol[g] * tol[g] * bb[g];
     }
-    tol2[g] = tol[g] * tol[g] * bb[g];
-    it_out[g] = 0;
   }
   int live = 0;
   for (int g = 0; g < G; ++g) {
-    c.h_seg_active[g] = r2[g] > tol2[g] && maxiter > 0;
+    c.h_seg_active[g] = c.h_seg_active[g] && r2[g] > tol2[g] && maxiter > 0;
     live += c.h_seg_active[g];
   }
-  MM_HIP(hipMemcpyAsync(c.d_seg_active, c.h_seg_active.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
+  auto shrink = [&]() -> int {
+    MM_HIP(hipMemcpyAsync(c.d_seg_active, c.h_seg_active.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
+    MM_TRY(build_wg_tables(c, c.h_seg_active.data(), c.wg_pcg));
+    MM_TRY(set_block_lists(c, c.h_seg_active.data()));
+    W = c.wg_pcg.n;
+    return 0;
+  };
+  MM_TRY(shrink());
   int it = 0, cur = 0;
   while (live > 0 && it < maxiter) {
-    MM_TRY(launch_apply(c, c.geom_mode, c.pp, c.pq, s, c.part_pq));  // q was cleared by k_seg_p
+    MM_TRY(timed_apply(c, c.pp, c.pq, s, c.part_pq));  // q was cleared by k_seg_p
     c.n_apply++;
-    k_seg_xr<<<W, kBlock, 0, s>>>(S, cur, c.part_pq, U, c.pr, c.pp, c.pq, c.pt, c.hseg_dev);
+    k_seg_xr<<<W, kBlock, 0, s>>>(S, cur, c.part_pq, U, c.pr, c.pp, c.pq, t64, tf, c.hseg_dev);
     MM_HIP(hipEventRecord(c.ev, s));
     c.seg_running = true;
     rc = precond(c, c.pr, c.pz, c.part_rz, s);
@@ -633,22 +788,23 @@ static int pcg_seg(Ctx& c, const double* b, double* U, const double* tol, int ma
     ++it;
     MM_HIP(hipEventSynchronize(c.ev));
     bool changed = false;
+    const std::vector<int>& wo = c.wg_pcg.h_off;
     for (int g = 0; g < G; ++g) {
+      if (!c.h_seg_active[g]) continue;
       r2[g] = 0.0;
       for (int w = wo[g]; w < wo[g + 1]; ++w) r2[g] += c.hseg[w];
-      if (c.h_seg_active[g]) {
-        it_out[g] = it;
-        if (r2[g] <= tol2[g] || it >= maxiter) {
-          c.h_seg_active[g] = 0;
-          --live;
-          changed = true;
-        }
+      it_out[g] = it;
+      if (r2[g] <= tol2[g] || it >= maxiter) {
+        c.h_seg_active[g] = 0;
+        --live;
+        changed = true;
       }
     }
-    if (changed)
-      MM_HIP(hipMemcpyAsync(c.d_seg_active, c.h_seg_active.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
+    if (changed && live > 0) MM_TRY(shrink());
   }
   MM_HIP(hipStreamSynchronize(s));
+  MM_TRY(resolve_apply_timing(c));
+  MM_TRY(set_block_lists(c, nullptr));
   c.n_pcg_it += it;
   for (int g = 0; g < G; ++g) {
     r2_out[g] = r2[g];
@@ -658,15 +814,21 @@ static int pcg_seg(Ctx& c, const double* b, double* U, const double* tol, int ma
 }
 
 int solve_U_seg(Ctx& c, const double* m, double* U, const double* tol, int maxiter, int* it, double* rel_res) {
-  MM_CHECK(c.lv.size() > 1 && c.lv[0].M != nullptr && c.precision == 0 && c.geom_mode == kGeomCoords,
-           "segmented solves need the FP64 AMG V-cycle and the on-the-fly apply");
+  return solve_U_seg_dev(c, m, U, tol, nullptr, maxiter, it, rel_res);
+}
+
+int solve_U_seg_dev(Ctx& c, const double* m, double* U, const double* tol, const int* want, int maxiter, int* it,
+                    double* rel_res) {
+  MM_CHECK(c.lv.size() > 1 && c.lv[0].M != nullptr && c.geom_mode == kGeomCoords,
+           "segmented solves need the AMG V-cycle and the on-the-fly apply");
   MM_TRY(ensure_seg(c));
   hipStream_t s = c.stream;
   vfill(c.pb, 0.0, c.N, s);
+  MM_TRY(set_block_lists(c, want));
   MM_TRY(launch_rhs(c, m, c.pb, s));
   c.n_rhs++;
   std::vector<double> r2(c.nseg), bb(c.nseg);
-  MM_TRY(pcg_seg(c, c.pb, U, tol, maxiter, it, r2.data(), bb.data()));
+  MM_TRY(pcg_seg(c, c.pb, U, tol, want, maxiter, it, r2.data(), bb.data()));
   for (int g = 0; g < c.nseg; ++g) rel_res[g] = std::sqrt(r2[g] / (bb[g] + 1e-30));
   return 0;
 }
@@ -683,11 +845,12 @@ static void vcycle_coarse(Ctx& c, int l, hipStream_t s) {
     return;
   }
   Level& Nx = c.lv[l + 1];
-  spmv(3, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.b, (const double*)L.M, L.t, s, L.x);   // t = A (M b), x = M b
-  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rx, L.b, L.t, Nx.b, s);                    // b' = R (b - t)
+  spmv(3, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.b, (const double*)L.M, L.t, s, L.x, &L.runs);  // t = A (M b), x = M b
+  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rx, L.b, L.t, Nx.b, s, (double*)nullptr, &Nx.runs);  // b' = R (b - t)
   vcycle_coarse(c, l + 1, s);
-  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Px, Nx.xs, (const double*)nullptr, L.x, s); // x += P e
-  spmv_post(L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, L.M, L.b, L.xs, s);              // xs = x + M (b - A x)
+  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Px, Nx.xs, (const double*)nullptr, L.x, s, (double*)nullptr,
+       &L.runs);                                                                      // x += P e
+  spmv_post(L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, L.M, L.b, L.xs, s, &L.runs);      // xs = x + M (b - A x)
 }
 
 // FP32 coarse part of the V-cycle (levels >= 1), same structure as vcycle_coarse.
@@ -699,11 +862,11 @@ static void vcycle_coarse_f(Ctx& c, int l, hipStream_t s) {
     return;
   }
   Level& Nx = c.lv[l + 1];
-  spmv(3, L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.bf, (const float*)L.Mf, L.tf, s, L.xf);
-  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rxf, L.bf, L.tf, Nx.bf, s);
+  spmv(3, L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.bf, (const float*)L.Mf, L.tf, s, L.xf, &L.runs);
+  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rxf, L.bf, L.tf, Nx.bf, s, (float*)nullptr, &Nx.runs);
   vcycle_coarse_f(c, l + 1, s);
-  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Pxf, Nx.xsf, (const float*)nullptr, L.xf, s);
-  spmv_post(L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.xf, L.Mf, L.bf, L.xsf, s);
+  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Pxf, Nx.xsf, (const float*)nullptr, L.xf, s, (float*)nullptr, &L.runs);
+  spmv_post(L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.xf, L.Mf, L.bf, L.xsf, s, &L.runs);
 }
 
 // Mixed-precision preconditioner: the whole V-cycle in FP32 (FP32 matrix-free fine applies,
@@ -714,11 +877,14 @@ static int precond_f32(Ctx& c, const double* r, double* z, double* rz_part, hipS
   Level& L0 = c.lv[0];
   Level& L1 = c.lv[1];
   MM_TRY(launch_apply_f32_scaled(c, r, L0.Mf, c.px0f, c.ptf, s));                 // x = M0 r, t = A x
-  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rxf, r, c.ptf, L1.bf, s);             // b1 = R0 (r - t)
+  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rxf, r, c.ptf, L1.bf, s, (float*)nullptr, &L1.runs);  // b1 = R0 (r - t)
   vcycle_coarse_f(c, 1, s);
-  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Pxf, L1.xsf, (const float*)nullptr, c.px0f, s, c.ptf);
+  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Pxf, L1.xsf, (const float*)nullptr, c.px0f, s, c.ptf, &L0.runs);
   MM_TRY(launch_apply_f32(c, c.px0f, c.ptf, s));                                  // t = A x
-  k_post_dot_mixed<<<red_grid(N), kBlock, 0, s>>>(c.px0f, L0.Mf, r, c.ptf, z, N, rz_part);
+  if (c.seg_running)
+    launch_seg_post_dot_mixed(c, c.px0f, L0.Mf, r, c.ptf, z, s);
+  else
+    k_post_dot_mixed<<<red_grid(N), kBlock, 0, s>>>(c.px0f, L0.Mf, r, c.ptf, z, N, rz_part);
   c.n_vcycle += 1;
   return 0;
 }
@@ -739,9 +905,10 @@ int precond(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s) 
   Level& L1 = c.lv[1];
   // the caller has cleared pt
   MM_TRY(timed_apply_scaled(c, r, L0.M, c.px0, c.pt, s));                          // x = M0 r, t = A x
-  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rx, r, c.pt, L1.b, s);                // b1 = R0 (r - t)
+  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rx, r, c.pt, L1.b, s, (double*)nullptr, &L1.runs);  // b1 = R0 (r - t)
   vcycle_coarse(c, 1, s);
-  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Px, L1.xs, (const double*)nullptr, c.px0, s, c.pt); // x += P0 e1, t = 0
+  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Px, L1.xs, (const double*)nullptr, c.px0, s, c.pt,
+       &L0.runs);  // x += P0 e1, t = 0
   MM_TRY(timed_apply(c, c.px0, c.pt, s));                                         // t = A x
   if (c.seg_running)
     launch_seg_post_dot(c, c.px0, L0.M, r, c.pt, z, s);

@@ -80,6 +80,26 @@ def test_segmented_solve_per_grain_tolerance(grainset, shape):
     assert its[0] < its[2] and its[1] <= its[2]
 
 
+def test_segmented_solve_mixed_precision(grainset):
+    """FP32 V-cycle inside the FP64 segmented PCG: every grain still meets its FP64 tolerance."""
+    from mumag_matrixfree2_amd.grainset import split
+
+    ctx, probs = grainset
+    m = union_m(ctx, 4)
+    ctx.set_precision("mixed")
+    try:
+        U, its, rrs = ctx.solve_U_segmented(m, np.zeros(ctx.N), 1e-10)
+    finally:
+        ctx.set_precision("fp64")
+    U64, its64, _ = ctx.solve_U_segmented(m, np.zeros(ctx.N), 1e-10)
+    print(f"[segmented mixed] its={its.tolist()} fp64 its={its64.tolist()}", flush=True)
+    for p, Ug, U64g, mg in zip(probs, split(ctx, U), split(ctx, U64), split(ctx, m)):
+        b = p.rhs(mg)
+        assert np.linalg.norm(b - p.A @ Ug) / np.linalg.norm(b) <= 1e-10 * 1.001
+        assert rel(Ug, U64g) <= 1e-7
+    assert np.all(np.abs(its - its64) <= 2)
+
+
 def test_converged_segment_takes_no_iteration(grainset):
     ctx, probs = grainset
     m = union_m(ctx, 5)
@@ -113,3 +133,43 @@ def test_segmented_rejects_bad_input(grainset):
             plain.solve_U_segmented(np.zeros((plain.N, 3)), np.zeros(plain.N), 1e-8)
     finally:
         plain.close()
+
+
+def test_minimize_segmented_equals_per_grain(grainset):
+    """One field step of every grain in shared launches against each grain minimised in a
+    context of its own. The two differ only through the AMG hierarchies (one on the union
+    matrix, one per grain) and the order of the FP64 atomic adds, so the solves agree to their
+    tolerance and the minimisers to well below the energy tolerance."""
+    from mumag_matrixfree2_amd.context import MeshContext, make_params
+    from mumag_matrixfree2_amd.grainset import split
+    from mumag_matrixfree2_amd.mesh import grain_set
+
+    ctx, _ = grainset
+    gs = grain_set(5, seed=7, t_min=3e3, t_max=3e4)
+    tables = [ndfeb_materials(axis=g["easy_axis"]) for g in gs]
+    p = make_params("pcohen_hs", tau_f=1e-10, cg_tol=1e-10)
+    B = np.array([0.017452, 0.0, 0.999848]) * -0.6
+    m0 = [tilted_m0(g["knt"].shape[0], seed=60 + i, sigma=0.05) for i, g in enumerate(gs)]
+    singles = []
+    for g, t, m in zip(gs, tables, m0):
+        c = MeshContext(g["knt"], g["ijk"][:, :4], g["ijk"][:, 4], *t, boundary_mask=g["boundary_mask"])
+        c.setup_precond("amgcl", max_coarse=300)
+        c.state_set(m, np.zeros(c.N))
+        info = c.minimize_resident(p, B)
+        singles.append((info, c.state_get()[0], c.state_j_parallel(B), c.V_mag))
+        c.close()
+    ctx.state_set(np.vstack(m0), np.zeros(ctx.N))
+    infos = ctx.minimize_segmented(p, B)
+    m_seg = split(ctx, ctx.state_get()[0])
+    J_seg = ctx.state_j_parallel_segmented(B)
+    V_seg = ctx.segment_volumes()
+    for i, ((a, m_a, J_a, V_a), b) in enumerate(zip(singles, infos)):
+        print(f"[grain {i}] single E={a['E']:.15e} it={a['iters']} nf={a['evals']} | segmented E={b['E']:.15e} "
+              f"it={b['iters']} nf={b['evals']} dm={np.max(np.abs(m_seg[i] - m_a)):.1e}", flush=True)
+        assert a["converged"] and b["converged"]
+        assert abs(a["E"] - b["E"]) <= 1e-12 * abs(a["E"])
+        assert abs(a["iters"] - b["iters"]) <= 1
+        assert np.max(np.abs(m_seg[i] - m_a)) < 1e-9
+        assert abs(J_seg[i] - J_a) <= 1e-8 and abs(V_seg[i] - V_a) <= 1e-12 * V_a
+    with pytest.raises(RuntimeError, match="segmented"):
+        ctx.minimize_resident(p, B)
