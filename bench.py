@@ -230,7 +230,7 @@ def run_grains(args, log):
                                   [f for f in ("segmented", "threads") if f != args.grains_form])
 
     ctxs, union = [], None
-    if "threads" in forms or not args.no_segmented:
+    if "threads" in forms or not args.no_segmented or args.fp64_sequential_baseline:
         for i, (g, t) in enumerate(zip(grains, tables)):
             c = MeshContext(g["knt"], g["ijk"][:, :4], g["ijk"][:, 4], *t, boundary_mask=g["boundary_mask"])
             c.setup_precond(args.precond, max_coarse=args.max_coarse)
@@ -249,8 +249,9 @@ def run_grains(args, log):
     E_tot = sum(g["ijk"].shape[0] for g in grains)
     N_tot = sum(g["knt"].shape[0] for g in grains)
 
-    def sweep(form, timed):
+    def sweep(form, timed, n_threads=None):
         """(seconds, J per step, per-grain energies per step, minimiser iterations, apply timing)"""
+        n_threads = args.threads if n_threads is None else n_threads
         if form == "segmented":
             union.state_set(np.vstack(m0), np.zeros(union.N))
             vol = union.segment_volumes()
@@ -271,12 +272,12 @@ def run_grains(args, log):
                 infos = union.minimize_segmented(p, B)
                 Jg = union.state_j_parallel_segmented(H_DIR)
             else:
-                infos = minimize_batched(ctxs, p, B, n_threads=args.threads)
+                infos = minimize_batched(ctxs, p, B, n_threads=n_threads)
                 Jg = np.array([c.state_j_parallel(H_DIR) for c in ctxs])
             J.append(float(vol @ Jg / vol.sum()))
             Es.append([i["E"] for i in infos])
             its += sum(i["iters"] for i in infos)
-            log(f"[bench:grains {form}] step {k:2d} B={B_vals[k] * JS:+.3f} T J={J[-1] * JS:+.5f} T "
+            log(f"[bench:grains {form} t={n_threads}] step {k:2d} B={B_vals[k] * JS:+.3f} T J={J[-1] * JS:+.5f} T "
                 f"iters={sum(i['iters'] for i in infos)} max={max(i['iters'] for i in infos)} "
                 f"t={time.perf_counter() - ts:.3f}s")
         el = time.perf_counter() - t_start
@@ -305,6 +306,19 @@ def run_grains(args, log):
                  "minimizer_iters": its1, "max_abs_J_diff": float(np.max(np.abs(J - J1))),
                  "max_rel_grain_energy_diff": float(np.max(np.abs(Es - Es1) / np.maximum(np.abs(Es1), 1.0))),
                  "switching_step": switching_step(J1)}
+    seq = None
+    if args.fp64_sequential_baseline and ctxs:
+        # the plain FP64 form: one grain at a time, each in its own context
+        precs = [c.precision for c in ctxs]
+        for c in ctxs:
+            c.set_precision("fp64")
+        t2, J2, Es2, its2, _ = sweep("threads", timed=False, n_threads=1)
+        for c, pr in zip(ctxs, precs):
+            c.set_precision(pr)
+        seq = {"field_steps_per_s": steps / t2, "speedup_of_headline_form": t2 / elapsed, "minimizer_iters": its2,
+               "max_abs_J_diff": float(np.max(np.abs(J - J2))),
+               "max_rel_grain_energy_diff": float(np.max(np.abs(Es - Es2) / np.maximum(np.abs(Es2), 1.0))),
+               "switching_step": switching_step(J2)}
     if args.grains_form == "segmented":
         # the sweep's applies cover shrinking segment subsets; the roofline is taken on
         # back-to-back applies over every grain (HIP events on the context's stream)
@@ -348,6 +362,8 @@ def run_grains(args, log):
     }
     if other is not None:
         result["other_form"] = other
+    if seq is not None:
+        result["fp64_sequential"] = seq
     if seg is not None:
         result["segmented_poisson"] = seg
     print(json.dumps(result), flush=True)
@@ -376,6 +392,8 @@ def main():
                     help="grains64: one segmented context for all grains, or one context per grain on host threads")
     ap.add_argument("--no-compare-forms", action="store_true",
                     help="grains64: do not run the sweep in the other form")
+    ap.add_argument("--fp64-sequential-baseline", action="store_true",
+                    help="grains64: also run the sweep in FP64 one grain at a time (one context per grain)")
     ap.add_argument("--no-segmented", action="store_true",
                     help="grains64: skip the segmented vs threaded Poisson solve comparison")
     args = ap.parse_args()
