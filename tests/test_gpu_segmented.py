@@ -144,7 +144,7 @@ def test_minimize_segmented_equals_per_grain(grainset):
     from mumag_matrixfree2_amd.grainset import split
     from mumag_matrixfree2_amd.mesh import grain_set
 
-    ctx, _ = grainset
+    ctx, probs = grainset
     gs = grain_set(5, seed=7, t_min=3e3, t_max=3e4)
     tables = [ndfeb_materials(axis=g["easy_axis"]) for g in gs]
     p = make_params("pcohen_hs", tau_f=1e-10, cg_tol=1e-10)
@@ -173,3 +173,7 @@ def test_minimize_segmented_equals_per_grain(grainset):
         assert abs(J_seg[i] - J_a) <= 1e-8 and abs(V_seg[i] - V_a) <= 1e-12 * V_a
     with pytest.raises(RuntimeError, match="segmented"):
         ctx.minimize_resident(p, B)
+    # the subset launches are undone: whole-context operators cover every grain again
+    x = np.random.default_rng(8).standard_normal(ctx.N)
+    for pr, yg, xg in zip(probs, split(ctx, ctx.apply_A(x)), split(ctx, x)):
+        assert rel(yg, pr.A @ xg) <= 1e-12
