@@ -387,6 +387,11 @@ def main():
     ap.add_argument("--max-coarse", type=int, default=None)
     ap.add_argument("--apply-shape", default="auto",
                     help="'auto' (autotuned at set-up) or unit:param, e.g. cluster:256, element:128, persistent:2")
+    ap.add_argument("--solve-guess", type=int, default=8,
+                    help="initial guess of the minimiser's Poisson solves: projection onto the last K solutions "
+                         "(0: the previous solution, as the reference)")
+    ap.add_argument("--no-guess-compare", action="store_true",
+                    help="do not rerun the sweep with the reference warm start for comparison")
     ap.add_argument("--threads", type=int, default=8, help="host threads driving the grains (grains64)")
     ap.add_argument("--grains-form", default="segmented", choices=["segmented", "threads"],
                     help="grains64: one segmented context for all grains, or one context per grain on host threads")
@@ -435,8 +440,9 @@ def main():
     m0 = initial_m(ctx.N)
     steps = min(args.steps or wl["steps"], len(B_vals))
 
-    def sweep(precision, timed):
+    def sweep(precision, timed, guess=None):
         ctx.set_precision(precision)
+        ctx.set_solve_guess(args.solve_guess if guess is None else guess)
         ctx.state_set(m0, np.zeros(ctx.N))
         if timed:
             ctx.apply_timing(True)
@@ -456,6 +462,7 @@ def main():
 
     # warm-up on a separate state
     ctx.set_precision(args.precision)
+    ctx.set_solve_guess(args.solve_guess)
     ctx.state_set(m0, np.zeros(ctx.N))
     for k in range(args.warmup):
         ctx.minimize_resident(p, B_vals[k % len(B_vals)] * H_DIR)
@@ -493,6 +500,16 @@ def main():
             "speedup_vs_fp64": t64 / elapsed,
         }
 
+    guess_cmp = None
+    if args.solve_guess > 0 and not args.no_guess_compare:
+        # the reference's warm start (each solve from the previous solution) on the same sweep
+        t0w, J0, E0w, _, it0 = sweep(args.precision, timed=False, guess=0)
+        ctx.set_solve_guess(args.solve_guess)
+        guess_cmp = {"field_steps_per_s": steps / t0w, "speedup_of_headline_form": t0w / elapsed,
+                     "minimizer_iters": int(sum(it0)), "headline_minimizer_iters": int(sum(iters)),
+                     "max_rel_energy_diff": float(np.max(np.abs(Es - E0w) / np.maximum(np.abs(E0w), 1.0))),
+                     "switching_step": switching_step(J0)}
+
     avg_apply_s = apply_ms / max(apply_n, 1) * 1e-3
     algo = apply_algorithmic_bytes(ctx)
     achieved = algo / avg_apply_s / 1e9
@@ -515,6 +532,8 @@ def main():
             "tets": ctx.E, "nodes": ctx.N, "method": "pcohen_hs", "precond": f"SA-AMG V-cycle ({args.precond})",
             "amg_levels": [int(l["A"].shape[0]) for l in ctx.amg_levels],
             "apply_shape": {"unit": shape["unit"], "param": shape["param"], "block_nodes": ctx.block_nodes},
+            "solve_guess": (f"projection onto the last {args.solve_guess} solutions" if args.solve_guess
+                            else "previous solution (reference warm start)"),
             "pcg_tol": 1e-10, "tau_f": 1e-10,
         },
         "pcg_iters_per_s": pcg_its / elapsed,
@@ -536,6 +555,8 @@ def main():
 
     if errors is not None:
         result["mixed_vs_fp64"] = errors
+    if guess_cmp is not None:
+        result["reference_warm_start"] = guess_cmp
     if not args.no_cpu_baseline and args.workload == "cube2M":
         it_cpu, dt, icg_cpu = cpu_baseline(mesh, mats, args.cpu_iters)
         it_per_step = sum(iters) / len(iters)

@@ -164,6 +164,16 @@ int mm_minimize(int64_t ctx, const mm_params* p, double* m_inout, double* U_inou
 /* J_parallel of the resident state. */
 int mm_state_j_parallel(int64_t ctx, const double* h, double* J);
 
+/* Initial guess of the Poisson solves inside mm_minimize / mm_minimize_resident (the line-
+ * search trials and the first solve of each field step): k = 0 (default) starts from the
+ * previous solution, as the reference warm-starts each trial from the previous trial's U
+ * (src/minimizers.py:305,348); k in 1..8 also forms the combination of the last k solutions
+ * whose right-hand sides best match the new one in the least-squares sense (Fischer 1998,
+ * successive right-hand sides) and starts from whichever of the two has the smaller true
+ * residual; the PCG then runs to the same tolerance. Resets the stored solutions. No
+ * reference counterpart. */
+int mm_set_solve_guess(int64_t ctx, int k);
+
 /* Field step of every segment of a segmented context at once (pcohen_hs only): the resident
  * state of segment g is minimised at field B as mm_minimize_resident would minimise it in a
  * context of its own, with its own line search, Steihaug loop and convergence test, while all

@@ -95,6 +95,7 @@ SIGNATURES = {
     "mm_minimize_segmented": [C.c_int64, C.POINTER(MMParams), f64p, C.POINTER(MMInfo)],
     "mm_state_j_parallel_segmented": [C.c_int64, f64p, f64p],
     "mm_seg_volumes": [C.c_int64, f64p],
+    "mm_set_solve_guess": [C.c_int64, C.c_int],
     "mm_minimize_batched": [C.c_int, C.POINTER(C.c_int64), C.POINTER(MMParams), f64p, C.c_int, C.POINTER(MMInfo)],
     "mm_solve_U_batched": [C.c_int, C.POINTER(C.c_int64), C.POINTER(f64p), C.POINTER(f64p), f64p, C.c_int, C.c_int,
                            i32p, f64p],

@@ -85,6 +85,7 @@ class MeshContext:
         self.n_clusters = int(info[10])
         self.block_nodes = int(info[11])
         self.apply_shape = {"unit": "cluster", "param": 256}
+        self.solve_guess = 0
         self.precond_type = None
         self.amg_levels = None
         self.precision = "fp64"
@@ -266,6 +267,13 @@ class MeshContext:
         J = np.zeros(1)
         call("mm_state_j_parallel", self.handle, ptr(h), ptr(J))
         return float(J[0])
+
+    def set_solve_guess(self, k: int):
+        """Initial guess of the minimiser's Poisson solves: 0 the previous solution (the
+        reference's warm start); k in 1..8 the better (by true residual) of that and the
+        combination of the last k solutions whose right-hand sides best match the new one."""
+        call("mm_set_solve_guess", self.handle, int(k))
+        self.solve_guess = int(k)
 
     def minimize_segmented(self, params: "_hip.MMParams", B) -> list[dict]:
         """Field step of every segment at once (pcohen_hs): one info dict per segment, as

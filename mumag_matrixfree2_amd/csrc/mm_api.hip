@@ -693,6 +693,7 @@ static void destroy(Ctx* c) {
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hpart) (void)hipHostFree(c->hpart);
   if (c->hseg) (void)hipHostFree(c->hseg);
+  if (c->hguess) (void)hipHostFree(c->hguess);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev) (void)hipEventDestroy(c->ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1228,6 +1229,11 @@ int mm_minimize_resident(int64_t h, const mm_params* p, const double* B, mm_info
   info->gnorm = mi.gnorm;
   info->time = mi.time;
   return 0;
+}
+
+int mm_set_solve_guess(int64_t h, int k) {
+  GET_CTX(h);
+  return set_solve_guess(c, k);
 }
 
 int mm_minimize_segmented(int64_t h, const mm_params* p, const double* B, mm_info* info) {

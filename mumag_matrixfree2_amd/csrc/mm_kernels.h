@@ -241,6 +241,15 @@ struct Ctx {
   double* sm_in = nullptr;            // nseg x kSegIn
   double *sm_pa = nullptr, *sm_pb = nullptr;  // seg_nwg x 4 partials each
 
+  // initial guess of the minimiser's Poisson solves: the combination of the last guess_k
+  // solutions whose right-hand sides best match the new one (0: the previous solution as
+  // given). Ring of solutions U_i and right-hand sides b_i; G[i][j] = b_i . b_j on the host.
+  int guess_k = 0, guess_n = 0, guess_head = 0, guess_new = -1;
+  double *guess_U = nullptr, *guess_b = nullptr;  // kGuessMax x N each
+  double* guess_x = nullptr;                      // N: the candidate guess
+  std::vector<double> guess_G;                    // kGuessMax x kGuessMax
+  double *hguess = nullptr, *hguess_dev = nullptr;  // host-coherent partials
+
   long long n_apply = 0, n_rhs = 0, n_energy = 0, n_local = 0, n_vcycle = 0, n_pcg_it = 0;
 
   // live timing of the Poisson apply kernel: event pairs recorded on `stream` around each
@@ -285,6 +294,11 @@ int vmaxabs(const Ctx& c, const double* a, long n, double* out, hipStream_t s);
 int precond_apply(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s);
 int precond(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s);
 int solve_U_dev(Ctx& c, const double* m, double* U, double tol, int maxiter, int* it, double* rel_res);
+// solve_U_dev whose initial guess is the projection onto the last guess_k solutions (when
+// guess_k > 0; U holds the fallback guess), and which adds its solution to them
+constexpr int kGuessMax = 8;
+int solve_U_guess(Ctx& c, const double* m, double* U, double tol, int maxiter, int* it, double* rel_res);
+int set_solve_guess(Ctx& c, int k);
 // solve_U for every segment of the context at once (one launch per kernel for all segments,
 // per-segment PCG scalars and stopping test): tol, it, rel_res have one entry per segment
 int solve_U_seg(Ctx& c, const double* m, double* U, const double* tol, int maxiter, int* it, double* rel_res);

@@ -560,7 +560,7 @@ static int line_search(Ctx& c, const MinParams& p, const double B[3], double pg,
     k_cayley<<<vg(c.N), kBlock, 0, c.stream>>>(c.m, c.H, s, c.m_t, c.N);
     int it = 0;
     double rr = 0;
-    MM_TRY(solve_U_dev(c, c.m_t, c.U_t, p.phi_tol, p.cg_maxiter, &it, &rr));
+    MM_TRY(solve_U_guess(c, c.m_t, c.U_t, p.phi_tol, p.cg_maxiter, &it, &rr));
     out.demag += it;
     MM_TRY(eval_energy(c, c.m_t, c.U_t, B, c.g_t));
     MM_TRY(read_slots(c, S_E, 1));
@@ -629,7 +629,7 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
   {
     int it = 0;
     double rr = 0;
-    MM_TRY(solve_U_dev(c, c.m, c.U, p.cg_tol, p.cg_maxiter, &it, &rr));
+    MM_TRY(solve_U_guess(c, c.m, c.U, p.cg_tol, p.cg_maxiter, &it, &rr));
     demag = it;
   }
   MM_TRY(eval_energy(c, c.m, c.U, B, c.g_raw));

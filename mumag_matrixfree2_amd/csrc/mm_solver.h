@@ -6,8 +6,9 @@
 namespace mm {
 
 // Device-resident PCG on A U = b (b, U device; U holds the initial guess).
+// q_ready: the caller has left A U in c.pq (the initial apply is skipped).
 int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it_out, double* r2_out,
-            double* bb_out);
+            double* bb_out, bool q_ready = false);
 
 // Minimiser parameters / statistics mirrored from the C-ABI structs.
 struct MinParams {

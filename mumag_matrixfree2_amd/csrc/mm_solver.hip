@@ -8,6 +8,7 @@
 // beta. p.Ap comes out of the Poisson apply itself (per-block element energies). The host
 // never waits inside an iteration except to sum the r.r partials for the stopping test, and
 // that copy overlaps the V-cycle of the same iteration.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -931,7 +932,7 @@ int precond_apply(Ctx& c, const double* r, double* z, double* rz_part, hipStream
 // Returns iterations, r.r and b.b; r.r is summed on the host from the per-workgroup
 // partials (written by the kernels into host-coherent memory) in a fixed order.
 int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it_out, double* r2_out,
-            double* bb_out) {
+            double* bb_out, bool q_ready) {
   hipStream_t s = c.stream;
   const long N = c.N;
   MM_CHECK(c.lv.size() >= 1 && c.lv[0].M != nullptr, "preconditioner not set up (call mm_amg_upload or mm_precond_jacobi)");
@@ -940,9 +941,11 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
   // p.Ap from the apply's per-block element energies (on-the-fly geometry), else a dot pass
   const bool fused = c.geom_mode == kGeomCoords;
   const int n_pq = fused ? apply_dot_count(c) : gv;
-  vfill(c.pq, 0.0, N, s);
-  MM_TRY(timed_apply(c, U, c.pq, s));
-  c.n_apply++;
+  if (!q_ready) {  // else the caller has left A U in pq
+    vfill(c.pq, 0.0, N, s);
+    MM_TRY(timed_apply(c, U, c.pq, s));
+    c.n_apply++;
+  }
   float* tf = c.precision == 1 ? c.ptf : nullptr;
   double* t64 = c.precision == 1 ? nullptr : c.pt;
   k_pcg_init<<<gv, kBlock, 0, s>>>(b, c.pq, c.pr, t64, tf, N, c.hpart_dev);
@@ -986,6 +989,221 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
 }
 
 // solve_U (src/poisson_solve.py:748-771): b = D m, PCG from the initial guess in U.
+// ------------------------------------------------------------ projected initial guess
+// Successive right-hand sides (the line-search trials and field steps of a minimisation) have
+// nearby solutions. The candidate initial guess of the next solve is the combination of the
+// last k solutions U_i whose right-hand sides b_i best match the new one,
+//   x_p = sum c_i U_i,  c = argmin || b - sum c_i b_i ||  (Gram matrix G_ij = b_i . b_j),
+// so that, with A U_i = b_i to the solver tolerance, b - A x_p is the least-squares residual
+// (the projection technique for successive right-hand sides of Fischer, CMAME 163, 1998, in
+// its residual-minimising form). The solve starts from x_p or from the given guess, whichever
+// has the smaller true residual; the PCG and its stopping test are unchanged.
+
+// per workgroup: h_i = b_i . b (i < n) and, for the newest slot j, b_j . b_i
+__global__ __launch_bounds__(kBlock) void k_guess_dots(long N, const double* __restrict__ BR,
+                                                       const double* __restrict__ b, int n, int j,
+                                                       double* __restrict__ part) {
+  double v[2 * kGuessMax];
+#pragma unroll
+  for (int k = 0; k < 2 * kGuessMax; ++k) v[k] = 0.0;
+  for (long x = blockIdx.x * (long)kBlock + threadIdx.x; x < N; x += (long)gridDim.x * kBlock) {
+    const double bx = b[x];
+    const double bj = j >= 0 ? BR[(long)j * N + x] : 0.0;
+#pragma unroll
+    for (int i = 0; i < kGuessMax; ++i) {
+      if (i >= n) break;
+      const double bi = BR[(long)i * N + x];
+      v[i] += bi * bx;
+      v[kGuessMax + i] += bi * bj;
+    }
+  }
+  block_partial<2 * kGuessMax>(v, part);
+}
+
+struct GuessCoef {
+  double c[kGuessMax];
+};
+
+// x = sum_i c_i U_i
+__global__ void k_guess_combine(long N, const double* __restrict__ UR, GuessCoef cf, int n,
+                                double* __restrict__ x) {
+  for (long k = blockIdx.x * (long)kBlock + threadIdx.x; k < N; k += (long)gridDim.x * kBlock) {
+    double u = 0.0;
+    for (int i = 0; i < n; ++i) u += cf.c[i] * UR[(long)i * N + k];
+    x[k] = u;
+  }
+}
+
+// per workgroup: |b - qa|^2 and |b - qb|^2
+__global__ __launch_bounds__(kBlock) void k_two_residuals(long N, const double* __restrict__ b,
+                                                          const double* __restrict__ qa,
+                                                          const double* __restrict__ qb,
+                                                          double* __restrict__ part) {
+  double v[2] = {0.0, 0.0};
+  for (long k = blockIdx.x * (long)kBlock + threadIdx.x; k < N; k += (long)gridDim.x * kBlock) {
+    const double ra = b[k] - qa[k], rb = b[k] - qb[k];
+    v[0] += ra * ra;
+    v[1] += rb * rb;
+  }
+  block_partial<2>(v, part);
+}
+
+// Every workgroup sums the two residual partials (fixed order) and, where x_p has the smaller
+// residual, copies x_p into U and A x_p into q.
+__global__ __launch_bounds__(kBlock) void k_guess_choose(long N, const double* __restrict__ part, int n_part,
+                                                         const double* __restrict__ xp,
+                                                         const double* __restrict__ qp, double* __restrict__ U,
+                                                         double* __restrict__ q) {
+  __shared__ double sh[2][kBlock / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double a = 0.0, b = 0.0;
+  for (int i = tid; i < n_part; i += kBlock) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  a = wave_red<false>(a);
+  b = wave_red<false>(b);
+  if (lane == 0) {
+    sh[0][wv] = a;
+    sh[1][wv] = b;
+  }
+  __syncthreads();
+  double rf = sh[0][0], rp = sh[1][0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) {
+    rf += sh[0][w];
+    rp += sh[1][w];
+  }
+  if (!(rp < rf)) return;
+  for (long k = blockIdx.x * (long)kBlock + tid; k < N; k += (long)gridDim.x * kBlock) {
+    U[k] = xp[k];
+    q[k] = qp[k];
+  }
+}
+
+// Eigen-decomposition of a small symmetric matrix (cyclic Jacobi): a = V diag(w) V^T.
+static void sym_eig(int n, std::vector<double> a, std::vector<double>& w, std::vector<double>& V) {
+  V.assign(n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int p = 0; p < n; ++p) {
+      diag += a[p * n + p] * a[p * n + p];
+      for (int q = p + 1; q < n; ++q) off += a[p * n + q] * a[p * n + q];
+    }
+    if (off <= 1e-34 * diag) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a[p * n + q];
+        if (apq == 0.0) continue;
+        const double th = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+        for (int k = 0; k < n; ++k) {
+          const double akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = cs * akp - sn * akq;
+          a[k * n + q] = sn * akp + cs * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = cs * apk - sn * aqk;
+          a[q * n + k] = sn * apk + cs * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = cs * vkp - sn * vkq;
+          V[k * n + q] = sn * vkp + cs * vkq;
+        }
+      }
+  }
+  w.resize(n);
+  for (int i = 0; i < n; ++i) w[i] = a[i * n + i];
+}
+
+int set_solve_guess(Ctx& c, int k) {
+  MM_CHECK(k >= 0 && k <= kGuessMax, "projection depth must be in [0, 8]");
+  if (k > 0 && c.guess_U == nullptr) {
+    for (double** v : {&c.guess_U, &c.guess_b}) {
+      MM_HIP(hipMalloc((void**)v, (size_t)kGuessMax * c.N * sizeof(double)));
+      c.allocs.push_back(*v);
+    }
+    MM_HIP(hipMalloc((void**)&c.guess_x, (size_t)c.N * sizeof(double)));
+    c.allocs.push_back(c.guess_x);
+    MM_HIP(hipHostMalloc((void**)&c.hguess, (size_t)kRedGridMax * 2 * kGuessMax * sizeof(double),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    MM_HIP(hipHostGetDevicePointer((void**)&c.hguess_dev, c.hguess, 0));
+  }
+  c.guess_k = k;
+  c.guess_n = 0;
+  c.guess_head = 0;
+  c.guess_new = -1;
+  c.guess_G.assign(kGuessMax * kGuessMax, 0.0);
+  return 0;
+}
+
+int solve_U_guess(Ctx& c, const double* m, double* U, double tol, int maxiter, int* it, double* rel_res) {
+  if (c.guess_k == 0) return solve_U_dev(c, m, U, tol, maxiter, it, rel_res);
+  hipStream_t s = c.stream;
+  const long N = c.N;
+  const int gr = red_grid(N);
+  vfill(c.pb, 0.0, N, s);
+  MM_TRY(launch_rhs(c, m, c.pb, s));
+  c.n_rhs++;
+  const int n = c.guess_n;
+  bool q_ready = false;
+  if (n > 0) {
+    k_guess_dots<<<gr, kBlock, 0, s>>>(N, c.guess_b, c.pb, n, c.guess_new, c.hguess_dev);
+    MM_HIP(hipStreamSynchronize(s));
+    double sums[2 * kGuessMax] = {};
+    for (int w = 0; w < gr; ++w)
+      for (int k = 0; k < 2 * kGuessMax; ++k) sums[k] += c.hguess[w * 2 * kGuessMax + k];
+    std::vector<double>& G = c.guess_G;
+    if (c.guess_new >= 0) {
+      const int j = c.guess_new;
+      for (int i = 0; i < n; ++i) G[j * kGuessMax + i] = G[i * kGuessMax + j] = sums[kGuessMax + i];
+      c.guess_new = -1;
+    }
+    // least squares on the well-conditioned eigenspace of G
+    std::vector<double> a(n * n), w, V;
+    for (int i = 0; i < n; ++i)
+      for (int q = 0; q < n; ++q) a[i * n + q] = G[i * kGuessMax + q];
+    sym_eig(n, a, w, V);
+    double wmax = 0.0;
+    for (double x : w) wmax = std::max(wmax, x);
+    if (wmax > 0.0) {
+      GuessCoef cf{};
+      for (int e = 0; e < n; ++e) {
+        if (!(w[e] > 1e-14 * wmax)) continue;
+        double vh = 0.0;
+        for (int i = 0; i < n; ++i) vh += V[i * n + e] * sums[i];
+        for (int i = 0; i < n; ++i) cf.c[i] += V[i * n + e] * vh / w[e];
+      }
+      k_guess_combine<<<vgrid(N), kBlock, 0, s>>>(N, c.guess_U, cf, n, c.guess_x);
+      // true residuals of the given guess (A U into pq) and of x_p (A x_p into pz)
+      vfill(c.pq, 0.0, N, s);
+      MM_TRY(timed_apply(c, U, c.pq, s));
+      vfill(c.pz, 0.0, N, s);
+      MM_TRY(timed_apply(c, c.guess_x, c.pz, s));
+      c.n_apply += 2;
+      k_two_residuals<<<gr, kBlock, 0, s>>>(N, c.pb, c.pq, c.pz, c.part_rr);
+      k_guess_choose<<<vgrid(N), kBlock, 0, s>>>(N, c.part_rr, gr, c.guess_x, c.pz, U, c.pq);
+      q_ready = true;
+    }
+  }
+  double r2 = 0, bb = 0;
+  MM_TRY(pcg_dev(c, c.pb, U, tol, maxiter, it, &r2, &bb, q_ready));
+  *rel_res = std::sqrt(r2 / (bb + 1e-30));
+  // add (U, b) to the ring; the new Gram entries are formed with the next solve's dots
+  const int j = c.guess_head;
+  MM_TRY(vcopy(c.guess_U + (long)j * N, U, N, s));
+  MM_TRY(vcopy(c.guess_b + (long)j * N, c.pb, N, s));
+  c.guess_head = (j + 1) % c.guess_k;
+  c.guess_n = std::min(c.guess_n + 1, c.guess_k);
+  c.guess_new = j;
+  for (int i = 0; i < kGuessMax; ++i) c.guess_G[j * kGuessMax + i] = c.guess_G[i * kGuessMax + j] = 0.0;
+  return 0;
+}
+
 int solve_U_dev(Ctx& c, const double* m, double* U, double tol, int maxiter, int* it, double* rel_res) {
   hipStream_t s = c.stream;
   vfill(c.pb, 0.0, c.N, s);

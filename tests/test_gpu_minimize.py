@@ -111,3 +111,38 @@ def test_sweep_writes_result_files(small_pair, tmp_path):
     mx_ref = [ctx.j_parallel(m_fin, e) for e in np.eye(3)]
     np.testing.assert_allclose(mh[-1, 2:5], mx_ref, atol=1e-12)
     assert sorted(f.name.split("_B")[0] for f in tmp_path.glob("*.vtu")) == ["state_cfg00000", "state_cfg00002"]
+
+
+@pytest.mark.parametrize("k", [4, 8])
+def test_projected_solve_guess_keeps_the_sweep(small_pair, k):
+    """The projected initial guess of the minimiser's Poisson solves changes where each PCG
+    starts, not where it stops: a +2 -> -8 T sweep gives the energies of the reference warm
+    start to 1e-9 of the energy scale, the same minimiser iterations within 1 per step and the
+    same switching step, and no more PCG iterations in all."""
+    from mumag_matrixfree2_amd.context import make_params
+    from mumag_matrixfree2_amd.hysteresis_loop import switching_step
+    from oracle.minimize import field_values
+
+    ctx, p, _ = small_pair
+    h = np.array([0.017452, 0.0, 0.999848])
+    par = make_params("pcohen_hs", tau_f=1e-10, eps_a=1e-12, cg_tol=1e-10)
+    m0 = tilted_m0(ctx.N, seed=0, sigma=0.01)
+    runs = {}
+    for kk in (0, k):
+        ctx.set_solve_guess(kk)
+        ctx.state_set(m0, np.zeros(ctx.N))
+        rows = []
+        for Bmag in field_values(2.0 / 1.61, -8.0 / 1.61, -0.5 / 1.61, False):
+            info = ctx.minimize_resident(par, Bmag * h)
+            rows.append((info["E"], info["iters"], info["demag_iters"], ctx.state_j_parallel(h)))
+        runs[kk] = np.array(rows)
+    ctx.set_solve_guess(0)
+    a, b = runs[0], runs[k]
+    terms = p.energy_terms(*ctx.state_get(), -8.0 / 1.61 * h)
+    scale = sum(abs(v) for v in terms.values())
+    print(f"[guess {k}] PCG iterations {int(a[:, 2].sum())} -> {int(b[:, 2].sum())}, "
+          f"max dE {np.max(np.abs(a[:, 0] - b[:, 0])):.2e}", flush=True)
+    assert np.max(np.abs(a[:, 0] - b[:, 0])) <= 1e-9 * scale
+    assert np.max(np.abs(a[:, 1] - b[:, 1])) <= 1
+    assert switching_step(a[:, 3]) == switching_step(b[:, 3])
+    assert b[:, 2].sum() <= a[:, 2].sum()
