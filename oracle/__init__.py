@@ -7,6 +7,10 @@ AMG V-cycle, and the minimiser pieces (Cayley update, Armijo line search,
 Steihaug-PCG preconditioner, pcohen_hs and L-BFGS steps, convergence test).
 Every function cites the reference file:line it follows.
 
+``cpu.py`` / ``cpu_minimizer.cpp`` restate the pcohen_hs field step of ``minimize.py`` and
+``solvers.py`` in C++ with OpenMP (the multi-core CPU baseline); ``tests/test_oracle_cpu.py``
+checks them against the NumPy code.
+
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import it, and only as the checker. The product package
 ``mumag_matrixfree2_amd`` never imports it and has no CPU fallback.
