@@ -333,7 +333,6 @@ static void build_cluster_partition(int N, std::vector<HostCluster>& cl, const s
   hp.cptr.assign(1, 0);
   hp.nptr.assign(1, 0);
   int blk = 0;
-  size_t c0 = 0;
   std::vector<int> members;
   auto close_block = [&]() {
     const int nb = (int)blk_nodes.size();
@@ -400,7 +399,6 @@ static void build_cluster_partition(int N, std::vector<HostCluster>& cl, const s
   if (!blk_nodes.empty()) close_block();
   for (auto& f : hp.nodes)
     if (touches[f & kIdMask] == 1) f |= kPrivate;
-  (void)c0;
 }
 
 static int upload_part(Ctx& c, const HostPart& hp, int E, Part& p) {
