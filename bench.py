@@ -376,6 +376,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precond", default="amgcl", choices=["amgcl", "amg"])
+    ap.add_argument("--method", default="pcohen_hs", choices=["pcohen_hs", "lbfgs"],
+                    help="field-step minimiser (cube2M / poly25M; the grain set runs pcohen_hs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="time budget of the CPU baseline (field steps from the start of the sweep)")
@@ -431,7 +433,7 @@ def main():
     log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: N={ctx.N} E={ctx.E} "
         f"AMG levels={[l['A'].shape[0] for l in ctx.amg_levels]}")
 
-    p = make_params("pcohen_hs", tau_f=1e-10, eps_a=1e-12, cg_tol=1e-10, max_iter=2000)
+    p = make_params(args.method, tau_f=1e-10, eps_a=1e-12, cg_tol=1e-10, max_iter=2000)
     B_vals = field_values(2.0 / JS, -8.0 / JS, wl["dB"] / JS, False)
     m0 = initial_m(ctx.N)
     steps = min(args.steps or wl["steps"], len(B_vals))
@@ -525,7 +527,7 @@ def main():
         "data": "synthetic (generated mesh, seeded initial magnetisation)",
         "config": {
             "workload": f"{wl['name']}_{args.precision}_sweep+2..-8T_{len(B_vals)}steps",
-            "tets": ctx.E, "nodes": ctx.N, "method": "pcohen_hs", "precond": f"SA-AMG V-cycle ({args.precond})",
+            "tets": ctx.E, "nodes": ctx.N, "method": args.method, "precond": f"SA-AMG V-cycle ({args.precond})",
             "amg_levels": [int(l["A"].shape[0]) for l in ctx.amg_levels],
             "apply_shape": {"unit": shape["unit"], "param": shape["param"], "block_nodes": ctx.block_nodes},
             "solve_guess": (f"projection onto the last {args.solve_guess} solutions" if args.solve_guess
@@ -553,7 +555,7 @@ def main():
         result["mixed_vs_fp64"] = errors
     if guess_cmp is not None:
         result["reference_warm_start"] = guess_cmp
-    if not args.no_cpu_baseline and args.workload == "cube2M":
+    if not args.no_cpu_baseline and args.workload == "cube2M" and args.method == "pcohen_hs":
         k_cpu, dt, it_cpu, nthr = cpu_baseline(mesh, mats, B_vals, args.cpu_seconds)
         t_gpu_k = sum(rows[:k_cpu])
         # the CPU's rate on the first k steps, carried to the whole sweep with the GPU's own ratio
