@@ -95,7 +95,17 @@ def run_hysteresis_loop(ctx: MeshContext, m0: np.ndarray, params: LoopParams, *,
     ``hysteresis.csv`` (one row per step), ``<name>.mh`` (B [T], J_par [T], mx, my, mz,
     E [J/m^3]) and, every ``snapshot_every`` steps, a ``state_cfg*.vtu`` with m and U. The
     host then downloads m once per step for the volume-averaged (mx, my, mz).
+
+    A segmented context (a grain set, ``grainset.grain_set_context``) sweeps every grain at
+    once (``minimize_segmented``, pcohen_hs): J_par and E of a row are the magnetic-volume
+    averages over the grains, iterations and evaluations the largest per grain, and the result
+    also holds ``grain_J`` (steps x grains).
     """
+    seg = ctx.segments is not None
+    if seg and params.method != "pcohen_hs":
+        raise ValueError("a segmented context sweeps with pcohen_hs")
+    vol = ctx.segment_volumes() if seg else None
+    grain_J = []
     h = np.asarray(params.h_dir, dtype=np.float64)
     h = h / (np.linalg.norm(h) + 1e-30)
     if ctx.precond_type != params.precond_type:
@@ -125,8 +135,18 @@ def run_hysteresis_loop(ctx: MeshContext, m0: np.ndarray, params: LoopParams, *,
     t_all = time.perf_counter()
     for k, Bmag in enumerate(B_vals):
         t0 = time.perf_counter()
-        info = ctx.minimize_resident(p, Bmag * h)
-        J = ctx.state_j_parallel(h)
+        if seg:
+            infos = ctx.minimize_segmented(p, Bmag * h)
+            Jg = ctx.state_j_parallel_segmented(h)
+            grain_J.append(Jg)
+            J = float(vol @ Jg / vol.sum())
+            info = {"E": float(vol @ np.array([i["E"] for i in infos]) / vol.sum()),
+                    "gnorm": max(i["gnorm"] for i in infos)}
+            for key in ("iters", "evals", "demag_iters", "preco_iters"):
+                info[key] = max(i[key] for i in infos)
+        else:
+            info = ctx.minimize_resident(p, Bmag * h)
+            J = ctx.state_j_parallel(h)
         dt = time.perf_counter() - t0
         row = [Bmag, J, info["E"], info["gnorm"], info["iters"], info["evals"], info["demag_iters"],
                info["preco_iters"], dt]
@@ -152,5 +172,8 @@ def run_hysteresis_loop(ctx: MeshContext, m0: np.ndarray, params: LoopParams, *,
     hist = np.asarray(hist)
     if out_dir is not None and mh:
         io_utils.write_mh(Path(out_dir) / name, np.asarray(mh))
-    return {"history": hist, "m": m, "U": U, "seconds": time.perf_counter() - t_all,
-            "switching_step": switching_step(hist[:, 1]) if len(hist) else -1}
+    out = {"history": hist, "m": m, "U": U, "seconds": time.perf_counter() - t_all,
+           "switching_step": switching_step(hist[:, 1]) if len(hist) else -1}
+    if seg:
+        out["grain_J"] = np.asarray(grain_J)
+    return out

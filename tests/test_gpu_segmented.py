@@ -200,3 +200,31 @@ def test_minimize_segmented_mixed_precision(grainset):
         assert a["converged"] and b["converged"]
         assert abs(a["E"] - b["E"]) <= 1e-8 * max(1.0, abs(a["E"]))
         assert abs(a["iters"] - b["iters"]) <= 2
+
+
+def test_hysteresis_sweep_of_a_grain_set(grainset, tmp_path):
+    """run_hysteresis_loop on the segmented context: per-grain J_par equal to each grain's own
+    sweep in a context of its own, the volume-averaged J_par of the rows, and the result files."""
+    from mumag_matrixfree2_amd.context import MeshContext
+    from mumag_matrixfree2_amd.grainset import split
+    from mumag_matrixfree2_amd.hysteresis_loop import LoopParams, run_hysteresis_loop
+    from mumag_matrixfree2_amd.mesh import grain_set
+
+    ctx, _ = grainset
+    gs = grain_set(5, seed=7, t_min=3e3, t_max=3e4)
+    h = np.array([0.017452, 0.0, 0.999848])
+    params = LoopParams(h_dir=h, B_start=1.0, B_end=-1.0, dB=-1.0, tau_f=1e-10, cg_tol=1e-10)
+    m0 = np.vstack([tilted_m0(g["knt"].shape[0], seed=80 + i, sigma=0.05) for i, g in enumerate(gs)])
+    res = run_hysteresis_loop(ctx, m0, params, out_dir=tmp_path, name="grains")
+    gJ = res["grain_J"]
+    assert gJ.shape == (3, len(gs))
+    vol = ctx.segment_volumes()
+    assert np.allclose(res["history"][:, 1], gJ @ vol / vol.sum(), rtol=0, atol=1e-14)
+    for i, (g, m_i) in enumerate(zip(gs, split(ctx, m0))):
+        c = MeshContext(g["knt"], g["ijk"][:, :4], g["ijk"][:, 4], *ndfeb_materials(axis=g["easy_axis"]),
+                        boundary_mask=g["boundary_mask"])
+        c.setup_precond("amgcl", max_coarse=300)
+        r = run_hysteresis_loop(c, m_i, params)
+        c.close()
+        assert np.max(np.abs(r["history"][:, 1] - gJ[:, i])) < 1e-9
+    assert (tmp_path / "hysteresis.csv").exists() and (tmp_path / "grains.mh").exists()
