@@ -907,6 +907,18 @@ int mm_poisson_diag(int64_t h, double* d) {
   return 0;
 }
 
+// CSR structure of a rows x cols matrix: offsets from 0, non-decreasing, column indices in range
+// (the SpMV kernels index the vectors with them unchecked)
+static bool valid_csr(const int32_t* p, const int32_t* j, int rows, int cols) {
+  if (!p || !j || p[0] != 0) return false;
+  for (int r = 0; r < rows; ++r) {
+    if (p[r + 1] < p[r]) return false;
+    for (int k = p[r]; k < p[r + 1]; ++k)
+      if (j[k] < 0 || j[k] >= cols) return false;
+  }
+  return true;
+}
+
 int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, const int32_t* Ai, const double* Ax,
                      const double* M, const int32_t* Pp, const int32_t* Pi, const double* Px, const int32_t* Rp,
                      const int32_t* Ri, const double* Rx, const double* dense_inv) {
@@ -914,6 +926,13 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
   MM_CHECK(l >= 0 && l < 32, "level index out of range");
   MM_CHECK(n > 0 && M, "level needs n > 0 and the smoother diagonal");
   MM_CHECK(l != 0 || n == c.N, "level 0 must have N rows");
+  MM_CHECK(n_next >= 0, "n_next must be >= 0");
+  if (l > 0 && Ap) MM_CHECK(Ax && valid_csr(Ap, Ai, n, n), "level matrix A: malformed CSR (n x n)");
+  if (n_next > 0) {
+    MM_CHECK(Pp && Pi && Px && Rp && Ri && Rx, "P and R are required above the last level");
+    MM_CHECK(valid_csr(Pp, Pi, n, n_next), "prolongation P: malformed CSR (n x n_next)");
+    MM_CHECK(valid_csr(Rp, Ri, n_next, n), "restriction R: malformed CSR (n_next x n)");
+  }
   if ((int)c.lv.size() <= l) c.lv.resize(l + 1);
   Level L;
   L.n = n;
@@ -925,7 +944,6 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
     MM_TRY(upload(c, &L.Ax, Ax, Ap[n]));
   }
   if (n_next > 0) {
-    MM_CHECK(Pp && Pi && Px && Rp && Ri && Rx, "P and R are required above the last level");
     MM_TRY(upload(c, &L.Pp, Pp, n + 1));
     MM_TRY(upload(c, &L.Pi, Pi, Pp[n]));
     MM_TRY(upload(c, &L.Px, Px, Pp[n]));
@@ -1139,6 +1157,7 @@ int mm_local_grad(int64_t h, const double* v, double* g) {
 
 int mm_j_parallel(int64_t h, const double* m, const double* hdir, double* J) {
   GET_CTX(h);
+  MM_CHECK(m && hdir && J, "null argument");
   MM_TRY(alloc_min_state(c));
   MM_HIP(hipMemcpyAsync(c.m_t, m, 3 * c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
   double hn = std::sqrt(hdir[0] * hdir[0] + hdir[1] * hdir[1] + hdir[2] * hdir[2]) + 1e-30;
@@ -1152,6 +1171,7 @@ int mm_j_parallel(int64_t h, const double* m, const double* hdir, double* J) {
 
 int mm_state_j_parallel(int64_t h, const double* hdir, double* J) {
   GET_CTX(h);
+  MM_CHECK(hdir && J, "null argument");
   MM_CHECK(c.have_state, "no resident state");
   double hn = std::sqrt(hdir[0] * hdir[0] + hdir[1] * hdir[1] + hdir[2] * hdir[2]) + 1e-30;
   const double hh[3] = {hdir[0] / hn, hdir[1] / hn, hdir[2] / hn};
@@ -1164,6 +1184,7 @@ int mm_state_j_parallel(int64_t h, const double* hdir, double* J) {
 
 int mm_state_set(int64_t h, const double* m, const double* U) {
   GET_CTX(h);
+  MM_CHECK(m, "null argument");
   MM_TRY(alloc_min_state(c));
   MM_HIP(hipMemcpyAsync(c.m, m, 3 * c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
   if (U)

@@ -94,3 +94,50 @@ def test_jacobi_pcg(cube_c0, oracle_c0):
     assert abs(it - it_ref) <= 2
     assert rel(U, U_ref) <= 1e-7
     ctx.close()
+
+
+def test_amg_upload_rejects_malformed_csr(cube_c0):
+    """A column index outside its matrix, or decreasing row offsets, in a level's A, P or R is an
+    error at upload, before any kernel could read through it; the good hierarchy still works."""
+    import scipy.sparse as sp
+
+    from mumag_matrixfree2_amd._hip import call, f64, i32, ptr
+    from mumag_matrixfree2_amd.amg import build_hierarchy, fine_matrix, upload_hierarchy
+    from mumag_matrixfree2_amd.context import MeshContext
+
+    A, K1, Js, k = ndfeb_materials()
+    m = cube_c0
+    ctx = MeshContext(m["knt"], m["ijk"][:, :4], m["ijk"][:, 4], A, K1, Js, k, boundary_mask=m["boundary_mask"])
+    levels = build_hierarchy(fine_matrix(ctx), smoother="amgcl")
+    assert len(levels) >= 2
+    lv0, lv1 = levels[0], levels[1]
+    n0, n1 = lv0["A"].shape[0], lv0["P"].shape[1]
+
+    def set0(P, R):
+        call("mm_amg_set_level", ctx.handle, 0, n0, n1, None, None, None, ptr(f64(lv0["M"])),
+             ptr(i32(P.indptr)), ptr(i32(P.indices)), ptr(f64(P.data)),
+             ptr(i32(R.indptr)), ptr(i32(R.indices)), ptr(f64(R.data)), None)
+
+    P, R = sp.csr_matrix(lv0["P"]), sp.csr_matrix(lv0["R"])
+    bad_P = P.copy()
+    bad_P.indices = bad_P.indices.copy()
+    bad_P.indices[len(bad_P.indices) // 2] = n1  # one past the last coarse unknown
+    with pytest.raises(RuntimeError, match="prolongation P: malformed"):
+        set0(bad_P, R)
+    bad_R = R.copy()
+    bad_R.indices = bad_R.indices.copy()
+    bad_R.indices[0] = -1
+    with pytest.raises(RuntimeError, match="restriction R: malformed"):
+        set0(P, bad_R)
+    A1 = sp.csr_matrix(lv1["A"])
+    ptr_bad = A1.indptr.copy()
+    ptr_bad[1], ptr_bad[2] = ptr_bad[2], ptr_bad[1]  # decreasing offsets
+    n_next1 = 0 if len(levels) == 2 else levels[1]["P"].shape[1]
+    with pytest.raises(RuntimeError, match="level matrix A: malformed"):
+        call("mm_amg_set_level", ctx.handle, 1, n1, n_next1, ptr(i32(ptr_bad)), ptr(i32(A1.indices)),
+             ptr(f64(A1.data)), ptr(f64(lv1["M"])), None, None, None, None, None, None,
+             ptr(f64(lv1["A_dense_inv"])) if len(levels) == 2 else None)
+    upload_hierarchy(ctx, levels, "amgcl")
+    r = np.random.default_rng(15).standard_normal(ctx.N)
+    assert float(r @ ctx.precond_apply(r)) > 0.0
+    ctx.close()
