@@ -383,6 +383,8 @@ def main():
                     help="time budget of the CPU baseline (field steps from the start of the sweep)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--max-coarse", type=int, default=None)
+    ap.add_argument("--block-nodes", type=int, default=256,
+                    help="distinct nodes per element block (16..256): the LDS staging size of the element kernels")
     ap.add_argument("--apply-shape", default="auto",
                     help="'auto' (autotuned at set-up) or unit:param, e.g. cluster:256, element:128, persistent:2")
     ap.add_argument("--solve-guess", type=int, default=8,
@@ -417,7 +419,7 @@ def main():
     mesh, mats = make_workload(args.workload)
     log(f"[bench] mesh {time.perf_counter() - t0:.1f}s")
     ctx = MeshContext(mesh["knt"], mesh["ijk"][:, :4], mesh["ijk"][:, 4], *mats,
-                      boundary_mask=mesh["boundary_mask"])
+                      boundary_mask=mesh["boundary_mask"], block_nodes=args.block_nodes)
     log(f"[bench] context {time.perf_counter() - t0:.1f}s")
     ctx.setup_precond(args.precond, max_coarse=args.max_coarse, log=log)
     ctx.set_precision(args.precision)
