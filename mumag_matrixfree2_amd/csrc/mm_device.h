@@ -277,6 +277,46 @@ __device__ void block_partial(double (&v)[K], double* __restrict__ partial, int 
     }
 }
 
+// Per-thread strided sums s[k] += part[i * K + k] for i = i0 + tid, i0 + tid + NT, ... < i1, added
+// in that order. A plain strided loop waits for each load before its add; here the loads of up
+// to 16 / K consecutive terms are in flight together, so the few thousand partials of an element
+// kernel cost one or two memory round trips per thread instead of one per term.
+template <int K, int NT>
+__device__ __forceinline__ void strided_sums(const double* __restrict__ part, int i0, int i1, double (&s)[K]) {
+  constexpr int B = 16 / K > 0 ? 16 / K : 1;
+  for (int base = i0 + (int)threadIdx.x; base < i1; base += B * NT) {
+    double v[B][K];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int i = base + u * NT;
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[u][k] = i < i1 ? part[(long)i * K + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+#pragma unroll
+      for (int k = 0; k < K; ++k) s[k] += v[u][k];
+  }
+}
+
+// Compensated form of strided_sums over (s, c) pairs part[2 i], part[2 i + 1], added in the
+// order i = i0 + tid, i0 + tid + NT, ... (a zero pair leaves the sum unchanged).
+template <int NT>
+__device__ __forceinline__ void strided_csums(const double* __restrict__ part, int i0, int i1, CSum& v) {
+  constexpr int B = 8;
+  for (int base = i0 + (int)threadIdx.x; base < i1; base += B * NT) {
+    CSum u[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int i = base + q * NT;
+      u[q].s = i < i1 ? part[2L * i] : 0.0;
+      u[q].c = i < i1 ? part[2L * i + 1] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < B; ++q) v.add(u[q]);
+  }
+}
+
 // Sum of n partials written by an earlier kernel on the same stream (the kernel boundary
 // makes them visible). Every workgroup that calls it computes the same total in the same
 // fixed order, so the result is deterministic and identical across the grid; no fence or
@@ -285,9 +325,9 @@ template <int NT>
 __device__ double sum_partials_all(const double* __restrict__ part, int n) {
   __shared__ double sh[NT / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  double s = 0.0;
-  for (int i = tid; i < n; i += NT) s += part[i];
-  s = wave_red<false>(s);
+  double sv[1] = {0.0};
+  strided_sums<1, NT>(part, 0, n, sv);
+  double s = wave_red<false>(sv[0]);
   if (lane == 0) sh[wv] = s;
   __syncthreads();
   double t = sh[0];

@@ -664,11 +664,10 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restric
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double v[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-    for (int i = tid; i < n; i += kBlock) s += partial[i * K + k];
-    v[k] = wave_red<false>(s);
-  }
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  strided_sums<K, kBlock>(partial, 0, n, v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_red<false>(v[k]);
   if (lane == 0)
 #pragma unroll
     for (int k = 0; k < K; ++k) sh[k][wv] = v[k];
@@ -688,12 +687,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials_cs(const double* __rest
   __shared__ double sh[2][kBlock / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   CSum v;
-  for (int i = tid; i < n; i += kBlock) {
-    CSum u;
-    u.s = partial[2 * i];
-    u.c = partial[2 * i + 1];
-    v.add(u);
-  }
+  strided_csums<kBlock>(partial, 0, n, v);
   v = wave_red_cs(v);
   if (lane == 0) {
     sh[0][wv] = v.s;
@@ -725,12 +719,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_block_sums(const int* __restrict
   if constexpr (CS) {
     __shared__ double sh[2][kBlock / 64];
     CSum v;
-    for (int i = b0 + tid; i < b1; i += kBlock) {
-      CSum u;
-      u.s = part[2 * i];
-      u.c = part[2 * i + 1];
-      v.add(u);
-    }
+    strided_csums<kBlock>(part, b0, b1, v);
     v = wave_red_cs(v);
     if (lane == 0) {
       sh[0][wv] = v.s;
@@ -752,9 +741,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_block_sums(const int* __restrict
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = 0.0;
-    for (int i = b0 + tid; i < b1; i += kBlock)
-#pragma unroll
-      for (int k = 0; k < K; ++k) v[k] += part[(long)i * K + k];
+    strided_sums<K, kBlock>(part, b0, b1, v);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       v[k] = wave_red<false>(v[k]);

@@ -261,10 +261,10 @@ __global__ __launch_bounds__(kBlock) void k_pc_finish(double* __restrict__ sc, c
   {
     // the three strided sums, each in the fixed order of sum_partials_all
     const int tid = threadIdx.x;
+    double s3[3] = {0.0, 0.0, 0.0};
+    strided_sums<3, kBlock>(part, 0, n_part, s3);
     for (int k = 0; k < 3; ++k) {
-      double sk = 0.0;
-      for (int i = tid; i < n_part; i += kBlock) sk += part[3 * i + k];
-      sk = wave_red<false>(sk);
+      const double sk = wave_red<false>(s3[k]);
       __shared__ double sh[kBlock / 64];
       if ((tid & 63) == 0) sh[tid >> 6] = sk;
       __syncthreads();

@@ -66,9 +66,7 @@ __device__ void seg_sums(const double* __restrict__ part, int w0, int w1, double
   double v[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = 0.0;
-  for (int w = w0 + tid; w < w1; w += kBlock)
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] += part[w * K + k];
+  strided_sums<K, kBlock>(part, w0, w1, v);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     v[k] = wave_red<false>(v[k]);

@@ -298,7 +298,17 @@ __global__ __launch_bounds__(256) void k_symv_reduce(int n, int T_, const double
   const int I = blockIdx.x, r = threadIdx.x & 63, g = threadIdx.x >> 6;
   const double* p = part + (size_t)I * T_ * 64 + r;
   double s = 0.0;
-  for (int J = g; J < T_; J += 4) s += p[(size_t)J * 64];
+  // J = g, g + 4, ... in order, 16 loads in flight per batch
+  for (int J0 = g; J0 < T_; J0 += 64) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int J = J0 + 4 * u;
+      v[u] = J < T_ ? p[(size_t)J * 64] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s += v[u];
+  }
   sh[g][r] = s;
   __syncthreads();
   const int i = I * 64 + threadIdx.x;
@@ -1056,13 +1066,10 @@ __global__ __launch_bounds__(kBlock) void k_guess_choose(long N, const double* _
                                                          double* __restrict__ q) {
   __shared__ double sh[2][kBlock / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  double a = 0.0, b = 0.0;
-  for (int i = tid; i < n_part; i += kBlock) {
-    a += part[2 * i];
-    b += part[2 * i + 1];
-  }
-  a = wave_red<false>(a);
-  b = wave_red<false>(b);
+  double ab[2] = {0.0, 0.0};
+  strided_sums<2, kBlock>(part, 0, n_part, ab);
+  const double a = wave_red<false>(ab[0]);
+  const double b = wave_red<false>(ab[1]);
   if (lane == 0) {
     sh[0][wv] = a;
     sh[1][wv] = b;
