@@ -692,6 +692,7 @@ static void destroy(Ctx* c) {
   if (c->hpart) (void)hipHostFree(c->hpart);
   if (c->hseg) (void)hipHostFree(c->hseg);
   if (c->hguess) (void)hipHostFree(c->hguess);
+  if (c->hmpart) (void)hipHostFree(c->hmpart);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev) (void)hipEventDestroy(c->ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -706,6 +707,8 @@ int alloc_min_state(Ctx& c) {
     MM_TRY(dalloc(c, v, n3));
   MM_TRY(dalloc(c, &c.U, n));
   MM_TRY(dalloc(c, &c.U_t, n));
+  MM_TRY(dalloc(c, &c.mpart, 6 * 2 * (size_t)kRedGridMax));
+  MM_HIP(hipHostMalloc((void**)&c.hmpart, 6 * 2 * kRedGridMax * sizeof(double), hipHostMallocDefault));
   return 0;
 }
 

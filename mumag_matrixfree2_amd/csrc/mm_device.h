@@ -336,6 +336,57 @@ __device__ double sum_partials_all(const double* __restrict__ part, int n) {
   return t;
 }
 
+// K sums of n partials part[i * K + k] (K per workgroup of the producer), as sum_partials_all:
+// same fixed order in every workgroup, valid in every thread.
+template <int K, int NT>
+__device__ void sum_partials_k(const double* __restrict__ part, int n, double (&out)[K]) {
+  __shared__ double sh[K][NT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double s[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) s[k] = 0.0;
+  strided_sums<K, NT>(part, 0, n, s);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    s[k] = wave_red<false>(s[k]);
+    if (lane == 0) sh[k][wv] = s[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double t = sh[k][0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) t += sh[k][w];
+    out[k] = t;
+  }
+}
+
+// Maximum of n partials written by an earlier kernel, valid in every thread.
+template <int NT>
+__device__ double max_partials_all(const double* __restrict__ part, int n) {
+  __shared__ double sh[NT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double m = 0.0;
+  constexpr int B = 16;
+  for (int base = tid; base < n; base += B * NT) {
+    double v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int i = base + u * NT;
+      v[u] = i < n ? part[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) m = fmax(m, v[u]);
+  }
+  m = wave_red<true>(m);
+  if (lane == 0) sh[wv] = m;
+  __syncthreads();
+  double t = sh[0];
+#pragma unroll
+  for (int w = 1; w < NT / 64; ++w) t = fmax(t, sh[w]);
+  return t;
+}
+
 // Block sum of v written by thread 0 to *out (no cross-block traffic).
 template <int NT>
 __device__ void block_sum_to(double v, double* __restrict__ out) {

@@ -210,6 +210,9 @@ struct Ctx {
   double *r3 = nullptr, *z3 = nullptr, *p3 = nullptr, *q3 = nullptr, *cv3 = nullptr;
   int mem = 10;
   double *S = nullptr, *Yh = nullptr;  // mem x 3N
+  // per-workgroup partials of the field-step reductions (6 banks of 2 x kRedGridMax) and a pinned
+  // host copy of the first banks
+  double *mpart = nullptr, *hmpart = nullptr;
   bool have_state = false;
 
   // independent systems (segments) sharing the context: contiguous node ranges; no element

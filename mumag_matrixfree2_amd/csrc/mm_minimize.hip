@@ -21,13 +21,8 @@ int alloc_min_state(Ctx& c);
 
 enum Slot {
   S_E = 16,
-  S_GN = 24,   // max |g_tan|
-  S_GS = 25,   // max |y|
-  S_NUM = 26,  // y . (g_ext - g_prev)
-  S_DEN = 27,  // d_prev . (g_ext - g_prev)
-  S_DG = 28,   // d . g_ext
-  S_PG = 29,   // g_raw . d
-  S_DM = 30,   // max |m_new - m|
+  // max |g_tan|, max |y|, the beta dots, d.g_ext, g_raw.d and max |m_new - m| are partial banks
+  // (RedBank below)
   S_PCIT = 31, // inner iterations (copy for the host)
   S_RHO = 32, S_Q = 33, S_IT = 34, S_DONE = 35, S_ACT = 36, S_ALPHA = 37, S_TGT = 38, S_DQ = 39, S_DN = 40,
   S_PAP = 41, S_RHON = 42, S_YY = 43, S_ZZ = 44, S_YG = 45,
@@ -62,11 +57,11 @@ __global__ void k_normalize(double* __restrict__ m, long N) {
   }
 }
 
-// g_tan = T(m, g invMr), g_ext = T(m, g); out = max |g_tan|  (src/minimizers.py:94-96, 963-966)
+// g_tan = T(m, g invMr), g_ext = T(m, g); per-workgroup max |g_tan| to part (src/minimizers.py:94-96,
+// 963-966)
 __global__ __launch_bounds__(kBlock) void k_tangents(const double* __restrict__ m, const double* __restrict__ g,
                                                      const double* __restrict__ invMr, double* __restrict__ g_tan,
-                                                     double* __restrict__ g_ext, long N, double* __restrict__ partial,
-                                                     unsigned* __restrict__ counter, double* __restrict__ out) {
+                                                     double* __restrict__ g_ext, long N, double* __restrict__ part) {
   double v[1] = {0.0};
   NODE_LOOP(i, N) {
     double t[3];
@@ -82,7 +77,15 @@ __global__ __launch_bounds__(kBlock) void k_tangents(const double* __restrict__ 
     g_tan[3 * i + 2] = t[2];
     v[0] = fmax(v[0], fmax(fabs(t[0]), fmax(fabs(t[1]), fabs(t[2]))));
   }
-  reduce_store<1, true>(v, partial, counter, out);
+  block_partial<1, kBlock, true>(v, part);
+}
+
+// per-workgroup max |a| to part
+__global__ __launch_bounds__(kBlock) void k_maxabs_part(const double* __restrict__ a, long n,
+                                                        double* __restrict__ part) {
+  double v[1] = {0.0};
+  NODE_LOOP(i, n) v[0] = fmax(v[0], fabs(a[i]));
+  block_partial<1, kBlock, true>(v, part);
 }
 
 // ------------------------------------------------------------- Steihaug-PCG (solve_Py_g)
@@ -127,12 +130,13 @@ __device__ __forceinline__ double pc_active(const double* bank, int max_iter) {
 // bank 0: rho = sum of the k_pc_init partials, forcing-sequence tolerance (src/minimizers.py:968-972,
 // 445-450), Q = 0, iteration 0, active flag. One workgroup.
 __global__ __launch_bounds__(kBlock) void k_pc_state_init(double* __restrict__ sc, const double* __restrict__ part,
-                                                          int n_part, int pc_auto, double eta, double alpha_f,
-                                                          int max_iter) {
+                                                          int n_part, const double* __restrict__ part_gn, int n_gn,
+                                                          int pc_auto, double eta, double alpha_f, int max_iter) {
   const double rho = sum_partials_all<kBlock>(part, n_part);
+  const double gn = max_partials_all<kBlock>(part_gn, n_gn);  // max |g_tan| (k_tangents)
   if (threadIdx.x != 0) return;
   double* bk = sc + pc_bank(0);
-  const double tol = pc_auto ? fmin(eta, pow(sc[S_GN], alpha_f)) : 0.0;
+  const double tol = pc_auto ? fmin(eta, pow(gn, alpha_f)) : 0.0;
   bk[B_RHO] = rho;
   bk[B_TGT] = tol * tol * rho;
   bk[B_Q] = 0.0;
@@ -295,25 +299,28 @@ __global__ __launch_bounds__(kBlock) void k_pc_finish(double* __restrict__ sc, c
 
 // ----------------------------------------------------------------- direction kernels
 // HS beta dots: y.(g_ext - g_prev), d_prev.(g_ext - g_prev)
-__global__ __launch_bounds__(kBlock) void k_beta_dots(double* __restrict__ sc, const double* __restrict__ y,
-                                                      const double* __restrict__ ge, const double* __restrict__ gp,
-                                                      const double* __restrict__ dp, long n3,
-                                                      double* __restrict__ partial, unsigned* __restrict__ counter) {
+__global__ __launch_bounds__(kBlock) void k_beta_dots(const double* __restrict__ y, const double* __restrict__ ge,
+                                                      const double* __restrict__ gp, const double* __restrict__ dp,
+                                                      long n3, double* __restrict__ part) {
   double v[2] = {0.0, 0.0};
   NODE_LOOP(i, n3) {
     const double dg = ge[i] - gp[i];
     v[0] += y[i] * dg;
     v[1] += dp[i] * dg;
   }
-  reduce_store<2, false>(v, partial, counter, sc + S_NUM);
+  block_partial<2>(v, part);
 }
 
-// d = -y + beta T(m, d_prev); d.g_ext  (src/minimizers.py:997-1012)
-__global__ __launch_bounds__(kBlock) void k_direction(double* __restrict__ sc, int restart, const double* __restrict__ m,
+// d = -y + beta T(m, d_prev) with beta from the k_beta_dots partials (y.dg, d_prev.dg); per-workgroup
+// d.g_ext to part_dg  (src/minimizers.py:997-1012)
+__global__ __launch_bounds__(kBlock) void k_direction(int restart, const double* __restrict__ m,
                                                       const double* __restrict__ y, const double* __restrict__ dp,
                                                       const double* __restrict__ ge, double* __restrict__ d, long N,
-                                                      double* __restrict__ partial, unsigned* __restrict__ counter) {
-  const double beta = restart ? 0.0 : fmax(0.0, sc[S_NUM] / (sc[S_DEN] + 1e-30));
+                                                      const double* __restrict__ part_nd, int n_nd,
+                                                      double* __restrict__ part_dg) {
+  double nd[2];
+  sum_partials_k<2, kBlock>(part_nd, n_nd, nd);
+  const double beta = restart ? 0.0 : fmax(0.0, nd[0] / (nd[1] + 1e-30));
   double v[1] = {0.0};
   NODE_LOOP(i, N) {
     double t[3];
@@ -325,16 +332,17 @@ __global__ __launch_bounds__(kBlock) void k_direction(double* __restrict__ sc, i
       v[0] += dk * ge[3 * i + k];
     }
   }
-  reduce_store<1, false>(v, partial, counter, sc + S_DG);
+  block_partial<1>(v, part_dg);
 }
 
-// if (force || d.g_ext > 0) d = -alt; H = m x d; pg = g_raw . d
-__global__ __launch_bounds__(kBlock) void k_direction_fix(double* __restrict__ sc, int force,
-                                                          const double* __restrict__ m, const double* __restrict__ alt,
-                                                          const double* __restrict__ g, double* __restrict__ d,
-                                                          double* __restrict__ H, long N, double* __restrict__ partial,
-                                                          unsigned* __restrict__ counter) {
-  const bool flip = force || sc[S_DG] > 0.0;
+// if (force || d.g_ext > 0) d = -alt (d.g_ext from the part_dg partials); H = m x d; per-workgroup
+// g_raw . d to part_pg
+__global__ __launch_bounds__(kBlock) void k_direction_fix(int force, const double* __restrict__ m,
+                                                          const double* __restrict__ alt, const double* __restrict__ g,
+                                                          double* __restrict__ d, double* __restrict__ H, long N,
+                                                          const double* __restrict__ part_dg, int n_dg,
+                                                          double* __restrict__ part_pg) {
+  const bool flip = force || sum_partials_all<kBlock>(part_dg, n_dg) > 0.0;
   double v[1] = {0.0};
   NODE_LOOP(i, N) {
     double dd[3];
@@ -349,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_direction_fix(double* __restrict__ s
     H[3 * i + 1] = mi[2] * dd[0] - mi[0] * dd[2];
     H[3 * i + 2] = mi[0] * dd[1] - mi[1] * dd[0];
   }
-  reduce_store<1, false>(v, partial, counter, sc + S_PG);
+  block_partial<1>(v, part_pg);
 }
 
 // Cayley update m_t = cayley(m, H, s) (src/minimizers.py:71-79)
@@ -373,11 +381,10 @@ __global__ void k_cayley(const double* __restrict__ m, const double* __restrict_
 }
 
 __global__ __launch_bounds__(kBlock) void k_maxdiff(const double* __restrict__ a, const double* __restrict__ b,
-                                                    long n, double* __restrict__ partial,
-                                                    unsigned* __restrict__ counter, double* __restrict__ out) {
+                                                    long n, double* __restrict__ part) {
   double v[1] = {0.0};
   NODE_LOOP(i, n) v[0] = fmax(v[0], fabs(a[i] - b[i]));
-  reduce_store<1, true>(v, partial, counter, out);
+  block_partial<1, kBlock, true>(v, part);
 }
 
 // ----------------------------------------------------------------------- L-BFGS kernels
@@ -435,16 +442,15 @@ __global__ void k_lb_second(const double* __restrict__ sc, int idx, double* __re
 }
 
 // d = -r and d.g_ext
-__global__ __launch_bounds__(kBlock) void k_neg_dot(double* __restrict__ sc, const double* __restrict__ r,
-                                                    const double* __restrict__ ge, double* __restrict__ d, long n,
-                                                    double* __restrict__ partial, unsigned* __restrict__ counter) {
+__global__ __launch_bounds__(kBlock) void k_neg_dot(const double* __restrict__ r, const double* __restrict__ ge,
+                                                    double* __restrict__ d, long n, double* __restrict__ part_dg) {
   double v[1] = {0.0};
   NODE_LOOP(i, n) {
     const double dk = -r[i];
     d[i] = dk;
     v[0] += dk * ge[i];
   }
-  reduce_store<1, false>(v, partial, counter, sc + S_DG);
+  block_partial<1>(v, part_dg);
 }
 
 // secant pair at m_new: s = T(m, d tau), y = T(m, g_new) - T(m, g_ext_old); curv = y.s, ss = s.s
@@ -508,6 +514,35 @@ static int read_slots(Ctx& c, int first, int count) {
   return 0;
 }
 
+// Per-workgroup partial banks of the field-step reductions (no fences or counters: a consuming
+// kernel sums a bank in a fixed order, the host reduces the banks it needs after the iteration's
+// synchronisation). At most 2 values per workgroup, at most kRedGridMax workgroups.
+enum RedBank { R_GN = 0, R_GS = 1, R_PG = 2, R_DM = 3, R_ND = 4, R_DG = 5 };
+constexpr int kBankStride = 2 * kRedGridMax;
+static inline double* rbank(const Ctx& c, int r) { return c.mpart + (size_t)r * kBankStride; }
+
+// Copies banks [0, nb) and the scalar slots S_E .. S_PCIT to the host and waits.
+static int fetch_host(Ctx& c, int nb) {
+  MM_HIP(hipMemcpyAsync(c.hmpart, c.mpart, (size_t)nb * kBankStride * sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+  MM_HIP(hipMemcpyAsync(c.hscal + S_E, c.dscal + S_E, (S_PCIT - S_E + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+static double host_bank_max(const Ctx& c, int r, int n) {
+  const double* b = c.hmpart + (size_t)r * kBankStride;
+  double m = 0.0;
+  for (int i = 0; i < n; ++i) m = std::fmax(m, b[i]);
+  return m;
+}
+static double host_bank_sum(const Ctx& c, int r, int n) {
+  const double* b = c.hmpart + (size_t)r * kBankStride;
+  double t = 0.0;
+  for (int i = 0; i < n; ++i) t += b[i];
+  return t;
+}
+
 // Energy and gradient of (m, U) into g (zeroed here); energy left in slot S_E.
 static int eval_energy(Ctx& c, const double* m, const double* U, const double B[3], double* g) {
   vfill(g, 0.0, 3L * c.N, c.stream);
@@ -525,7 +560,8 @@ static int solve_P(Ctx& c, const MinParams& p) {
   double* part_a = c.part_rz;  // p.q partials (the PCG is not running meanwhile)
   double* part_b = c.part_rr;  // r.z partials
   k_pc_init<<<gr, kBlock, 0, s>>>(c.g_ext, c.inv_M_prec, c.y, c.r3, c.z3, c.p3, c.cv3, N, part_b);
-  k_pc_state_init<<<1, kBlock, 0, s>>>(sc, part_b, gr, p.pc_auto, p.pc_force_eta, p.pc_force_alpha, p.pc_iters);
+  k_pc_state_init<<<1, kBlock, 0, s>>>(sc, part_b, gr, rbank(c, R_GN), red_grid(N), p.pc_auto, p.pc_force_eta,
+                                       p.pc_force_alpha, p.pc_iters);
   for (int k = 0; k < p.pc_iters; ++k) {
     double* bank = sc + pc_bank(k);
     MM_TRY(launch_local(c, c.p3, c.cv3, bank + B_ACT, s));
@@ -633,11 +669,11 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
     demag = it;
   }
   MM_TRY(eval_energy(c, c.m, c.U, B, c.g_raw));
-  k_tangents<<<red_grid(N), kBlock, 0, s>>>(c.m, c.g_raw, c.inv_M_rel, c.g_tan, c.g_ext, N, c.partial, cnt, sc + S_GN);
-  MM_TRY(read_slots(c, S_E, 1));
+  const int nN = red_grid(N), n3g = red_grid(n3);  // workgroups (= partials) of the N- and 3N-loops
+  k_tangents<<<nN, kBlock, 0, s>>>(c.m, c.g_raw, c.inv_M_rel, c.g_tan, c.g_ext, N, rbank(c, R_GN));
+  MM_TRY(fetch_host(c, R_GN + 1));
   double E = c.hscal[S_E];
-  MM_TRY(read_slots(c, S_GN, 1));
-  double gnorm = c.hscal[S_GN];
+  double gnorm = host_bank_max(c, R_GN, nN);
   int evals = 1, preco = 0, it = 0;
   bool conv = false;
   const int Lp = p.L > 0 ? p.L : (int)N;
@@ -645,8 +681,7 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
   if (p.method == 0) {
     // PCGState init: g = g_ext, y = g_ext, d = -g_ext
     MM_TRY(vcopy(c.g_prev, c.g_ext, n3, s));
-    hipLaunchKernelGGL(k_neg_dot, dim3(red_grid(n3)), dim3(kBlock), 0, s, sc, c.g_ext, c.g_ext, c.d_prev, n3, c.partial,
-                       cnt);
+    k_neg_dot<<<n3g, kBlock, 0, s>>>(c.g_ext, c.g_ext, c.d_prev, n3, rbank(c, R_DG));
   } else {
     vfill(c.S, 0.0, (long)mem * n3, s);
     vfill(c.Yh, 0.0, (long)mem * n3, s);
@@ -656,16 +691,15 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
   std::vector<double> h_rho(mem, 0.0);
 
   while (!conv && it < p.max_iter) {
-    k_tangents<<<red_grid(N), kBlock, 0, s>>>(c.m, c.g_raw, c.inv_M_rel, c.g_tan, c.g_ext, N, c.partial, cnt,
-                                               sc + S_GN);
+    k_tangents<<<nN, kBlock, 0, s>>>(c.m, c.g_raw, c.inv_M_rel, c.g_tan, c.g_ext, N, rbank(c, R_GN));
     double s0 = 1.0;
     if (p.method == 0) {
       MM_TRY(solve_P(c, p));
-      vmaxabs(c, c.y, n3, sc + S_GS, s);
-      k_beta_dots<<<red_grid(n3), kBlock, 0, s>>>(sc, c.y, c.g_ext, c.g_prev, c.d_prev, n3, c.partial, cnt);
-      k_direction<<<red_grid(N), kBlock, 0, s>>>(sc, (it % Lp) == 0, c.m, c.y, c.d_prev, c.g_ext, c.d, N, c.partial,
-                                                  cnt);
-      k_direction_fix<<<red_grid(N), kBlock, 0, s>>>(sc, 0, c.m, c.y, c.g_raw, c.d, c.H, N, c.partial, cnt);
+      k_maxabs_part<<<n3g, kBlock, 0, s>>>(c.y, n3, rbank(c, R_GS));
+      k_beta_dots<<<n3g, kBlock, 0, s>>>(c.y, c.g_ext, c.g_prev, c.d_prev, n3, rbank(c, R_ND));
+      k_direction<<<nN, kBlock, 0, s>>>((it % Lp) == 0, c.m, c.y, c.d_prev, c.g_ext, c.d, N, rbank(c, R_ND), n3g,
+                                         rbank(c, R_DG));
+      k_direction_fix<<<nN, kBlock, 0, s>>>(0, c.m, c.y, c.g_raw, c.d, c.H, N, rbank(c, R_DG), nN, rbank(c, R_PG));
     } else {
       // two-loop recursion on q = g_ext (src/minimizers.py:1284-1312); q in r3, r in z3
       MM_TRY(vcopy(c.r3, c.g_ext, n3, s));
@@ -686,14 +720,15 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
         k_dot_slot<<<red_grid(n3), kBlock, 0, s>>>(sc, S_TMP, c.Yh + (long)idx * n3, c.z3, n3, c.partial, cnt);
         k_lb_second<<<vg(n3), kBlock, 0, s>>>(sc, idx, c.z3, c.S + (long)idx * n3, n3);
       }
-      k_neg_dot<<<red_grid(n3), kBlock, 0, s>>>(sc, c.z3, c.g_ext, c.d, n3, c.partial, cnt);
-      k_direction_fix<<<red_grid(N), kBlock, 0, s>>>(sc, it == 0, c.m, c.g_tan, c.g_raw, c.d, c.H, N, c.partial, cnt);
+      k_neg_dot<<<n3g, kBlock, 0, s>>>(c.z3, c.g_ext, c.d, n3, rbank(c, R_DG));
+      k_direction_fix<<<nN, kBlock, 0, s>>>(it == 0, c.m, c.g_tan, c.g_raw, c.d, c.H, N, rbank(c, R_DG), n3g,
+                                             rbank(c, R_PG));
       s0 = (it == 0) ? p.tau0 : 1.0;
     }
-    MM_TRY(read_slots(c, S_GN, 8));  // GN, GS, NUM, DEN, DG, PG, DM, PCIT
-    const double gn_tan = c.hscal[S_GN];
-    const double gs = c.hscal[S_GS];
-    const double pg = c.hscal[S_PG];
+    MM_TRY(fetch_host(c, R_PG + 1));  // banks GN, GS, PG and the slots (PCIT)
+    const double gn_tan = host_bank_max(c, R_GN, nN);
+    const double gs = (p.method == 0) ? host_bank_max(c, R_GS, n3g) : 0.0;
+    const double pg = host_bank_sum(c, R_PG, nN);
     const int pit = (p.method == 0) ? (int)c.hscal[S_PCIT] : 0;
     const double gconv = (p.method == 0) ? gs : gn_tan;
 
@@ -701,9 +736,9 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
     MM_TRY(line_search(c, p, B, pg, E, s0, ls));
     double dm = 0.0;
     if (ls.moved) {
-      k_maxdiff<<<red_grid(n3), kBlock, 0, s>>>(c.m_t, c.m, n3, c.partial, cnt, sc + S_DM);
-      MM_TRY(read_slots(c, S_DM, 1));
-      dm = c.hscal[S_DM];
+      k_maxdiff<<<n3g, kBlock, 0, s>>>(c.m_t, c.m, n3, rbank(c, R_DM));
+      MM_TRY(fetch_host(c, R_DM + 1));
+      dm = host_bank_max(c, R_DM, n3g);
     }
     const double E_new = ls.E;
     // check_convergence (src/minimizers.py:160-172)
