@@ -663,8 +663,6 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   // reductions and scalars
   const int max_grid = std::max({kRedBlocks, 8 * ((c.part_all.nblk + 7) / 8), 8 * ((c.part_mag.nblk + 7) / 8)});
   MM_TRY(dalloc(c, &c.partial, 4 * (size_t)max_grid));
-  MM_TRY(dalloc(c, &c.counter, 16));
-  MM_HIP(hipMemset(c.counter, 0, 16 * sizeof(unsigned)));
   MM_TRY(dalloc(c, &c.dscal, kNumScal));
   MM_HIP(hipMemset(c.dscal, 0, kNumScal * sizeof(double)));
   MM_HIP(hipHostMalloc((void**)&c.hscal, kNumScal * sizeof(double), hipHostMallocDefault));
@@ -707,8 +705,8 @@ int alloc_min_state(Ctx& c) {
     MM_TRY(dalloc(c, v, n3));
   MM_TRY(dalloc(c, &c.U, n));
   MM_TRY(dalloc(c, &c.U_t, n));
-  MM_TRY(dalloc(c, &c.mpart, 6 * 2 * (size_t)kRedGridMax));
-  MM_HIP(hipHostMalloc((void**)&c.hmpart, 6 * 2 * kRedGridMax * sizeof(double), hipHostMallocDefault));
+  MM_TRY(dalloc(c, &c.mpart, 9 * 2 * (size_t)kRedGridMax));
+  MM_HIP(hipHostMalloc((void**)&c.hmpart, 9 * 2 * kRedGridMax * sizeof(double), hipHostMallocDefault));
   return 0;
 }
 

@@ -1,5 +1,5 @@
 // Device helpers shared by the kernel translation units: block lookup, tet geometry,
-// LDS/global scatter, and the deterministic last-block reduction.
+// LDS/global scatter, and the fence-free per-workgroup reductions and their consumers.
 #pragma once
 
 #include "mm_kernels.h"
@@ -177,84 +177,6 @@ __device__ void block_partial_cs(CSum v, double* __restrict__ partial, int idx) 
   }
 }
 
-// Returns true in exactly one workgroup: the last one to arrive. Every workgroup of the
-// grid must call it. The counter is reset by the last workgroup.
-__device__ __forceinline__ bool last_block(unsigned* counter) {
-  __shared__ int is_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
-    is_last = (t == gridDim.x - 1);
-    if (is_last) *counter = 0u;
-  }
-  __syncthreads();
-  if (is_last) __threadfence();
-  return is_last;
-}
-
-// Block-reduce K values into per-workgroup partials; the last workgroup reduces all
-// partials in a fixed order (deterministic for a fixed grid) and returns true with the
-// totals in v[] (valid in thread 0).
-template <int K, bool MAX>
-__device__ bool reduce_last(double (&v)[K], double* __restrict__ partial, unsigned* __restrict__ counter) {
-  __shared__ double sh[K][kBlock / 64];
-  __shared__ int is_last;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-#pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = wave_red<MAX>(v[k]);
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < K; ++k) sh[k][wv] = v[k];
-  __syncthreads();
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double s = sh[k][0];
-      for (int w = 1; w < kBlock / 64; ++w) s = MAX ? fmax(s, sh[k][w]) : s + sh[k][w];
-      partial[blockIdx.x * K + k] = s;
-    }
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-    for (int i0 = 0; i0 < (int)gridDim.x; i0 += 4 * kBlock) {
-      double q[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = i0 + u * kBlock + tid;
-        q[u] = i < (int)gridDim.x
-                   ? __hip_atomic_load(&partial[i * K + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                   : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) s = MAX ? fmax(s, q[u]) : s + q[u];
-    }
-    v[k] = wave_red<MAX>(s);
-  }
-  __syncthreads();
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < K; ++k) sh[k][wv] = v[k];
-  __syncthreads();
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double s = sh[k][0];
-      for (int w = 1; w < kBlock / 64; ++w) s = MAX ? fmax(s, sh[k][w]) : s + sh[k][w];
-      v[k] = s;
-    }
-    *counter = 0u;
-  }
-  return true;
-}
-
 // Block sum (or maximum) of K values written to partial[idx * K + k], idx = blockIdx.x by
 // default (no cross-block traffic); sum_partials then reduces them in a fixed order.
 template <int K, int NT = kBlock, bool MAX = false>
@@ -402,18 +324,8 @@ __device__ void block_sum_to(double v, double* __restrict__ out) {
   }
 }
 
-// reduce_last + store of the totals to out[0..K-1]
-template <int K, bool MAX>
-__device__ void reduce_store(double (&v)[K], double* __restrict__ partial, unsigned* __restrict__ counter,
-                             double* __restrict__ out) {
-  if (reduce_last<K, MAX>(v, partial, counter) && threadIdx.x == 0)
-#pragma unroll
-    for (int k = 0; k < K; ++k) out[k] = v[k];
-}
-
-// Grid size for grid-stride vector kernels with reductions. Every workgroup of a reducing
-// kernel ends with a fence and one atomic on a shared counter; those serialise (~20 ns each),
-// so the grid stays at <= 256 workgroups (one per CU) and each thread loops over more items.
+// Grid size for grid-stride vector kernels with reductions: at most 256 workgroups (one per CU),
+// so a consumer sums at most 256 partials per value and each thread loops over more items.
 constexpr int kRedGridMax = 256;
 inline int red_grid(long n) {
   long g = (n + kBlock - 1) / kBlock;

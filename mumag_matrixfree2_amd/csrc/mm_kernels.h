@@ -182,7 +182,6 @@ struct Ctx {
 
   // reductions / scalars
   double* partial = nullptr;          // kRedBlocks x 4
-  unsigned* counter = nullptr;        // 16 last-block counters
   double* dscal = nullptr;            // kNumScal device scalars
   double* hscal = nullptr;            // pinned host mirror
   // fence-free PCG reductions: per-workgroup partials consumed by the next kernel or the host
@@ -210,7 +209,7 @@ struct Ctx {
   double *r3 = nullptr, *z3 = nullptr, *p3 = nullptr, *q3 = nullptr, *cv3 = nullptr;
   int mem = 10;
   double *S = nullptr, *Yh = nullptr;  // mem x 3N
-  // per-workgroup partials of the field-step reductions (6 banks of 2 x kRedGridMax) and a pinned
+  // per-workgroup partials of the field-step reductions (9 banks of 2 x kRedGridMax) and a pinned
   // host copy of the first banks
   double *mpart = nullptr, *hmpart = nullptr;
   bool have_state = false;
@@ -289,8 +288,6 @@ int launch_jpar(const Ctx& c, const double* m, const double h[3], double* out, h
 int vfill(double* x, double v, long n, hipStream_t s);
 int vcopy(double* dst, const double* src, long n, hipStream_t s);
 // out[0] = a.b (deterministic)
-int vdot(const Ctx& c, const double* a, const double* b, long n, double* out, hipStream_t s);
-int vmaxabs(const Ctx& c, const double* a, long n, double* out, hipStream_t s);
 
 // solver (mm_solver.hip)
 // z = M^-1 r; rz_part receives red_grid(N) partials of r.z

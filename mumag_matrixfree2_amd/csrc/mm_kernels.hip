@@ -559,8 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(Part p, Geo geo, const double* _
 // partials, reduced by k_sum_partials_cs). Output g and E are divided by V_mag.
 __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double* __restrict__ m,
                                                    const double* __restrict__ U, double Bx, double By, double Bz,
-                                                   double inv_V, double* __restrict__ g, double* __restrict__ partial,
-                                                   unsigned* __restrict__ counter, double* __restrict__ E_out) {
+                                                   double inv_V, double* __restrict__ g, double* __restrict__ partial) {
   __shared__ double sm[3][kMaxBlockNodes];
   __shared__ double sX[3][kMaxBlockNodes];
   __shared__ double su[kMaxBlockNodes];
@@ -896,8 +895,7 @@ __global__ __launch_bounds__(kBlock) void k_diag(Part p, Geo geo, double reg, do
 // ---------------------------------------------------------- volume-averaged J along h
 // out[0] = sum_e Ve Js_e mean_a(m_a).h, out[1] = sum_{Js_e > 0} Ve (``src/hysteresis_loop.py:153-190``).
 __global__ __launch_bounds__(kBlock) void k_jpar(Part p, Geo geo, const double* __restrict__ m, double hx, double hy,
-                                                 double hz, double* __restrict__ partial, unsigned* __restrict__ counter,
-                                                 double* __restrict__ out) {
+                                                 double hz, double* __restrict__ partial) {
   __shared__ double sX[3][kMaxBlockNodes];
   __shared__ double sh[kMaxBlockNodes];
   const BlkInfo bi = blk_info(p);
@@ -1050,8 +1048,7 @@ int launch_energy(const Ctx& c, const double* m, const double* U, const double B
   const Part& p = c.part_mag;
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
-  k_energy<<<grid, kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, c.partial, c.counter + 1,
-                                   E_out);
+  k_energy<<<grid, kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, c.partial);
   k_sum_partials_cs<<<1, kBlock, 0, s>>>(c.partial, p.nblk, E_out);  // partials by block id
   return 0;
 }
@@ -1095,7 +1092,7 @@ int launch_jpar(const Ctx& c, const double* m, const double h[3], double* out, h
   const Part& p = c.part_mag;
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
-  k_jpar<<<grid, kBlock, 0, s>>>(p, c.geo, m, h[0], h[1], h[2], c.partial, c.counter + 2, out);
+  k_jpar<<<grid, kBlock, 0, s>>>(p, c.geo, m, h[0], h[1], h[2], c.partial);
   k_sum_partials<2><<<1, kBlock, 0, s>>>(c.partial, p.nblk, out);  // partials by block id
   return 0;
 }

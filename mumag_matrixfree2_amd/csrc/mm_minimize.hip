@@ -26,7 +26,6 @@ enum Slot {
   S_PCIT = 31, // inner iterations (copy for the host)
   S_RHO = 32, S_Q = 33, S_IT = 34, S_DONE = 35, S_ACT = 36, S_ALPHA = 37, S_TGT = 38, S_DQ = 39, S_DN = 40,
   S_PAP = 41, S_RHON = 42, S_YY = 43, S_ZZ = 44, S_YG = 45,
-  S_SY = 48, S_YMY = 49, S_CURV = 50, S_SS = 51, S_TMP = 52,
   S_ALPHAS = 64,  // + history slot
   S_RHOS = 96,    // + history slot
 };
@@ -388,28 +387,27 @@ __global__ __launch_bounds__(kBlock) void k_maxdiff(const double* __restrict__ a
 }
 
 // ----------------------------------------------------------------------- L-BFGS kernels
-// out[slot] = a.b (generic 1-value dot into a scalar slot)
-__global__ __launch_bounds__(kBlock) void k_dot_slot(double* __restrict__ sc, int slot, const double* __restrict__ a,
-                                                     const double* __restrict__ b, long n, double* __restrict__ partial,
-                                                     unsigned* __restrict__ counter) {
+// per-workgroup partials of a.b
+__global__ __launch_bounds__(kBlock) void k_lb_dot(const double* __restrict__ a, const double* __restrict__ b, long n,
+                                                   double* __restrict__ part) {
   double v[1] = {0.0};
   NODE_LOOP(i, n) v[0] += a[i] * b[i];
-  reduce_store<1, false>(v, partial, counter, sc + slot);
+  block_partial<1>(v, part);
 }
 
-// first loop: alpha_idx = rho_idx * sc[S_TMP]; q -= alpha_idx Y_idx
-__global__ void k_lb_first(double* __restrict__ sc, int idx, double* __restrict__ q, const double* __restrict__ Yi,
-                           long n) {
-  const double alpha = sc[S_RHOS + idx] * sc[S_TMP];
+// first loop: alpha_idx = rho_idx * (S_idx . q) from the k_lb_dot partials; q -= alpha_idx Y_idx
+__global__ __launch_bounds__(kBlock) void k_lb_first(double* __restrict__ sc, int idx, double* __restrict__ q,
+                                                     const double* __restrict__ Yi, long n,
+                                                     const double* __restrict__ part, int n_part) {
+  const double alpha = sc[S_RHOS + idx] * sum_partials_all<kBlock>(part, n_part);
   NODE_LOOP(i, n) q[i] -= alpha * Yi[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_ALPHAS + idx] = alpha;
 }
 
-// y.y_invM for gamma: S_last.Y_last and Y_last.(Y_last invMr)
-__global__ __launch_bounds__(kBlock) void k_lb_gamma_dots(double* __restrict__ sc, const double* __restrict__ S,
-                                                          const double* __restrict__ Y, const double* __restrict__ invMr,
-                                                          long N, double* __restrict__ partial,
-                                                          unsigned* __restrict__ counter) {
+// per-workgroup partials of the gamma dots S_last.Y_last and Y_last.(Y_last invMr)
+__global__ __launch_bounds__(kBlock) void k_lb_gamma_dots(const double* __restrict__ S, const double* __restrict__ Y,
+                                                          const double* __restrict__ invMr, long N,
+                                                          double* __restrict__ part) {
   double v[2] = {0.0, 0.0};
   NODE_LOOP(i, N) {
     const double s = invMr[i];
@@ -419,13 +417,17 @@ __global__ __launch_bounds__(kBlock) void k_lb_gamma_dots(double* __restrict__ s
       v[1] += Y[3 * i + k] * Y[3 * i + k] * s;
     }
   }
-  reduce_store<2, false>(v, partial, counter, sc + S_SY);
+  block_partial<2>(v, part);
 }
 
-// r = gamma q invMr, gamma = clip(sy/(ymy + 1e-30), 1e-3, 1e3) or 1 without history
-__global__ void k_lb_scale(const double* __restrict__ sc, int have_hist, const double* __restrict__ q,
-                           const double* __restrict__ invMr, double* __restrict__ r, long N) {
-  double gam = have_hist ? sc[S_SY] / (sc[S_YMY] + 1e-30) : 1.0;
+// r = gamma q invMr, gamma = clip(sy/(ymy + 1e-30), 1e-3, 1e3) from the k_lb_gamma_dots partials, or 1
+// without history
+__global__ __launch_bounds__(kBlock) void k_lb_scale(int have_hist, const double* __restrict__ q,
+                                                     const double* __restrict__ invMr, double* __restrict__ r, long N,
+                                                     const double* __restrict__ part, int n_part) {
+  double sy[2] = {0.0, 1.0};
+  if (have_hist) sum_partials_k<2, kBlock>(part, n_part, sy);  // uniform per launch
+  double gam = have_hist ? sy[0] / (sy[1] + 1e-30) : 1.0;
   gam = fmin(fmax(gam, 1e-3), 1e3);
   NODE_LOOP(i, N) {
     const double s = gam * invMr[i];
@@ -434,10 +436,11 @@ __global__ void k_lb_scale(const double* __restrict__ sc, int have_hist, const d
   }
 }
 
-// second loop: beta = rho_idx * sc[S_TMP]; r += S_idx (alpha_idx - beta)
-__global__ void k_lb_second(const double* __restrict__ sc, int idx, double* __restrict__ r,
-                            const double* __restrict__ Si, long n) {
-  const double f = sc[S_ALPHAS + idx] - sc[S_RHOS + idx] * sc[S_TMP];
+// second loop: beta = rho_idx * (Y_idx . r) from the k_lb_dot partials; r += S_idx (alpha_idx - beta)
+__global__ __launch_bounds__(kBlock) void k_lb_second(const double* __restrict__ sc, int idx, double* __restrict__ r,
+                                                      const double* __restrict__ Si, long n,
+                                                      const double* __restrict__ part, int n_part) {
+  const double f = sc[S_ALPHAS + idx] - sc[S_RHOS + idx] * sum_partials_all<kBlock>(part, n_part);
   NODE_LOOP(i, n) r[i] += Si[i] * f;
 }
 
@@ -454,11 +457,10 @@ __global__ __launch_bounds__(kBlock) void k_neg_dot(const double* __restrict__ r
 }
 
 // secant pair at m_new: s = T(m, d tau), y = T(m, g_new) - T(m, g_ext_old); curv = y.s, ss = s.s
-__global__ __launch_bounds__(kBlock) void k_lb_pair(double* __restrict__ sc, const double* __restrict__ m,
-                                                    const double* __restrict__ d, double tau,
-                                                    const double* __restrict__ gn, const double* __restrict__ ge,
-                                                    double* __restrict__ s_out, double* __restrict__ y_out, long N,
-                                                    double* __restrict__ partial, unsigned* __restrict__ counter) {
+__global__ __launch_bounds__(kBlock) void k_lb_pair(const double* __restrict__ m, const double* __restrict__ d,
+                                                    double tau, const double* __restrict__ gn,
+                                                    const double* __restrict__ ge, double* __restrict__ s_out,
+                                                    double* __restrict__ y_out, long N, double* __restrict__ part) {
   double v[2] = {0.0, 0.0};
   NODE_LOOP(i, N) {
     double s3[3], a[3], b[3];
@@ -477,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_lb_pair(double* __restrict__ sc, con
       v[1] += s3[k] * s3[k];
     }
   }
-  reduce_store<2, false>(v, partial, counter, sc + S_CURV);
+  block_partial<2>(v, part);
 }
 
 // transport every history vector to the tangent space at m (src/minimizers.py:1342-1347)
@@ -503,11 +505,6 @@ __global__ void k_lb_transport(const double* __restrict__ m, double* __restrict_
 __global__ void k_set_slot(double* __restrict__ sc, int slot, double v) { sc[slot] = v; }
 
 // ------------------------------------------------------------------------- host driver
-struct Red {
-  double* partial;
-  unsigned* counter;
-};
-
 static int read_slots(Ctx& c, int first, int count) {
   MM_HIP(hipMemcpyAsync(c.hscal + first, c.dscal + first, count * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   MM_HIP(hipStreamSynchronize(c.stream));
@@ -517,7 +514,7 @@ static int read_slots(Ctx& c, int first, int count) {
 // Per-workgroup partial banks of the field-step reductions (no fences or counters: a consuming
 // kernel sums a bank in a fixed order, the host reduces the banks it needs after the iteration's
 // synchronisation). At most 2 values per workgroup, at most kRedGridMax workgroups.
-enum RedBank { R_GN = 0, R_GS = 1, R_PG = 2, R_DM = 3, R_ND = 4, R_DG = 5 };
+enum RedBank { R_GN = 0, R_GS = 1, R_PG = 2, R_DM = 3, R_CURV = 4, R_ND = 5, R_DG = 6, R_TMP = 7, R_SY = 8 };
 constexpr int kBankStride = 2 * kRedGridMax;
 static inline double* rbank(const Ctx& c, int r) { return c.mpart + (size_t)r * kBankStride; }
 
@@ -645,7 +642,6 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
   const long N = c.N, n3 = 3 * N;
   hipStream_t s = c.stream;
   double* sc = c.dscal;
-  unsigned* cnt = c.counter + 9;
   const int mem = p.memory > 0 ? p.memory : 10;
   if (p.method == 1 && (!c.S || c.mem != mem)) {
     c.mem = mem;
@@ -706,19 +702,19 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
       const int nh = std::min(hist, mem);
       for (int i = 0; i < nh; ++i) {
         const int idx = ((hist - 1 - i) % mem + mem) % mem;
-        k_dot_slot<<<red_grid(n3), kBlock, 0, s>>>(sc, S_TMP, c.S + (long)idx * n3, c.r3, n3, c.partial, cnt);
-        k_lb_first<<<vg(n3), kBlock, 0, s>>>(sc, idx, c.r3, c.Yh + (long)idx * n3, n3);
+        k_lb_dot<<<n3g, kBlock, 0, s>>>(c.S + (long)idx * n3, c.r3, n3, rbank(c, R_TMP));
+        k_lb_first<<<n3g, kBlock, 0, s>>>(sc, idx, c.r3, c.Yh + (long)idx * n3, n3, rbank(c, R_TMP), n3g);
       }
       if (hist > 0) {
         const int last = ((hist - 1) % mem + mem) % mem;
-        k_lb_gamma_dots<<<red_grid(N), kBlock, 0, s>>>(sc, c.S + (long)last * n3, c.Yh + (long)last * n3, c.inv_M_rel,
-                                                        N, c.partial, cnt);
+        k_lb_gamma_dots<<<nN, kBlock, 0, s>>>(c.S + (long)last * n3, c.Yh + (long)last * n3, c.inv_M_rel, N,
+                                               rbank(c, R_SY));
       }
-      k_lb_scale<<<vg(N), kBlock, 0, s>>>(sc, hist > 0, c.r3, c.inv_M_rel, c.z3, N);
+      k_lb_scale<<<nN, kBlock, 0, s>>>(hist > 0, c.r3, c.inv_M_rel, c.z3, N, rbank(c, R_SY), nN);
       for (int i = 0; i < nh; ++i) {
         const int idx = ((hist - nh + i) % mem + mem) % mem;
-        k_dot_slot<<<red_grid(n3), kBlock, 0, s>>>(sc, S_TMP, c.Yh + (long)idx * n3, c.z3, n3, c.partial, cnt);
-        k_lb_second<<<vg(n3), kBlock, 0, s>>>(sc, idx, c.z3, c.S + (long)idx * n3, n3);
+        k_lb_dot<<<n3g, kBlock, 0, s>>>(c.Yh + (long)idx * n3, c.z3, n3, rbank(c, R_TMP));
+        k_lb_second<<<n3g, kBlock, 0, s>>>(sc, idx, c.z3, c.S + (long)idx * n3, n3, rbank(c, R_TMP), n3g);
       }
       k_neg_dot<<<n3g, kBlock, 0, s>>>(c.z3, c.g_ext, c.d, n3, rbank(c, R_DG));
       k_direction_fix<<<nN, kBlock, 0, s>>>(it == 0, c.m, c.g_tan, c.g_raw, c.d, c.H, N, rbank(c, R_DG), n3g,
@@ -760,9 +756,13 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
       gnorm = gs;
     } else {
       // secant pair, transport of the history, conditional insertion (src/minimizers.py:1330-1360)
-      k_lb_pair<<<red_grid(N), kBlock, 0, s>>>(sc, c.m, c.d, ls.tau, c.g_raw, c.g_ext, c.q3, c.cv3, N, c.partial, cnt);
-      MM_TRY(read_slots(c, S_CURV, 2));
-      const double curv = c.hscal[S_CURV], ss = c.hscal[S_SS];
+      k_lb_pair<<<nN, kBlock, 0, s>>>(c.m, c.d, ls.tau, c.g_raw, c.g_ext, c.q3, c.cv3, N, rbank(c, R_CURV));
+      MM_TRY(fetch_host(c, R_CURV + 1));
+      double curv = 0.0, ss = 0.0;
+      for (int w = 0; w < nN; ++w) {  // fixed order
+        curv += c.hmpart[(size_t)R_CURV * kBankStride + 2 * w];
+        ss += c.hmpart[(size_t)R_CURV * kBankStride + 2 * w + 1];
+      }
       const bool ok = curv > 1e-12 * ss;
       const int idx = hist % mem;
       k_lb_transport<<<vg(N), kBlock, 0, s>>>(c.m, c.S, c.Yh, mem, N);

@@ -28,21 +28,6 @@ __global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, lon
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) d[i] = s[i];
 }
 
-__global__ __launch_bounds__(kBlock) void k_dot(const double* __restrict__ a, const double* __restrict__ b, long n,
-                                                double* __restrict__ partial, unsigned* __restrict__ counter,
-                                                double* __restrict__ out) {
-  double v[1] = {0.0};
-  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) v[0] += a[i] * b[i];
-  reduce_store<1, false>(v, partial, counter, out);
-}
-
-__global__ __launch_bounds__(kBlock) void k_maxabs(const double* __restrict__ a, long n, double* __restrict__ partial,
-                                                   unsigned* __restrict__ counter, double* __restrict__ out) {
-  double v[1] = {0.0};
-  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) v[0] = fmax(v[0], fabs(a[i]));
-  reduce_store<1, true>(v, partial, counter, out);
-}
-
 // per-workgroup partials of a.b (no cross-workgroup traffic)
 __global__ __launch_bounds__(kBlock) void k_dot_part(const double* __restrict__ a, const double* __restrict__ b, long n,
                                                      double* __restrict__ part) {
@@ -63,14 +48,6 @@ int vfill(double* x, double v, long n, hipStream_t s) {
 }
 int vcopy(double* d, const double* src, long n, hipStream_t s) {
   if (n > 0) k_copy<<<vgrid(n), kBlock, 0, s>>>(d, src, n);
-  return 0;
-}
-int vdot(const Ctx& c, const double* a, const double* b, long n, double* out, hipStream_t s) {
-  k_dot<<<red_grid(n), kBlock, 0, s>>>(a, b, n, c.partial, c.counter + 3, out);
-  return 0;
-}
-int vmaxabs(const Ctx& c, const double* a, long n, double* out, hipStream_t s) {
-  k_maxabs<<<red_grid(n), kBlock, 0, s>>>(a, n, c.partial, c.counter + 3, out);
   return 0;
 }
 
