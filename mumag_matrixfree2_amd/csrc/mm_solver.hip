@@ -136,48 +136,70 @@ __global__ void k_post(double* __restrict__ x, const double* __restrict__ M, con
 
 constexpr int kUnroll = 4;  // entries per lane whose loads are in flight together
 
+// Products of up to kUnroll strided entries of one row: the index and value loads are issued
+// together, then the gathers of x together, so a lane waits for two round trips per kUnroll
+// entries. No branch per entry: a padding entry re-reads the row's last entry (j1 - 1 >= jb is a
+// valid entry inside the loop) and contributes an exact zero through the select. MODE as in
+// k_spmv.
+template <int MODE, int LANES, typename TA, typename TX, typename TX2>
+__device__ __forceinline__ TA row_products(int j0, int j1, int lane, const int* __restrict__ Ai,
+                                           const TA* __restrict__ Ax, const TX* __restrict__ x,
+                                           const TX2* __restrict__ x2) {
+  TA s = 0;
+  for (int jb = j0 + lane; jb < j1; jb += LANES * kUnroll) {
+    int cu[kUnroll];
+    TA au[kUnroll];
+    bool ok[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int j = jb + u * LANES;
+      ok[u] = j < j1;
+      const int jj = ok[u] ? j : j1 - 1;
+      cu[u] = Ai[jj];
+      au[u] = Ax[jj];
+    }
+    TA xv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int c = cu[u];
+      xv[u] = (MODE == 2) ? (TA)((TA)x[c] - (TA)x2[c]) : (TA)x[c];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) s += ok[u] ? au[u] * xv[u] : (TA)0;
+  }
+  return s;
+}
+
 // CSR SpMV with LANES lanes per row, accumulation in the matrix value type TA.
-// MODE 0: y = A x; 1: y += A x (and z = 0 on the same rows when z != nullptr); 2: y = A (x - x2);
-// 3: y = A (x2 .* x) and z = x2 .* x on the same rows (pre-smoothing from a zero guess, fused)
+// MODE 0: y = A x; 1: y += A x (and z = 0 on the same rows when z != nullptr); 2: y = A (x - x2)
+// and, when z != nullptr, z = zw .* y on the same rows (a restriction that also forms the next
+// level's pre-smoothed guess M b). The row's own entries of the epilogue are loaded first, so
+// their latency overlaps the products.
 template <int MODE, int LANES, typename TA, typename TX, typename TX2, typename TY>
 __global__ __launch_bounds__(kBlock) void k_spmv(int row0, int nrows, const int* __restrict__ Ap,
                                                  const int* __restrict__ Ai, const TA* __restrict__ Ax,
                                                  const TX* __restrict__ x, const TX2* __restrict__ x2,
-                                                 TY* __restrict__ y, TY* __restrict__ z) {
+                                                 TY* __restrict__ y, TY* __restrict__ z, const TY* __restrict__ zw) {
   const int lane = threadIdx.x & (LANES - 1);
   const int row = row0 + (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
-  TA s = 0;
-  if (row < nrows) {
-    const int j0 = Ap[row], j1 = Ap[row + 1];
-    // kUnroll strided entries per lane: their index and value loads are all issued before the
-    // dependent gathers of x, so a lane waits for one round trip per kUnroll entries
-    for (int jb = j0 + lane; jb < j1; jb += LANES * kUnroll) {
-      int cu[kUnroll];
-      TA au[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int j = jb + u * LANES;
-        cu[u] = j < j1 ? Ai[j] : -1;
-        au[u] = j < j1 ? Ax[j] : (TA)0;
-      }
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        if (cu[u] < 0) continue;
-        const int c = cu[u];
-        const TA xv = (MODE == 2) ? (TA)((TA)x[c] - (TA)x2[c]) : (MODE == 3) ? (TA)((TA)x2[c] * (TA)x[c]) : (TA)x[c];
-        s += au[u] * xv;
-      }
-    }
+  const bool live = row < nrows;
+  TY y_old = 0, w = 0;
+  if (live && lane == 0) {
+    if (MODE == 1) y_old = y[row];
+    if (MODE == 2 && z) w = zw[row];
   }
+  TA s = 0;
+  if (live) s = row_products<MODE, LANES>(Ap[row], Ap[row + 1], lane, Ai, Ax, x, x2);
 #pragma unroll
   for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LANES);
-  if (row < nrows && lane == 0) {
+  if (live && lane == 0) {
     if (MODE == 1) {
-      y[row] += (TY)s;
+      y[row] = y_old + (TY)s;
       if (z) z[row] = 0;
     } else {
-      y[row] = (TY)s;
-      if (MODE == 3) z[row] = (TY)((TA)x2[row] * (TA)x[row]);
+      const TY yv = (TY)s;
+      y[row] = yv;
+      if (MODE == 2 && z) z[row] = w * yv;
     }
   }
 }
@@ -190,26 +212,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv_post(int row0, int nrows, const
                                                       const T* __restrict__ b, T* __restrict__ xs) {
   const int lane = threadIdx.x & (LANES - 1);
   const int row = row0 + (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
-  T s = 0;
-  if (row < nrows) {
-    const int j0 = Ap[row], j1 = Ap[row + 1];
-    for (int jb = j0 + lane; jb < j1; jb += LANES * kUnroll) {
-      int cu[kUnroll];
-      T au[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int j = jb + u * LANES;
-        cu[u] = j < j1 ? Ai[j] : -1;
-        au[u] = j < j1 ? Ax[j] : (T)0;
-      }
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u)
-        if (cu[u] >= 0) s += au[u] * x[cu[u]];
-    }
+  const bool live = row < nrows;
+  T xr = 0, mr = 0, br = 0;
+  if (live && lane == 0) {
+    xr = x[row];
+    mr = M[row];
+    br = b[row];
   }
+  T s = 0;
+  if (live) s = row_products<0, LANES>(Ap[row], Ap[row + 1], lane, Ai, Ax, x, (const T*)nullptr);
 #pragma unroll
   for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LANES);
-  if (row < nrows && lane == 0) xs[row] = x[row] + M[row] * (b[row] - s);
+  if (live && lane == 0) xs[row] = xr + mr * (br - s);
 }
 
 // y = A b for the symmetric coarsest-level inverse stored as its lower-triangle 64 x 64 tiles
@@ -302,14 +316,14 @@ static void symv(const Level& L, const T* tiles, const T* b, T* y, hipStream_t s
 
 template <int MODE, typename TA, typename TX, typename TX2, typename TY>
 static void spmv_lanes(int lanes, int row0, int row1, const int* Ap, const int* Ai, const TA* Ax, const TX* x,
-                       const TX2* x2, TY* y, TY* z, hipStream_t s) {
+                       const TX2* x2, TY* y, TY* z, const TY* zw, hipStream_t s) {
   const int grid = (int)(((long)(row1 - row0) * lanes + kBlock - 1) / kBlock);
   switch (lanes) {
-    case 4: k_spmv<MODE, 4, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z); break;
-    case 8: k_spmv<MODE, 8, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z); break;
-    case 16: k_spmv<MODE, 16, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z); break;
-    case 32: k_spmv<MODE, 32, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z); break;
-    default: k_spmv<MODE, 64, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z); break;
+    case 4: k_spmv<MODE, 4, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    case 8: k_spmv<MODE, 8, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    case 16: k_spmv<MODE, 16, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    case 32: k_spmv<MODE, 32, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    default: k_spmv<MODE, 64, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
   }
 }
 
@@ -324,19 +338,18 @@ static inline void for_runs(const Runs* runs, int nrows, const std::function<voi
     if (r.second > r.first) f(r.first, r.second);
 }
 
-// y = A x (mode 0), y += A x and z = 0 (mode 1), y = A (x - x2) (mode 2)
+// y = A x (mode 0), y += A x and z = 0 (mode 1), y = A (x - x2) and z = zw .* y (mode 2; z optional)
 template <typename TA, typename TX, typename TX2, typename TY>
 static void spmv(int mode, int lanes, int nrows, const int* Ap, const int* Ai, const TA* Ax, const TX* x,
-                 const TX2* x2, TY* y, hipStream_t s, TY* z = nullptr, const Runs* runs = nullptr) {
+                 const TX2* x2, TY* y, hipStream_t s, TY* z = nullptr, const Runs* runs = nullptr,
+                 const TY* zw = nullptr) {
   for_runs(runs, nrows, [&](int r0, int r1) {
     if (mode == 0)
-      spmv_lanes<0>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, s);
+      spmv_lanes<0>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, (const TY*)nullptr, s);
     else if (mode == 1)
-      spmv_lanes<1>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, z, s);
-    else if (mode == 2)
-      spmv_lanes<2>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, s);
+      spmv_lanes<1>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, z, (const TY*)nullptr, s);
     else
-      spmv_lanes<3>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, z, s);
+      spmv_lanes<2>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, zw ? z : (TY*)nullptr, zw, s);
   });
 }
 
@@ -823,8 +836,8 @@ int solve_U_seg_dev(Ctx& c, const double* m, double* U, const double* tol, const
 
 // ----------------------------------------------------------------------------- V-cycle
 // Coarse part of the V-cycle on levels l >= 1; the correction is returned in lv[l].xs.
-// Pre-smoothing from a zero guess is fused into the first SpMV (t = A (M b), x = M b) and the
-// post-smoothing x + M (b - A x) is one SpMV into xs.
+// Pre-smoothing from a zero guess, x = M b, comes with b from the restriction that formed b;
+// the residual SpMV t = A x follows and the post-smoothing x + M (b - A x) is one SpMV into xs.
 static void vcycle_coarse(Ctx& c, int l, hipStream_t s) {
   Level& L = c.lv[l];
   const int last = (int)c.lv.size() - 1;
@@ -833,8 +846,10 @@ static void vcycle_coarse(Ctx& c, int l, hipStream_t s) {
     return;
   }
   Level& Nx = c.lv[l + 1];
-  spmv(3, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.b, (const double*)L.M, L.t, s, L.x, &L.runs);  // t = A (M b), x = M b
-  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rx, L.b, L.t, Nx.b, s, (double*)nullptr, &Nx.runs);  // b' = R (b - t)
+  const bool nx_smooth = l + 1 < last;  // the next level forms x = M b with its b
+  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, (const double*)nullptr, L.t, s, (double*)nullptr, &L.runs);  // t = A x
+  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rx, L.b, L.t, Nx.b, s, nx_smooth ? Nx.x : (double*)nullptr, &Nx.runs,
+       nx_smooth ? (const double*)Nx.M : (const double*)nullptr);  // b' = R (b - t), x' = M' b'
   vcycle_coarse(c, l + 1, s);
   spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Px, Nx.xs, (const double*)nullptr, L.x, s, (double*)nullptr,
        &L.runs);                                                                      // x += P e
@@ -850,8 +865,10 @@ static void vcycle_coarse_f(Ctx& c, int l, hipStream_t s) {
     return;
   }
   Level& Nx = c.lv[l + 1];
-  spmv(3, L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.bf, (const float*)L.Mf, L.tf, s, L.xf, &L.runs);
-  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rxf, L.bf, L.tf, Nx.bf, s, (float*)nullptr, &Nx.runs);
+  const bool nx_smooth = l + 1 < last;
+  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.xf, (const float*)nullptr, L.tf, s, (float*)nullptr, &L.runs);
+  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rxf, L.bf, L.tf, Nx.bf, s, nx_smooth ? Nx.xf : (float*)nullptr, &Nx.runs,
+       nx_smooth ? (const float*)Nx.Mf : (const float*)nullptr);
   vcycle_coarse_f(c, l + 1, s);
   spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Pxf, Nx.xsf, (const float*)nullptr, L.xf, s, (float*)nullptr, &L.runs);
   spmv_post(L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.xf, L.Mf, L.bf, L.xsf, s, &L.runs);
@@ -865,7 +882,9 @@ static int precond_f32(Ctx& c, const double* r, double* z, double* rz_part, hipS
   Level& L0 = c.lv[0];
   Level& L1 = c.lv[1];
   MM_TRY(launch_apply_f32_scaled(c, r, L0.Mf, c.px0f, c.ptf, s));                 // x = M0 r, t = A x
-  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rxf, r, c.ptf, L1.bf, s, (float*)nullptr, &L1.runs);  // b1 = R0 (r - t)
+  const bool l1_smooth = c.lv.size() > 2;  // level 1 is not the coarsest: x1 = M1 b1 with b1
+  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rxf, r, c.ptf, L1.bf, s, l1_smooth ? L1.xf : (float*)nullptr, &L1.runs,
+       l1_smooth ? (const float*)L1.Mf : (const float*)nullptr);  // b1 = R0 (r - t)
   vcycle_coarse_f(c, 1, s);
   spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Pxf, L1.xsf, (const float*)nullptr, c.px0f, s, c.ptf, &L0.runs);
   MM_TRY(launch_apply_f32(c, c.px0f, c.ptf, s));                                  // t = A x
@@ -893,7 +912,9 @@ int precond(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s) 
   Level& L1 = c.lv[1];
   // the caller has cleared pt
   MM_TRY(timed_apply_scaled(c, r, L0.M, c.px0, c.pt, s));                          // x = M0 r, t = A x
-  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rx, r, c.pt, L1.b, s, (double*)nullptr, &L1.runs);  // b1 = R0 (r - t)
+  const bool l1_smooth = c.lv.size() > 2;  // level 1 is not the coarsest: x1 = M1 b1 with b1
+  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rx, r, c.pt, L1.b, s, l1_smooth ? L1.x : (double*)nullptr, &L1.runs,
+       l1_smooth ? (const double*)L1.M : (const double*)nullptr);  // b1 = R0 (r - t), x1 = M1 b1
   vcycle_coarse(c, 1, s);
   spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Px, L1.xs, (const double*)nullptr, c.px0, s, c.pt,
        &L0.runs);  // x += P0 e1, t = 0
