@@ -511,6 +511,40 @@ __global__ __launch_bounds__(kMaxBlockNodes) void k_apply_clp(Part p, Geo geo, c
   }
 }
 
+// ------------------------------------------- element loop of the per-tet magnetic kernels
+// Thread tid takes elements e0 + tid, e0 + tid + kBlock, ... of its block. The words (local
+// corners, material) of its first kElemPre elements are loaded before the node staging, so
+// their latency overlaps it instead of opening each loop iteration; the loop over them is
+// unrolled (register arrays indexed by constants) and further elements, if any, follow in a
+// plain loop. One staged node per thread: a block has at most kMaxBlockNodes <= kBlock nodes.
+static_assert(kMaxBlockNodes <= kBlock, "one staged node per thread");
+constexpr int kElemPre = 4;
+struct ElemWords {
+  uint32_t lc[kElemPre];
+  uint32_t mt[kElemPre];
+};
+
+__device__ __forceinline__ ElemWords elem_prefetch(const Part& p, const BlkInfo& bi, int tid) {
+  ElemWords w;
+#pragma unroll
+  for (int q = 0; q < kElemPre; ++q) {
+    // clamped into the block's element range (a launched partition has elements), so the loads
+    // need no branch; for_elems skips the clamped slots
+    const int e = max(bi.e0, min(bi.e0 + tid + q * kBlock, bi.e1 - 1));
+    w.lc[q] = p.lconn[e];
+    w.mt[q] = p.mat[e];
+  }
+  return w;
+}
+
+template <typename F>
+__device__ __forceinline__ void for_elems(const Part& p, const BlkInfo& bi, int tid, const ElemWords& w, F&& f) {
+#pragma unroll
+  for (int q = 0; q < kElemPre; ++q)
+    if (bi.e0 + tid + q * kBlock < bi.e1) f(w.lc[q], (int)w.mt[q]);
+  for (int e = bi.e0 + tid + kElemPre * kBlock; e < bi.e1; e += kBlock) f(p.lconn[e], (int)p.mat[e]);
+}
+
 // ------------------------------------------------------------- demag right-hand side (K2)
 // b_a = sum_e (Js_e Ve / 4) grad phi_a . (m_0 + m_1 + m_2 + m_3) over magnetic elements.
 __global__ __launch_bounds__(kBlock) void k_rhs(Part p, Geo geo, const double* __restrict__ m,
@@ -521,21 +555,24 @@ __global__ __launch_bounds__(kBlock) void k_rhs(Part p, Geo geo, const double* _
   const BlkInfo bi = blk_info(p);
   if (!bi.valid) return;
   const int tid = threadIdx.x;
-  for (int i = tid; i < bi.nb; i += kBlock) {
-    const uint32_t id = p.nodes[bi.n0 + i] & kIdMask;
+  const bool has_node = tid < bi.nb;
+  const uint32_t fl = has_node ? p.nodes[bi.n0 + tid] : 0u;  // first: the staging gathers wait for it only
+  const ElemWords ew = elem_prefetch(p, bi, tid);
+  if (has_node) {
+    const uint32_t id = fl & kIdMask;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      sm[k][i] = m[3 * id + k];
-      sX[k][i] = geo.knt[3 * id + k];
+      sm[k][tid] = m[3 * id + k];
+      sX[k][tid] = geo.knt[3 * id + k];
     }
-    sb[i] = 0.0;
+    sb[tid] = 0.0;
   }
   __syncthreads();
-  for (int e = bi.e0 + tid; e < bi.e1; e += kBlock) {
-    const double Js = geo.mt->Js[p.mat[e]];
-    if (Js == 0.0) continue;
+  for_elems(p, bi, tid, ew, [&](uint32_t lc, int mi) {
+    const double Js = geo.mt->Js[mi];
+    if (Js == 0.0) return;
     int l0, l1, l2, l3;
-    unpack(p.lconn[e], l0, l1, l2, l3);
+    unpack(lc, l0, l1, l2, l3);
     double g[4][3], vol;
     tet_grads(sX, l0, l1, l2, l3, g, vol);
     const double f = Js * vol * 0.25;
@@ -546,9 +583,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs(Part p, Geo geo, const double* _
     atomicAdd(&sb[l1], g[1][0] * sx_ + g[1][1] * sy_ + g[1][2] * sz_);
     atomicAdd(&sb[l2], g[2][0] * sx_ + g[2][1] * sy_ + g[2][2] * sz_);
     atomicAdd(&sb[l3], g[3][0] * sx_ + g[3][1] * sy_ + g[3][2] * sz_);
-  }
+  });
   __syncthreads();
-  for (int i = tid; i < bi.nb; i += kBlock) node_out(b, p.nodes[bi.n0 + i], sb[i]);
+  if (has_node) node_out(b, fl, sb[tid]);
 }
 
 // ------------------------------------------------------------ energy + gradient (K7)
@@ -565,27 +602,32 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
   __shared__ double su[kMaxBlockNodes];
   __shared__ double sg[3][kMaxBlockNodes];
   const BlkInfo bi = blk_info(p);
+  if (!bi.valid) return;
   const int tid = threadIdx.x;
+  const bool has_node = tid < bi.nb;
+  const uint32_t fl = has_node ? p.nodes[bi.n0 + tid] : 0u;  // first: the staging gathers wait for it only
+  const ElemWords ew = elem_prefetch(p, bi, tid);
   // a block lies inside one segment: one 1 / V_mag for all its nodes
   const double iv = (geo.inv_V_node && bi.nb > 0) ? geo.inv_V_node[p.nodes[bi.n0] & kIdMask] : inv_V;
   CSum acc;
-  for (int i = tid; i < bi.nb; i += kBlock) {
-    const uint32_t id = p.nodes[bi.n0 + i] & kIdMask;
+  double Mi = 0.0;  // M_nodal of the thread's node (Zeeman term of its owner)
+  if (has_node) {
+    const uint32_t id = fl & kIdMask;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      sm[k][i] = m[3 * id + k];
-      sX[k][i] = geo.knt[3 * id + k];
-      sg[k][i] = 0.0;
+      sm[k][tid] = m[3 * id + k];
+      sX[k][tid] = geo.knt[3 * id + k];
+      sg[k][tid] = 0.0;
     }
-    su[i] = U[id];
+    su[tid] = U[id];
+    Mi = geo.M_nodal[id];
   }
   __syncthreads();
-  for (int e = bi.e0 + tid; e < bi.e1; e += kBlock) {
-    const int mi = p.mat[e];
+  for_elems(p, bi, tid, ew, [&](uint32_t lc, int mi) {
     const double A = geo.mt->A[mi], K1 = geo.mt->K1[mi], Js = geo.mt->Js[mi];
     const double kx = geo.mt->k[mi][0], ky = geo.mt->k[mi][1], kz = geo.mt->k[mi][2];
     int l[4];
-    unpack(p.lconn[e], l[0], l[1], l[2], l[3]);
+    unpack(lc, l[0], l[1], l[2], l[3]);
     double g4[4][3], vol;
     tet_grads(sX, l[0], l[1], l[2], l[3], g4, vol);
     const double a_ve = 2.0 * A * vol, q_ve = 2.0 * K1 * vol / 20.0, j_ve = 2.0 * Js * vol / 4.0;
@@ -626,14 +668,14 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
       for (int c = 0; c < 3; ++c) atomicAdd(&sg[c][l[a]], c3[c]);
     }
     acc.add(0.5 * e_el);
-  }
+  });
   __syncthreads();
-  for (int i = tid; i < bi.nb; i += kBlock) {
-    const uint32_t f = p.nodes[bi.n0 + i];
+  if (has_node) {
+    const uint32_t f = fl;
     const uint32_t id = f & kIdMask;
+    const int i = tid;
     double v0 = sg[0][i], v1 = sg[1][i], v2 = sg[2][i];
     if (f & kOwner) {
-      const double Mi = geo.M_nodal[id];
       const double z0 = -2.0 * Mi * Bx, z1 = -2.0 * Mi * By, z2 = -2.0 * Mi * Bz;
       v0 += z0;
       v1 += z1;
@@ -769,23 +811,25 @@ __global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const double*
   const BlkInfo bi = blk_info(p);
   if (!bi.valid) return;
   const int tid = threadIdx.x;
+  const bool has_node = tid < bi.nb;
+  const uint32_t fl = has_node ? p.nodes[bi.n0 + tid] : 0u;  // first: the staging gathers wait for it only
+  const ElemWords ew = elem_prefetch(p, bi, tid);
   const double iv = (geo.inv_V_node && bi.nb > 0) ? geo.inv_V_node[p.nodes[bi.n0] & kIdMask] : inv_V;
-  for (int i = tid; i < bi.nb; i += kBlock) {
-    const uint32_t id = p.nodes[bi.n0 + i] & kIdMask;
+  if (has_node) {
+    const uint32_t id = fl & kIdMask;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      sm[k][i] = vin[3 * id + k];
-      sX[k][i] = geo.knt[3 * id + k];
-      sg[k][i] = 0.0;
+      sm[k][tid] = vin[3 * id + k];
+      sX[k][tid] = geo.knt[3 * id + k];
+      sg[k][tid] = 0.0;
     }
   }
   __syncthreads();
-  for (int e = bi.e0 + tid; e < bi.e1; e += kBlock) {
-    const int mi = p.mat[e];
+  for_elems(p, bi, tid, ew, [&](uint32_t lc, int mi) {
     const double A = geo.mt->A[mi], K1 = geo.mt->K1[mi];
     const double kx = geo.mt->k[mi][0], ky = geo.mt->k[mi][1], kz = geo.mt->k[mi][2];
     int l[4];
-    unpack(p.lconn[e], l[0], l[1], l[2], l[3]);
+    unpack(lc, l[0], l[1], l[2], l[3]);
     double g4[4][3], vol;
     tet_grads(sX, l[0], l[1], l[2], l[3], g4, vol);
     const double a_ve = 2.0 * A * vol, q_ve = 2.0 * K1 * vol / 20.0;
@@ -810,17 +854,16 @@ __global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const double*
       atomicAdd(&sg[1][l[a]], g4[a][0] * gm[1][0] + g4[a][1] * gm[1][1] + g4[a][2] * gm[1][2] + fa * ky);
       atomicAdd(&sg[2][l[a]], g4[a][0] * gm[2][0] + g4[a][1] * gm[2][1] + g4[a][2] * gm[2][2] + fa * kz);
     }
-  }
+  });
   __syncthreads();
-  for (int i = tid; i < bi.nb; i += kBlock) {
-    const uint32_t f = p.nodes[bi.n0 + i];
-    const uint32_t id = f & kIdMask;
-    if (f & kPrivate) {
+  if (has_node) {
+    const uint32_t id = fl & kIdMask;
+    if (fl & kPrivate) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) out[3 * id + k] = sg[k][i] * iv;
+      for (int k = 0; k < 3; ++k) out[3 * id + k] = sg[k][tid] * iv;
     } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&out[3 * id + k], sg[k][i] * iv);
+      for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&out[3 * id + k], sg[k][tid] * iv);
     }
   }
 }
