@@ -4,10 +4,11 @@
 // (src/minimizers.py:265-385), the Steihaug-PCG preconditioner solve
 // (src/minimizers.py:393-492) and the Gill-Murray convergence test (src/minimizers.py:160-172).
 //
-// Vectors (N x 3 interleaved) never leave the GPU. Scalars (dots, step lengths, flags)
-// live in the device scalar block and are consumed by the next kernel; the host reads
-// back only what drives control flow: the slope pg once per iteration, the trial energy
-// once per line-search evaluation, and the convergence quantities.
+// Vectors (N x 3 interleaved) never leave the GPU. Reductions are per-workgroup partials
+// (no fences): the next kernel sums the ones it needs, and the host reads back only what
+// drives control flow: the slope pg and the convergence quantities once per iteration, the
+// trial energy once per line-search evaluation. Loop state and step lengths live in the
+// device scalar block.
 #include <chrono>
 #include <cmath>
 #include <utility>
