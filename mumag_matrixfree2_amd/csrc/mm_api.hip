@@ -691,6 +691,9 @@ static void destroy(Ctx* c) {
   if (c->hseg) (void)hipHostFree(c->hseg);
   if (c->hguess) (void)hipHostFree(c->hguess);
   if (c->hmpart) (void)hipHostFree(c->hmpart);
+  if (c->seg_stage) (void)hipHostFree(c->seg_stage);
+  for (hipEvent_t e : c->seg_stage_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev) (void)hipEventDestroy(c->ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -35,6 +35,7 @@ constexpr uint32_t kIdMask = (1u << 29) - 1;
 constexpr int kMaxMat = 255;               // materials (uint8 index per element)
 constexpr int kRedBlocks = 2048;         // max workgroups of a reducing kernel
 constexpr int kNumScal = 256;            // device scalar slots
+constexpr int kSegStageSlots = 4;        // pinned staging slots of the segment tables
 
 enum Geom { kGeomCoords = 0, kGeomStored64 = 1, kGeomStored32 = 2 };
 
@@ -227,6 +228,20 @@ struct Ctx {
   int seg_nwg = 0;
   int *d_seg_node_off = nullptr, *d_seg_blk_all = nullptr, *d_seg_blk_cl = nullptr;
   int* d_seg_active = nullptr;
+  // The tables of a segment subset (d_seg_active, the workgroup tables wg_pcg / wg_min, the three
+  // block lists) are sub-arrays of one device buffer seg_tab with a host master copy. An update
+  // writes the master and widens the dirty range; a flush copies that range through a ring of
+  // pinned staging slots, a slot being reused only after its previous copy has completed. While
+  // seg_defer is set, updates only mark the range, so several updates travel in one copy.
+  int* seg_tab = nullptr;
+  int seg_tab_n = 0;
+  std::vector<int> seg_master;
+  int* seg_stage = nullptr;                       // kSegStageSlots x seg_tab_n, pinned
+  hipEvent_t seg_stage_ev[kSegStageSlots] = {};
+  bool seg_stage_rec[kSegStageSlots] = {};
+  int seg_stage_next = 0;
+  long seg_dirty_lo = -1, seg_dirty_hi = -1;
+  bool seg_defer = false;
   double* d_seg_rz = nullptr;         // 2 x nseg r.z ping-pong
   double* seg_part_rz = nullptr;      // seg_nwg r.z partials
   double *hseg = nullptr, *hseg_dev = nullptr;  // host-coherent r.r / b.b partials, 2 x seg_nwg
@@ -312,6 +327,18 @@ int ensure_seg(Ctx& c);
 int set_block_lists(Ctx& c, const int* mask);
 // workgroup tables of the segments with mask[g] != 0 (null: all)
 int build_wg_tables(Ctx& c, const int* mask, WgTables& T);
+// n ints of src to the sub-array dev of seg_tab (through the master copy; flushed now unless
+// seg_defer is set); seg_flush copies the dirty range
+int seg_upload(Ctx& c, int* dev, const int* src, int n);
+int seg_flush(Ctx& c);
+// defers the flushes of the segment-table updates made while it lives; restores the previous
+// setting on every path (batches nest), after which the owner calls seg_flush
+struct SegBatch {
+  Ctx& c;
+  bool prev;
+  explicit SegBatch(Ctx& c_) : c(c_), prev(c_.seg_defer) { c.seg_defer = true; }
+  ~SegBatch() { c.seg_defer = prev; }
+};
 // per-segment sums of the magnetic-partition block partials: out[g * stride + k] for k < K
 // (part holds K values per logical block; compensated pairs when CS)
 int launch_seg_block_sums(const Ctx& c, const double* part, int K, bool cs, double* out, int stride, int slot,

@@ -568,8 +568,12 @@ int minimize_segmented(Ctx& c, const MinParams& p, const double B[3], MinInfo* i
   auto T = [&](int g, int slot) { return tab[(size_t)g * kSegSlots + slot]; };
   // restrict the vector kernels and the element kernels to the segments in mask
   auto use = [&](const std::vector<int>& mask) -> int {
-    MM_TRY(build_wg_tables(c, mask.data(), c.wg_min));
-    MM_TRY(set_block_lists(c, mask.data()));
+    {
+      SegBatch batch(c);  // workgroup tables and block lists in one copy
+      MM_TRY(build_wg_tables(c, mask.data(), c.wg_min));
+      MM_TRY(set_block_lists(c, mask.data()));
+    }
+    MM_TRY(seg_flush(c));
     W = c.wg_min.n;
     return 0;
   };

@@ -615,17 +615,39 @@ int ensure_seg(Ctx& c) {
   MM_TRY(up(&c.d_seg_node_off, c.h_seg_node_off));
   MM_TRY(up(&c.d_seg_blk_all, c.h_seg_blk_all));
   MM_TRY(up(&c.d_seg_blk_cl, c.h_seg_blk_cl));
-  MM_TRY(up(&c.d_seg_active, std::vector<int>(c.nseg, 1)));
-  for (WgTables* T : {&c.wg_pcg, &c.wg_min}) {
-    MM_TRY(up(&T->seg, std::vector<int>(c.seg_nwg, 0)));
-    MM_TRY(up(&T->chunk, std::vector<int>(c.seg_nwg, 0)));
-    MM_TRY(up(&T->off, std::vector<int>(c.nseg + 1, 0)));
-    T->key.clear();
+  // the subset tables, one buffer: active mask, the two workgroup tables, the three block lists
+  // (each list holds at most every block of its partition)
+  {
+    const int G = c.nseg, W = c.seg_nwg;
+    const int sizes[10] = {G, W, W, G + 1, W, W, G + 1, std::max(c.part_cl.nblk, 1), std::max(c.part_all.nblk, 1),
+                           std::max(c.part_mag.nblk, 1)};
+    int off[11] = {0};
+    for (int q = 0; q < 10; ++q) off[q + 1] = off[q] + sizes[q];
+    c.seg_tab_n = off[10];
+    c.seg_master.assign(c.seg_tab_n, 0);
+    std::fill(c.seg_master.begin(), c.seg_master.begin() + G, 1);  // every segment active
+    MM_TRY(up(&c.seg_tab, c.seg_master));
+    int* t = c.seg_tab;
+    c.d_seg_active = t + off[0];
+    c.wg_pcg.seg = t + off[1];
+    c.wg_pcg.chunk = t + off[2];
+    c.wg_pcg.off = t + off[3];
+    c.wg_min.seg = t + off[4];
+    c.wg_min.chunk = t + off[5];
+    c.wg_min.off = t + off[6];
+    c.d_blist_cl = t + off[7];
+    c.d_blist_all = t + off[8];
+    c.d_blist_mag = t + off[9];
+    c.wg_pcg.key.clear();
+    c.wg_min.key.clear();
+    MM_HIP(hipHostMalloc((void**)&c.seg_stage, (size_t)kSegStageSlots * c.seg_tab_n * sizeof(int),
+                         hipHostMallocDefault));
+    for (int k = 0; k < kSegStageSlots; ++k) {
+      MM_HIP(hipEventCreateWithFlags(&c.seg_stage_ev[k], hipEventDisableTiming));
+      c.seg_stage_rec[k] = false;
+    }
+    c.seg_dirty_lo = c.seg_dirty_hi = -1;
   }
-  // block lists hold at most every block of their partition
-  MM_TRY(up(&c.d_blist_cl, std::vector<int>(std::max(c.part_cl.nblk, 1), 0)));
-  MM_TRY(up(&c.d_blist_all, std::vector<int>(std::max(c.part_all.nblk, 1), 0)));
-  MM_TRY(up(&c.d_blist_mag, std::vector<int>(std::max(c.part_mag.nblk, 1), 0)));
   c.blist_key.clear();
   MM_HIP(hipMalloc((void**)&c.d_seg_rz, 2 * (size_t)c.nseg * sizeof(double)));
   c.allocs.push_back(c.d_seg_rz);
@@ -635,6 +657,31 @@ int ensure_seg(Ctx& c) {
                        hipHostMallocMapped | hipHostMallocCoherent));
   MM_HIP(hipHostGetDevicePointer((void**)&c.hseg_dev, c.hseg, 0));
   return 0;
+}
+
+int seg_flush(Ctx& c) {
+  if (c.seg_dirty_lo < 0) return 0;
+  const int k = c.seg_stage_next;
+  c.seg_stage_next = (k + 1) % kSegStageSlots;
+  if (c.seg_stage_rec[k]) MM_HIP(hipEventSynchronize(c.seg_stage_ev[k]));  // the slot's last copy is done
+  int* slot = c.seg_stage + (size_t)k * c.seg_tab_n;
+  const long lo = c.seg_dirty_lo, n = c.seg_dirty_hi - c.seg_dirty_lo;
+  std::copy(c.seg_master.begin() + lo, c.seg_master.begin() + lo + n, slot + lo);
+  MM_HIP(hipMemcpyAsync(c.seg_tab + lo, slot + lo, n * sizeof(int), hipMemcpyHostToDevice, c.stream));
+  MM_HIP(hipEventRecord(c.seg_stage_ev[k], c.stream));
+  c.seg_stage_rec[k] = true;
+  c.seg_dirty_lo = c.seg_dirty_hi = -1;
+  return 0;
+}
+
+int seg_upload(Ctx& c, int* dev, const int* src, int n) {
+  if (n <= 0) return 0;
+  const long lo = dev - c.seg_tab;
+  MM_CHECK(c.seg_tab && lo >= 0 && lo + n <= c.seg_tab_n, "segment table upload out of range");
+  std::copy(src, src + n, c.seg_master.begin() + lo);
+  c.seg_dirty_lo = c.seg_dirty_lo < 0 ? lo : std::min(c.seg_dirty_lo, lo);
+  c.seg_dirty_hi = std::max(c.seg_dirty_hi, lo + n);
+  return c.seg_defer ? 0 : seg_flush(c);
 }
 
 static std::vector<int> full_mask(const Ctx& c, const int* mask) {
@@ -661,11 +708,13 @@ int build_wg_tables(Ctx& c, const int* mask, WgTables& T) {
   }
   T.n = off[c.nseg];
   // stream-ordered after the kernels that read the previous tables
-  if (T.n > 0) {
-    MM_HIP(hipMemcpyAsync(T.seg, seg.data(), T.n * sizeof(int), hipMemcpyHostToDevice, c.stream));
-    MM_HIP(hipMemcpyAsync(T.chunk, chunk.data(), T.n * sizeof(int), hipMemcpyHostToDevice, c.stream));
+  {
+    SegBatch batch(c);
+    MM_TRY(seg_upload(c, T.seg, seg.data(), T.n));
+    MM_TRY(seg_upload(c, T.chunk, chunk.data(), T.n));
+    MM_TRY(seg_upload(c, T.off, off.data(), (int)off.size()));
   }
-  MM_HIP(hipMemcpyAsync(T.off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+  if (!c.seg_defer) MM_TRY(seg_flush(c));
   T.h_off = std::move(off);
   T.key = std::move(key);
   return 0;
@@ -687,14 +736,18 @@ int set_block_lists(Ctx& c, const int* mask) {
     for (int g = 0; g < c.nseg; ++g)
       if (key[g])
         for (int b = off[g]; b < off[g + 1]; ++b) h.push_back(b);
-    if (!h.empty()) MM_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    MM_TRY(seg_upload(c, d, h.data(), (int)h.size()));
     p.blist = d;
     p.nlist = (int)h.size();
     return 0;
   };
-  MM_TRY(fill(c.part_cl, c.h_seg_blk_cl, c.h_blist_cl, c.d_blist_cl));
-  MM_TRY(fill(c.part_all, c.h_seg_blk_all, c.h_blist_all, c.d_blist_all));
-  MM_TRY(fill(c.part_mag, c.h_seg_blk_mag, c.h_blist_mag, c.d_blist_mag));
+  {
+    SegBatch batch(c);
+    MM_TRY(fill(c.part_cl, c.h_seg_blk_cl, c.h_blist_cl, c.d_blist_cl));
+    MM_TRY(fill(c.part_all, c.h_seg_blk_all, c.h_blist_all, c.d_blist_all));
+    MM_TRY(fill(c.part_mag, c.h_seg_blk_mag, c.h_blist_mag, c.d_blist_mag));
+  }
+  if (!c.seg_defer) MM_TRY(seg_flush(c));
   // AMG rows of the same segments: runs of consecutive segments (at most kMaxRuns launches per
   // SpMV, else the span from the first to the last); every row where a level's unknowns are not
   // ordered by segment
@@ -728,9 +781,13 @@ static int pcg_seg(Ctx& c, const double* b, double* U, const double* tol, const 
   const long N = c.N;
   const int G = c.nseg;
   c.h_seg_active = full_mask(c, want);
-  MM_HIP(hipMemcpyAsync(c.d_seg_active, c.h_seg_active.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
-  MM_TRY(build_wg_tables(c, c.h_seg_active.data(), c.wg_pcg));
-  MM_TRY(set_block_lists(c, c.h_seg_active.data()));
+  {
+    SegBatch batch(c);  // mask, workgroup tables and block lists in one copy
+    MM_TRY(seg_upload(c, c.d_seg_active, c.h_seg_active.data(), G));
+    MM_TRY(build_wg_tables(c, c.h_seg_active.data(), c.wg_pcg));
+    MM_TRY(set_block_lists(c, c.h_seg_active.data()));
+  }
+  MM_TRY(seg_flush(c));
   std::vector<double> r2(G, 0.0), bb(G, 0.0), tol2(G, 0.0);
   for (int g = 0; g < G; ++g) it_out[g] = 0;
   int W = c.wg_pcg.n;
@@ -767,9 +824,13 @@ static int pcg_seg(Ctx& c, const double* b, double* U, const double* tol, const 
     live += c.h_seg_active[g];
   }
   auto shrink = [&]() -> int {
-    MM_HIP(hipMemcpyAsync(c.d_seg_active, c.h_seg_active.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
-    MM_TRY(build_wg_tables(c, c.h_seg_active.data(), c.wg_pcg));
-    MM_TRY(set_block_lists(c, c.h_seg_active.data()));
+    {
+      SegBatch batch(c);  // one copy for the mask, the workgroup tables and the block lists
+      MM_TRY(seg_upload(c, c.d_seg_active, c.h_seg_active.data(), G));
+      MM_TRY(build_wg_tables(c, c.h_seg_active.data(), c.wg_pcg));
+      MM_TRY(set_block_lists(c, c.h_seg_active.data()));
+    }
+    MM_TRY(seg_flush(c));
     W = c.wg_pcg.n;
     return 0;
   };
