@@ -141,3 +141,48 @@ def test_amg_upload_rejects_malformed_csr(cube_c0):
     r = np.random.default_rng(15).standard_normal(ctx.N)
     assert float(r @ ctx.precond_apply(r)) > 0.0
     ctx.close()
+
+
+def test_partials_in_several_batches_match_oracle():
+    """16-node element blocks on an 83k-tet cube: more than 4,096 cluster blocks, so the p.Ap
+    partials of the PCG apply are summed in several load batches by their consumer, and the
+    per-tet kernels see blocks with fewer elements than threads (their prefetched element words
+    are clamped). The apply, rhs, energy and gradient, local Hessian and solve_U match the oracle."""
+    from mumag_matrixfree2_amd.context import MeshContext
+    from mumag_matrixfree2_amd.mesh import config_cube_mesh
+    from oracle.fem import Problem
+    from oracle.solvers import solve_U
+
+    mesh = config_cube_mesh(20.0, 18, 3)
+    A, K1, Js, k = ndfeb_materials()
+    ctx = MeshContext(mesh["knt"], mesh["ijk"][:, :4], mesh["ijk"][:, 4], A, K1, Js, k,
+                      boundary_mask=mesh["boundary_mask"], block_nodes=16)
+    ctx.setup_precond("amgcl")
+    print(f"[many blocks] N={ctx.N} E={ctx.E} cluster blocks={ctx.n_blocks_cl} "
+          f"magnet blocks={ctx.n_blocks_mag}", flush=True)
+    assert ctx.n_blocks_cl > 4096
+    ref = Problem(mesh["knt"], mesh["ijk"], A, K1, Js, k, boundary_mask=mesh["boundary_mask"])
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(ctx.N)
+    assert rel(ctx.apply_A(x), ref.A @ x) <= 1e-12
+    m = tilted_m0(ctx.N, seed=22, sigma=0.3)
+    assert rel(ctx.rhs(m), ref.rhs(m)) <= 1e-12
+    U = rng.standard_normal(ctx.N) * 0.1
+    B = np.array([0.01, -0.02, -0.3])
+    E, g = ctx.energy_grad(m, U, B)
+    E_ref, g_ref = ref.energy_and_grad(m, U, B)
+    scale = sum(abs(v) for v in ref.energy_terms(m, U, B).values())
+    assert abs(E - E_ref) <= 1e-12 * scale
+    assert rel(g, g_ref) <= 1e-12
+    v = rng.standard_normal((ctx.N, 3))
+    assert rel(ctx.local_grad(v), ref.local_grad(v)) <= 1e-12
+    x0 = np.zeros(ctx.N)
+    Us, it, rr = ctx.solve_U(m, x0, 1e-10)
+    U_ref, it_ref, _ = solve_U(ref, m, x0, 1e-10, precond="amg", levels=oracle_levels(ctx.amg_levels, "amgcl"),
+                               kind="amgcl")
+    print(f"[many blocks] it={it} it_ref={it_ref} rr={rr:.3e}", flush=True)
+    assert abs(it - it_ref) <= 1
+    assert rel(Us, U_ref) <= 1e-9
+    b = ref.rhs(m)
+    assert np.linalg.norm(b - ref.A @ Us) <= 1.0001e-10 * np.linalg.norm(b)
+    ctx.close()
