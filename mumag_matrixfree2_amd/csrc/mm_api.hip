@@ -474,6 +474,8 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
     const double det = cc[0][0] * (cc[1][1] * cc[2][2] - cc[1][2] * cc[2][1]) -
                        cc[0][1] * (cc[1][0] * cc[2][2] - cc[1][2] * cc[2][0]) +
                        cc[0][2] * (cc[1][0] * cc[2][1] - cc[1][1] * cc[2][0]);
+    // a flat element has no J^-1 (the reference's inversion of J fails there too)
+    MM_CHECK(det != 0.0 && std::isfinite(det), "degenerate element (zero or non-finite volume)");
     if (det < 0) std::swap(conn[4 * e + 1], conn[4 * e + 2]);
   }
   // element order: by (smallest node id, largest node id), stable
