@@ -41,3 +41,38 @@ def test_binding_table_matches_header():
     from mumag_matrixfree2_amd import _hip
 
     assert set(_hip.SIGNATURES) == declared_entry_points()
+
+
+def test_missing_library_raises(monkeypatch):
+    """Without the built library every entry point raises: there is no CPU fallback."""
+    from mumag_matrixfree2_amd import _hip
+
+    monkeypatch.setattr(_hip, "LIB_PATH", ROOT / "no_such_dir" / "libmumag_hip.so")
+    monkeypatch.setattr(_hip, "_lib", None)
+    with pytest.raises(RuntimeError, match="not built"):
+        _hip.load()
+
+
+def test_without_a_device_creation_is_an_error():
+    """On a host without a GPU, creating a context reports the HIP error instead of computing
+    anywhere else (run in a child process; on a GPU host the context simply builds)."""
+    import subprocess
+    import sys
+
+    code = (
+        "import sys; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
+        "from conftest import ndfeb_materials\n"
+        "from mumag_matrixfree2_amd.mesh import config_cube_mesh\n"
+        "from mumag_matrixfree2_amd.context import MeshContext\n"
+        "m = config_cube_mesh(8.0, 4, 2)\n"
+        "A, K1, Js, k = ndfeb_materials()\n"
+        "try:\n"
+        "    MeshContext(m['knt'], m['ijk'][:, :4], m['ijk'][:, 4], A, K1, Js, k, boundary_mask=m['boundary_mask'])\n"
+        "    print('BUILT')\n"
+        "except RuntimeError as e:\n"
+        "    print('ERROR', e)\n" % (str(ROOT), str(ROOT / "tests"))
+    )
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = r.stdout.strip().splitlines()[-1]
+    assert out == "BUILT" or (out.startswith("ERROR") and "mm_ctx_create" in out), out
