@@ -383,6 +383,8 @@ def main():
                     help="time budget of the CPU baseline (field steps from the start of the sweep)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--max-coarse", type=int, default=None)
+    ap.add_argument("--amg-theta", default=None,
+                    help="strength thresholds of the SA set-up, fine level first, comma-separated (default 0.1,0.05)")
     ap.add_argument("--block-nodes", type=int, default=256,
                     help="distinct nodes per element block (16..256): the LDS staging size of the element kernels")
     ap.add_argument("--apply-shape", default="auto",
@@ -421,7 +423,8 @@ def main():
     ctx = MeshContext(mesh["knt"], mesh["ijk"][:, :4], mesh["ijk"][:, 4], *mats,
                       boundary_mask=mesh["boundary_mask"], block_nodes=args.block_nodes)
     log(f"[bench] context {time.perf_counter() - t0:.1f}s")
-    ctx.setup_precond(args.precond, max_coarse=args.max_coarse, log=log)
+    amg_kw = {} if args.amg_theta is None else {"theta": tuple(float(t) for t in args.amg_theta.split(","))}
+    ctx.setup_precond(args.precond, max_coarse=args.max_coarse, log=log, **amg_kw)
     ctx.set_precision(args.precision)
     from mumag_matrixfree2_amd.autotune import tune_apply
 
