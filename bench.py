@@ -116,6 +116,17 @@ def reference_layout_bytes(ctx) -> int:
     return ctx.E * (16 + 10 * 8) + ctx.N * 2 * 8
 
 
+def amg_kwargs(args) -> dict:
+    """Options of the host SA set-up from the command line (defaults of ``amg.build_hierarchy``
+    where not given)."""
+    kw = {"max_coarse": args.max_coarse}
+    if args.amg_theta is not None:
+        kw["theta"] = tuple(float(t) for t in args.amg_theta.split(","))
+    if args.amg_omega is not None:
+        kw["omega"] = args.amg_omega
+    return kw
+
+
 def measured_traffic():
     """HBM bytes per k_apply launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
     (profiles/k_apply_cl_cube2M_fp64.md), or None if absent."""
@@ -167,7 +178,7 @@ def segmented_poisson(grains, ctxs, args, log, union=None, reps=3, tol=1e-10):
         A, K1, Js, _ = materials()
         tables = [(A, K1, Js, np.array([g["easy_axis"], g["easy_axis"]])) for g in grains]
         u = grain_set_context(grains, tables)
-        u.setup_precond(args.precond, max_coarse=args.max_coarse)
+        u.setup_precond(args.precond, **amg_kwargs(args))
         log(f"[bench:segmented] union context N={u.N} E={u.E}, AMG levels "
             f"{[l['A'].shape[0] for l in u.amg_levels]}, {time.perf_counter() - t0:.1f}s")
     precisions = [c.precision for c in ctxs] + [u.precision]
@@ -229,14 +240,14 @@ def run_grains(args, log):
     if "threads" in forms or not args.no_segmented or args.fp64_sequential_baseline:
         for i, (g, t) in enumerate(zip(grains, tables)):
             c = MeshContext(g["knt"], g["ijk"][:, :4], g["ijk"][:, 4], *t, boundary_mask=g["boundary_mask"])
-            c.setup_precond(args.precond, max_coarse=args.max_coarse)
+            c.setup_precond(args.precond, **amg_kwargs(args))
             c.set_precision(args.precision)
             tune_apply(c)
             ctxs.append(c)
         log(f"[bench] {len(ctxs)} grain contexts set up, {time.perf_counter() - t0:.1f}s")
     if "segmented" in forms:
         union = grain_set_context(grains, tables)
-        union.setup_precond(args.precond, max_coarse=args.max_coarse)
+        union.setup_precond(args.precond, **amg_kwargs(args))
         union.set_precision(args.precision)
         shape = tune_apply(union)
         log(f"[bench] union context N={union.N} E={union.E} AMG levels "
@@ -385,6 +396,8 @@ def main():
     ap.add_argument("--max-coarse", type=int, default=None)
     ap.add_argument("--amg-theta", default=None,
                     help="strength thresholds of the SA set-up, fine level first, comma-separated (default 0.1,0.05)")
+    ap.add_argument("--amg-omega", type=float, default=None,
+                    help="prolongator smoothing weight omega / rho(D^-1 A) of the SA set-up (default 1.5)")
     ap.add_argument("--block-nodes", type=int, default=256,
                     help="distinct nodes per element block (16..256): the LDS staging size of the element kernels")
     ap.add_argument("--apply-shape", default="auto",
@@ -423,8 +436,7 @@ def main():
     ctx = MeshContext(mesh["knt"], mesh["ijk"][:, :4], mesh["ijk"][:, 4], *mats,
                       boundary_mask=mesh["boundary_mask"], block_nodes=args.block_nodes)
     log(f"[bench] context {time.perf_counter() - t0:.1f}s")
-    amg_kw = {} if args.amg_theta is None else {"theta": tuple(float(t) for t in args.amg_theta.split(","))}
-    ctx.setup_precond(args.precond, max_coarse=args.max_coarse, log=log, **amg_kw)
+    ctx.setup_precond(args.precond, log=log, **amg_kwargs(args))
     ctx.set_precision(args.precision)
     from mumag_matrixfree2_amd.autotune import tune_apply
 
