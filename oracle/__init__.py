@@ -31,6 +31,8 @@ How each part is pinned (the tests under ``tests/test_oracle_*.py``):
   line search        closed-form quadratic model (known accepted step)
   pcohen_hs / lbfgs  energy decrease per accepted step, |m| = 1, S,Y tangent, s.y > 0,
                      and both methods converge to the same minimum.
+  lbfgs_two_loop     the explicit product of BFGS inverse updates from H0 = gam diag(invMr),
+                     with the ring of pairs wrapped, and the secant condition H y = s.
 Parity of the L-BFGS iterate trajectory against the reference JAX code is
-unpinned: JAX is not installed here, so only the invariants above check it.
+unpinned: JAX is not installed here, so only the checks above cover it.
 """

@@ -136,6 +136,35 @@ def solve_Py_g(local_grad, m, g_ext, g_tan_ext, inv_M_prec, max_iter=10, tol=0.0
     return y, it
 
 
+def lbfgs_two_loop(q, S, Y, rho, hist, invMr):
+    """H q for the L-BFGS inverse Hessian (two-loop recursion, src/minimizers.py:1284-1312).
+
+    S, Y (M, N, 3) are a ring of M pairs, rho[i] = 1 / (s_i . y_i); the newest of the ``hist``
+    pairs stored so far sits at (hist - 1) % M. H0 = gam * diag(invMr) with
+    gam = s.y / (y . invMr y) of the newest pair, clipped to [1e-3, 1e3] (1 with no pairs).
+    """
+    M = len(rho)
+    nh = min(hist, M)
+    q = np.array(q, dtype=np.float64, copy=True)
+    alphas = np.zeros(M)
+    for i in range(nh):
+        idx = (hist - 1 - i) % M
+        alphas[idx] = rho[idx] * np.vdot(S[idx], q)
+        q = q - alphas[idx] * Y[idx]
+    last = (hist - 1) % M
+    if hist > 0:
+        gam = np.vdot(S[last], Y[last]) / (np.vdot(Y[last], Y[last] * invMr) + 1e-30)
+    else:
+        gam = 1.0
+    gam = float(np.clip(gam, 1e-3, 1e3))
+    r = gam * q * invMr
+    for i in range(nh):
+        idx = (hist - nh + i) % M
+        b = rho[idx] * np.vdot(Y[idx], r)
+        r = r + S[idx] * (alphas[idx] - b)
+    return r
+
+
 class Minimizer:
     """Field-step minimiser: ``make_minimizer`` + ``solve_and_minimize`` + ``kernel`` (``src/minimizers.py:3402-3452``).
 
@@ -209,26 +238,7 @@ class Minimizer:
                 g_tan = tangent_grad(m, g_raw * invMr)
                 g_tan_ext = tangent_grad(m, g_raw)
                 gi = float(np.max(np.abs(g_tan)))
-                # two-loop recursion (src/minimizers.py:1284-1312)
-                nh = min(hist, M)
-                q = g_tan_ext.copy()
-                alphas = np.zeros(M)
-                for i in range(nh):
-                    idx = (hist - 1 - i) % M
-                    alphas[idx] = rho[idx] * np.vdot(S[idx], q)
-                    q = q - alphas[idx] * Y[idx]
-                last = (hist - 1) % M
-                if hist > 0:
-                    gam = np.vdot(S[last], Y[last]) / (np.vdot(Y[last], Y[last] * invMr) + 1e-30)
-                else:
-                    gam = 1.0
-                gam = float(np.clip(gam, 1e-3, 1e3))
-                r = gam * q * invMr
-                for i in range(nh):
-                    idx = (hist - nh + i) % M
-                    b = rho[idx] * np.vdot(Y[idx], r)
-                    r = r + S[idx] * (alphas[idx] - b)
-                d = -r
+                d = -lbfgs_two_loop(g_tan_ext, S, Y, rho, hist, invMr)
                 if it == 0 or np.vdot(d, g_tan_ext) > 0:
                     d = -g_tan
                 H = np.cross(m, d)

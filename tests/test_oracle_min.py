@@ -170,3 +170,50 @@ def test_field_values():
     v = field_values(0.0, 1.0, 0.3)
     assert np.allclose(v, [0.0, 0.3, 0.6, 0.9, 1.0])
     assert field_values(0.0, 1.0, 0.5, loop=True).tolist() == [0.0, 0.5, 1.0, 0.5, 0.0]
+
+
+def _dense_inverse_bfgs(S, Y, rho, hist, invMr):
+    """The L-BFGS inverse Hessian as an explicit matrix: H0 = gam D, then the BFGS inverse update
+    H <- (I - rho s y^T)^T H (I - rho y s^T) + rho s s^T for each stored pair, oldest first
+    (Nocedal & Wright, Numerical Optimization, eqs. 6.17 and 7.19)."""
+    M = len(rho)
+    nh = min(hist, M)
+    n = S[0].size
+    d = np.repeat(invMr.ravel(), 3)
+    if hist > 0:
+        s, y = S[(hist - 1) % M].ravel(), Y[(hist - 1) % M].ravel()
+        gam = float(np.clip(s @ y / (y @ (d * y)), 1e-3, 1e3))
+    else:
+        gam = 1.0
+    H = gam * np.diag(d)
+    for i in range(nh):
+        idx = (hist - nh + i) % M
+        s, y = S[idx].ravel(), Y[idx].ravel()
+        V = np.eye(n) - rho[idx] * np.outer(y, s)
+        H = V.T @ H @ V + rho[idx] * np.outer(s, s)
+    return H
+
+
+@pytest.mark.parametrize("hist", [0, 1, 3, 4, 7])
+def test_lbfgs_two_loop_is_the_dense_inverse_bfgs_update(hist):
+    """The two-loop recursion equals the explicit product of BFGS inverse updates on a ring of
+    4 pairs, also after the ring has wrapped (hist 7): a wrong loop order, ring index or sign of
+    a correction term fails it."""
+    from oracle.minimize import lbfgs_two_loop
+
+    rng = np.random.default_rng(11 + hist)
+    N, M = 6, 4
+    S = rng.standard_normal((M, N, 3))
+    Y = 0.7 * S + 0.2 * rng.standard_normal((M, N, 3))
+    rho = np.array([1.0 / np.vdot(S[i], Y[i]) for i in range(M)])
+    assert np.all(rho > 0)
+    invMr = rng.uniform(0.5, 2.0, (N, 1))
+    q = rng.standard_normal((N, 3))
+    r = lbfgs_two_loop(q, S, Y, rho, hist, invMr)
+    H = _dense_inverse_bfgs(S, Y, rho, hist, invMr)
+    np.testing.assert_allclose(r.ravel(), H @ q.ravel(), rtol=1e-12, atol=1e-12)
+    if hist > 0:
+        # secant condition of the newest pair: H y = s
+        last = (hist - 1) % M
+        np.testing.assert_allclose(lbfgs_two_loop(Y[last], S, Y, rho, hist, invMr), S[last],
+                                   rtol=1e-11, atol=1e-11)
