@@ -156,6 +156,11 @@ int mm_j_parallel(int64_t ctx, const double* m, const double* h, double* J);
 /* Device-resident state for a field sweep: upload / download m (N,3) and U (N,). */
 int mm_state_set(int64_t ctx, const double* m, const double* U);
 int mm_state_get(int64_t ctx, double* m, double* U);
+/* Validate a parameter struct without touching a device: method 0 or 1, memory in [1, 32]
+ * (0 selects 10), non-negative iteration limits, finite non-negative tolerances. The
+ * minimiser entry points run the same checks. The reference validates nothing here; the cap on
+ * memory is this library's (the history's scalars live in a fixed device table). */
+int mm_params_check(const mm_params* p);
 /* Minimise the resident state at field B (one field step, src/minimizers.py:3402-3452). */
 int mm_minimize_resident(int64_t ctx, const mm_params* p, const double* B, mm_info* info);
 /* mm_state_set + mm_minimize_resident + mm_state_get; the role of

@@ -89,6 +89,7 @@ SIGNATURES = {
     "mm_j_parallel": [C.c_int64, f64p, f64p, f64p],
     "mm_state_set": [C.c_int64, f64p, f64p],
     "mm_state_get": [C.c_int64, f64p, f64p],
+    "mm_params_check": [C.POINTER(MMParams)],
     "mm_minimize_resident": [C.c_int64, C.POINTER(MMParams), f64p, C.POINTER(MMInfo)],
     "mm_minimize": [C.c_int64, C.POINTER(MMParams), f64p, f64p, f64p, C.POINTER(MMInfo)],
     "mm_state_j_parallel": [C.c_int64, f64p, f64p],

@@ -389,4 +389,5 @@ def make_params(method="pcohen_hs", *, max_iter=2000, tau_f=1e-8, eps_a=1e-12, t
     p.phi_tol = float(min(cg_tol, 0.1 * tau_f) if phi_tol is None else phi_tol)
     p.cg_maxiter = int(cg_maxiter)
     p.memory = int(memory)
+    call("mm_params_check", C.byref(p))
     return p

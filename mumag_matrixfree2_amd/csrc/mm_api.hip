@@ -451,11 +451,14 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
       MM_CHECK(seg_node_off[g + 1] > seg_node_off[g], "segments must be non-empty and increasing");
       for (int v = seg_node_off[g]; v < seg_node_off[g + 1]; ++v) seg_of[v] = g;
     }
-    for (int e = 0; e < E; ++e)
+    for (int e = 0; e < E; ++e) {
+      for (int q = 0; q < 4; ++q)
+        MM_CHECK(conn_in[4 * (size_t)e + q] >= 0 && conn_in[4 * (size_t)e + q] < N,
+                 "connectivity index out of range");
       for (int q = 1; q < 4; ++q)
-        MM_CHECK(conn_in[4 * (size_t)e + q] >= 0 && conn_in[4 * (size_t)e + q] < N &&
-                     seg_of[conn_in[4 * (size_t)e + q]] == seg_of[conn_in[4 * (size_t)e]],
+        MM_CHECK(seg_of[conn_in[4 * (size_t)e + q]] == seg_of[conn_in[4 * (size_t)e]],
                  "an element couples two segments");
+    }
   }
   c.nseg = (int)c.h_seg_node_off.size() - 1;
 
@@ -1235,11 +1238,27 @@ static void to_minparams(const mm_params* p, MinParams& q) {
   q.memory = p->memory;
 }
 
+// Parameter checks shared by the minimiser entry points. The L-BFGS history keeps its alphas and
+// rhos in device scalar slots S_ALPHAS.. and S_RHOS.. (32 each), so memory is capped at 32.
+static int check_params(const mm_params* p) {
+  MM_CHECK(p, "null parameter struct");
+  MM_CHECK(p->method == 0 || p->method == 1, "method must be 0 (pcohen_hs) or 1 (lbfgs)");
+  MM_CHECK(p->memory >= 0 && p->memory <= kMaxLbfgsMemory, "memory (L-BFGS history) must be in [1, 32] (0: 10)");
+  MM_CHECK(p->max_iter >= 0 && p->ls_max_evals >= 1 && p->pc_iters >= 0 && p->cg_maxiter >= 1 && p->L >= 0,
+           "iteration limits must be non-negative (ls_max_evals, cg_maxiter >= 1)");
+  for (double v : {p->tau_f, p->eps_a, p->tau0, p->ls_eta1, p->ls_eta2, p->ls_C, p->ls_c, p->pc_force_eta,
+                   p->pc_force_alpha, p->pc_stagnation_nu, p->pc_reg, p->cg_tol, p->phi_tol})
+    MM_CHECK(std::isfinite(v) && v >= 0.0, "tolerances and line-search constants must be finite and non-negative");
+  return 0;
+}
+
+int mm_params_check(const mm_params* p) { return check_params(p); }
+
 int mm_minimize_resident(int64_t h, const mm_params* p, const double* B, mm_info* info) {
   GET_CTX(h);
   MM_CHECK(p && B && info, "null argument");
   MM_CHECK(c.have_state, "no resident state (call mm_state_set)");
-  MM_CHECK(p->method == 0 || p->method == 1, "method must be 0 (pcohen_hs) or 1 (lbfgs)");
+  MM_TRY(check_params(p));
   MM_CHECK(c.nseg == 1, "the context holds several independent systems: use mm_minimize_segmented");
   MinParams q;
   to_minparams(p, q);
@@ -1266,6 +1285,8 @@ int mm_minimize_segmented(int64_t h, const mm_params* p, const double* B, mm_inf
   MM_CHECK(p && B && info, "null argument");
   MM_CHECK(c.have_state, "no resident state (call mm_state_set)");
   MM_CHECK(c.nseg > 1, "mm_minimize_segmented needs a context created with mm_ctx_create_segmented");
+  MM_TRY(check_params(p));
+  MM_CHECK(p->method == 0, "segmented minimisation runs pcohen_hs (method 0)");
   MM_CHECK(c.lv.size() > 1 && c.lv[0].M != nullptr && c.geom_mode == kGeomCoords,
            "segmented minimisation needs the AMG V-cycle and the on-the-fly apply");
   MinParams q;

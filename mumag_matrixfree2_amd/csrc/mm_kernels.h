@@ -32,6 +32,7 @@ constexpr uint32_t kOwner = 1u << 31;    // this block adds reg*x / writes the b
 constexpr uint32_t kPrivate = 1u << 30;  // no other block touches the node: plain store
 constexpr uint32_t kBoundary = 1u << 29; // Dirichlet node (outer surface)
 constexpr uint32_t kIdMask = (1u << 29) - 1;
+constexpr int kMaxLbfgsMemory = 32;       // L-BFGS history pairs (device scalar slots)
 constexpr int kMaxMat = 255;               // materials (uint8 index per element)
 constexpr int kRedBlocks = 2048;         // max workgroups of a reducing kernel
 constexpr int kNumScal = 256;            // device scalar slots

@@ -84,15 +84,46 @@ def volume_averaged_m(m: np.ndarray, conn: np.ndarray, Ve: np.ndarray, magnetic:
     return (w[:, None] * me).sum(axis=0) / (w.sum() + 1e-30)
 
 
+def _params_table(params: LoopParams) -> dict:
+    """The LoopParams fields as the rows of params.log (h_dir as a list, ``extra`` merged in)."""
+    from dataclasses import fields
+
+    d = {}
+    for f in fields(params):
+        v = getattr(params, f.name)
+        if f.name == "extra":
+            continue
+        d[f.name] = np.asarray(v).tolist() if isinstance(v, np.ndarray) else v
+    d.update(params.extra)
+    return d
+
+
+def _params_sources(params: LoopParams) -> dict:
+    """'default' where a field holds its LoopParams default, 'caller' where it was set."""
+    from dataclasses import MISSING, fields
+
+    src = {}
+    for f in fields(params):
+        if f.name == "extra":
+            continue
+        default = f.default if f.default is not MISSING else MISSING
+        v = getattr(params, f.name)
+        same = default is not MISSING and not isinstance(v, np.ndarray) and v == default
+        src[f.name] = "default" if same else "caller"
+    src.update({k: "caller" for k in params.extra})
+    return src
+
+
 def run_hysteresis_loop(ctx: MeshContext, m0: np.ndarray, params: LoopParams, *, steps: int | None = None,
                         on_step=None, out_dir=None, name: str = "hysteresis", snapshot_every: int = 0) -> dict:
     """Sweep the field along ``params.h_dir`` and minimise at each value.
 
-    Returns a dict with ``history`` rows (B_ext [Js_ref units], J_par [Js_ref units], E, gnorm,
+    Returns a dict with the ``switching_step``, the ``nucleation`` field and index
+    (``io_utils.nucleation_field``), ``history`` rows (B_ext [Js_ref units], J_par [Js_ref units], E, gnorm,
     iters, evals, demag_iters, preco_iters, seconds) and the final ``m``, ``U``.
 
     With ``out_dir``, the result files of ``src/hysteresis_loop.py:455-657`` are written there:
-    ``hysteresis.csv`` (one row per step), ``<name>.mh`` (B [T], J_par [T], mx, my, mz,
+    ``params.log`` (``src/loop.py:1046-1058``), ``hysteresis.csv`` (one row per step), ``<name>.mh`` (B [T], J_par [T], mx, my, mz,
     E [J/m^3]) and, every ``snapshot_every`` steps, a ``state_cfg*.vtu`` with m and U. The
     host then downloads m once per step for the volume-averaged (mx, my, mz).
 
@@ -129,6 +160,7 @@ def run_hysteresis_loop(ctx: MeshContext, m0: np.ndarray, params: LoopParams, *,
 
         out = Path(out_dir)
         out.mkdir(parents=True, exist_ok=True)
+        io_utils.write_params_log(out, _params_table(params), _params_sources(params))
         csv_path = out / "hysteresis.csv"
         io_utils.write_hysteresis_header(csv_path)
         conn, Ve, mag = ctx.element_table()
@@ -174,6 +206,10 @@ def run_hysteresis_loop(ctx: MeshContext, m0: np.ndarray, params: LoopParams, *,
         io_utils.write_mh(Path(out_dir) / name, np.asarray(mh))
     out = {"history": hist, "m": m, "U": U, "seconds": time.perf_counter() - t_all,
            "switching_step": switching_step(hist[:, 1]) if len(hist) else -1}
+    if len(hist) > 1:
+        from .io_utils import nucleation_field
+
+        out["nucleation"] = nucleation_field(hist[:, 0] * params.Js_ref, hist[:, 1] * params.Js_ref)
     if seg:
         out["grain_J"] = np.asarray(grain_J)
     return out

@@ -5,7 +5,13 @@
 * ``<name>.mh``: the mammos-mumag magnetisation history, a ``#`` header line and columns
   B_ext [T], J_parallel [T], mx, my, mz, E [J/m^3] as ``%.9e`` (``src/io_utils.py:78-95``);
 * ``.vtu``: an ASCII VTK UnstructuredGrid of tetrahedra with point and cell data
-  (``src/io_utils.py:139-268``), e.g. the magnetisation and the potential of a state.
+  (``src/io_utils.py:139-268``), e.g. the magnetisation and the potential of a state;
+* ``params.log``: the run's parameters as a Markdown table ``| Parameter | Value | Source |``,
+  keys sorted, with the derived ``phi_tol`` row after ``cg_tol`` (``src/loop.py:1046-1058``).
+
+``nucleation_field`` reads the nucleation field off a sweep as ``src/extract_nucleation.py:8-44``
+does (first field where dJ/dB exceeds 1% of its maximum, and the field of the largest
+|d2J/dB2|).
 
 Plain host code: the values come from the GPU state (``MeshContext.state_get``,
 ``state_j_parallel``) and are written here.
@@ -99,3 +105,44 @@ def write_vtu_tetra(path, points, tets, point_data: Mapping[str, np.ndarray] | N
     out = Path(path)
     out.write_text("\n".join(L) + "\n", encoding="utf-8")
     return out
+
+
+def write_params_log(out_dir, params: Mapping[str, object], sources: Mapping[str, str] | None = None) -> Path:
+    """``params.log`` in ``out_dir``: one row per parameter, sorted by name, keys containing
+    'extrapolate' skipped, and after ``cg_tol`` the derived ``phi_tol = min(cg_tol, 0.1 tau_f)``
+    (``src/loop.py:1046-1058``). ``sources`` maps a key to its source ('default' otherwise)."""
+    sources = sources or {}
+    out = Path(out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    lines = ["| Parameter | Value | Source |", "| :--- | :--- | :--- |"]
+    for k in sorted(params):
+        if "extrapolate" in k:
+            continue
+        lines.append(f"| {k} | {params[k]} | {sources.get(k, 'default')} |")
+        if k == "cg_tol":
+            phi_tol = float(min(float(params["cg_tol"]), float(params["tau_f"]) * 0.1))
+            lines.append(f"| phi_tol | {phi_tol} | derived |")
+    path = out / "params.log"
+    path.write_text("\n".join(lines) + "\n", encoding="utf-8")
+    return path
+
+
+def nucleation_field(B_T, J_T) -> dict | None:
+    """Nucleation field of a sweep (``src/extract_nucleation.py:8-44``): dJ/dB by
+    ``np.gradient`` over the field values; the first index where it exceeds 1% of its maximum
+    (``index``, ``B_nuc``) and the index of the largest |d2J/dB2| (``index_2nd``, ``B_nuc_2nd``).
+    None when the largest dJ/dB is below 1e-3 (no nucleation in the range)."""
+    B = np.asarray(B_T, dtype=np.float64)
+    J = np.asarray(J_T, dtype=np.float64)
+    if B.size < 2:
+        return None
+    dJ = np.gradient(J, B)
+    d2J = np.gradient(dJ, B)
+    mx = float(np.max(dJ))
+    if mx < 1e-3:
+        return None
+    idx = np.where(dJ > 0.01 * mx)[0]
+    if idx.size == 0:
+        return None
+    i2 = int(np.argmax(np.abs(d2J)))
+    return {"index": int(idx[0]), "B_nuc": float(B[idx[0]]), "index_2nd": i2, "B_nuc_2nd": float(B[i2])}

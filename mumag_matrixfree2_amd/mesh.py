@@ -168,7 +168,8 @@ def morton_encode(q: np.ndarray) -> np.ndarray:
 def morton_reorder(knt: np.ndarray, ijk: np.ndarray, extra_node_arrays: tuple = ()) -> dict:
     """Morton-sort nodes and elements (``src/reorder_mesh.py:86-101``, target 'gpu').
 
-    Sorting uses a stable argsort so the permutation is reproducible.
+    Both sorts use NumPy's default argsort kind, as the reference does, so keys that tie (the
+    centroids of a hex's tets can share a quantised key) are ordered the same way.
 
     Returns a dict with the permuted ``knt``, ``ijk``, ``node_perm`` (new -> old),
     ``inv_node_perm`` (old -> new), ``elem_perm`` and the permuted ``extra`` node arrays.
@@ -179,12 +180,12 @@ def morton_reorder(knt: np.ndarray, ijk: np.ndarray, extra_node_arrays: tuple = 
     denom = knt.max(axis=0) - lo
     denom = np.where(denom == 0, 1.0, denom)
     qn = ((knt - lo) / denom * 1023).astype(np.int32)
-    node_perm = np.argsort(morton_encode(qn), kind="stable")
+    node_perm = np.argsort(morton_encode(qn))
     inv = np.empty(knt.shape[0], dtype=np.int32)
     inv[node_perm] = np.arange(knt.shape[0], dtype=np.int32)
     cen = knt[ijk[:, :4].astype(np.int64)].mean(axis=1)
     qc = ((cen - lo) / denom * 1023).astype(np.int32)
-    elem_perm = np.argsort(morton_encode(qc), kind="stable")
+    elem_perm = np.argsort(morton_encode(qc))
     ijk_new = ijk[elem_perm].copy()
     ijk_new[:, :4] = inv[ijk_new[:, :4].astype(np.int64)]
     return {

@@ -39,6 +39,11 @@ def make_minimizer(geom: TetGeom, A_lookup, K1_lookup, Js_lookup, k_easy_lookup,
     def minimize(m0, B_ext, **params):
         U0 = params.pop("U0", None)
         params.pop("sparse_ops", None)
+        # the sweep passes memory=params.memory with every call (src/hysteresis_loop.py:504-531), but
+        # the L-BFGS history length is the factory's (src/minimizers.py:3099): the call's value is
+        # dropped, as the reference drops it
+        params.pop("memory", None)
+        params.pop("cg_tol", None)
         p = make_params(method, cg_tol=cg_tol, memory=memory, cg_maxiter=params.pop("cg_maxiter", cg_maxiter),
                         **params)
         m0 = np.asarray(m0, dtype=np.float64)

@@ -76,3 +76,22 @@ def test_without_a_device_creation_is_an_error():
     assert r.returncode == 0, r.stderr[-2000:]
     out = r.stdout.strip().splitlines()[-1]
     assert out == "BUILT" or (out.startswith("ERROR") and "mm_ctx_create" in out), out
+
+
+@pytest.mark.parametrize("bad", [dict(memory=33), dict(memory=-1), dict(ls_max_evals=0), dict(tau_f=float("nan")),
+                                 dict(cg_maxiter=0), dict(pc_iters=-1)])
+def test_params_check_rejects_out_of_range(bad):
+    """The C-ABI rejects parameter structs the device code cannot run (host-only check, no GPU):
+    the L-BFGS history holds at most 32 pairs, limits are non-negative, tolerances finite."""
+    from mumag_matrixfree2_amd.context import make_params
+
+    with pytest.raises(RuntimeError, match="mm_params_check"):
+        make_params("lbfgs", **bad)
+
+
+def test_params_check_accepts_reference_defaults():
+    from mumag_matrixfree2_amd.context import make_params
+
+    for method in ("pcohen_hs", "lbfgs"):
+        for mem in (1, 10, 32):
+            assert make_params(method, memory=mem).memory == mem

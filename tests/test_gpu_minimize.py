@@ -111,6 +111,9 @@ def test_sweep_writes_result_files(small_pair, tmp_path):
     mx_ref = [ctx.j_parallel(m_fin, e) for e in np.eye(3)]
     np.testing.assert_allclose(mh[-1, 2:5], mx_ref, atol=1e-12)
     assert sorted(f.name.split("_B")[0] for f in tmp_path.glob("*.vtu")) == ["state_cfg00000", "state_cfg00002"]
+    plog = (tmp_path / "params.log").read_text().splitlines()
+    assert plog[0] == "| Parameter | Value | Source |"
+    assert "| method | pcohen_hs | default |" in plog and f"| phi_tol | {1e-10 * 0.1} | derived |" in plog
 
 
 @pytest.mark.parametrize("k", [4, 8])

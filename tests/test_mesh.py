@@ -35,6 +35,43 @@ def test_morton_reorder_is_a_relabelling():
     assert np.array_equal(np.sort(a["boundary_mask"]), np.sort(b["boundary_mask"]))
 
 
+def test_morton_key_bit_layout():
+    """x bit i at 3i+2, y at 3i+1, z at 3i (src/reorder_mesh.py:19-30), checked on single bits
+    and against a bit-string interleave."""
+    from mumag_matrixfree2_amd.mesh import morton_encode
+
+    q = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [1023, 1023, 1023], [512, 0, 0], [0, 0, 512]])
+    assert morton_encode(q).tolist() == [4, 2, 1, 2**30 - 1, 1 << 29, 1 << 27]
+    rng = np.random.default_rng(5)
+    q = rng.integers(0, 1024, (200, 3))
+    ref = [int("".join(f"{x:010b}"[i] + f"{y:010b}"[i] + f"{z:010b}"[i] for i in range(10)), 2) for x, y, z in q]
+    assert morton_encode(q).tolist() == ref
+
+
+def test_morton_order_follows_the_reference_sort_with_ties():
+    """Nodes and elements are sorted by Morton key with the reference's argsort (default kind);
+    keys are non-decreasing along the new order. A far-away unused node stretches the bounding
+    box so that the tets of a hex share quantised centroids: the keys tie, and the tie order is
+    the reference's."""
+    from mumag_matrixfree2_amd.mesh import graded_airbox_cube, morton_encode, morton_reorder
+
+    mesh = graded_airbox_cube(8.0, 4, 2, 2.0)
+    knt = np.vstack([mesh["knt"], [[4000.0, 4000.0, 4000.0]]])
+    ijk = mesh["ijk"]
+    r = morton_reorder(knt, ijk)
+    lo, hi = knt.min(axis=0), knt.max(axis=0)
+    qn = ((knt - lo) / (hi - lo) * 1023).astype(np.int32)
+    kn = morton_encode(qn)
+    qc = ((knt[ijk[:, :4]].mean(axis=1) - lo) / (hi - lo) * 1023).astype(np.int32)
+    kc = morton_encode(qc)
+    assert np.unique(kc).size < kc.size  # ties exist
+    assert np.array_equal(r["node_perm"], np.argsort(kn))
+    assert np.array_equal(r["elem_perm"], np.argsort(kc))
+    assert np.all(np.diff(kn[r["node_perm"]]) >= 0) and np.all(np.diff(kc[r["elem_perm"]]) >= 0)
+    assert np.array_equal(r["knt"], knt[r["node_perm"]])
+    assert np.array_equal(r["node_perm"][r["ijk"][:, :4]], ijk[r["elem_perm"], :4])
+
+
 def test_polycrystal_phases_by_brute_force():
     """Grain = nearest centre, grain boundary = within gb_width of the bisector plane of the
     two nearest centres, air = outside the cube; checked tet by tet with a full sort."""
