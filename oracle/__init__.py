@@ -24,15 +24,28 @@ How each part is pinned (the tests under ``tests/test_oracle_*.py``):
                      45 deg anisotropy, cube demag ~ 1/3), central differences of
                      ``tests/test_gradients.py``, E = 1/2 m.(g + g_z)
   PCG / solve_U      scipy ``spsolve`` of the same masked matrix; residual <= tol
-  V-cycle            SPD check (z.r > 0) and a dense reference on small levels
-  Cayley / tangent   |m| = 1, orthogonality of the rotation, closed-form small rotations
-  Steihaug PCG       one iteration equals the closed-form Cauchy step; exact solve
-                     on an SPD-projected system
-  line search        closed-form quadratic model (known accepted step)
-  pcohen_hs / lbfgs  energy decrease per accepted step, |m| = 1, S,Y tangent, s.y > 0,
-                     and both methods converge to the same minimum.
+  V-cycle            one level = the exact inverse; on two and three levels the error
+                     propagation equals (I - MA)(I - P B_c R A)(I - MA) with the coarse
+                     V-cycle B_c (dense random Galerkin hierarchies, both smoothers);
+                     symmetric positive definite on the mesh hierarchy; planted mistakes
+                     (no post-smoothing, subtracted coarse correction) fail the check
+  Cayley / tangent   |m| = 1, rotation by 2 atan|k| about k, orthogonality
+  Steihaug PCG       the stagnation exit after one step returns the Cauchy point (2 iterations),
+                     negative curvature and zero iterations fall back to z = M g_tan, the 10x
+                     clip gives y = 10 z on a near-singular P, exact solve on an SPD system
+  line search        a one-node Cayley path with a prescribed ratio d(s): hand-derived accepted
+                     step and evaluation count for expansion, contraction (towards s_min),
+                     failure (tau = 0 after 16 evaluations), the 15-evaluation expansion cap,
+                     non-finite energies and the warm-start chain of the solves; planted
+                     changes of the rule fail
+  pcohen_hs          restart period 1 reproduces preconditioned steepest descent iterate for
+                     iterate; the HS direction is conjugate to the gradient difference up to
+                     the projection defect (the Polak-Ribiere beta fails this); restart period
   lbfgs_two_loop     the explicit product of BFGS inverse updates from H0 = gam diag(invMr),
                      with the ring of pairs wrapped, and the secant condition H y = s.
-Parity of the L-BFGS iterate trajectory against the reference JAX code is
-unpinned: JAX is not installed here, so only the checks above cover it.
+  minimisers         energy decrease per accepted step, |m| = 1, S,Y tangent, s.y > 0, and
+                     pcohen_hs and L-BFGS converge to the same minimum.
+Parity of the L-BFGS iterate trajectory against the reference JAX code rests on the two-loop
+check and these invariants: JAX is not installed here. The SA hierarchy (pyamg's role) is
+the build's own and has no reference to be pinned against.
 """

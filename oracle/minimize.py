@@ -182,7 +182,11 @@ class Minimizer:
     def minimize(self, m0, B_ext, U0=None, tau_f=1e-10, eps_a=1e-12, max_iter=2000, tau0=1e-2,
                  ls_eta1=0.1, ls_eta2=0.1, ls_C=2.0, ls_c=0.5, pc_iters=10, pc_auto=True,
                  pc_force_eta=0.5, pc_force_alpha=0.5, pc_stagnation_nu=0.01, pc_reg=0.0, L=None,
-                 trace=None):
+                 trace=None, trace_vectors=False, beta_type="hs"):
+        """One field step. ``beta_type`` 'hs' (Hestenes-Stiefel, the reference's pcohen_hs) or 'pr'
+        (Polak-Ribiere), the two branches of ``make_pcohen_minimizer`` (``src/minimizers.py:1000-1010``).
+        ``trace`` (a list) receives one dict per iteration; with ``trace_vectors`` the pcohen_hs
+        entries also hold m, y, the gradient difference, d_prev and d."""
         p = self.p
         N = p.N
         phi_tol = float(min(self.cg_tol, tau_f * 0.1))
@@ -211,9 +215,14 @@ class Minimizer:
                                     pc_reg, pc_stagnation_nu)
                 gs = float(np.max(np.abs(y)))
                 dg = g_tan_ext - g_prev
-                beta = 0.0 if it % Lp == 0 else max(0.0, np.vdot(y, dg) / (np.vdot(d_prev, dg) + 1e-30))
+                if beta_type == "pr":
+                    den = np.vdot(y_prev, g_prev) + 1e-30
+                else:
+                    den = np.vdot(d_prev, dg) + 1e-30
+                beta = 0.0 if it % Lp == 0 else max(0.0, np.vdot(y, dg) / den)
                 d = -y + beta * tangent_grad(m, d_prev)
-                if np.vdot(d, g_tan_ext) > 0:
+                fallback = bool(np.vdot(d, g_tan_ext) > 0)
+                if fallback:
                     d = -y
                 H = np.cross(m, d)
                 pg = float(np.vdot(g_raw, d))
@@ -221,7 +230,11 @@ class Minimizer:
                     eg, self.solve_U, m, pg, H, E, U, g_raw, B_ext, phi_tol, s0=1.0, **ls_args)
                 conv = check_convergence(it, E_new, E, m, m_new, gs, tau_f, eps_a)
                 if trace is not None:
-                    trace.append(dict(it=it, E=E_new, tau=tau, pg=pg, evals=le, preco=pit, gnorm=gs))
+                    t = dict(it=it, E=E_new, tau=tau, pg=pg, evals=le, preco=pit, gnorm=gs, beta=beta,
+                             fallback=fallback)
+                    if trace_vectors:
+                        t.update(m=m, y=y, dg=dg, d_prev=d_prev, d=d)
+                    trace.append(t)
                 g_prev, y_prev, d_prev = g_tan_ext, y, d
                 m, U, g_raw, E, gnorm = m_new, U_new, g_new, E_new, gs
                 it += 1
