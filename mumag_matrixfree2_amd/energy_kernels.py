@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .fem_utils import TetGeom, context_for
+from .fem_utils import TetGeom, context_for, context_with_tables
 
 
 def make_energy_kernels(geom: TetGeom, A_lookup, K1_lookup, Js_lookup, k_easy_lookup, V_mag, M_nodal, *,
@@ -22,7 +22,8 @@ def make_energy_kernels(geom: TetGeom, A_lookup, K1_lookup, Js_lookup, k_easy_lo
     """
     if B_bias is not None and np.any(np.asarray(B_bias) != 0):
         raise NotImplementedError("per-node bias fields are not supported by the HIP backend")
-    ctx = context or context_for(geom, A_lookup, K1_lookup, Js_lookup, k_easy_lookup, boundary_mask, reg)
+    ctx = (context or context_with_tables(geom, A_lookup, K1_lookup, Js_lookup, k_easy_lookup)
+           or context_for(geom, A_lookup, K1_lookup, Js_lookup, k_easy_lookup, boundary_mask, reg))
     if abs(ctx.V_mag - float(V_mag)) > 1e-9 * max(1.0, abs(float(V_mag))):
         raise ValueError(f"V_mag {V_mag} differs from the mesh's magnetic volume {ctx.V_mag}")
     if M_nodal is not None:
@@ -54,6 +55,14 @@ def compute_exchange_diagonal(geom: TetGeom, A_lookup, V_mag, *, context=None, *
     """
     if context is None:
         A = np.asarray(A_lookup, dtype=np.float64)
-        context = context_for(geom, A, np.zeros_like(A), (A != 0).astype(np.float64), np.tile([0, 0, 1.0], (A.size, 1)))
+        from .fem_utils import _CACHE
+
+        for e in _CACHE:  # any context of this geometry with the same exchange constants
+            if e["geom"] is geom and e["A"].shape == A.shape and np.array_equal(e["A"], A):
+                context = e["ctx"]
+                break
+        else:
+            context = context_for(geom, A, np.zeros_like(A), (A != 0).astype(np.float64),
+                                  np.tile([0, 0, 1.0], (A.size, 1)))
     d = context.exchange_diagonal()
     return d * context.V_mag / float(V_mag)

@@ -12,7 +12,7 @@ from __future__ import annotations
 import numpy as np
 
 from .context import make_params
-from .fem_utils import TetGeom, context_for
+from .fem_utils import TetGeom, cached_inputs, context_for, tables_match
 
 SUPPORTED = ("pcohen_hs", "lbfgs")
 
@@ -28,9 +28,20 @@ def make_minimizer(geom: TetGeom, A_lookup, K1_lookup, Js_lookup, k_easy_lookup,
     """
     if method not in SUPPORTED:
         raise NotImplementedError(f"method {method!r} is not implemented by the HIP backend ({SUPPORTED})")
-    ctx = context or getattr(solve_U, "_ctx", None)
+    ctx = context
     if ctx is None:
-        ctx = context_for(geom, A_lookup, K1_lookup, Js_lookup, k_easy_lookup, boundary_mask)
+        sctx = getattr(solve_U, "_ctx", None)
+        if sctx is not None and tables_match(sctx, A_lookup, K1_lookup, Js_lookup, k_easy_lookup):
+            ctx = sctx
+        else:
+            # the solver's context lacks the exchange/anisotropy tables: a context with the full
+            # tables and the solver's Dirichlet mask, regularisation and preconditioner
+            mask, reg = (boundary_mask, 1e-12) if sctx is None else (cached_inputs(sctx) or (boundary_mask, sctx.reg))
+            ctx = context_for(geom, A_lookup, K1_lookup, Js_lookup, k_easy_lookup, mask, reg)
+            if sctx is not None and sctx.precond_type is not None and ctx.precond_type != sctx.precond_type:
+                ctx.setup_precond(sctx.precond_type)
+            if sctx is not None:
+                ctx.set_precision(sctx.precision)
     if ctx.precond_type is None:
         ctx.setup_precond("amgcl")
     memory = int(kwargs.get("memory", 10))  # src/minimizers.py:3099
@@ -44,6 +55,9 @@ def make_minimizer(geom: TetGeom, A_lookup, K1_lookup, Js_lookup, k_easy_lookup,
         # dropped, as the reference drops it
         params.pop("memory", None)
         params.pop("cg_tol", None)
+        # both methods call the line search with 15 evaluations per phase whatever the caller's
+        # ls_max_evals (src/minimizers.py:1035,1335)
+        params.pop("ls_max_evals", None)
         p = make_params(method, cg_tol=cg_tol, memory=memory, cg_maxiter=params.pop("cg_maxiter", cg_maxiter),
                         **params)
         m0 = np.asarray(m0, dtype=np.float64)

@@ -14,12 +14,19 @@ from collections.abc import Callable
 
 import numpy as np
 
-from .fem_utils import TetGeom, context_for
+from .fem_utils import TetGeom, context_for, material_tables_of
 
 
 def _poisson_context(geom, Js_lookup, boundary_mask, reg, context):
+    """The Poisson operators depend on the geometry, Js, the Dirichlet mask and reg. When the
+    energy kernels of the same geometry were built first (the order of
+    src/hysteresis_loop.py:309-338), their full material tables are used, so the minimiser built
+    on this solver shares its context."""
     if context is not None:
         return context
+    full = material_tables_of(geom, Js_lookup)
+    if full is not None:
+        return context_for(geom, *full, boundary_mask, reg)
     Js = np.asarray(Js_lookup, dtype=np.float64)
     z = np.zeros_like(Js)
     return context_for(geom, z, z, Js, np.tile([0.0, 0.0, 1.0], (Js.size, 1)), boundary_mask, reg)
