@@ -147,6 +147,15 @@ int mm_solve_U_segmented(int64_t ctx, const double* m, double* U_inout, const do
 
 /* energy_and_grad (src/energy_kernels.py:340-459): E scalar, g (N,3). */
 int mm_energy_grad(int64_t ctx, const double* m, const double* U, const double* B, double* E, double* g);
+/* energy_and_grad on device-resident arrays, allocating nothing: m_dev (N,3), U_dev (N,), E_dev
+ * (1 double) and g_dev (N,3) are device pointers, B three host doubles; launched on `stream`
+ * (a hipStream_t; NULL: the context's stream) without synchronising. The energy's block partials
+ * go to a fixed 512 KiB device array of the library instead of a context buffer, so meshes with
+ * more than 32768 magnetic element blocks are rejected. Every call on one GPU shares that array:
+ * calls that overlap on one GPU (two streams, two threads) race. No reference counterpart; for
+ * callers that cannot allocate device memory (HIP graph capture, serving loops). */
+int mm_energy_grad_noalloc(int64_t ctx, const double* m_dev, const double* U_dev, const double* B, double* E_dev,
+                           double* g_dev, void* stream);
 /* local_grad_only (src/energy_kernels.py:272-338): g = (K_ex + K_an) v / V_mag. */
 int mm_local_grad(int64_t ctx, const double* v, double* g);
 

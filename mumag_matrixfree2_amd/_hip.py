@@ -86,6 +86,7 @@ SIGNATURES = {
     "mm_solve_U_segmented": [C.c_int64, f64p, f64p, f64p, C.c_int, i32p, f64p],
     "mm_energy_grad": [C.c_int64, f64p, f64p, f64p, f64p, f64p],
     "mm_local_grad": [C.c_int64, f64p, f64p],
+    "mm_energy_grad_noalloc": [C.c_int64, C.c_void_p, C.c_void_p, f64p, C.c_void_p, C.c_void_p, C.c_void_p],
     "mm_j_parallel": [C.c_int64, f64p, f64p, f64p],
     "mm_state_set": [C.c_int64, f64p, f64p],
     "mm_state_get": [C.c_int64, f64p, f64p],

@@ -19,6 +19,33 @@ namespace mm {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 
+// Scratch of the allocation-free energy entry point (mm_energy_grad_noalloc): the per-block
+// (sum, compensation) partials of the energy, 2 doubles per magnetic element block, in a
+// file-scope device array of 512 KiB that every GPU the library runs on holds until the process
+// ends. Host code never uses the array's name as an address: the device address is resolved
+// once per device with hipGetSymbolAddress (with that device current, when the first context
+// on it is created) and kept in g_scratch_dev.
+constexpr long kScratchDoubles = 1L << 16;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_dev;  // device id -> address of that device's scratch
+
+static int resolve_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_dev.size() <= device) g_scratch_dev.resize(device + 1, nullptr);
+  if (g_scratch_dev[device]) return 0;
+  double* ptr = nullptr;
+  MM_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+  MM_CHECK(ptr != nullptr, "scratch array has no device address");
+  g_scratch_dev[device] = ptr;
+  return 0;
+}
+
+static double* scratch_on(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device < (int)g_scratch_dev.size() ? g_scratch_dev[device] : nullptr;
+}
+
 static std::mutex g_mu;
 static std::vector<Ctx*> g_ctx;  // index + 1 is the public handle
 
@@ -437,6 +464,7 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   c.node_cap = (flags >> 8) & 511 ? (flags >> 8) & 511 : kMaxBlockNodes;
   MM_CHECK(c.geom_mode <= 2, "invalid geometry mode");
   MM_HIP(hipGetDevice(&c.device));
+  MM_TRY(resolve_scratch(c.device));
   MM_HIP(hipDeviceGetAttribute(&c.num_cu, hipDeviceAttributeMultiprocessorCount, c.device));
   MM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   MM_HIP(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming));
@@ -1146,6 +1174,22 @@ int mm_energy_grad(int64_t h, const double* m, const double* U, const double* B,
   MM_HIP(hipMemcpyAsync(g, c.g_t, 3 * c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   MM_HIP(hipMemcpyAsync(E, c.dscal + 16, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   MM_HIP(hipStreamSynchronize(c.stream));
+  c.n_energy++;
+  return 0;
+}
+
+int mm_energy_grad_noalloc(int64_t h, const double* m_dev, const double* U_dev, const double* B, double* E_dev,
+                           double* g_dev, void* stream) {
+  GET_CTX(h);
+  MM_CHECK(m_dev && U_dev && B && E_dev && g_dev, "null argument");
+  MM_CHECK(2L * c.part_mag.nblk <= kScratchDoubles,
+           "mesh too large for the allocation-free energy: 2 x magnetic blocks exceeds the 65536-double scratch");
+  double* sp = scratch_on(c.device);
+  MM_CHECK(sp != nullptr, "scratch address not resolved for the context's device");
+  hipStream_t s = stream ? (hipStream_t)stream : c.stream;
+  vfill(g_dev, 0.0, 3L * c.N, s);
+  MM_TRY(launch_energy_buf(c, m_dev, U_dev, B, g_dev, E_dev, sp, s));
+  MM_HIP(hipGetLastError());
   c.n_energy++;
   return 0;
 }

@@ -293,6 +293,9 @@ int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s);
 int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s);
 int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float* xo, float* y, hipStream_t s);
 int launch_rhs(const Ctx& c, const double* m, double* b, hipStream_t s);
+// launch_energy with the block partials (2 doubles per magnetic block) in a caller-given buffer
+int launch_energy_buf(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,
+                      double* partial, hipStream_t s);
 int launch_energy(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,
                   hipStream_t s);
 int launch_local(const Ctx& c, const double* v, double* out, const double* gate, hipStream_t s);

@@ -1086,14 +1086,19 @@ int launch_rhs(const Ctx& c, const double* m, double* b, hipStream_t s) {
   return 0;
 }
 
-int launch_energy(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,
-                  hipStream_t s) {
+int launch_energy_buf(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,
+                      double* partial, hipStream_t s) {
   const Part& p = c.part_mag;
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
-  k_energy<<<grid, kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, c.partial);
-  k_sum_partials_cs<<<1, kBlock, 0, s>>>(c.partial, p.nblk, E_out);  // partials by block id
+  k_energy<<<grid, kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, partial);
+  k_sum_partials_cs<<<1, kBlock, 0, s>>>(partial, p.nblk, E_out);  // 2 x nblk partials by block id
   return 0;
+}
+
+int launch_energy(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,
+                  hipStream_t s) {
+  return launch_energy_buf(c, m, U, B, g, E_out, c.partial, s);
 }
 
 int launch_local(const Ctx& c, const double* v, double* out, const double* gate, hipStream_t s) {
