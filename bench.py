@@ -1,30 +1,30 @@
-"""Hysteresis benchmark: field-steps/s of the GPU path on the 2.06M-tet cube or the 24.6M-tet polycrystal.
+"""Hysteresis benchmark: field-steps/s of the GPU path, mixed precision against FP64.
 
-Default workload (``--workload cube2M``, the plain FP64 GPU configuration): 80 nm NdFeB-like
-cube, 60^3 hex cells x 6 tets (1,296,000 magnet tets, 226,981 magnet nodes) inside a 5-layer
-graded airbox (2,058,000 tets, 357,911 nodes), Morton-ordered; Js = 1.61 T, A = 7.7 pJ/m,
-K1 = 4.3 MJ/m^3, easy axis z; m0 = z + N(0, 0.01) (seed 0), renormalised; field +2 T -> -8 T
-in 41 steps along (sin 1 deg, 0, cos 1 deg); pcohen_hs, tau_f = 1e-10, PCG tol 1e-10
-(line-search solves at min(cg_tol, 0.1 tau_f)); smoothed-aggregation AMG V-cycle with one pre-
-and one post-smoothing sweep.
+Default workload (``--workload poly25M``, the configuration the metric is quoted on): 200 nm cube
+of 27 Voronoi grains (centres from default_rng(4)), 150^3 hex cells x 6 tets (20.25M magnet tets)
+in a 5-layer graded airbox (24,576,000 tets, 4,173,281 nodes), Morton-ordered; tets within 2 nm of
+a grain-boundary bisector plane form a grain-boundary phase with A = 1.5 pJ/m, K1 = 0; easy axes
+per grain uniform on the sphere (default_rng(40)); Js = 1.61 T, A = 7.7 pJ/m, K1 = 4.3 MJ/m^3;
+m0 = z + N(0, 0.01) (seed 0), renormalised; 81 field values +2 T -> -8 T (dB = -0.125 T) along
+(sin 1 deg, 0, cos 1 deg); pcohen_hs, tau_f = 1e-10, PCG tol 1e-10 (line-search solves at
+min(cg_tol, 0.1 tau_f)); smoothed-aggregation AMG V-cycle with one pre- and one post-smoothing
+sweep. The headline form is mixed precision (``--precision mixed``); the FP64 form runs the same
+field steps in the same process and the line carries the comparison (``mixed_vs_fp64``).
 
-``--workload grains64``: 64 independent grains (src of the set: ``mesh.grain_set``, default_rng(3)):
-total tets per grain log-uniform in [2e4, 4e5] (7,752,942 tets, 1,431,475 nodes in all), cube or
-sphere by coin flip, easy axis uniform on the sphere, each in its own 3-layer graded airbox with
-2 nm cells; 41 field values +2 T -> -8 T; one field step minimises every grain: by default all
-grains in one segmented context (one launch per kernel for all grains, per-grain scalars);
-``--grains-form threads``: one context per grain, host threads driving them (``--threads``).
+``--workload cube2M``: the plain FP64 GPU configuration, 80 nm cube, 60^3 hex cells x 6 tets +
+5-layer graded airbox (2,058,000 tets, 357,911 nodes), 41 field values (dB = -0.25 T).
 
-``--workload poly25M``: 200 nm cube of 27 Voronoi grains (centres from default_rng(4)), 150^3
-hex cells x 6 tets (20.25M magnet tets) in a 5-layer graded airbox (24,576,000 tets, 4,173,281
-nodes); tets within 2 nm of a grain-boundary bisector plane form a grain-boundary phase with
-A = 1.5 pJ/m, K1 = 0; easy axes per grain uniform on the sphere (default_rng(40)); 81 field
-values +2 T -> -8 T (dB = -0.125 T); everything else as above.
+``--workload grains64``: 64 independent grains (``mesh.grain_set``, default_rng(3)): total tets per
+grain log-uniform in [2e4, 4e5] (7,752,942 tets, 1,431,475 nodes in all), cube or sphere by coin
+flip, easy axis uniform on the sphere, each in its own 3-layer graded airbox with 2 nm cells; 41
+field values; one field step minimises every grain: by default all grains in one segmented
+context; ``--grains-form threads``: one context per grain, host threads driving them.
 
 A step is one field step: a full energy minimisation at one field value, with the
 magnetisation and potential resident on the GPU. ``--warmup W`` field steps run first on
-a separate state (untimed); then ``--steps K`` field steps of the sweep from +2 T are
-timed. One JSON line is printed on stdout.
+a separate state (untimed); then the first ``--steps K`` field steps of the sweep from +2 T are
+timed (the line's ``config.workload`` names K and the field range). One JSON line is printed on
+stdout.
 """
 
 from __future__ import annotations
@@ -110,12 +110,6 @@ def apply_algorithmic_bytes(ctx) -> int:
     return 4 * ctx.E + 4 * ctx.node_list_entries + 40 * ctx.N
 
 
-def reference_layout_bytes(ctx) -> int:
-    """Bytes of the same apply in the reference's stored-J^-T layout: per tet 4 x int32 corners,
-    9 J^-T entries and Ve in FP64; per node x read and y written."""
-    return ctx.E * (16 + 10 * 8) + ctx.N * 2 * 8
-
-
 def amg_kwargs(args) -> dict:
     """Options of the host SA set-up from the command line (defaults of ``amg.build_hierarchy``
     where not given)."""
@@ -127,23 +121,25 @@ def amg_kwargs(args) -> dict:
     return kw
 
 
-def measured_traffic():
-    """HBM bytes per k_apply launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    (profiles/k_apply_cl_cube2M_fp64.md), or None if absent."""
-    f = ROOT / "profiles" / "k_apply_cl_cube2M_fp64.md"
+def measured_traffic(workload):
+    """HBM bytes per launch of the PCG's FP64 apply from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (``profiles/k_apply_cl_<workload>_fp64.md``, line
+    ``traffic_bytes_per_launch: N``), or None if there is none."""
+    f = ROOT / "profiles" / f"k_apply_cl_{workload}_fp64.md"
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":")[1])
+            return {"bytes": int(line.split(":")[1]), "source": str(f.relative_to(ROOT))}
     return None
 
 
-def cpu_baseline(mesh, mats, B_vals, budget_s):
+def cpu_baseline(mesh, mats, B_vals, budget_s, log=None, max_iter=2000):
     """The multi-core CPU restatement of the field step (``oracle/cpu_minimizer.cpp``: pcohen_hs on
-    the assembled operators with the same AMG V-cycle, OpenMP over all host threads; kind 'port')
-    on the same mesh, hierarchy and sweep: field steps from +2 T in order until ``budget_s``
-    seconds are spent or the sweep ends. Returns (steps, seconds, minimiser iterations, threads)."""
+    the assembled operators with an SA-AMG V-cycle, OpenMP over all host threads; kind 'port') on
+    ``mesh``: field steps of ``B_vals`` in order from +2 T until ``budget_s`` seconds are spent or
+    the list ends; ``max_iter`` caps the minimiser iterations of each step. Returns (steps,
+    seconds, minimiser iterations, threads, tets)."""
     from mumag_matrixfree2_amd.amg import build_hierarchy
     from oracle import cpu
     from oracle.fem import Problem
@@ -151,18 +147,25 @@ def cpu_baseline(mesh, mats, B_vals, budget_s):
     if not cpu.LIB.exists():
         cpu.build()
     A, K1, Js, k = mats
+    t0 = time.perf_counter()
     p = Problem(mesh["knt"], mesh["ijk"], A, K1, Js, k, boundary_mask=mesh["boundary_mask"])
     cm = cpu.CpuMinimizer(p, build_hierarchy(p.A, "amgcl"))
+    if log is not None:
+        log(f"[bench:cpu] operators and hierarchy of {p.N} nodes in {time.perf_counter() - t0:.1f}s, "
+            f"{cpu.threads()} threads")
     m, U = initial_m(p.N), np.zeros(p.N)
     steps, its = 0, 0
     t0 = time.perf_counter()
     while steps < len(B_vals) and time.perf_counter() - t0 < budget_s:
-        m, U, info = cm.minimize(m, B_vals[steps] * H_DIR, U0=U, tau_f=1e-10, eps_a=1e-12, cg_tol=1e-10)
+        m, U, info = cm.minimize(m, B_vals[steps] * H_DIR, U0=U, tau_f=1e-10, eps_a=1e-12, cg_tol=1e-10,
+                                 max_iter=max_iter)
         its += info["iters"]
         steps += 1
+        if log is not None:
+            log(f"[bench:cpu] step {steps - 1} it={info['iters']} t={time.perf_counter() - t0:.2f}s")
     dt = time.perf_counter() - t0
     cm.close()
-    return steps, dt, its, cpu.threads()
+    return steps, dt, its, cpu.threads(), int(mesh["ijk"].shape[0])
 
 
 def segmented_poisson(grains, ctxs, args, log, union=None, reps=3, tol=1e-10):
@@ -352,7 +355,8 @@ def run_grains(args, log):
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64+f32",
         "data": "synthetic (generated grain set, seeded initial magnetisation)",
-        "config": {"workload": f"{wl['name']}_{args.precision}_sweep+2..-8T_{len(B_vals)}steps", "grains": len(grains),
+        "config": {"workload": f"{wl['name']}_{args.precision}_sweep_first{steps}of{len(B_vals)}steps_from+2T",
+                   "grains": len(grains),
                    "tets": E_tot, "nodes": N_tot, "method": "pcohen_hs", "form": args.grains_form,
                    "threads": args.threads if args.grains_form == "threads" else 1,
                    "precond": f"SA-AMG V-cycle ({args.precond})", "pcg_tol": 1e-10, "tau_f": 1e-10},
@@ -380,19 +384,51 @@ def run_grains(args, log):
         union.close()
 
 
+def cpu_sample_mesh(workload):
+    """The mesh the CPU restatement runs on: the workload's own mesh for the cube; for the 24.6M-tet
+    polycrystal (whose assembled operators the CPU restatement cannot hold in a bounded run) the
+    same 200 nm, 27-grain polycrystal with its grain-boundary phase and airbox at 40^3 hexes."""
+    from mumag_matrixfree2_amd.mesh import config_cube_mesh, polycrystal_cube
+
+    if workload == "cube2M":
+        return config_cube_mesh(80.0, 60, 5), materials(), "the same mesh"
+    return (polycrystal_cube(200.0, 40, 5, n_grains=N_GRAINS, gb_width=2.0, seed=4), poly_materials(),
+            "the same 200 nm 27-grain polycrystal (grain-boundary phase, airbox) meshed at 40^3 hexes x 6 tets")
+
+
+def host_cpu_description() -> dict:
+    """CPU model, sockets, physical cores and logical CPUs of the host (lscpu, /proc/cpuinfo)."""
+    import subprocess
+
+    out = {"model": None, "sockets": None, "physical_cores": None, "logical_cpus": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {ln.split(":", 1)[0].strip(): ln.split(":", 1)[1].strip() for ln in txt.splitlines() if ":" in ln}
+        out["model"] = kv.get("Model name")
+        if kv.get("Socket(s)", "").isdigit():
+            out["sockets"] = int(kv["Socket(s)"])
+        if out["sockets"] and kv.get("Core(s) per socket", "").isdigit():
+            out["physical_cores"] = out["sockets"] * int(kv["Core(s) per socket"])
+    except Exception:
+        pass
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="cube2M", choices=sorted(WORKLOADS))
-    ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="poly25M", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=20, help="field steps timed, from +2 T (at most the sweep length)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed field steps first, on a separate state")
     ap.add_argument("--precond", default="amgcl", choices=["amgcl", "amg"])
     ap.add_argument("--method", default="pcohen_hs", choices=["pcohen_hs", "lbfgs"],
                     help="field-step minimiser (cube2M / poly25M; the grain set runs pcohen_hs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="time budget of the CPU baseline (field steps from the start of the sweep)")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
+                    help="headline form; with 'mixed' the FP64 form runs the same steps for the comparison")
+    ap.add_argument("--no-fp64-compare", action="store_true", help="mixed headline without the FP64 sweep")
     ap.add_argument("--max-coarse", type=int, default=None)
     ap.add_argument("--amg-theta", default=None,
                     help="strength thresholds of the SA set-up, fine level first, comma-separated (default 0.1,0.05)")
@@ -405,8 +441,14 @@ def main():
     ap.add_argument("--solve-guess", type=int, default=8,
                     help="initial guess of the minimiser's Poisson solves: projection onto the last K solutions "
                          "(0: the previous solution, as the reference)")
-    ap.add_argument("--no-guess-compare", action="store_true",
-                    help="do not rerun the sweep with the reference warm start for comparison")
+    ap.add_argument("--max-iter", type=int, default=2000,
+                    help="minimiser iterations per field step at most (the workload's is 2000; lower only to "
+                         "bound profiler runs)")
+    ap.add_argument("--calibrate-copies", type=int, default=0,
+                    help="after the sweep, launch this many plain copies of N doubles (k_copy_calib), the "
+                         "known-size reference of the HBM byte counters")
+    ap.add_argument("--guess-compare", action="store_true",
+                    help="also rerun the headline sweep with the reference warm start")
     ap.add_argument("--threads", type=int, default=8, help="host threads driving the grains (grains64)")
     ap.add_argument("--grains-form", default="segmented", choices=["segmented", "threads"],
                     help="grains64: one segmented context for all grains, or one context per grain on host threads")
@@ -425,6 +467,7 @@ def main():
 
     from mumag_matrixfree2_amd.context import MeshContext, make_params
     from mumag_matrixfree2_amd.hysteresis_loop import field_values, switching_step
+    from mumag_matrixfree2_amd.io_utils import nucleation_field
 
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     if args.workload == "grains64":
@@ -437,7 +480,6 @@ def main():
                       boundary_mask=mesh["boundary_mask"], block_nodes=args.block_nodes)
     log(f"[bench] context {time.perf_counter() - t0:.1f}s")
     ctx.setup_precond(args.precond, log=log, **amg_kwargs(args))
-    ctx.set_precision(args.precision)
     from mumag_matrixfree2_amd.autotune import tune_apply
 
     if args.apply_shape == "auto":
@@ -450,10 +492,10 @@ def main():
     log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: N={ctx.N} E={ctx.E} "
         f"AMG levels={[l['A'].shape[0] for l in ctx.amg_levels]}")
 
-    p = make_params(args.method, tau_f=1e-10, eps_a=1e-12, cg_tol=1e-10, max_iter=2000)
+    p = make_params(args.method, tau_f=1e-10, eps_a=1e-12, cg_tol=1e-10, max_iter=args.max_iter)
     B_vals = field_values(2.0 / JS, -8.0 / JS, wl["dB"] / JS, False)
     m0 = initial_m(ctx.N)
-    steps = min(args.steps or wl["steps"], len(B_vals))
+    steps = max(1, min(args.steps, len(B_vals)))
 
     def sweep(precision, timed, guess=None):
         ctx.set_precision(precision)
@@ -461,19 +503,24 @@ def main():
         ctx.state_set(m0, np.zeros(ctx.N))
         if timed:
             ctx.apply_timing(True)
-        J, E, rows, iters = [], [], [], []
+        c0 = ctx.counters()
+        J, E, iters = [], [], []
         t_start = time.perf_counter()
         for k in range(steps):
             ts = time.perf_counter()
             info = ctx.minimize_resident(p, B_vals[k] * H_DIR)
             J.append(ctx.state_j_parallel(H_DIR))
             E.append(info["E"])
-            rows.append(time.perf_counter() - ts)
             iters.append(info["iters"])
             log(f"[bench:{precision}] step {k:2d} B={B_vals[k] * JS:+.3f} T J={J[-1] * JS:+.5f} T "
                 f"E={info['E']:+.10e} it={info['iters']} nf={info['evals']} icg={info['demag_iters']} "
-                f"t={rows[-1]:.3f}s")
-        return time.perf_counter() - t_start, np.array(J), np.array(E), rows, iters
+                f"t={time.perf_counter() - ts:.3f}s")
+        el = time.perf_counter() - t_start
+        c1 = ctx.counters()
+        timing = ctx.apply_timing(False) if timed else None
+        return {"seconds": el, "J": np.array(J), "E": np.array(E), "iters": iters,
+                "pcg_iters": c1["pcg_iters"] - c0["pcg_iters"], "applies": c1["apply"] - c0["apply"],
+                "timing": timing}
 
     # warm-up on a separate state
     ctx.set_precision(args.precision)
@@ -482,20 +529,25 @@ def main():
     for k in range(args.warmup):
         ctx.minimize_resident(p, B_vals[k % len(B_vals)] * H_DIR)
 
-    c0 = ctx.counters()
-    elapsed, J, Es, rows, iters = sweep(args.precision, timed=True)
-    apply_ms, apply_n = ctx.apply_timing(False)
-    c1 = ctx.counters()
-    pcg_its = c1["pcg_iters"] - c0["pcg_iters"]
-    errors = None
-    if args.precision == "mixed":
-        # reference FP64 sweep in the same run: per-step relative energy error (energies in units
-        # of Kd V_mag, floored at 1), switching step, and the final states' potential, gradient
-        # and magnetisation; plus the mixed and FP64 solvers on the same magnetisation
+    head = sweep(args.precision, timed=True)
+    elapsed = head["seconds"]
+    apply_ms, apply_n = head["timing"]
+    B_T = B_vals[:steps] * JS
+
+    def nuc(J):
+        n = nucleation_field(B_T, J * JS)
+        return None if n is None else n["index"]
+
+    compare = None
+    if args.precision == "mixed" and not args.no_fp64_compare:
+        # the FP64 form on the same steps in the same run: per-step relative energy error (energies in
+        # units of Kd V_mag, the denominator floored at 1 since E crosses 0 during the sweep), switching
+        # step and nucleation index, the final states' potential, gradient and magnetisation, and the
+        # two solvers on the same magnetisation
         B_last = B_vals[steps - 1] * H_DIR
         m_mx, U_mx = ctx.state_get()
         _, g_mx = ctx.energy_grad(m_mx, U_mx, B_last)
-        t64, J64, E64, _, _ = sweep("fp64", timed=False)
+        r64 = sweep("fp64", timed=False)
         m64, U64 = ctx.state_get()
         _, g64 = ctx.energy_grad(m64, U64, B_last)
         U_same64, _, _ = ctx.solve_U(m64, np.zeros(ctx.N), 1e-10)
@@ -506,29 +558,38 @@ def main():
         def rel(a, b):
             return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
-        errors = {
-            "max_rel_energy_err_vs_fp64": float(np.max(np.abs(Es - E64) / np.maximum(np.abs(E64), 1.0))),
-            "switching_step_fp64": switching_step(J64),
+        err = np.abs(head["E"] - r64["E"]) / np.maximum(np.abs(r64["E"]), 1.0)
+        compare = {
+            "fp64_field_steps_per_s": steps / r64["seconds"],
+            "speedup_vs_fp64": r64["seconds"] / elapsed,
+            "max_rel_energy_err": float(err.max()),
+            "energy_err_bar": 1e-8,
+            "switching_step": [switching_step(head["J"]), switching_step(r64["J"])],
+            "nucleation_index": [nuc(head["J"]), nuc(r64["J"])],
             "final_state_rel_err": {"U": rel(U_mx, U64), "grad": rel(g_mx, g64), "m": rel(m_mx, m64)},
             "same_m_potential_rel_err": rel(U_samemx, U_same64),
-            "fp64_field_steps_per_s_same_run": steps / t64,
-            "speedup_vs_fp64": t64 / elapsed,
+            "minimizer_iters": [int(sum(head["iters"])), int(sum(r64["iters"]))],
+            "pcg_iters": [head["pcg_iters"], r64["pcg_iters"]],
         }
 
     guess_cmp = None
-    if args.solve_guess > 0 and not args.no_guess_compare:
-        # the reference's warm start (each solve from the previous solution) on the same sweep
-        t0w, J0, E0w, _, it0 = sweep(args.precision, timed=False, guess=0)
+    if args.solve_guess > 0 and args.guess_compare:
+        rg = sweep(args.precision, timed=False, guess=0)
         ctx.set_solve_guess(args.solve_guess)
-        guess_cmp = {"field_steps_per_s": steps / t0w, "speedup_of_headline_form": t0w / elapsed,
-                     "minimizer_iters": int(sum(it0)), "headline_minimizer_iters": int(sum(iters)),
-                     "max_rel_energy_diff": float(np.max(np.abs(Es - E0w) / np.maximum(np.abs(E0w), 1.0))),
-                     "switching_step": switching_step(J0)}
+        guess_cmp = {"field_steps_per_s": steps / rg["seconds"], "speedup_of_headline_form": rg["seconds"] / elapsed,
+                     "minimizer_iters": int(sum(rg["iters"])), "headline_minimizer_iters": int(sum(head["iters"])),
+                     "max_rel_energy_diff": float(np.max(np.abs(head["E"] - rg["E"]) /
+                                                         np.maximum(np.abs(rg["E"]), 1.0))),
+                     "switching_step": switching_step(rg["J"])}
 
+    if args.calibrate_copies > 0:
+        ms = ctx.bench_copy(args.calibrate_copies)
+        log(f"[bench] {args.calibrate_copies} copies of {ctx.N} doubles ({16 * ctx.N} bytes each): {ms * 1e3:.2f} us")
     avg_apply_s = apply_ms / max(apply_n, 1) * 1e-3
     algo = apply_algorithmic_bytes(ctx)
     achieved = algo / avg_apply_s / 1e9
-    label = "FP64" if args.precision == "fp64" else "mixed FP32 V-cycle / FP64 PCG"
+    traffic = measured_traffic(args.workload)
+    label = "FP64" if args.precision == "fp64" else "mixed: FP32 V-cycle, FP64 PCG, FP64 energies"
     result = {
         "metric": f"hysteresis field-steps/s ({label}, {wl['name']})",
         "value": steps / elapsed,
@@ -543,7 +604,8 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f64+f32",
         "data": "synthetic (generated mesh, seeded initial magnetisation)",
         "config": {
-            "workload": f"{wl['name']}_{args.precision}_sweep+2..-8T_{len(B_vals)}steps",
+            "workload": (f"{wl['name']}_{args.precision}_sweep_first{steps}of{len(B_vals)}steps_"
+                         f"+2T..{B_T[-1]:+.3f}T"),
             "tets": ctx.E, "nodes": ctx.N, "method": args.method, "precond": f"SA-AMG V-cycle ({args.precond})",
             "amg_levels": [int(l["A"].shape[0]) for l in ctx.amg_levels],
             "apply_shape": {"unit": shape["unit"], "param": shape["param"], "block_nodes": ctx.block_nodes},
@@ -551,43 +613,71 @@ def main():
                             else "previous solution (reference warm start)"),
             "pcg_tol": 1e-10, "tau_f": 1e-10,
         },
-        "pcg_iters_per_s": pcg_its / elapsed,
-        "switching_step": switching_step(J),
+        "pcg_iters_per_s": head["pcg_iters"] / elapsed,
+        "switching_step": switching_step(head["J"]),
+        "nucleation_index": nuc(head["J"]),
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(),
-            "kernel": "k_apply<on_the_fly> (matrix-free Poisson apply)",
+            "traffic": None if traffic is None else traffic["bytes"],
+            "kernel": ("k_apply_cl<double, false, true> (FP64 matrix-free Poisson apply of the PCG, edge-star "
+                       "clusters)" if shape["unit"] != "element" else "k_apply (FP64 matrix-free Poisson apply)"),
             "avg_launch_us": avg_apply_s * 1e6,
-            "timed_launches": apply_n, "launches": c1["apply"] - c0["apply"],
+            "timed_launches": apply_n, "launches": head["applies"],
             "algorithmic_bytes_per_launch": algo,
-            "reference_layout_GBps": reference_layout_bytes(ctx) / avg_apply_s / 1e9,
+            "traffic_source": None if traffic is None else traffic["source"],
         },
     }
-
-    if errors is not None:
-        result["mixed_vs_fp64"] = errors
+    if compare is not None:
+        result["mixed_vs_fp64"] = compare
     if guess_cmp is not None:
         result["reference_warm_start"] = guess_cmp
-    if not args.no_cpu_baseline and args.workload == "cube2M" and args.method == "pcohen_hs":
-        k_cpu, dt, it_cpu, nthr = cpu_baseline(mesh, mats, B_vals, args.cpu_seconds)
-        t_gpu_k = sum(rows[:k_cpu])
-        # the CPU's rate on the first k steps, carried to the whole sweep with the GPU's own ratio
-        # of whole-sweep to first-k-steps rate (step 0 costs more than the average step)
-        scale = (steps / elapsed) / (k_cpu / t_gpu_k) if k_cpu <= len(rows) and t_gpu_k > 0 else 1.0
-        result["cpu_baseline"] = {
-            "value": k_cpu / dt * scale, "unit": "field-steps/s", "cores": nthr, "kind": "port",
-            "sample": f"oracle/cpu_minimizer.cpp (C++/OpenMP restatement of the oracle's pcohen_hs field step on the "
-                      f"assembled operators, same mesh and AMG hierarchy, {nthr} threads): the first {k_cpu} field "
-                      f"steps of the sweep from +2 T ({it_cpu} minimiser iterations) took {dt:.2f} s "
-                      f"({k_cpu / dt:.3f} steps/s); the GPU ran the same steps in {t_gpu_k:.3f} s, and the value is "
-                      f"the CPU rate times the GPU's whole-sweep / first-{k_cpu}-steps rate ratio ({scale:.3f})",
-        }
+    if not args.no_cpu_baseline and args.method == "pcohen_hs":
+        result["cpu_baseline"] = cpu_baseline_line(args, ctx, head, steps, log)
     print(json.dumps(result), flush=True)
     ctx.close()
+
+
+def cpu_baseline_line(args, ctx, head, steps, log) -> dict:
+    """The CPU restatement's rate in field-steps/s of the timed workload. On the cube it runs the
+    same mesh and sweep; on the polycrystal it runs the first 40 minimiser iterations of the
+    first field step on the 40^3-hex polycrystal (the same physical problem on a mesh whose
+    assembled operators it builds in seconds), and its seconds per minimiser iteration are
+    scaled to the headline mesh by the tet ratio and multiplied by the minimiser iterations the
+    GPU needed per field step on the headline mesh."""
+    from mumag_matrixfree2_amd.hysteresis_loop import field_values
+
+    mesh_c, mats_c, what = cpu_sample_mesh(args.workload)
+    wl = WORKLOADS[args.workload]
+    B_vals = field_values(2.0 / JS, -8.0 / JS, wl["dB"] / JS, False)
+    k_max = steps
+    if args.workload == "cube2M":
+        k_cpu, dt, it_cpu, nthr, E_c = cpu_baseline(mesh_c, mats_c, B_vals[:k_max], args.cpu_seconds, log)
+    else:  # a bounded sample: the first minimiser iterations of the first field step
+        k_cpu, dt, it_cpu, nthr, E_c = cpu_baseline(mesh_c, mats_c, B_vals[:1], args.cpu_seconds, log, max_iter=40)
+    cpu = host_cpu_description()
+    gpu_its = head["iters"][:k_cpu]
+    if args.workload == "cube2M":
+        gpu_t = None
+        value = k_cpu / dt
+        how = (f"the first {k_cpu} field steps of the same sweep ({it_cpu} minimiser iterations; the GPU took "
+               f"{int(sum(gpu_its))}) in {dt:.2f} s")
+    else:
+        s_per_it = dt / max(it_cpu, 1) * (ctx.E / E_c)
+        its_per_step = float(np.mean(head["iters"]))
+        value = 1.0 / (s_per_it * its_per_step)
+        how = (f"{what} ({E_c} tets): the first {it_cpu} minimiser iterations of the first field step took "
+               f"{dt:.2f} s ({dt / max(it_cpu, 1) * 1e3:.1f} ms per iteration); scaled by the tet ratio "
+               f"{ctx.E / E_c:.2f} to {s_per_it:.3f} s per iteration and multiplied by the {its_per_step:.1f} "
+               f"minimiser iterations per field step of the GPU run on the {ctx.E}-tet mesh (an estimate: PCG "
+               f"iterations per solve differ between the meshes)")
+    return {"value": value, "unit": "field-steps/s", "cores": nthr, "kind": "port",
+            "sample": (f"oracle/cpu_minimizer.cpp (C++/OpenMP restatement of the oracle's pcohen_hs field step on "
+                       f"the assembled operators, SA-AMG PCG, {nthr} threads): " + how),
+            "host_cpu": cpu}
 
 
 if __name__ == "__main__":

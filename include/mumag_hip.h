@@ -243,6 +243,10 @@ int mm_counters(int64_t ctx, int64_t* out6);
  * HIP events on the context's stream; returns the mean milliseconds per apply. */
 int mm_bench_apply(int64_t ctx, int geom, int reps, double* ms_per_apply);
 
+/* Benchmark hook: `reps` copies of N doubles between two context vectors with a plain copy kernel
+ * (k_copy_calib, 16 N bytes per launch), the known-size reference for HBM byte counters; mean ms. */
+int mm_bench_copy(int64_t ctx, int reps, double* ms_per_copy);
+
 /* Benchmark hook: `reps` preconditioner applications on device-resident vectors, launched
  * directly (use_graph = 0) or replayed from a captured hipGraph (use_graph = 1); mean ms. */
 int mm_bench_precond(int64_t ctx, int use_graph, int reps, double* ms_per_apply);

@@ -106,6 +106,7 @@ SIGNATURES = {
     "mm_apply_timing": [C.c_int64, C.c_int, f64p, i64p],
     "mm_counters": [C.c_int64, i64p],
     "mm_bench_apply": [C.c_int64, C.c_int, C.c_int, f64p],
+    "mm_bench_copy": [C.c_int64, C.c_int, f64p],
     "mm_bench_precond": [C.c_int64, C.c_int, C.c_int, f64p],
 }
 

@@ -310,6 +310,12 @@ class MeshContext:
         call("mm_apply_timing", self.handle, int(bool(enable)), C.byref(ms), C.byref(n))
         return float(ms.value), int(n.value)
 
+    def bench_copy(self, reps=16):
+        """Mean ms of a plain copy of N doubles (16 N bytes), ``reps`` launches."""
+        ms = C.c_double(0.0)
+        call("mm_bench_copy", self.handle, int(reps), C.byref(ms))
+        return float(ms.value)
+
     def bench_apply(self, geom="on_the_fly", reps=50):
         ms = C.c_double(0.0)
         call("mm_bench_apply", self.handle, GEOM_MODES[geom], int(reps), C.byref(ms))

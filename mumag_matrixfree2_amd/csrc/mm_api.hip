@@ -1590,6 +1590,25 @@ int mm_counters(int64_t h, int64_t* out6) {
   return 0;
 }
 
+int mm_bench_copy(int64_t h, int reps, double* ms) {
+  GET_CTX(h);
+  MM_CHECK(reps > 0 && ms, "bad arguments");
+  hipEvent_t e0, e1;
+  MM_HIP(hipEventCreate(&e0));
+  MM_HIP(hipEventCreate(&e1));
+  vfill(c.pp, 1.0, c.N, c.stream);
+  MM_HIP(hipEventRecord(e0, c.stream));
+  for (int r = 0; r < reps; ++r) MM_TRY(launch_copy_calib(c.pq, c.pp, c.N, c.stream));
+  MM_HIP(hipEventRecord(e1, c.stream));
+  MM_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  MM_HIP(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
 int mm_bench_apply(int64_t h, int geom, int reps, double* ms) {
   GET_CTX(h);
   MM_CHECK(geom >= 0 && geom <= 3 && reps > 0, "bad arguments");
