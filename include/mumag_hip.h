@@ -77,9 +77,10 @@ int mm_ctx_create_segmented(int N, int E, const double* knt, const int32_t* conn
                             const double* bnd_mask, double reg, int flags, int nseg, const int32_t* seg_node_off,
                             int64_t* out_ctx);
 
-/* info[0..11] = V_mag, N, E, element blocks (all), element blocks (magnetic),
+/* info[0..12] = V_mag, N, E, element blocks (all), element blocks (magnetic),
  * largest node list of a block, magnetic elements, node-list entries of all blocks,
- * cluster blocks, node-list entries of the cluster blocks, edge-star clusters, block node cap */
+ * cluster blocks, node-list entries of the cluster blocks, edge-star clusters, block node cap,
+ * cluster block node cap */
 int mm_ctx_info(int64_t ctx, double* info);
 
 /* Oriented connectivity (E,4), Ve (E,) and shape-function gradients of corners 1..3 (E,3,3)
@@ -107,6 +108,13 @@ int mm_set_geometry_mode(int64_t ctx, int mode);
  * its next block while computing the current one (FP64 applies; the FP32 V-cycle keeps unit 1).
  * Results agree to rounding; the choice is a tuning knob with no reference counterpart. */
 int mm_set_apply_shape(int64_t ctx, int unit, int param);
+
+/* Distinct nodes per block of the edge-star cluster partition of the Poisson apply: 256 (the
+ * default), 512 or 1024. Larger blocks stage fewer node entries and share fewer nodes with other
+ * blocks (fewer global atomics); they run 256 threads per workgroup and not on the persistent
+ * grid. Rebuilds the cluster partition (not allowed on segmented contexts). Results agree to
+ * rounding; a tuning knob with no reference counterpart. */
+int mm_set_cluster_block_nodes(int64_t ctx, int nb);
 
 /* y = A x: assembled-mode apply_A (src/poisson_solve.py:166-171) computed matrix-free. */
 int mm_poisson_apply(int64_t ctx, const double* x, double* y);

@@ -9,9 +9,11 @@ result beyond rounding:
 * the workgroup size (64, 128 or 256 threads per element block), or for the persistent grid
   the workgroups per CU.
 
-A third knob, the number of distinct nodes per element block (the LDS staging size, hence
-the elements per block), is fixed when a context is created; ``tune_block_nodes`` times
-contexts built with each candidate value.
+A third knob is the number of distinct nodes per block of the cluster partition (256, 512 or
+1024; larger blocks stage fewer node entries and leave fewer shared nodes to the write-back
+atomics): ``tune_apply`` rebuilds the cluster partition for each value it tries. The node cap
+of the element blocks (the per-tet kernels) is fixed when a context is created;
+``tune_block_nodes`` times contexts built with each candidate value.
 
 The timings come from back-to-back applies on device-resident vectors, bracketed by HIP
 events on the context's stream (``mm_bench_apply``). The reference has no counterpart: its
@@ -24,18 +26,34 @@ SHAPES = (("persistent", 2), ("persistent", 3), ("cluster", 256), ("cluster", 12
           ("element", 256), ("element", 128), ("element", 64))
 
 
-def tune_apply(ctx, shapes=SHAPES, reps: int = 40) -> dict:
-    """Time every (unit, param) shape on ``ctx``, keep the fastest set on the context and return
-    ``{"unit", "param", "us_per_apply": {"unit:param": microseconds}}``."""
+CLUSTER_BLOCK_NODES = (512, 1024)
+
+
+def tune_apply(ctx, shapes=SHAPES, reps: int = 40, cluster_block_nodes=CLUSTER_BLOCK_NODES) -> dict:
+    """Time every (unit, param) shape on ``ctx`` with 256-node cluster blocks, then 256-thread
+    cluster applies on ``cluster_block_nodes``-node blocks (not on segmented contexts); keep the
+    fastest on the context and return ``{"unit", "param", "cluster_block_nodes",
+    "us_per_apply": {"unit:param[:nodes]": microseconds}}``."""
     times = {}
+    if getattr(ctx, "cluster_block_nodes", 256) != 256:
+        ctx.set_cluster_block_nodes(256)
     for unit, param in shapes:
         ctx.set_apply_shape(unit, param)
         ctx.bench_apply(reps=3)
-        times[(unit, param)] = ctx.bench_apply(reps=reps) * 1e3
+        times[(unit, param, 256)] = ctx.bench_apply(reps=reps) * 1e3
+    if ctx.segments is None:
+        for nb in cluster_block_nodes:
+            ctx.set_cluster_block_nodes(nb)
+            ctx.set_apply_shape("cluster", 256)
+            ctx.bench_apply(reps=3)
+            times[("cluster", 256, nb)] = ctx.bench_apply(reps=reps) * 1e3
     best = min(times, key=times.get)
-    ctx.set_apply_shape(*best)
-    return {"unit": best[0], "param": best[1],
-            "us_per_apply": {f"{u}:{q}": round(v, 3) for (u, q), v in times.items()}}
+    if ctx.segments is None:
+        ctx.set_cluster_block_nodes(best[2])
+    ctx.set_apply_shape(best[0], best[1])
+    return {"unit": best[0], "param": best[1], "cluster_block_nodes": best[2],
+            "us_per_apply": {f"{u}:{q}" + ("" if nb == 256 else f":{nb}"): round(v, 3)
+                             for (u, q, nb), v in times.items()}}
 
 
 def tune_block_nodes(make_ctx, caps=(128, 192, 256), reps: int = 40):

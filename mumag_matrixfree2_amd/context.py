@@ -72,7 +72,7 @@ class MeshContext:
                  ptr(K1_red), ptr(Js_red), ptr(k_easy), ptr(bnd), self.reg, flags, self.segments.size - 1,
                  ptr(self.segments), C.byref(h))
         self.handle = h.value
-        info = np.zeros(12)
+        info = np.zeros(13)
         call("mm_ctx_info", self.handle, ptr(info))
         self.V_mag = float(info[0])
         self.n_blocks_all = int(info[3])
@@ -84,6 +84,7 @@ class MeshContext:
         self.cluster_node_list_entries = int(info[9])
         self.n_clusters = int(info[10])
         self.block_nodes = int(info[11])
+        self.cluster_block_nodes = int(info[12])
         self.apply_shape = {"unit": "cluster", "param": 256}
         self.solve_guess = 0
         self.precond_type = None
@@ -144,6 +145,20 @@ class MeshContext:
         ``param`` = workgroups per CU (1..4)."""
         call("mm_set_apply_shape", self.handle, APPLY_UNITS[unit], int(param))
         self.apply_shape = {"unit": unit, "param": int(param)}
+
+    def set_cluster_block_nodes(self, nb: int):
+        """Distinct nodes per cluster block of the Poisson apply (256, 512 or 1024); rebuilds the
+        cluster partition and refreshes the partition counts."""
+        call("mm_set_cluster_block_nodes", self.handle, int(nb))
+        info = np.zeros(13)
+        call("mm_ctx_info", self.handle, ptr(info))
+        self.n_blocks_cl = int(info[8])
+        self.cluster_node_list_entries = int(info[9])
+        self.cluster_block_nodes = int(info[12])
+        if self.cluster_block_nodes != 256 and self.apply_shape["unit"] == "persistent":
+            self.apply_shape = {"unit": "cluster", "param": 256}
+        if self.cluster_block_nodes != 256 and self.apply_shape["unit"] == "cluster":
+            self.apply_shape = {"unit": "cluster", "param": 256}
 
     def set_geometry_mode(self, geom: str):
         call("mm_set_geometry_mode", self.handle, GEOM_MODES[geom])

@@ -373,8 +373,10 @@ static void build_cluster_partition(int N, std::vector<HostCluster>& cl, const s
       uint32_t w[2] = {0u, 0u}, m = h.tets;
       for (int q = 0; q < 8; ++q)
         if (h.node[q] >= 0) {
-          w[q >> 2] |= (uint32_t)remap[local[h.node[q]]] << (8 * (q & 3));
+          const uint32_t li = (uint32_t)remap[local[h.node[q]]];
+          w[q >> 2] |= (li & 255u) << (8 * (q & 3));
           m |= 1u << (8 + q);
+          m |= (li >> 8) << (16 + 2 * q);
         }
       hp.w0.push_back(w[0]);
       hp.w1.push_back(w[1]);
@@ -426,6 +428,47 @@ static void build_cluster_partition(int N, std::vector<HostCluster>& cl, const s
   if (!blk_nodes.empty()) close_block();
   for (auto& f : hp.nodes)
     if (touches[f & kIdMask] == 1) f |= kPrivate;
+}
+
+// Cut the context's clusters into blocks of at most `cap` distinct nodes and upload them as
+// part_cl, freeing a previous cluster partition. hc receives the host partition.
+static int upload_cluster_part(Ctx& c, int cap, const std::vector<int>& seg_of, HostCPart& hc) {
+  std::vector<HostCluster> cl(c.h_cl_tets.size());
+  for (size_t i = 0; i < cl.size(); ++i) {
+    for (int q = 0; q < 8; ++q) cl[i].node[q] = c.h_cl_nodes[8 * i + q];
+    cl[i].tets = c.h_cl_tets[i];
+  }
+  build_cluster_partition(c.N, cl, c.h_is_bnd, hc, cap, seg_of);
+  Part& q = c.part_cl;
+  for (const void* old : {(const void*)q.eptr, (const void*)q.nptr, (const void*)q.nodes, (const void*)q.lconn,
+                          (const void*)q.lconn2, (const void*)q.cmask})
+    if (old) {
+      auto it = std::find(c.allocs.begin(), c.allocs.end(), old);
+      if (it != c.allocs.end()) {
+        (void)hipFree(*it);
+        c.allocs.erase(it);
+      }
+    }
+  q = Part();
+  q.nblk = (int)hc.cptr.size() - 1;
+  q.E = (int)hc.w0.size();
+  int *cp, *np;
+  uint32_t *nodes, *w0, *w1, *wm;
+  MM_TRY(upload(c, &cp, hc.cptr.data(), hc.cptr.size()));
+  MM_TRY(upload(c, &np, hc.nptr.data(), hc.nptr.size()));
+  MM_TRY(upload(c, &nodes, hc.nodes.data(), hc.nodes.size()));
+  MM_TRY(upload(c, &w0, hc.w0.data(), hc.w0.size()));
+  MM_TRY(upload(c, &w1, hc.w1.data(), hc.w1.size()));
+  MM_TRY(upload(c, &wm, hc.wm.data(), hc.wm.size()));
+  q.eptr = cp;
+  q.nptr = np;
+  q.nodes = nodes;
+  q.lconn = w0;
+  q.lconn2 = w1;
+  q.cmask = wm;
+  c.nodes_cl = (long long)hc.nodes.size();
+  c.cl_cap = cap;
+  return 0;
 }
 
 static int upload_part(Ctx& c, const HostPart& hp, int E, Part& p) {
@@ -571,26 +614,15 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   {
     std::vector<HostCluster> cl;
     build_clusters(knt, E, conn, cl);
+    c.h_cl_nodes.resize(8 * cl.size());
+    c.h_cl_tets.resize(cl.size());
+    for (size_t i = 0; i < cl.size(); ++i) {
+      for (int q = 0; q < 8; ++q) c.h_cl_nodes[8 * i + q] = cl[i].node[q];
+      c.h_cl_tets[i] = cl[i].tets;
+    }
+    c.h_is_bnd = is_bnd;
     HostCPart hc;
-    build_cluster_partition(N, cl, is_bnd, hc, node_cap, seg_of);
-    Part& q = c.part_cl;
-    q.nblk = (int)hc.cptr.size() - 1;
-    q.E = (int)hc.w0.size();
-    int *cp, *np;
-    uint32_t *nodes, *w0, *w1, *wm;
-    MM_TRY(upload(c, &cp, hc.cptr.data(), hc.cptr.size()));
-    MM_TRY(upload(c, &np, hc.nptr.data(), hc.nptr.size()));
-    MM_TRY(upload(c, &nodes, hc.nodes.data(), hc.nodes.size()));
-    MM_TRY(upload(c, &w0, hc.w0.data(), hc.w0.size()));
-    MM_TRY(upload(c, &w1, hc.w1.data(), hc.w1.size()));
-    MM_TRY(upload(c, &wm, hc.wm.data(), hc.wm.size()));
-    q.eptr = cp;
-    q.nptr = np;
-    q.nodes = nodes;
-    q.lconn = w0;
-    q.lconn2 = w1;
-    q.cmask = wm;
-    c.nodes_cl = (long long)hc.nodes.size();
+    MM_TRY(upload_cluster_part(c, node_cap, seg_of, hc));
     c.h_seg_blk_cl = seg_blocks(hc.nptr, hc.nodes);
   }
   c.h_seg_blk_mag = seg_blocks(hp_mag.nptr, hp_mag.nodes);
@@ -851,6 +883,7 @@ int mm_ctx_info(int64_t h, double* info) {
   info[9] = c.nodes_cl;
   info[10] = c.part_cl.E;
   info[11] = c.node_cap;
+  info[12] = c.cl_cap;
   return 0;
 }
 
@@ -891,17 +924,36 @@ int mm_set_geometry_mode(int64_t h, int mode) {
   return 0;
 }
 
+int mm_set_cluster_block_nodes(int64_t h, int nb) {
+  GET_CTX(h);
+  MM_CHECK(nb == 256 || nb == 512 || nb == 1024, "cluster block nodes must be 256, 512 or 1024");
+  MM_CHECK(c.nseg == 1, "segmented contexts keep 256-node cluster blocks");
+  if (nb == c.cl_cap) return 0;
+  MM_HIP(hipStreamSynchronize(c.stream));
+  HostCPart hc;
+  MM_TRY(upload_cluster_part(c, nb, std::vector<int>(c.N, 0), hc));
+  if (nb != kMaxBlockNodes && c.apply_persist > 0) {
+    c.apply_persist = 0;  // the persistent grid stages 256-node blocks
+    c.elem_threads = 256;
+  }
+  if (nb != kMaxBlockNodes) c.elem_threads = 256;
+  return 0;
+}
+
 int mm_set_apply_shape(int64_t h, int unit, int param) {
   GET_CTX(h);
   MM_CHECK(unit >= 0 && unit <= 2, "unit must be 0 (element), 1 (cluster) or 2 (persistent cluster)");
   if (unit == 2) {
     MM_CHECK(param >= 1 && param <= 4, "persistent cluster apply: 1..4 workgroups per CU");
+    MM_CHECK(c.cl_cap == kMaxBlockNodes, "the persistent cluster apply needs 256-node cluster blocks");
     c.apply_clusters = true;
     c.apply_persist = param;
     c.elem_threads = kMaxBlockNodes;
     return 0;
   }
   MM_CHECK(param == 64 || param == 128 || param == 256, "threads must be 64, 128 or 256");
+  MM_CHECK(unit == 0 || c.cl_cap == kMaxBlockNodes || param == 256,
+           "cluster blocks of more than 256 nodes run 256 threads per workgroup");
   c.apply_clusters = unit == 1;
   c.apply_persist = 0;
   c.elem_threads = param;

@@ -243,14 +243,18 @@ __global__ __launch_bounds__(NT) void k_apply_f32(Part p, Geo geo, const float* 
 // from LDS once, the edge vector c1 = x1 - x0 and the cross products c1 x (x_s - x0) are
 // shared by the tets, and the slot sums leave through 8 LDS atomics. Adds the tets' energies
 // sum_a x_a y_a to acc_e when DOT.
-template <typename T, bool DOT>
-__device__ __forceinline__ void cluster_apply(uint32_t w0, uint32_t w1, uint32_t wm, const T (*sX)[kMaxBlockNodes],
+template <typename T, bool DOT, int NB>
+__device__ __forceinline__ void cluster_apply(uint32_t w0, uint32_t w1, uint32_t wm, const T (*sX)[NB],
                                               const T* sx, double* sy, double& acc_e) {
   int li[8];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     li[q] = (w0 >> (8 * q)) & 255u;
     li[4 + q] = (w1 >> (8 * q)) & 255u;
+  }
+  if constexpr (NB > 256) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) li[q] |= (int)((wm >> (16 + 2 * q)) & 3u) << 8;
   }
   T X[8][3], v[8];
 #pragma unroll
@@ -316,18 +320,21 @@ __device__ __forceinline__ void cluster_apply(uint32_t w0, uint32_t w1, uint32_t
 // operator (x, y FP64, DOT as in k_apply); T = float: the FP32 operator of the
 // mixed-precision V-cycle (block-relative FP32 coordinates, FP64 LDS accumulation).
 // SCALED: input w .* r from the FP64 residual r.
-template <typename T, bool SCALED, bool DOT, int NT>
+// NB: distinct nodes per block (256, 512 or 1024; indices above 255 carry their high bits in
+// cmask), NB / NT staged nodes per thread.
+template <typename T, bool SCALED, bool DOT, int NT, int NB>
 __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __restrict__ x,
                                                  const double* __restrict__ r, const T* __restrict__ w,
                                                  T* __restrict__ xo, T* __restrict__ y, T reg,
                                                  double* __restrict__ dot_part) {
-  __shared__ T sx[kMaxBlockNodes];
-  __shared__ double sy[kMaxBlockNodes];
-  __shared__ T sX[3][kMaxBlockNodes];
+  static_assert(NB % NT == 0 && NB <= kMaxClusterBlockNodes, "block nodes per thread");
+  __shared__ T sx[NB];
+  __shared__ double sy[NB];
+  __shared__ T sX[3][NB];
   const BlkInfo bi = blk_info(p);
   if (!bi.valid) return;
   const int tid = threadIdx.x;
-  constexpr int kNode = kMaxBlockNodes / NT;
+  constexpr int kNode = NB / NT;
   uint32_t fv[kNode];
   const int c0 = bi.e0 + tid;
   uint32_t cw0 = 0u, cw1 = 0u, cwm = 0u;
@@ -370,9 +377,9 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
   }
   __syncthreads();
   double acc_e = 0.0;
-  if (c0 < bi.e1) cluster_apply<T, DOT>(cw0, cw1, cwm, sX, sx, sy, acc_e);
+  if (c0 < bi.e1) cluster_apply<T, DOT, NB>(cw0, cw1, cwm, sX, sx, sy, acc_e);
   for (int cc = c0 + NT; cc < bi.e1; cc += NT)
-    cluster_apply<T, DOT>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
+    cluster_apply<T, DOT, NB>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kNode; ++q) {
@@ -476,9 +483,9 @@ __global__ __launch_bounds__(kMaxBlockNodes) void k_apply_clp(Part p, Geo geo, c
     fetch_meta(bn);
     fetch_node();
     double acc_e = 0.0;
-    cluster_apply<double, DOT>(cw0, cw1, cwm, sX, sx, sy, acc_e);  // cwm == 0: no tets, no slots
+    cluster_apply<double, DOT, kMaxBlockNodes>(cw0, cw1, cwm, sX, sx, sy, acc_e);  // cwm == 0: no tets, no slots
     for (int cc = c0 + NT; cc < ce1; cc += NT)
-      cluster_apply<double, DOT>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
+      cluster_apply<double, DOT, kMaxBlockNodes>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
     fetch_vals();
     __syncthreads();
     if (tid < cnb) {
@@ -978,6 +985,18 @@ static void with_nt(int nt, F&& f) {
   }
 }
 
+// workgroup size and block node count of the cluster apply: blocks of more than 256 nodes run
+// 256 threads
+template <typename F>
+static void with_cl_shape(const Ctx& c, F&& f) {
+  if (c.cl_cap == 1024)
+    f(std::integral_constant<int, 256>{}, std::integral_constant<int, 1024>{});
+  else if (c.cl_cap == 512)
+    f(std::integral_constant<int, 256>{}, std::integral_constant<int, 512>{});
+  else
+    with_nt(c.elem_threads, [&](auto nt) { f(nt, std::integral_constant<int, kMaxBlockNodes>{}); });
+}
+
 // persistent grid: apply_persist workgroups per CU, a multiple of 8 (one share per XCD)
 static int persist_grid(const Ctx& c) {
   const int g = 8 * std::max(1, (c.num_cu * c.apply_persist) / 8);
@@ -1003,14 +1022,14 @@ int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t
     const Part& p = c.part_cl;
     if (part_len(p) == 0) return 0;
     const int grid = grid_for(p);
-    with_nt(c.elem_threads, [&](auto nt) {
-      constexpr int NT = decltype(nt)::value;
+    with_cl_shape(c, [&](auto nt, auto nb) {
+      constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
       if (dot_part)
-        k_apply_cl<double, false, true, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, c.reg,
-                                                                 dot_part);
+        k_apply_cl<double, false, true, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, c.reg,
+                                                                     dot_part);
       else
-        k_apply_cl<double, false, false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, c.reg,
-                                                                  nullptr);
+        k_apply_cl<double, false, false, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
+                                                                      c.reg, nullptr);
     });
     return 0;
   }
@@ -1040,12 +1059,16 @@ int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* 
     return 0;
   }
   const int grid = grid_for(p);
+  if (c.apply_clusters) {
+    with_cl_shape(c, [&](auto nt, auto nb) {
+      constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
+      k_apply_cl<double, true, false, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, c.reg, nullptr);
+    });
+    return 0;
+  }
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
-    if (c.apply_clusters)
-      k_apply_cl<double, true, false, NT><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, c.reg, nullptr);
-    else
-      k_apply<kGeomCoords, true, false, NT><<<grid, NT, 0, s>>>(p, c.geo, r, w, xo, y, c.reg, nullptr);
+    k_apply<kGeomCoords, true, false, NT><<<grid, NT, 0, s>>>(p, c.geo, r, w, xo, y, c.reg, nullptr);
   });
   return 0;
 }
@@ -1054,13 +1077,17 @@ int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s) {
   const Part& p = c.apply_clusters ? c.part_cl : c.part_all;
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
+  if (c.apply_clusters) {
+    with_cl_shape(c, [&](auto nt, auto nb) {
+      constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
+      k_apply_cl<float, false, false, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
+                                                                  (float)c.reg, nullptr);
+    });
+    return 0;
+  }
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
-    if (c.apply_clusters)
-      k_apply_cl<float, false, false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, (float)c.reg,
-                                                              nullptr);
-    else
-      k_apply_f32<false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, (float)c.reg);
+    k_apply_f32<false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, (float)c.reg);
   });
   return 0;
 }
@@ -1069,12 +1096,17 @@ int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float
   const Part& p = c.apply_clusters ? c.part_cl : c.part_all;
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
+  if (c.apply_clusters) {
+    with_cl_shape(c, [&](auto nt, auto nb) {
+      constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
+      k_apply_cl<float, true, false, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg,
+                                                                 nullptr);
+    });
+    return 0;
+  }
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
-    if (c.apply_clusters)
-      k_apply_cl<float, true, false, NT><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg, nullptr);
-    else
-      k_apply_f32<true, NT><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg);
+    k_apply_f32<true, NT><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg);
   });
   return 0;
 }

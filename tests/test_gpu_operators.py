@@ -53,15 +53,17 @@ def test_poisson_apply(ctx_c0, oracle_c0, geom):
 
 
 @pytest.mark.parametrize("shape", ["persistent:1", "persistent:3", "cluster:256", "cluster:64", "element:128",
-                                   "element:64"])
+                                   "element:64", "cluster:256:512", "cluster:256:1024"])
 def test_poisson_apply_every_work_shape(ctx_c0, oracle_c0, shape):
     """Every apply shape the autotuner can pick gives the oracle's A x, in one apply and inside
-    a PCG solve (the persistent grid writes its x.Ax partials per block)."""
+    a PCG solve (the persistent grid writes its x.Ax partials per block); the third field is the
+    cluster block size (slot indices above 255 carry their high bits in the cluster mask)."""
     from oracle.solvers import solve_U as oracle_solve_U
 
-    unit, param = shape.split(":")
+    unit, param, *nb = shape.split(":")
     rng = np.random.default_rng(5)
     x = rng.standard_normal(ctx_c0.N)
+    ctx_c0.set_cluster_block_nodes(int(nb[0]) if nb else 256)
     ctx_c0.set_apply_shape(unit, int(param))
     try:
         assert rel(ctx_c0.apply_A(x), oracle_c0.A @ x) <= TOL
@@ -71,7 +73,16 @@ def test_poisson_apply_every_work_shape(ctx_c0, oracle_c0, shape):
         U, it, rr = ctx_c0.solve_U(m, np.zeros(ctx_c0.N), 1e-10)
         U_ref, _, _ = oracle_solve_U(oracle_c0, m, np.zeros(ctx_c0.N), 1e-12, precond="direct")
         assert rr <= 1e-10 and rel(U, U_ref) <= 1e-7
+        if nb:
+            assert ctx_c0.cluster_block_nodes == int(nb[0]) and ctx_c0.n_blocks_cl >= 1
+            ctx_c0.set_precision("mixed")
+            try:
+                Um, _, rrm = ctx_c0.solve_U(m, np.zeros(ctx_c0.N), 1e-10)
+            finally:
+                ctx_c0.set_precision("fp64")
+            assert rrm <= 1e-10 and rel(Um, U_ref) <= 1e-7
     finally:
+        ctx_c0.set_cluster_block_nodes(256)
         ctx_c0.set_apply_shape("cluster", 256)
 
 
@@ -80,11 +91,15 @@ def test_autotuner_keeps_the_fastest_shape(ctx_c0):
 
     res = tune_apply(ctx_c0, reps=5)
     times = res["us_per_apply"]
-    from mumag_matrixfree2_amd.autotune import SHAPES
+    from mumag_matrixfree2_amd.autotune import CLUSTER_BLOCK_NODES, SHAPES
 
-    assert len(times) == len(SHAPES) and all(t > 0 for t in times.values())
-    assert times[f"{res['unit']}:{res['param']}"] == min(times.values())
+    assert len(times) == len(SHAPES) + len(CLUSTER_BLOCK_NODES) and all(t > 0 for t in times.values())
+    key = f"{res['unit']}:{res['param']}" + ("" if res["cluster_block_nodes"] == 256 else
+                                            f":{res['cluster_block_nodes']}")
+    assert times[key] == min(times.values())
     assert ctx_c0.apply_shape == {"unit": res["unit"], "param": res["param"]}
+    assert ctx_c0.cluster_block_nodes == res["cluster_block_nodes"]
+    ctx_c0.set_cluster_block_nodes(256)
     ctx_c0.set_apply_shape("cluster", 256)
 
 

@@ -55,16 +55,19 @@ def delaunay_pair():
     return ctx, prob
 
 
-@pytest.mark.parametrize("shape", ["cluster:256", "cluster:64", "persistent:2", "element:256"])
+@pytest.mark.parametrize("shape", ["cluster:256", "cluster:64", "persistent:2", "element:256", "cluster:256:512",
+                                   "cluster:256:1024"])
 def test_apply_on_unstructured_mesh(delaunay_pair, shape):
     ctx, prob = delaunay_pair
     assert ctx.n_clusters > ctx.E / 6  # partial stars: more clusters than on a hex split
-    unit, param = shape.split(":")
+    unit, param, *nb = shape.split(":")
+    ctx.set_cluster_block_nodes(int(nb[0]) if nb else 256)
     ctx.set_apply_shape(unit, int(param))
     try:
         x = np.random.default_rng(2).standard_normal(ctx.N)
         assert rel(ctx.apply_A(x), prob.A @ x) <= TOL
     finally:
+        ctx.set_cluster_block_nodes(256)
         ctx.set_apply_shape("cluster", 256)
 
 
