@@ -100,14 +100,15 @@ def initial_m(N, seed=0, sigma=0.01):
     return m / np.linalg.norm(m, axis=1, keepdims=True)
 
 
-def apply_algorithmic_bytes(ctx) -> int:
+def apply_algorithmic_bytes(ctx, fp32: bool = False) -> int:
     """Bytes one on-the-fly Poisson apply must move if every stream is touched once: node
-    coordinates (24 B/node), x (8 B/node), y (8 B/node), the blocks' node lists (4 B/entry)
-    and the work items: per edge-star cluster 3 words (12 B: 8 slot indices, tet and slot
-    masks), or per tet 4 packed corner indices (4 B)."""
+    coordinates (24 B/node, FP64), x and y (8 B/node each, 4 in the FP32 apply), the blocks' node
+    lists (4 B/entry) and the work items: per edge-star cluster 3 words (12 B: slot indices, tet
+    and slot masks), or per tet 4 packed corner indices (4 B)."""
+    xy = 8 if fp32 else 16
     if ctx.apply_shape["unit"] in ("cluster", "persistent"):
-        return 12 * ctx.n_clusters + 4 * ctx.cluster_node_list_entries + 40 * ctx.N
-    return 4 * ctx.E + 4 * ctx.node_list_entries + 40 * ctx.N
+        return 12 * ctx.n_clusters + 4 * ctx.cluster_node_list_entries + (24 + xy) * ctx.N
+    return 4 * ctx.E + 4 * ctx.node_list_entries + (24 + xy) * ctx.N
 
 
 def amg_kwargs(args) -> dict:
@@ -426,8 +427,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="time budget of the CPU baseline (field steps from the start of the sweep)")
-    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
-                    help="headline form; with 'mixed' the FP64 form runs the same steps for the comparison")
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed", "fp32pcg"],
+                    help="headline form: 'mixed' (FP32 V-cycle under the FP64 PCG) or 'fp32pcg' (FP32 PCG with "
+                         "FP64 refinement); the FP64 form runs the same steps for the comparison")
     ap.add_argument("--no-fp64-compare", action="store_true", help="mixed headline without the FP64 sweep")
     ap.add_argument("--max-coarse", type=int, default=None)
     ap.add_argument("--amg-theta", default=None,
@@ -542,7 +544,7 @@ def main():
         return None if n is None else n["index"]
 
     compare = None
-    if args.precision == "mixed" and not args.no_fp64_compare:
+    if args.precision != "fp64" and not args.no_fp64_compare:
         # the FP64 form on the same steps in the same run: per-step relative energy error (energies in
         # units of Kd V_mag, the denominator floored at 1 since E crosses 0 during the sweep), switching
         # step and nucleation index, the final states' potential, gradient and magnetisation, and the
@@ -554,9 +556,8 @@ def main():
         m64, U64 = ctx.state_get()
         _, g64 = ctx.energy_grad(m64, U64, B_last)
         U_same64, _, _ = ctx.solve_U(m64, np.zeros(ctx.N), 1e-10)
-        ctx.set_precision("mixed")
-        U_samemx, _, _ = ctx.solve_U(m64, np.zeros(ctx.N), 1e-10)
         ctx.set_precision(args.precision)
+        U_samemx, _, _ = ctx.solve_U(m64, np.zeros(ctx.N), 1e-10)
 
         def rel(a, b):
             return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
@@ -589,10 +590,12 @@ def main():
         ms = ctx.bench_copy(args.calibrate_copies)
         log(f"[bench] {args.calibrate_copies} copies of {ctx.N} doubles ({16 * ctx.N} bytes each): {ms * 1e3:.2f} us")
     avg_apply_s = apply_ms / max(apply_n, 1) * 1e-3
-    algo = apply_algorithmic_bytes(ctx)
+    algo = apply_algorithmic_bytes(ctx, fp32=args.precision == "fp32pcg")
     achieved = algo / avg_apply_s / 1e9
     traffic = measured_traffic(args.workload)
-    label = "FP64" if args.precision == "fp64" else "mixed: FP32 V-cycle, FP64 PCG, FP64 energies"
+    label = {"fp64": "FP64", "mixed": "mixed: FP32 V-cycle, FP64 PCG, FP64 energies",
+             "fp32pcg": "FP32 PCG and V-cycle with FP64 dots and FP64 true-residual refinement, FP64 energies"
+             }[args.precision]
     result = {
         "metric": f"hysteresis field-steps/s ({label}, {wl['name']})",
         "value": steps / elapsed,
@@ -626,8 +629,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None if traffic is None else traffic["bytes"],
-            "kernel": ("k_apply_cl<double, false, true> (FP64 matrix-free Poisson apply of the PCG, edge-star "
+            "traffic": None if traffic is None or args.precision == "fp32pcg" else traffic["bytes"],
+            "kernel": ("k_apply_cl<float, false, true> (FP32 matrix-free Poisson apply of the PCG, edge-star "
+                       "clusters)" if args.precision == "fp32pcg" else
+                       "k_apply_cl<double, false, true> (FP64 matrix-free Poisson apply of the PCG, edge-star "
                        "clusters)" if shape["unit"] != "element" else "k_apply (FP64 matrix-free Poisson apply)"),
             "avg_launch_us": avg_apply_s * 1e6,
             "timed_launches": apply_n, "launches": head["applies"],

@@ -93,10 +93,13 @@ int mm_ctx_get_nodal(int64_t ctx, double* M_nodal, double* inv_M_rel, double* in
 /* diag(K_ex)/V_mag (N,): compute_exchange_diagonal (src/energy_kernels.py:492-566). */
 int mm_ctx_get_exchange_diag(int64_t ctx, double* d);
 
-/* Precision of the Poisson preconditioner: 0 = FP64 V-cycle; 1 = mixed: the whole V-cycle in
- * FP32 (FP32 matrix-free applies and coarse levels) while the PCG recurrence, its operator
- * apply (the true residual), the dot products and alpha/beta stay FP64, so solutions still
- * meet the FP64 tolerance. */
+/* Precision of the Poisson solve: 0 = FP64; 1 = mixed: the whole V-cycle in FP32 (FP32
+ * matrix-free applies and coarse levels) while the PCG recurrence, its operator apply, the dot
+ * products and alpha/beta stay FP64; 2 = FP32 PCG with FP64 iterative refinement: the PCG runs on
+ * FP32 vectors with the FP32 apply and V-cycle (dots, alpha and beta in FP64) and an outer loop
+ * recomputes the true residual b - A U with the FP64 apply and restarts the inner PCG on it until
+ * r.r <= tol^2 b.b (needs the AMG V-cycle and the on-the-fly cluster apply). Every form meets the
+ * FP64 tolerance. */
 int mm_set_precision(int64_t ctx, int precision);
 
 /* Select the geometry source of the Poisson apply (see flags of mm_ctx_create). */

@@ -18,6 +18,7 @@ from ._hip import call, f64, i32, ptr
 GEOM_MODES = {"on_the_fly": 0, "stored_f64": 1, "stored_f32": 2, "on_the_fly_f32": 3}
 METHODS = {"pcohen_hs": 0, "lbfgs": 1}
 APPLY_UNITS = {"element": 0, "cluster": 1, "persistent": 2}
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp32pcg": 2}
 
 
 class MeshContext:
@@ -134,8 +135,10 @@ class MeshContext:
         return d
 
     def set_precision(self, precision: str):
-        """'fp64' or 'mixed' (FP32 V-cycle under an FP64 PCG)."""
-        call("mm_set_precision", self.handle, {"fp64": 0, "mixed": 1}[precision])
+        """'fp64'; 'mixed' (FP32 V-cycle under an FP64 PCG); 'fp32pcg' (the PCG itself on FP32
+        vectors with the FP32 apply and V-cycle, FP64 dots and alpha/beta, and an FP64
+        true-residual refinement loop to the same tolerance)."""
+        call("mm_set_precision", self.handle, PRECISIONS[precision])
         self.precision = precision
 
     def set_apply_shape(self, unit: str = "cluster", param: int = 256):

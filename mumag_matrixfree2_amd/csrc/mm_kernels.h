@@ -207,8 +207,11 @@ struct Ctx {
   std::vector<Level> lv;
   std::vector<std::vector<int>> amg_seg_off;  // per level: rows of each segment (set-up)
   double* diagA = nullptr;            // N
-  int precision = 0;                  // 0: FP64 V-cycle, 1: FP32 V-cycle (mixed precision)
+  int precision = 0;                  // 0: FP64; 1: FP32 V-cycle; 2: FP32 PCG with FP64 refinement
   float *px0f = nullptr, *ptf = nullptr;  // level-0 FP32 work vectors
+  // FP32 PCG with FP64 refinement (precision 2): correction, residual, direction, A p, V-cycle output
+  float *rxf = nullptr, *rrf = nullptr, *rpf = nullptr, *rqf = nullptr, *rzf = nullptr;
+  int ref_cycles = 0;                 // refinement cycles of the last solve
 
   // minimiser state (3N unless noted)
   double *m = nullptr, *U = nullptr /*N*/, *g_raw = nullptr, *g_tan = nullptr, *g_ext = nullptr, *g_prev = nullptr;
@@ -298,6 +301,8 @@ int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s);
 // y = A (w .* r) with xo = w .* r written by the node owners (fused pre-smoothing); y zeroed first
 int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s);
 int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float* xo, float* y, hipStream_t s);
+int launch_apply_f32_scaled_f(const Ctx& c, const float* r, const float* w, float* xo, float* y, hipStream_t s);
+int launch_apply_f32_dot(const Ctx& c, const float* x, float* y, double* dot_part, hipStream_t s);
 int launch_rhs(const Ctx& c, const double* m, double* b, hipStream_t s);
 // launch_energy with the block partials (2 doubles per magnetic block) in a caller-given buffer
 int launch_energy_buf(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,

@@ -322,9 +322,15 @@ __device__ __forceinline__ void cluster_apply(uint32_t w0, uint32_t w1, uint32_t
 // SCALED: input w .* r from the FP64 residual r.
 // NB: distinct nodes per block (256, 512 or 1024; indices above 255 carry their high bits in
 // cmask), NB / NT staged nodes per thread.
-template <typename T, bool SCALED, bool DOT, int NT, int NB>
+template <typename T>
+struct same_type {  // a non-deduced parameter type: the template argument is given or defaulted
+  using type = T;
+};
+
+template <typename T, bool SCALED, bool DOT, int NT, int NB, typename TR = double>
 __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __restrict__ x,
-                                                 const double* __restrict__ r, const T* __restrict__ w,
+                                                 const typename same_type<TR>::type* __restrict__ r,
+                                                 const T* __restrict__ w,
                                                  T* __restrict__ xo, T* __restrict__ y, T reg,
                                                  double* __restrict__ dot_part) {
   static_assert(NB % NT == 0 && NB <= kMaxClusterBlockNodes, "block nodes per thread");
@@ -1088,6 +1094,35 @@ int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s) {
   with_nt(c.elem_threads, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
     k_apply_f32<false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, (float)c.reg);
+  });
+  return 0;
+}
+
+// FP32 cluster apply of the FP32 PCG: y = A x with per-block partials of x . A x (FP64 sums of
+// the FP32 element energies) in dot_part
+int launch_apply_f32_dot(const Ctx& c, const float* x, float* y, double* dot_part, hipStream_t s) {
+  MM_CHECK(c.apply_clusters, "the FP32 PCG needs the cluster apply");
+  const Part& p = c.part_cl;
+  if (part_len(p) == 0) return 0;
+  const int grid = grid_for(p);
+  with_cl_shape(c, [&](auto nt, auto nb) {
+    constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
+    k_apply_cl<float, false, true, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
+                                                               (float)c.reg, dot_part);
+  });
+  return 0;
+}
+
+// y = A (w .* r) with FP32 r (the V-cycle of the FP32 PCG)
+int launch_apply_f32_scaled_f(const Ctx& c, const float* r, const float* w, float* xo, float* y, hipStream_t s) {
+  MM_CHECK(c.apply_clusters, "the FP32 PCG needs the cluster apply");
+  const Part& p = c.part_cl;
+  if (part_len(p) == 0) return 0;
+  const int grid = grid_for(p);
+  with_cl_shape(c, [&](auto nt, auto nb) {
+    constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
+    k_apply_cl<float, true, false, NT, NB, float><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg,
+                                                                      nullptr);
   });
   return 0;
 }

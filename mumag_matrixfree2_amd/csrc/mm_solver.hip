@@ -809,8 +809,8 @@ static int pcg_seg(Ctx& c, const double* b, double* U, const double* tol, const 
   vfill(c.pq, 0.0, N, s);
   MM_TRY(timed_apply(c, U, c.pq, s));
   c.n_apply++;
-  float* tf = c.precision == 1 ? c.ptf : nullptr;
-  double* t64 = c.precision == 1 ? nullptr : c.pt;
+  float* tf = c.precision >= 1 ? c.ptf : nullptr;
+  double* t64 = c.precision >= 1 ? nullptr : c.pt;
   k_seg_init<<<W, kBlock, 0, s>>>(S, b, c.pq, c.pr, t64, tf, c.hseg_dev);
   c.seg_running = true;
   int rc = precond(c, c.pr, c.pz, c.part_rz, s);
@@ -971,7 +971,7 @@ static int precond_f32(Ctx& c, const double* r, double* z, double* rz_part, hipS
 int precond(Ctx& c, const double* r, double* z, double* rz_part, hipStream_t s) {
   const long N = c.N;
   Level& L0 = c.lv[0];
-  if (c.precision == 1 && c.lv.size() > 1 && L0.Mf != nullptr) return precond_f32(c, r, z, rz_part, s);
+  if (c.precision >= 1 && c.lv.size() > 1 && L0.Mf != nullptr) return precond_f32(c, r, z, rz_part, s);
   if (c.lv.size() <= 1) {
     if (L0.dense_inv)  // single-level hierarchy: exact dense solve
       symv<double>(L0, L0.dense_inv, r, z, s);
@@ -1006,6 +1006,176 @@ int precond_apply(Ctx& c, const double* r, double* z, double* rz_part, hipStream
   return precond(c, r, z, rz_part, s);
 }
 
+// ------------------------------------------------- FP32 PCG with FP64 iterative refinement
+// precision 2: the PCG runs on FP32 vectors (correction, residual, direction, A p, V-cycle
+// output) with the FP32 matrix-free apply and the FP32 V-cycle, while its dot products and the
+// alpha / beta recurrences stay FP64. An outer FP64 loop recomputes the true residual
+// r = b - A U with the FP64 apply after every inner solve and restarts the inner PCG on it until
+// r.r <= tol^2 b.b: the stopping test, and so the solution's accuracy, are those of the FP64 PCG.
+
+// rf = (float) r, xc = 0, tf = 0
+__global__ void k_ref_start(const double* __restrict__ r, float* __restrict__ rf, float* __restrict__ xc,
+                            float* __restrict__ tf, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    rf[i] = (float)r[i];
+    xc[i] = 0.0f;
+    tf[i] = 0.0f;
+  }
+}
+
+// z = x + M (r - t) in FP32 from the V-cycle's level-0 vectors, partials of r.z in FP64
+__global__ __launch_bounds__(kBlock) void k_post_dot_ff(const float* __restrict__ x, const float* __restrict__ M,
+                                                        const float* __restrict__ r, const float* __restrict__ t,
+                                                        float* __restrict__ z, long n, double* __restrict__ part) {
+  double v[1] = {0.0};
+  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    const float zi = x[i] + M[i] * (r[i] - t[i]);
+    z[i] = zi;
+    v[0] += (double)r[i] * (double)zi;
+  }
+  block_partial<1>(v, part);
+}
+
+// alpha = rz / (p.q + 1e-30) in FP64; xc += alpha p, r -= alpha q (FP32 vectors, FP64
+// arithmetic); partials of r.r; tf cleared for the next V-cycle
+__global__ __launch_bounds__(kBlock) void k_ref_xr(const double* __restrict__ sc, int rz_slot,
+                                                   const double* __restrict__ part_pq, int n_pq,
+                                                   float* __restrict__ xc, float* __restrict__ r,
+                                                   const float* __restrict__ p, const float* __restrict__ q,
+                                                   float* __restrict__ tf, long n, double* __restrict__ part_rr) {
+  const double alpha = sc[rz_slot] / (sum_partials_all<kBlock>(part_pq, n_pq) + 1e-30);
+  double v[1] = {0.0};
+  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    tf[i] = 0.0f;
+    xc[i] = (float)((double)xc[i] + alpha * (double)p[i]);
+    const double ri = (double)r[i] - alpha * (double)q[i];
+    const float rf = (float)ri;
+    r[i] = rf;
+    v[0] += (double)rf * (double)rf;
+  }
+  block_partial<1>(v, part_rr);
+}
+
+// rz_new from partials (kept in sc[new_slot]); beta = rz_new / sc[old_slot]; p = z + beta p
+// (first: p = z); q = 0
+__global__ __launch_bounds__(kBlock) void k_ref_p(double* __restrict__ sc, const double* __restrict__ part_rz,
+                                                  int n_rz, int new_slot, int old_slot, int first,
+                                                  const float* __restrict__ z, float* __restrict__ p,
+                                                  float* __restrict__ q, long n) {
+  const double rz = sum_partials_all<kBlock>(part_rz, n_rz);
+  const double beta = first ? 0.0 : rz / (sc[old_slot] + 1e-30);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[new_slot] = rz;
+  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    p[i] = first ? z[i] : (float)((double)z[i] + beta * (double)p[i]);
+    q[i] = 0.0f;
+  }
+}
+
+// U += xc; q = 0 for the FP64 apply of the true residual
+__global__ void k_ref_update(double* __restrict__ U, const float* __restrict__ xc, double* __restrict__ q, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    U[i] += (double)xc[i];
+    q[i] = 0.0;
+  }
+}
+
+// The FP32 V-cycle on an FP32 residual: z (FP32) and FP64 partials of r.z. The caller has
+// cleared ptf.
+static int precond_ff(Ctx& c, const float* r, float* z, double* rz_part, hipStream_t s) {
+  const long N = c.N;
+  Level& L0 = c.lv[0];
+  Level& L1 = c.lv[1];
+  MM_TRY(launch_apply_f32_scaled_f(c, r, L0.Mf, c.px0f, c.ptf, s));               // x = M0 r, t = A x
+  const bool l1_smooth = c.lv.size() > 2;
+  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rxf, r, c.ptf, L1.bf, s, l1_smooth ? L1.xf : (float*)nullptr, &L1.runs,
+       l1_smooth ? (const float*)L1.Mf : (const float*)nullptr);  // b1 = R0 (r - t)
+  vcycle_coarse_f(c, 1, s);
+  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Pxf, L1.xsf, (const float*)nullptr, c.px0f, s, c.ptf, &L0.runs);
+  MM_TRY(launch_apply_f32(c, c.px0f, c.ptf, s));                                  // t = A x
+  k_post_dot_ff<<<red_grid(N), kBlock, 0, s>>>(c.px0f, L0.Mf, r, c.ptf, z, N, rz_part);
+  c.n_vcycle += 1;
+  return 0;
+}
+
+static int timed_apply_f32_dot(Ctx& c, const float* x, float* y, double* dot_part, hipStream_t s) {
+  if (!c.time_apply || (c.apply_seen++ % kApplySample) != 0) return launch_apply_f32_dot(c, x, y, dot_part, s);
+  if (c.ev_pool.empty()) {
+    c.ev_pool.resize(512);
+    for (auto& e : c.ev_pool) MM_HIP(hipEventCreate(&e));
+  }
+  if (2 * c.ev_used >= (int)c.ev_pool.size()) MM_TRY(resolve_apply_timing(c));
+  MM_HIP(hipEventRecord(c.ev_pool[2 * c.ev_used], s));
+  MM_TRY(launch_apply_f32_dot(c, x, y, dot_part, s));
+  MM_HIP(hipEventRecord(c.ev_pool[2 * c.ev_used + 1], s));
+  ++c.ev_used;
+  return 0;
+}
+
+// Inner reduction per refinement cycle: about what an FP32 residual can carry
+constexpr double kRefEta = 1e-6;
+
+static int pcg_ref(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it_out, double* r2_out,
+                   double* bb_out, bool q_ready) {
+  hipStream_t s = c.stream;
+  const long N = c.N;
+  double* sc = c.dscal;
+  const int gv = red_grid(N);
+  const int n_pq = apply_dot_count(c);
+  if (!q_ready) {
+    vfill(c.pq, 0.0, N, s);
+    MM_TRY(launch_apply(c, kGeomCoords, U, c.pq, s));
+    c.n_apply++;
+  }
+  k_pcg_init<<<gv, kBlock, 0, s>>>(b, c.pq, c.pr, nullptr, nullptr, N, c.hpart_dev);  // true FP64 residual
+  MM_HIP(hipStreamSynchronize(s));
+  double r2 = 0.0, bb = 0.0;
+  for (int i = 0; i < gv; ++i) {
+    r2 += c.hpart[2 * i];
+    bb += c.hpart[2 * i + 1];
+  }
+  const double tol2 = tol * tol * bb;
+  int it = 0, cycles = 0;
+  while (it < maxiter && r2 > tol2) {
+    // inner FP32 PCG on A xc = r, to 0.5 tol of b or a 1e-6 reduction of this cycle's residual
+    const double target = std::max(0.25 * tol2, kRefEta * kRefEta * r2);
+    k_ref_start<<<vgrid(N), kBlock, 0, s>>>(c.pr, c.rrf, c.rxf, c.ptf, N);
+    MM_TRY(precond_ff(c, c.rrf, c.rzf, c.part_rz, s));
+    k_ref_p<<<gv, kBlock, 0, s>>>(sc, c.part_rz, gv, 0, 0, 1, c.rzf, c.rpf, c.rqf, N);
+    int cur = 0;
+    double r2in = r2;
+    while (it < maxiter && r2in > target) {
+      MM_TRY(timed_apply_f32_dot(c, c.rpf, c.rqf, c.part_pq, s));  // q was cleared by k_ref_p
+      c.n_apply++;
+      k_ref_xr<<<gv, kBlock, 0, s>>>(sc, cur, c.part_pq, n_pq, c.rxf, c.rrf, c.rpf, c.rqf, c.ptf, N, c.hpart_dev);
+      MM_HIP(hipEventRecord(c.ev, s));
+      MM_TRY(precond_ff(c, c.rrf, c.rzf, c.part_rz, s));
+      k_ref_p<<<gv, kBlock, 0, s>>>(sc, c.part_rz, gv, 1 - cur, cur, 0, c.rzf, c.rpf, c.rqf, N);
+      cur = 1 - cur;
+      ++it;
+      MM_HIP(hipEventSynchronize(c.ev));
+      r2in = 0.0;
+      for (int i = 0; i < gv; ++i) r2in += c.hpart[i];
+    }
+    // U += xc and the true residual in FP64
+    k_ref_update<<<vgrid(N), kBlock, 0, s>>>(U, c.rxf, c.pq, N);
+    MM_TRY(launch_apply(c, kGeomCoords, U, c.pq, s));
+    c.n_apply++;
+    k_pcg_init<<<gv, kBlock, 0, s>>>(b, c.pq, c.pr, nullptr, nullptr, N, c.hpart_dev);
+    MM_HIP(hipStreamSynchronize(s));
+    r2 = 0.0;
+    for (int i = 0; i < gv; ++i) r2 += c.hpart[2 * i];
+    ++cycles;
+  }
+  MM_HIP(hipStreamSynchronize(s));
+  MM_TRY(resolve_apply_timing(c));
+  c.n_pcg_it += it;
+  c.ref_cycles = cycles;
+  *it_out = it;
+  *r2_out = r2;
+  *bb_out = bb;
+  return 0;
+}
+
 // Device-resident PCG: U (in/out, device) solves A U = b (device), stop when
 // it >= maxiter or r.r <= tol^2 b.b (make_pcg_solve, src/poisson_solve.py:422-478).
 // Returns iterations, r.r and b.b; r.r is summed on the host from the per-workgroup
@@ -1015,6 +1185,12 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
   hipStream_t s = c.stream;
   const long N = c.N;
   MM_CHECK(c.lv.size() >= 1 && c.lv[0].M != nullptr, "preconditioner not set up (call mm_amg_upload or mm_precond_jacobi)");
+  if (c.precision == 2) {
+    MM_CHECK(c.lv.size() > 1 && c.lv[0].Mf != nullptr && c.rxf != nullptr && c.geom_mode == kGeomCoords &&
+                 c.apply_clusters,
+             "the FP32 PCG needs the AMG V-cycle, the on-the-fly cluster apply and mm_set_precision(ctx, 2)");
+    return pcg_ref(c, b, U, tol, maxiter, it_out, r2_out, bb_out, q_ready);
+  }
   double* sc = c.dscal;  // slots 0/1: r.z ping-pong
   const int gv = red_grid(N);
   // p.Ap from the apply's per-block element energies (on-the-fly geometry), else a dot pass
@@ -1025,8 +1201,8 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
     MM_TRY(timed_apply(c, U, c.pq, s));
     c.n_apply++;
   }
-  float* tf = c.precision == 1 ? c.ptf : nullptr;
-  double* t64 = c.precision == 1 ? nullptr : c.pt;
+  float* tf = c.precision >= 1 ? c.ptf : nullptr;
+  double* t64 = c.precision >= 1 ? nullptr : c.pt;
   k_pcg_init<<<gv, kBlock, 0, s>>>(b, c.pq, c.pr, t64, tf, N, c.hpart_dev);
   MM_TRY(precond(c, c.pr, c.pz, c.part_rz, s));
   k_pcg_p<<<gv, kBlock, 0, s>>>(sc, c.part_rz, gv, 0, 0, 1, c.pz, c.pp, c.pq, N);

@@ -101,3 +101,50 @@ def test_mixed_sweep_same_switching_step(small_cube):
     E64, Emx = r64["history"][:, 2], rmx["history"][:, 2]
     assert r64["switching_step"] == rmx["switching_step"] > 0
     assert np.max(np.abs(Emx - E64) / np.maximum(np.abs(E64), 1.0)) <= 1e-8
+
+
+def test_fp32_pcg_meets_the_fp64_tolerance(mixed_ctx, oracle_c0):
+    """The FP32 PCG with FP64 refinement stops on the FP64 true residual: r.r <= tol^2 b.b
+    checked by the oracle in FP64, and the solution agrees with the FP64 solve to the tolerance
+    (from zero and from a warm start close to the solution)."""
+    ctx = mixed_ctx
+    m = tilted_m0(ctx.N, seed=34, sigma=0.4)
+    U64, it64, _ = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+    b = oracle_c0.rhs(m)
+    ctx.set_precision("fp32pcg")
+    try:
+        U, it, rr = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+        U2, it2, rr2 = ctx.solve_U(m, U64 * (1 + 1e-4), 1e-12)
+    finally:
+        ctx.set_precision("fp64")
+    true_rr = np.linalg.norm(b - oracle_c0.A @ U) / np.linalg.norm(b)
+    true_rr2 = np.linalg.norm(b - oracle_c0.A @ U2) / np.linalg.norm(b)
+    print(f"[fp32pcg solve] it={it} (fp64 {it64}) rr={rr:.3e} true={true_rr:.3e}; warm it={it2} true={true_rr2:.3e}",
+          flush=True)
+    assert rr <= 1e-10 and true_rr <= 1.001e-10
+    assert rr2 <= 1e-12 and true_rr2 <= 1.001e-12
+    assert rel(U, U64) <= 1e-8
+    assert it <= 2 * it64 + 5
+
+
+@pytest.mark.parametrize("method", ["pcohen_hs", "lbfgs"])
+def test_fp32_pcg_field_step_energy(mixed_ctx, oracle_c0, method):
+    from mumag_matrixfree2_amd.context import make_params
+
+    ctx = mixed_ctx
+    m0 = tilted_m0(ctx.N, seed=35, sigma=0.1)
+    B = np.array([0.017452, 0.0, 0.999848]) * (-0.4)
+    p = make_params(method, tau_f=1e-10, cg_tol=1e-10)
+    out = {}
+    for prec in ("fp64", "fp32pcg"):
+        ctx.set_precision(prec)
+        ctx.state_set(m0, np.zeros(ctx.N))
+        out[prec] = ctx.minimize_resident(p, B)
+    ctx.set_precision("fp64")
+    m, U = ctx.state_get()
+    scale = sum(abs(v) for v in oracle_c0.energy_terms(m, U, B).values())
+    print(f"[fp32pcg {method}] E64={out['fp64']['E']:.15e} E32={out['fp32pcg']['E']:.15e} "
+          f"it={out['fp64']['iters']}/{out['fp32pcg']['iters']} pcg={out['fp64']['demag_iters']}/"
+          f"{out['fp32pcg']['demag_iters']}", flush=True)
+    assert out["fp32pcg"]["converged"]
+    assert abs(out["fp32pcg"]["E"] - out["fp64"]["E"]) <= 1e-10 * scale
