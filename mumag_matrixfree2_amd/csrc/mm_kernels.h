@@ -219,7 +219,8 @@ struct Ctx {
   double *m_t = nullptr, *g_t = nullptr, *U_t = nullptr /*N*/;
   double *r3 = nullptr, *z3 = nullptr, *p3 = nullptr, *q3 = nullptr, *cv3 = nullptr;
   int mem = 10;
-  double *S = nullptr, *Yh = nullptr;  // mem x 3N
+  double *S = nullptr, *Yh = nullptr;  // mem x 3N (FP64 history)
+  float *Sf = nullptr, *Yf = nullptr;  // mem x 3N (FP32 history of the mixed-precision forms)
   // per-workgroup partials of the field-step reductions (9 banks of 2 x kRedGridMax) and a pinned
   // host copy of the first banks
   double *mpart = nullptr, *hmpart = nullptr;

@@ -388,25 +388,32 @@ __global__ __launch_bounds__(kBlock) void k_maxdiff(const double* __restrict__ a
 }
 
 // ----------------------------------------------------------------------- L-BFGS kernels
+// The history vectors S, Y are FP64, or FP32 in the mixed-precision forms (TS = float): the
+// two-loop recursion reads them in FP64 arithmetic and keeps q, r, alpha, beta, rho and gamma
+// in FP64.
+
 // per-workgroup partials of a.b
-__global__ __launch_bounds__(kBlock) void k_lb_dot(const double* __restrict__ a, const double* __restrict__ b, long n,
+template <typename TS>
+__global__ __launch_bounds__(kBlock) void k_lb_dot(const TS* __restrict__ a, const double* __restrict__ b, long n,
                                                    double* __restrict__ part) {
   double v[1] = {0.0};
-  NODE_LOOP(i, n) v[0] += a[i] * b[i];
+  NODE_LOOP(i, n) v[0] += (double)a[i] * b[i];
   block_partial<1>(v, part);
 }
 
 // first loop: alpha_idx = rho_idx * (S_idx . q) from the k_lb_dot partials; q -= alpha_idx Y_idx
+template <typename TS>
 __global__ __launch_bounds__(kBlock) void k_lb_first(double* __restrict__ sc, int idx, double* __restrict__ q,
-                                                     const double* __restrict__ Yi, long n,
+                                                     const TS* __restrict__ Yi, long n,
                                                      const double* __restrict__ part, int n_part) {
   const double alpha = sc[S_RHOS + idx] * sum_partials_all<kBlock>(part, n_part);
-  NODE_LOOP(i, n) q[i] -= alpha * Yi[i];
+  NODE_LOOP(i, n) q[i] -= alpha * (double)Yi[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_ALPHAS + idx] = alpha;
 }
 
 // per-workgroup partials of the gamma dots S_last.Y_last and Y_last.(Y_last invMr)
-__global__ __launch_bounds__(kBlock) void k_lb_gamma_dots(const double* __restrict__ S, const double* __restrict__ Y,
+template <typename TS>
+__global__ __launch_bounds__(kBlock) void k_lb_gamma_dots(const TS* __restrict__ S, const TS* __restrict__ Y,
                                                           const double* __restrict__ invMr, long N,
                                                           double* __restrict__ part) {
   double v[2] = {0.0, 0.0};
@@ -414,8 +421,9 @@ __global__ __launch_bounds__(kBlock) void k_lb_gamma_dots(const double* __restri
     const double s = invMr[i];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      v[0] += S[3 * i + k] * Y[3 * i + k];
-      v[1] += Y[3 * i + k] * Y[3 * i + k] * s;
+      const double yk = (double)Y[3 * i + k];
+      v[0] += (double)S[3 * i + k] * yk;
+      v[1] += yk * yk * s;
     }
   }
   block_partial<2>(v, part);
@@ -438,11 +446,23 @@ __global__ __launch_bounds__(kBlock) void k_lb_scale(int have_hist, const double
 }
 
 // second loop: beta = rho_idx * (Y_idx . r) from the k_lb_dot partials; r += S_idx (alpha_idx - beta)
+template <typename TS>
 __global__ __launch_bounds__(kBlock) void k_lb_second(const double* __restrict__ sc, int idx, double* __restrict__ r,
-                                                      const double* __restrict__ Si, long n,
+                                                      const TS* __restrict__ Si, long n,
                                                       const double* __restrict__ part, int n_part) {
   const double f = sc[S_ALPHAS + idx] - sc[S_RHOS + idx] * sum_partials_all<kBlock>(part, n_part);
-  NODE_LOOP(i, n) r[i] += Si[i] * f;
+  NODE_LOOP(i, n) r[i] += (double)Si[i] * f;
+}
+
+template <typename TS>
+__global__ void k_lb_zero(TS* __restrict__ dst, long n) {
+  NODE_LOOP(i, n) dst[i] = (TS)0;
+}
+
+// history slot = (TS) v
+template <typename TS>
+__global__ void k_lb_store(TS* __restrict__ dst, const double* __restrict__ v, long n) {
+  NODE_LOOP(i, n) dst[i] = (TS)v[i];
 }
 
 // d = -r and d.g_ext
@@ -484,21 +504,24 @@ __global__ __launch_bounds__(kBlock) void k_lb_pair(const double* __restrict__ m
 }
 
 // transport every history vector to the tangent space at m (src/minimizers.py:1342-1347)
-__global__ void k_lb_transport(const double* __restrict__ m, double* __restrict__ S, double* __restrict__ Y, int mem,
+template <typename TS>
+__global__ void k_lb_transport(const double* __restrict__ m, TS* __restrict__ S, TS* __restrict__ Y, int mem,
                                long N) {
   NODE_LOOP(i, N) {
     const double m0 = m[3 * i], m1 = m[3 * i + 1], m2 = m[3 * i + 2];
     for (int h = 0; h < mem; ++h) {
-      double* s = S + (long)h * 3 * N + 3 * i;
-      double* y = Y + (long)h * 3 * N + 3 * i;
-      const double ds = m0 * s[0] + m1 * s[1] + m2 * s[2];
-      s[0] -= ds * m0;
-      s[1] -= ds * m1;
-      s[2] -= ds * m2;
-      const double dy = m0 * y[0] + m1 * y[1] + m2 * y[2];
-      y[0] -= dy * m0;
-      y[1] -= dy * m1;
-      y[2] -= dy * m2;
+      TS* s = S + (long)h * 3 * N + 3 * i;
+      TS* y = Y + (long)h * 3 * N + 3 * i;
+      const double s0 = s[0], s1 = s[1], s2 = s[2];
+      const double ds = m0 * s0 + m1 * s1 + m2 * s2;
+      s[0] = (TS)(s0 - ds * m0);
+      s[1] = (TS)(s1 - ds * m1);
+      s[2] = (TS)(s2 - ds * m2);
+      const double y0 = y[0], y1 = y[1], y2 = y[2];
+      const double dy = m0 * y0 + m1 * y1 + m2 * y2;
+      y[0] = (TS)(y0 - dy * m0);
+      y[1] = (TS)(y1 - dy * m1);
+      y[2] = (TS)(y2 - dy * m2);
     }
   }
 }
@@ -644,16 +667,33 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
   hipStream_t s = c.stream;
   double* sc = c.dscal;
   const int mem = p.memory > 0 ? p.memory : 10;
-  if (p.method == 1 && (!c.S || c.mem != mem)) {
+  // L-BFGS history: FP64, or FP32 in the mixed-precision forms
+  const bool lb32 = c.precision >= 1;
+  if (p.method == 1 && ((lb32 ? !c.Sf : !c.S) || c.mem != mem)) {
+    if (c.mem != mem) {
+      for (void* old : {(void*)c.S, (void*)c.Yh, (void*)c.Sf, (void*)c.Yf})
+        if (old) {
+          auto itp = std::find(c.allocs.begin(), c.allocs.end(), old);
+          if (itp != c.allocs.end()) c.allocs.erase(itp);
+          (void)hipFree(old);
+        }
+      c.S = c.Yh = nullptr;
+      c.Sf = c.Yf = nullptr;
+    }
     c.mem = mem;
-    double* S = nullptr;
-    double* Y = nullptr;
-    MM_HIP(hipMalloc(&S, (size_t)mem * n3 * sizeof(double)));
-    MM_HIP(hipMalloc(&Y, (size_t)mem * n3 * sizeof(double)));
+    const size_t bytes = (size_t)mem * n3 * (lb32 ? sizeof(float) : sizeof(double));
+    void *S = nullptr, *Y = nullptr;
+    MM_HIP(hipMalloc(&S, bytes));
+    MM_HIP(hipMalloc(&Y, bytes));
     c.allocs.push_back(S);
     c.allocs.push_back(Y);
-    c.S = S;
-    c.Yh = Y;
+    if (lb32) {
+      c.Sf = (float*)S;
+      c.Yf = (float*)Y;
+    } else {
+      c.S = (double*)S;
+      c.Yh = (double*)Y;
+    }
   }
 
   // solve_and_minimize (src/minimizers.py:3408-3435)
@@ -680,8 +720,13 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
     MM_TRY(vcopy(c.g_prev, c.g_ext, n3, s));
     k_neg_dot<<<n3g, kBlock, 0, s>>>(c.g_ext, c.g_ext, c.d_prev, n3, rbank(c, R_DG));
   } else {
-    vfill(c.S, 0.0, (long)mem * n3, s);
-    vfill(c.Yh, 0.0, (long)mem * n3, s);
+    if (lb32) {
+      k_lb_zero<float><<<vg(mem * n3), kBlock, 0, s>>>(c.Sf, mem * n3);
+      k_lb_zero<float><<<vg(mem * n3), kBlock, 0, s>>>(c.Yf, mem * n3);
+    } else {
+      vfill(c.S, 0.0, (long)mem * n3, s);
+      vfill(c.Yh, 0.0, (long)mem * n3, s);
+    }
     for (int h = 0; h < mem; ++h) k_set_slot<<<1, 1, 0, s>>>(sc, S_RHOS + h, 0.0);
   }
   int hist = 0;
@@ -701,22 +746,29 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
       // two-loop recursion on q = g_ext (src/minimizers.py:1284-1312); q in r3, r in z3
       MM_TRY(vcopy(c.r3, c.g_ext, n3, s));
       const int nh = std::min(hist, mem);
-      for (int i = 0; i < nh; ++i) {
-        const int idx = ((hist - 1 - i) % mem + mem) % mem;
-        k_lb_dot<<<n3g, kBlock, 0, s>>>(c.S + (long)idx * n3, c.r3, n3, rbank(c, R_TMP));
-        k_lb_first<<<n3g, kBlock, 0, s>>>(sc, idx, c.r3, c.Yh + (long)idx * n3, n3, rbank(c, R_TMP), n3g);
-      }
-      if (hist > 0) {
-        const int last = ((hist - 1) % mem + mem) % mem;
-        k_lb_gamma_dots<<<nN, kBlock, 0, s>>>(c.S + (long)last * n3, c.Yh + (long)last * n3, c.inv_M_rel, N,
-                                               rbank(c, R_SY));
-      }
-      k_lb_scale<<<nN, kBlock, 0, s>>>(hist > 0, c.r3, c.inv_M_rel, c.z3, N, rbank(c, R_SY), nN);
-      for (int i = 0; i < nh; ++i) {
-        const int idx = ((hist - nh + i) % mem + mem) % mem;
-        k_lb_dot<<<n3g, kBlock, 0, s>>>(c.Yh + (long)idx * n3, c.z3, n3, rbank(c, R_TMP));
-        k_lb_second<<<n3g, kBlock, 0, s>>>(sc, idx, c.z3, c.S + (long)idx * n3, n3, rbank(c, R_TMP), n3g);
-      }
+      auto two_loop = [&](auto* S, auto* Y) {
+        using TS = std::remove_const_t<std::remove_pointer_t<decltype(S)>>;
+        for (int i = 0; i < nh; ++i) {
+          const int idx = ((hist - 1 - i) % mem + mem) % mem;
+          k_lb_dot<TS><<<n3g, kBlock, 0, s>>>(S + (long)idx * n3, c.r3, n3, rbank(c, R_TMP));
+          k_lb_first<TS><<<n3g, kBlock, 0, s>>>(sc, idx, c.r3, Y + (long)idx * n3, n3, rbank(c, R_TMP), n3g);
+        }
+        if (hist > 0) {
+          const int last = ((hist - 1) % mem + mem) % mem;
+          k_lb_gamma_dots<TS><<<nN, kBlock, 0, s>>>(S + (long)last * n3, Y + (long)last * n3, c.inv_M_rel, N,
+                                                     rbank(c, R_SY));
+        }
+        k_lb_scale<<<nN, kBlock, 0, s>>>(hist > 0, c.r3, c.inv_M_rel, c.z3, N, rbank(c, R_SY), nN);
+        for (int i = 0; i < nh; ++i) {
+          const int idx = ((hist - nh + i) % mem + mem) % mem;
+          k_lb_dot<TS><<<n3g, kBlock, 0, s>>>(Y + (long)idx * n3, c.z3, n3, rbank(c, R_TMP));
+          k_lb_second<TS><<<n3g, kBlock, 0, s>>>(sc, idx, c.z3, S + (long)idx * n3, n3, rbank(c, R_TMP), n3g);
+        }
+      };
+      if (lb32)
+        two_loop(c.Sf, c.Yf);
+      else
+        two_loop(c.S, c.Yh);
       k_neg_dot<<<n3g, kBlock, 0, s>>>(c.z3, c.g_ext, c.d, n3, rbank(c, R_DG));
       k_direction_fix<<<nN, kBlock, 0, s>>>(it == 0, c.m, c.g_tan, c.g_raw, c.d, c.H, N, rbank(c, R_DG), n3g,
                                              rbank(c, R_PG));
@@ -766,10 +818,18 @@ int minimize_resident(Ctx& c, const MinParams& p, const double B[3], MinInfo& in
       }
       const bool ok = curv > 1e-12 * ss;
       const int idx = hist % mem;
-      k_lb_transport<<<vg(N), kBlock, 0, s>>>(c.m, c.S, c.Yh, mem, N);
+      if (lb32)
+        k_lb_transport<float><<<vg(N), kBlock, 0, s>>>(c.m, c.Sf, c.Yf, mem, N);
+      else
+        k_lb_transport<double><<<vg(N), kBlock, 0, s>>>(c.m, c.S, c.Yh, mem, N);
       if (ok) {
-        MM_TRY(vcopy(c.S + (long)idx * n3, c.q3, n3, s));
-        MM_TRY(vcopy(c.Yh + (long)idx * n3, c.cv3, n3, s));
+        if (lb32) {
+          k_lb_store<float><<<vg(n3), kBlock, 0, s>>>(c.Sf + (long)idx * n3, c.q3, n3);
+          k_lb_store<float><<<vg(n3), kBlock, 0, s>>>(c.Yf + (long)idx * n3, c.cv3, n3);
+        } else {
+          MM_TRY(vcopy(c.S + (long)idx * n3, c.q3, n3, s));
+          MM_TRY(vcopy(c.Yh + (long)idx * n3, c.cv3, n3, s));
+        }
         k_set_slot<<<1, 1, 0, s>>>(sc, S_RHOS + idx, 1.0 / (curv + 1e-30));
         ++hist;
       }
