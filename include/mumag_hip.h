@@ -121,6 +121,11 @@ int mm_set_cluster_block_nodes(int64_t ctx, int nb);
 
 /* y = A x: assembled-mode apply_A (src/poisson_solve.py:166-171) computed matrix-free. */
 int mm_poisson_apply(int64_t ctx, const double* x, double* y);
+/* y_r = A x_r for k in {1, 2, 4, 8} vectors at once: x, y (k, N) row-major. One staging of the
+ * mesh serves all k vectors (the unit of a multi-right-hand-side solve); needs the on-the-fly
+ * cluster apply on 256-node cluster blocks. No reference counterpart (the reference applies A
+ * to one vector per call, src/poisson_solve.py:166-171). */
+int mm_poisson_apply_multi(int64_t ctx, int k, const double* x, double* y);
 /* b = D m: rhs_from_m (src/poisson_solve.py:173-186, 254-280); m (N,3). */
 int mm_poisson_rhs(int64_t ctx, const double* m, double* b);
 /* diag(A): assemble_diag (src/poisson_solve.py:282-299) of the masked, regularised A. */
@@ -253,6 +258,10 @@ int mm_counters(int64_t ctx, int64_t* out6);
 /* Benchmark hook: `reps` back-to-back Poisson applies on device-resident vectors, timed with
  * HIP events on the context's stream; returns the mean milliseconds per apply. */
 int mm_bench_apply(int64_t ctx, int geom, int reps, double* ms_per_apply);
+
+/* Benchmark hook: `reps` back-to-back k-vector applies (with their x.Ax partials) on device-resident
+ * vectors; mean milliseconds per k-vector apply. */
+int mm_bench_apply_multi(int64_t ctx, int k, int reps, double* ms);
 
 /* Benchmark hook: `reps` copies of N doubles between two context vectors with a plain copy kernel
  * (k_copy_calib, 16 N bytes per launch), the known-size reference for HBM byte counters; mean ms. */

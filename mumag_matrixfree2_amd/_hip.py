@@ -108,6 +108,8 @@ SIGNATURES = {
     "mm_counters": [C.c_int64, i64p],
     "mm_bench_apply": [C.c_int64, C.c_int, C.c_int, f64p],
     "mm_bench_copy": [C.c_int64, C.c_int, f64p],
+    "mm_bench_apply_multi": [C.c_int64, C.c_int, C.c_int, f64p],
+    "mm_poisson_apply_multi": [C.c_int64, C.c_int, f64p, f64p],
     "mm_bench_precond": [C.c_int64, C.c_int, C.c_int, f64p],
 }
 

@@ -72,3 +72,31 @@ def tune_block_nodes(make_ctx, caps=(128, 192, 256), reps: int = 40):
         else:
             ctx.close()
     return best_ctx, {"block_nodes": best_ctx.block_nodes, "shapes": report}
+
+
+RHS_WIDTHS = (1, 2, 4, 8)
+
+
+def tune_rhs_width(ctx, widths=RHS_WIDTHS, reps: int = 20) -> dict:
+    """Right-hand-side width of a multi-vector solve: time the k-vector apply (one staging of
+    the mesh for k node-major vectors, ``mm_bench_apply_multi``) for each k and the best
+    single-vector apply of ``ctx``, and return ``{"width": k with the lowest time per vector,
+    "us_per_vector": {k: us}, "single_us": us}``. A width of 1 means batching vectors saves
+    nothing on this mesh. Runs on 256-node cluster blocks; restores the context's block size."""
+    nb = getattr(ctx, "cluster_block_nodes", 256)
+    shape = dict(ctx.apply_shape)
+    ctx.bench_apply(reps=3)
+    single = ctx.bench_apply(reps=reps) * 1e3
+    if ctx.segments is None:
+        ctx.set_cluster_block_nodes(256)
+    ctx.set_apply_shape("cluster", 256)
+    per = {}
+    for k in widths:
+        ctx.bench_apply_multi(k, reps=2)
+        per[k] = ctx.bench_apply_multi(k, reps=reps) * 1e3 / k
+    if ctx.segments is None:
+        ctx.set_cluster_block_nodes(nb)
+    ctx.set_apply_shape(shape["unit"], shape["param"])
+    best = min(per, key=per.get)
+    width = best if per[best] < single else 1
+    return {"width": width, "us_per_vector": {k: round(v, 3) for k, v in per.items()}, "single_us": round(single, 3)}

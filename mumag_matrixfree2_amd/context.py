@@ -173,6 +173,19 @@ class MeshContext:
         call("mm_poisson_apply", self.handle, ptr(x), ptr(y))
         return y
 
+    def apply_A_multi(self, X):
+        """Y = A X for k in {1, 2, 4, 8} vectors X (k, N) in one multi-vector apply."""
+        X = f64(X)
+        Y = np.empty_like(X)
+        call("mm_poisson_apply_multi", self.handle, X.shape[0], ptr(X), ptr(Y))
+        return Y
+
+    def bench_apply_multi(self, k, reps=40):
+        """Mean ms of a k-vector apply (device-resident vectors)."""
+        ms = C.c_double(0.0)
+        call("mm_bench_apply_multi", self.handle, int(k), int(reps), C.byref(ms))
+        return float(ms.value)
+
     def rhs(self, m):
         m = f64(m).reshape(-1)
         b = np.empty(self.N)

@@ -976,6 +976,32 @@ int mm_poisson_apply(int64_t h, const double* x, double* y) {
   return 0;
 }
 
+int mm_poisson_apply_multi(int64_t h, int k, const double* x, double* y) {
+  GET_CTX(h);
+  MM_CHECK(x && y, "null array");
+  MM_CHECK(k == 1 || k == 2 || k == 4 || k == 8, "k must be 1, 2, 4 or 8");
+  MM_CHECK(c.geom_mode == kGeomCoords && c.apply_clusters && c.cl_cap == kMaxBlockNodes,
+           "the multi-vector apply needs the on-the-fly cluster apply on 256-node blocks");
+  if (!c.mx) {
+    MM_TRY(dalloc(c, &c.mx, 8 * (size_t)c.N));
+    MM_TRY(dalloc(c, &c.my, 8 * (size_t)c.N));
+  }
+  const long N = c.N;
+  std::vector<double> t((size_t)k * N);
+  for (int r = 0; r < k; ++r)
+    for (long i = 0; i < N; ++i) t[(size_t)k * i + r] = x[(size_t)r * N + i];
+  MM_HIP(hipMemcpyAsync(c.mx, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  vfill(c.my, 0.0, (long)k * N, c.stream);
+  MM_TRY(launch_apply_multi(c, k, c.mx, c.my, nullptr, c.stream));
+  MM_HIP(hipGetLastError());
+  MM_HIP(hipMemcpyAsync(t.data(), c.my, t.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  for (int r = 0; r < k; ++r)
+    for (long i = 0; i < N; ++i) y[(size_t)r * N + i] = t[(size_t)k * i + r];
+  c.n_apply += k;
+  return 0;
+}
+
 int mm_poisson_rhs(int64_t h, const double* m, double* b) {
   GET_CTX(h);
   MM_CHECK(m && b, "null array");
@@ -1642,6 +1668,37 @@ int mm_counters(int64_t h, int64_t* out6) {
   out6[3] = c.n_local;
   out6[4] = c.n_vcycle;
   out6[5] = c.n_pcg_it;
+  return 0;
+}
+
+int mm_bench_apply_multi(int64_t h, int k, int reps, double* ms) {
+  GET_CTX(h);
+  MM_CHECK(reps > 0 && ms, "bad arguments");
+  MM_CHECK(k == 1 || k == 2 || k == 4 || k == 8, "k must be 1, 2, 4 or 8");
+  MM_CHECK(c.apply_clusters && c.cl_cap == kMaxBlockNodes, "the multi-vector apply needs 256-node cluster blocks");
+  if (!c.mx) {
+    MM_TRY(dalloc(c, &c.mx, 8 * (size_t)c.N));
+    MM_TRY(dalloc(c, &c.my, 8 * (size_t)c.N));
+  }
+  if (c.mdot_n < c.part_cl.nblk) {  // k partials per block
+    MM_TRY(dalloc(c, &c.mdot, 8 * (size_t)c.part_cl.nblk));
+    c.mdot_n = c.part_cl.nblk;
+  }
+  hipEvent_t e0, e1;
+  MM_HIP(hipEventCreate(&e0));
+  MM_HIP(hipEventCreate(&e1));
+  vfill(c.mx, 1.0, (long)k * c.N, c.stream);
+  vfill(c.my, 0.0, (long)k * c.N, c.stream);
+  MM_TRY(launch_apply_multi(c, k, c.mx, c.my, c.mdot, c.stream));  // warm-up
+  MM_HIP(hipEventRecord(e0, c.stream));
+  for (int r = 0; r < reps; ++r) MM_TRY(launch_apply_multi(c, k, c.mx, c.my, c.mdot, c.stream));
+  MM_HIP(hipEventRecord(e1, c.stream));
+  MM_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  MM_HIP(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return 0;
 }
 

@@ -221,6 +221,9 @@ struct Ctx {
   int mem = 10;
   double *S = nullptr, *Yh = nullptr;  // mem x 3N (FP64 history)
   float *Sf = nullptr, *Yf = nullptr;  // mem x 3N (FP32 history of the mixed-precision forms)
+  double *mx = nullptr, *my = nullptr;  // 8N each: node-major vectors of the multi-vector apply
+  double* mdot = nullptr;               // 8 x (cluster blocks at 256 nodes): its x.Ax partials
+  int mdot_n = 0;
   // per-workgroup partials of the field-step reductions (9 banks of 2 x kRedGridMax) and a pinned
   // host copy of the first banks
   double *mpart = nullptr, *hmpart = nullptr;
@@ -304,6 +307,9 @@ int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* 
 int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float* xo, float* y, hipStream_t s);
 int launch_apply_f32_scaled_f(const Ctx& c, const float* r, const float* w, float* xo, float* y, hipStream_t s);
 int launch_apply_f32_dot(const Ctx& c, const float* x, float* y, double* dot_part, hipStream_t s);
+// y_r = A x_r for K in {1, 2, 4, 8} node-major vectors (x[K id + r]); y zeroed first; dot_part
+// (nullable) receives K partials per cluster block
+int launch_apply_multi(const Ctx& c, int K, const double* x, double* y, double* dot_part, hipStream_t s);
 int launch_rhs(const Ctx& c, const double* m, double* b, hipStream_t s);
 // launch_energy with the block partials (2 doubles per magnetic block) in a caller-given buffer
 int launch_energy_buf(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,

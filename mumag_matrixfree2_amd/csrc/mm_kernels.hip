@@ -415,6 +415,149 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
     if (bi.b >= 0) block_sum_to<NT>(acc_e, dot_part + bi.b);
 }
 
+// ------------------------------- multi-RHS Poisson apply over edge-star clusters (K vectors)
+// y_r = A x_r for K right-hand sides stored node-major (x[K id + r]): one staging of the block's
+// node list and coordinates, and one set of edge vectors and cross products per cluster, serve
+// all K value vectors; the tet normals are re-formed from them for each vector (registers, no
+// memory traffic). Per vector: 8 LDS value reads and 8 LDS atomics per cluster, and K values per
+// node in the write-back. 256-node blocks (LDS: (2K + 3) x 2 KiB). With DOT, dot_part[K b + r]
+// receives block b's share of x_r . A x_r.
+template <int K, bool DOT, int NT>
+__global__ __launch_bounds__(NT) void k_apply_cl_multi(Part p, Geo geo, const double* __restrict__ x,
+                                                       double* __restrict__ y, double reg,
+                                                       double* __restrict__ dot_part) {
+  constexpr int NB = kMaxBlockNodes;
+  __shared__ double sx[K][NB];
+  __shared__ double sy[K][NB];
+  __shared__ double sX[3][NB];
+  const BlkInfo bi = blk_info(p);
+  if (!bi.valid) return;
+  const int tid = threadIdx.x;
+  constexpr int kNode = NB / NT;
+  uint32_t fv[kNode];
+#pragma unroll
+  for (int q = 0; q < kNode; ++q) {
+    const int i = tid + q * NT;
+    fv[q] = i < bi.nb ? p.nodes[bi.n0 + i] : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < kNode; ++q) {
+    const int i = tid + q * NT;
+    if (i < bi.nb) {
+      const uint32_t f = fv[q];
+      const uint32_t id = f & kIdMask;
+#pragma unroll
+      for (int r = 0; r < K; ++r) {
+        sx[r][i] = (f & kBoundary) ? 0.0 : x[(long)K * id + r];
+        sy[r][i] = 0.0;
+      }
+      sX[0][i] = geo.knt[3 * id + 0];
+      sX[1][i] = geo.knt[3 * id + 1];
+      sX[2][i] = geo.knt[3 * id + 2];
+    }
+  }
+  __syncthreads();
+  double acc_e[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) acc_e[r] = 0.0;
+  for (int cc = bi.e0 + tid; cc < bi.e1; cc += NT) {
+    const uint32_t w0 = p.lconn[cc], w1 = p.lconn2[cc], wm = p.cmask[cc];
+    int li[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      li[q] = (w0 >> (8 * q)) & 255u;
+      li[4 + q] = (w1 >> (8 * q)) & 255u;
+    }
+    double c[8][3], mm[8][3];
+    {
+      double X0[3] = {sX[0][li[0]], sX[1][li[0]], sX[2][li[0]]};
+#pragma unroll
+      for (int q = 1; q < 8; ++q) {
+        const bool used = wm & (1u << (8 + q));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c[q][k] = used ? sX[k][li[q]] - X0[k] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 2; q < 8; ++q) {
+      mm[q][0] = c[1][1] * c[q][2] - c[1][2] * c[q][1];
+      mm[q][1] = c[1][2] * c[q][0] - c[1][0] * c[q][2];
+      mm[q][2] = c[1][0] * c[q][1] - c[1][1] * c[q][0];
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      double v[8], acc[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        v[q] = (wm & (1u << (8 + q))) ? sx[r][li[q]] : 0.0;
+        acc[q] = 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        if (wm & (1u << j)) {
+          const int s2 = 2 + j, s3 = 2 + (j + 1) % 6;
+          double n[3][3];
+          n[0][0] = c[s2][1] * c[s3][2] - c[s2][2] * c[s3][1];
+          n[0][1] = c[s2][2] * c[s3][0] - c[s2][0] * c[s3][2];
+          n[0][2] = c[s2][0] * c[s3][1] - c[s2][1] * c[s3][0];
+          n[1][0] = -mm[s3][0];
+          n[1][1] = -mm[s3][1];
+          n[1][2] = -mm[s3][2];
+          n[2][0] = mm[s2][0];
+          n[2][1] = mm[s2][1];
+          n[2][2] = mm[s2][2];
+          const double det = c[1][0] * n[0][0] + c[1][1] * n[0][1] + c[1][2] * n[0][2];
+          double ya[4];
+          const double en = tet_laplace(n, det, v[1] - v[0], v[s2] - v[0], v[s3] - v[0], ya);
+          if constexpr (DOT) acc_e[r] += en;
+          acc[0] += ya[0];
+          acc[1] += ya[1];
+          acc[s2] += ya[2];
+          acc[s3] += ya[3];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (wm & (1u << (8 + q))) atomicAdd(&sy[r][li[q]], acc[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kNode; ++q) {
+    const int i = tid + q * NT;
+    if (i >= bi.nb) continue;
+    const uint32_t f = fv[q];
+    const uint32_t id = f & kIdMask;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      if (f & kBoundary) {
+        if (f & kOwner) {
+          const double xb = x[(long)K * id + r];
+          y[(long)K * id + r] = xb;
+          if constexpr (DOT) acc_e[r] += xb * xb;
+        }
+        continue;
+      }
+      double val = sy[r][i];
+      if (f & kOwner) {
+        val += reg * sx[r][i];
+        if constexpr (DOT) acc_e[r] += reg * sx[r][i] * sx[r][i];
+      }
+      if (f & kPrivate)
+        y[(long)K * id + r] = val;
+      else
+        unsafeAtomicAdd(&y[(long)K * id + r], val);
+    }
+  }
+  if constexpr (DOT) {
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      __syncthreads();
+      if (bi.b >= 0) block_sum_to<NT>(acc_e[r], dot_part + (long)K * bi.b + r);
+    }
+  }
+}
+
 // Persistent form of k_apply_cl (FP64): a grid of a few workgroups per CU walks the blocks of
 // its XCD's range, and while it computes the clusters of one block it already has the node
 // flags, values and coordinates of its next block in flight (register double buffering), so
@@ -1095,6 +1238,31 @@ int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s) {
     constexpr int NT = decltype(nt)::value;
     k_apply_f32<false, NT><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, (float)c.reg);
   });
+  return 0;
+}
+
+int launch_apply_multi(const Ctx& c, int K, const double* x, double* y, double* dot_part, hipStream_t s) {
+  MM_CHECK(c.apply_clusters && c.cl_cap == kMaxBlockNodes,
+           "the multi-vector apply runs the cluster apply on 256-node blocks");
+  MM_CHECK(K == 1 || K == 2 || K == 4 || K == 8, "vectors per multi-vector apply: 1, 2, 4 or 8");
+  const Part& p = c.part_cl;
+  if (part_len(p) == 0) return 0;
+  const int grid = grid_for(p);
+  auto go = [&](auto kc) {
+    constexpr int KK = decltype(kc)::value;
+    if (dot_part)
+      k_apply_cl_multi<KK, true, 256><<<grid, 256, 0, s>>>(p, c.geo, x, y, c.reg, dot_part);
+    else
+      k_apply_cl_multi<KK, false, 256><<<grid, 256, 0, s>>>(p, c.geo, x, y, c.reg, nullptr);
+  };
+  if (K == 1)
+    go(std::integral_constant<int, 1>{});
+  else if (K == 2)
+    go(std::integral_constant<int, 2>{});
+  else if (K == 4)
+    go(std::integral_constant<int, 4>{});
+  else
+    go(std::integral_constant<int, 8>{});
   return 0;
 }
 

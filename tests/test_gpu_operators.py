@@ -144,3 +144,24 @@ def test_j_parallel(ctx_c0, oracle_c0):
     m = tilted_m0(ctx_c0.N, seed=7, sigma=0.5)
     h = np.array([0.017452, 0.0, 0.999848])
     assert abs(ctx_c0.j_parallel(m, h) - oracle_c0.J_parallel(m, h)) <= 1e-13
+
+
+@pytest.mark.parametrize("k", [1, 2, 4, 8])
+def test_multi_vector_apply(ctx_c0, oracle_c0, k):
+    """k vectors at once: each column equals the oracle's A x (and the single-vector apply)."""
+    X = np.random.default_rng(70 + k).standard_normal((k, ctx_c0.N))
+    Y = ctx_c0.apply_A_multi(X)
+    for r in range(k):
+        assert rel(Y[r], oracle_c0.A @ X[r]) <= TOL
+        assert rel(Y[r], ctx_c0.apply_A(X[r])) <= 1e-14
+
+
+def test_rhs_width_tuner_reports_per_vector_cost(ctx_c0):
+    from mumag_matrixfree2_amd.autotune import RHS_WIDTHS, tune_rhs_width
+
+    res = tune_rhs_width(ctx_c0, reps=3)
+    assert set(res["us_per_vector"]) == set(RHS_WIDTHS) and res["single_us"] > 0
+    assert res["width"] in RHS_WIDTHS
+    if res["width"] > 1:
+        assert res["us_per_vector"][res["width"]] < res["single_us"]
+    assert ctx_c0.cluster_block_nodes == 256 and ctx_c0.apply_shape == {"unit": "cluster", "param": 256}
