@@ -427,7 +427,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="time budget of the CPU baseline (field steps from the start of the sweep)")
-    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed", "fp32pcg"],
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed", "fp32pcg", "fp32all"],
                     help="headline form: 'mixed' (FP32 V-cycle under the FP64 PCG) or 'fp32pcg' (FP32 PCG with "
                          "FP64 refinement); the FP64 form runs the same steps for the comparison")
     ap.add_argument("--no-fp64-compare", action="store_true", help="mixed headline without the FP64 sweep")
@@ -590,12 +590,13 @@ def main():
         ms = ctx.bench_copy(args.calibrate_copies)
         log(f"[bench] {args.calibrate_copies} copies of {ctx.N} doubles ({16 * ctx.N} bytes each): {ms * 1e3:.2f} us")
     avg_apply_s = apply_ms / max(apply_n, 1) * 1e-3
-    algo = apply_algorithmic_bytes(ctx, fp32=args.precision == "fp32pcg")
+    algo = apply_algorithmic_bytes(ctx, fp32=args.precision in ("fp32pcg", "fp32all"))
     achieved = algo / avg_apply_s / 1e9
     traffic = measured_traffic(args.workload)
     label = {"fp64": "FP64", "mixed": "mixed: FP32 V-cycle, FP64 PCG, FP64 energies",
-             "fp32pcg": "FP32 PCG and V-cycle with FP64 dots and FP64 true-residual refinement, FP64 energies"
-             }[args.precision]
+             "fp32pcg": "FP32 PCG and V-cycle with FP64 dots and FP64 true-residual refinement, FP64 energies",
+             "fp32all": "FP32 PCG and V-cycle with FP64 dots and FP64 true-residual refinement, FP32 energy "
+                        "densities with compensated FP64 sums"}[args.precision]
     result = {
         "metric": f"hysteresis field-steps/s ({label}, {wl['name']})",
         "value": steps / elapsed,
@@ -629,9 +630,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None if traffic is None or args.precision == "fp32pcg" else traffic["bytes"],
+            "traffic": None if traffic is None or args.precision in ("fp32pcg", "fp32all") else traffic["bytes"],
             "kernel": ("k_apply_cl<float, false, true> (FP32 matrix-free Poisson apply of the PCG, edge-star "
-                       "clusters)" if args.precision == "fp32pcg" else
+                       "clusters)" if args.precision in ("fp32pcg", "fp32all") else
                        "k_apply_cl<double, false, true> (FP64 matrix-free Poisson apply of the PCG, edge-star "
                        "clusters)" if shape["unit"] != "element" else "k_apply (FP64 matrix-free Poisson apply)"),
             "avg_launch_us": avg_apply_s * 1e6,

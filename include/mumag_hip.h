@@ -98,8 +98,11 @@ int mm_ctx_get_exchange_diag(int64_t ctx, double* d);
  * products and alpha/beta stay FP64; 2 = FP32 PCG with FP64 iterative refinement: the PCG runs on
  * FP32 vectors with the FP32 apply and V-cycle (dots, alpha and beta in FP64) and an outer loop
  * recomputes the true residual b - A U with the FP64 apply and restarts the inner PCG on it until
- * r.r <= tol^2 b.b (needs the AMG V-cycle and the on-the-fly cluster apply). Every form meets the
- * FP64 tolerance. */
+ * r.r <= tol^2 b.b (needs the AMG V-cycle and the on-the-fly cluster apply); 3 = form 2 with the
+ * energy densities and gradient terms of mm_energy_grad and of the minimiser in FP32 element
+ * arithmetic (FP64 accumulation, FP64 Zeeman term, compensated FP64 energy sum). Every form meets
+ * the FP64 solver tolerance; forms 1-3 run the minimiser's local Hessian action (the Steihaug loop)
+ * in FP32 element arithmetic and keep the L-BFGS history in FP32. */
 int mm_set_precision(int64_t ctx, int precision);
 
 /* Select the geometry source of the Poisson apply (see flags of mm_ctx_create). */

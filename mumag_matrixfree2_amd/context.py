@@ -18,7 +18,7 @@ from ._hip import call, f64, i32, ptr
 GEOM_MODES = {"on_the_fly": 0, "stored_f64": 1, "stored_f32": 2, "on_the_fly_f32": 3}
 METHODS = {"pcohen_hs": 0, "lbfgs": 1}
 APPLY_UNITS = {"element": 0, "cluster": 1, "persistent": 2}
-PRECISIONS = {"fp64": 0, "mixed": 1, "fp32pcg": 2}
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp32pcg": 2, "fp32all": 3}
 
 
 class MeshContext:
@@ -137,7 +137,9 @@ class MeshContext:
     def set_precision(self, precision: str):
         """'fp64'; 'mixed' (FP32 V-cycle under an FP64 PCG); 'fp32pcg' (the PCG itself on FP32
         vectors with the FP32 apply and V-cycle, FP64 dots and alpha/beta, and an FP64
-        true-residual refinement loop to the same tolerance)."""
+        true-residual refinement loop to the same tolerance); 'fp32all' ('fp32pcg' with FP32
+        energy densities). The mixed forms also run the Steihaug Hessian action in FP32 and keep
+        the L-BFGS history in FP32."""
         call("mm_set_precision", self.handle, PRECISIONS[precision])
         self.precision = precision
 

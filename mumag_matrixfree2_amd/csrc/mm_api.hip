@@ -912,9 +912,10 @@ int mm_ctx_get_exchange_diag(int64_t h, double* d) {
 
 int mm_set_precision(int64_t h, int precision) {
   GET_CTX(h);
-  MM_CHECK(precision >= 0 && precision <= 2,
-           "precision must be 0 (FP64), 1 (FP32 preconditioner) or 2 (FP32 PCG with FP64 refinement)");
-  if (precision == 2 && c.rxf == nullptr)
+  MM_CHECK(precision >= 0 && precision <= 3,
+           "precision must be 0 (FP64), 1 (FP32 preconditioner), 2 (FP32 PCG with FP64 refinement) or 3 "
+           "(2 with FP32 energy densities)");
+  if (precision >= 2 && c.rxf == nullptr)
     for (float** v : {&c.rxf, &c.rrf, &c.rpf, &c.rqf, &c.rzf}) MM_TRY(dalloc(c, v, (size_t)c.N));
   c.precision = precision;
   return 0;

@@ -32,24 +32,25 @@ __device__ __forceinline__ BlkInfo blk_info(const Part& p) {
 // Shape-function gradients and volume of one tet from LDS-staged corner coordinates.
 // grad phi_b (b = 1..3) are the rows of J^-1, J = [x1-x0, x2-x0, x3-x0]:
 // (c2 x c3)/det, (c3 x c1)/det, (c1 x c2)/det; grad phi_0 = -(their sum); Ve = |det|/6.
-__device__ __forceinline__ void tet_grads(const double (*X)[kMaxBlockNodes], int l0, int l1, int l2, int l3,
-                                          double g[4][3], double& vol) {
-  const double ox = X[0][l0], oy = X[1][l0], oz = X[2][l0];
-  const double c1x = X[0][l1] - ox, c1y = X[1][l1] - oy, c1z = X[2][l1] - oz;
-  const double c2x = X[0][l2] - ox, c2y = X[1][l2] - oy, c2z = X[2][l2] - oz;
-  const double c3x = X[0][l3] - ox, c3y = X[1][l3] - oy, c3z = X[2][l3] - oz;
-  const double n1x = c2y * c3z - c2z * c3y, n1y = c2z * c3x - c2x * c3z, n1z = c2x * c3y - c2y * c3x;
-  const double n2x = c3y * c1z - c3z * c1y, n2y = c3z * c1x - c3x * c1z, n2z = c3x * c1y - c3y * c1x;
-  const double n3x = c1y * c2z - c1z * c2y, n3y = c1z * c2x - c1x * c2z, n3z = c1x * c2y - c1y * c2x;
-  const double det = c1x * n1x + c1y * n1y + c1z * n1z;
-  const double inv = 1.0 / det;
+template <typename T>
+__device__ __forceinline__ void tet_grads(const T (*X)[kMaxBlockNodes], int l0, int l1, int l2, int l3,
+                                          T g[4][3], T& vol) {
+  const T ox = X[0][l0], oy = X[1][l0], oz = X[2][l0];
+  const T c1x = X[0][l1] - ox, c1y = X[1][l1] - oy, c1z = X[2][l1] - oz;
+  const T c2x = X[0][l2] - ox, c2y = X[1][l2] - oy, c2z = X[2][l2] - oz;
+  const T c3x = X[0][l3] - ox, c3y = X[1][l3] - oy, c3z = X[2][l3] - oz;
+  const T n1x = c2y * c3z - c2z * c3y, n1y = c2z * c3x - c2x * c3z, n1z = c2x * c3y - c2y * c3x;
+  const T n2x = c3y * c1z - c3z * c1y, n2y = c3z * c1x - c3x * c1z, n2z = c3x * c1y - c3y * c1x;
+  const T n3x = c1y * c2z - c1z * c2y, n3y = c1z * c2x - c1x * c2z, n3z = c1x * c2y - c1y * c2x;
+  const T det = c1x * n1x + c1y * n1y + c1z * n1z;
+  const T inv = (T)1 / det;
   g[1][0] = n1x * inv; g[1][1] = n1y * inv; g[1][2] = n1z * inv;
   g[2][0] = n2x * inv; g[2][1] = n2y * inv; g[2][2] = n2z * inv;
   g[3][0] = n3x * inv; g[3][1] = n3y * inv; g[3][2] = n3z * inv;
   g[0][0] = -(g[1][0] + g[2][0] + g[3][0]);
   g[0][1] = -(g[1][1] + g[2][1] + g[3][1]);
   g[0][2] = -(g[1][2] + g[2][2] + g[3][2]);
-  vol = fabs(det) * (1.0 / 6.0);
+  vol = fabs(det) * ((T)1 / (T)6);
 }
 
 // 1/a from the hardware estimate (v_rcp_f64) refined by two Newton-Raphson steps

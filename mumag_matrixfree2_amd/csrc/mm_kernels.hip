@@ -750,12 +750,16 @@ __global__ __launch_bounds__(kBlock) void k_rhs(Part p, Geo geo, const double* _
 // Owners add the Zeeman term -2 M_i B. Energy E = 1/2 sum m.(g_quad) + sum m.g_z with
 // compensated summation from the element terms up to the total (per-block (sum, error)
 // partials, reduced by k_sum_partials_cs). Output g and E are divided by V_mag.
+// T = float (precision form 3): the element energy densities and gradient terms in FP32
+// on block-relative FP32 coordinates and FP32 copies of m and U; LDS accumulation, the Zeeman
+// term (from the FP64 m), the compensated energy sum and the output stay FP64.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double* __restrict__ m,
                                                    const double* __restrict__ U, double Bx, double By, double Bz,
                                                    double inv_V, double* __restrict__ g, double* __restrict__ partial) {
-  __shared__ double sm[3][kMaxBlockNodes];
-  __shared__ double sX[3][kMaxBlockNodes];
-  __shared__ double su[kMaxBlockNodes];
+  __shared__ T sm[3][kMaxBlockNodes];
+  __shared__ T sX[3][kMaxBlockNodes];
+  __shared__ T su[kMaxBlockNodes];
   __shared__ double sg[3][kMaxBlockNodes];
   const BlkInfo bi = blk_info(p);
   if (!bi.valid) return;
@@ -767,42 +771,52 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
   const double iv = (geo.inv_V_node && bi.nb > 0) ? geo.inv_V_node[p.nodes[bi.n0] & kIdMask] : inv_V;
   CSum acc;
   double Mi = 0.0;  // M_nodal of the thread's node (Zeeman term of its owner)
+  double mt[3] = {0.0, 0.0, 0.0};  // the thread's node's m in FP64
+  double o[3] = {0.0, 0.0, 0.0};   // FP32: coordinates relative to the block's first node
+  if constexpr (sizeof(T) == 4)
+    if (bi.nb > 0) {
+      const uint32_t id0 = p.nodes[bi.n0] & kIdMask;
+      o[0] = geo.knt[3 * id0];
+      o[1] = geo.knt[3 * id0 + 1];
+      o[2] = geo.knt[3 * id0 + 2];
+    }
   if (has_node) {
     const uint32_t id = fl & kIdMask;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      sm[k][tid] = m[3 * id + k];
-      sX[k][tid] = geo.knt[3 * id + k];
+      mt[k] = m[3 * id + k];
+      sm[k][tid] = (T)mt[k];
+      sX[k][tid] = (T)(geo.knt[3 * id + k] - o[k]);
       sg[k][tid] = 0.0;
     }
-    su[tid] = U[id];
+    su[tid] = (T)U[id];
     Mi = geo.M_nodal[id];
   }
   __syncthreads();
   for_elems(p, bi, tid, ew, [&](uint32_t lc, int mi) {
-    const double A = geo.mt->A[mi], K1 = geo.mt->K1[mi], Js = geo.mt->Js[mi];
-    const double kx = geo.mt->k[mi][0], ky = geo.mt->k[mi][1], kz = geo.mt->k[mi][2];
+    const T A = (T)geo.mt->A[mi], K1 = (T)geo.mt->K1[mi], Js = (T)geo.mt->Js[mi];
+    const T kx = (T)geo.mt->k[mi][0], ky = (T)geo.mt->k[mi][1], kz = (T)geo.mt->k[mi][2];
     int l[4];
     unpack(lc, l[0], l[1], l[2], l[3]);
-    double g4[4][3], vol;
-    tet_grads(sX, l[0], l[1], l[2], l[3], g4, vol);
-    const double a_ve = 2.0 * A * vol, q_ve = 2.0 * K1 * vol / 20.0, j_ve = 2.0 * Js * vol / 4.0;
+    T g4[4][3], vol;
+    tet_grads<T>(sX, l[0], l[1], l[2], l[3], g4, vol);
+    const T a_ve = (T)2 * A * vol, q_ve = (T)2 * K1 * vol / (T)20, j_ve = (T)2 * Js * vol / (T)4;
     // grad m_c (c = x,y,z) and grad U, relative to corner 0
-    double gm[3][3], gu[3];
+    T gm[3][3], gu[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double m0 = sm[c][l[0]];
-      const double d1 = sm[c][l[1]] - m0, d2 = sm[c][l[2]] - m0, d3 = sm[c][l[3]] - m0;
+      const T m0 = sm[c][l[0]];
+      const T d1 = sm[c][l[1]] - m0, d2 = sm[c][l[2]] - m0, d3 = sm[c][l[3]] - m0;
 #pragma unroll
       for (int i = 0; i < 3; ++i) gm[c][i] = d1 * g4[1][i] + d2 * g4[2][i] + d3 * g4[3][i];
     }
     {
-      const double u0 = su[l[0]];
-      const double d1 = su[l[1]] - u0, d2 = su[l[2]] - u0, d3 = su[l[3]] - u0;
+      const T u0 = su[l[0]];
+      const T d1 = su[l[1]] - u0, d2 = su[l[2]] - u0, d3 = su[l[3]] - u0;
 #pragma unroll
       for (int i = 0; i < 3; ++i) gu[i] = (d1 * g4[1][i] + d2 * g4[2][i] + d3 * g4[3][i]) * j_ve;
     }
-    double v[4], vs = 0.0;
+    T v[4], vs = 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       v[a] = sm[0][l[a]] * kx + sm[1][l[a]] * ky + sm[2][l[a]] * kz;
@@ -811,17 +825,17 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
     double e_el = 0.0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      const double fa = -q_ve * (vs + v[a]);
-      double c3[3];
+      const T fa = -q_ve * (vs + v[a]);
+      T c3[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
         c3[c] = a_ve * (g4[a][0] * gm[c][0] + g4[a][1] * gm[c][1] + g4[a][2] * gm[c][2]) + gu[c];
       c3[0] += fa * kx;
       c3[1] += fa * ky;
       c3[2] += fa * kz;
-      e_el += sm[0][l[a]] * c3[0] + sm[1][l[a]] * c3[1] + sm[2][l[a]] * c3[2];
+      e_el += (double)(sm[0][l[a]] * c3[0] + sm[1][l[a]] * c3[1] + sm[2][l[a]] * c3[2]);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) atomicAdd(&sg[c][l[a]], c3[c]);
+      for (int c = 0; c < 3; ++c) atomicAdd(&sg[c][l[a]], (double)c3[c]);
     }
     acc.add(0.5 * e_el);
   });
@@ -836,7 +850,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(Part p, Geo geo, const double
       v0 += z0;
       v1 += z1;
       v2 += z2;
-      acc.add(sm[0][i] * z0 + sm[1][i] * z1 + sm[2][i] * z2);
+      acc.add(mt[0] * z0 + mt[1] * z1 + mt[2] * z2);
     }
     if (f & kPrivate) {
       g[3 * id + 0] = v0 * iv;
@@ -957,12 +971,15 @@ __global__ __launch_bounds__(kBlock) void k_seg_block_sums(const int* __restrict
 
 // --------------------------------------------------- local Hessian action C v (K8)
 // C v = (K_ex + K_an) v / V_mag: exchange and anisotropy parts of K7 without demag/Zeeman.
+// T = float: element arithmetic in FP32 on block-relative FP32 coordinates and FP32 copies of v
+// (the mixed-precision forms' Steihaug loop), accumulation and output in FP64.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const double* __restrict__ vin,
                                                   double inv_V, double* __restrict__ out,
                                                   const double* __restrict__ gate) {
   if (gate != nullptr && *gate == 0.0) return;
-  __shared__ double sm[3][kMaxBlockNodes];
-  __shared__ double sX[3][kMaxBlockNodes];
+  __shared__ T sm[3][kMaxBlockNodes];
+  __shared__ T sX[3][kMaxBlockNodes];
   __shared__ double sg[3][kMaxBlockNodes];
   const BlkInfo bi = blk_info(p);
   if (!bi.valid) return;
@@ -971,33 +988,41 @@ __global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const double*
   const uint32_t fl = has_node ? p.nodes[bi.n0 + tid] : 0u;  // first: the staging gathers wait for it only
   const ElemWords ew = elem_prefetch(p, bi, tid);
   const double iv = (geo.inv_V_node && bi.nb > 0) ? geo.inv_V_node[p.nodes[bi.n0] & kIdMask] : inv_V;
+  double o[3] = {0.0, 0.0, 0.0};  // FP32: coordinates relative to the block's first node
+  if constexpr (sizeof(T) == 4)
+    if (bi.nb > 0) {
+      const uint32_t id0 = p.nodes[bi.n0] & kIdMask;
+      o[0] = geo.knt[3 * id0];
+      o[1] = geo.knt[3 * id0 + 1];
+      o[2] = geo.knt[3 * id0 + 2];
+    }
   if (has_node) {
     const uint32_t id = fl & kIdMask;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      sm[k][tid] = vin[3 * id + k];
-      sX[k][tid] = geo.knt[3 * id + k];
+      sm[k][tid] = (T)vin[3 * id + k];
+      sX[k][tid] = (T)(geo.knt[3 * id + k] - o[k]);
       sg[k][tid] = 0.0;
     }
   }
   __syncthreads();
   for_elems(p, bi, tid, ew, [&](uint32_t lc, int mi) {
-    const double A = geo.mt->A[mi], K1 = geo.mt->K1[mi];
-    const double kx = geo.mt->k[mi][0], ky = geo.mt->k[mi][1], kz = geo.mt->k[mi][2];
+    const T A = (T)geo.mt->A[mi], K1 = (T)geo.mt->K1[mi];
+    const T kx = (T)geo.mt->k[mi][0], ky = (T)geo.mt->k[mi][1], kz = (T)geo.mt->k[mi][2];
     int l[4];
     unpack(lc, l[0], l[1], l[2], l[3]);
-    double g4[4][3], vol;
-    tet_grads(sX, l[0], l[1], l[2], l[3], g4, vol);
-    const double a_ve = 2.0 * A * vol, q_ve = 2.0 * K1 * vol / 20.0;
-    double gm[3][3];
+    T g4[4][3], vol;
+    tet_grads<T>(sX, l[0], l[1], l[2], l[3], g4, vol);
+    const T a_ve = (T)2 * A * vol, q_ve = (T)2 * K1 * vol / (T)20;
+    T gm[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double m0 = sm[c][l[0]];
-      const double d1 = sm[c][l[1]] - m0, d2 = sm[c][l[2]] - m0, d3 = sm[c][l[3]] - m0;
+      const T m0 = sm[c][l[0]];
+      const T d1 = sm[c][l[1]] - m0, d2 = sm[c][l[2]] - m0, d3 = sm[c][l[3]] - m0;
 #pragma unroll
       for (int i = 0; i < 3; ++i) gm[c][i] = (d1 * g4[1][i] + d2 * g4[2][i] + d3 * g4[3][i]) * a_ve;
     }
-    double v[4], vs = 0.0;
+    T v[4], vs = 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       v[a] = sm[0][l[a]] * kx + sm[1][l[a]] * ky + sm[2][l[a]] * kz;
@@ -1005,10 +1030,10 @@ __global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const double*
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      const double fa = -q_ve * (vs + v[a]);
-      atomicAdd(&sg[0][l[a]], g4[a][0] * gm[0][0] + g4[a][1] * gm[0][1] + g4[a][2] * gm[0][2] + fa * kx);
-      atomicAdd(&sg[1][l[a]], g4[a][0] * gm[1][0] + g4[a][1] * gm[1][1] + g4[a][2] * gm[1][2] + fa * ky);
-      atomicAdd(&sg[2][l[a]], g4[a][0] * gm[2][0] + g4[a][1] * gm[2][1] + g4[a][2] * gm[2][2] + fa * kz);
+      const T fa = -q_ve * (vs + v[a]);
+      atomicAdd(&sg[0][l[a]], (double)(g4[a][0] * gm[0][0] + g4[a][1] * gm[0][1] + g4[a][2] * gm[0][2] + fa * kx));
+      atomicAdd(&sg[1][l[a]], (double)(g4[a][0] * gm[1][0] + g4[a][1] * gm[1][1] + g4[a][2] * gm[1][2] + fa * ky));
+      atomicAdd(&sg[2][l[a]], (double)(g4[a][0] * gm[2][0] + g4[a][1] * gm[2][1] + g4[a][2] * gm[2][2] + fa * kz));
     }
   });
   __syncthreads();
@@ -1326,7 +1351,10 @@ int launch_energy_buf(const Ctx& c, const double* m, const double* U, const doub
   const Part& p = c.part_mag;
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
-  k_energy<<<grid, kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, partial);
+  if (c.precision >= 3)
+    k_energy<float><<<grid, kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, partial);
+  else
+    k_energy<double><<<grid, kBlock, 0, s>>>(p, c.geo, m, U, B[0], B[1], B[2], 1.0 / c.V_mag, g, partial);
   k_sum_partials_cs<<<1, kBlock, 0, s>>>(partial, p.nblk, E_out);  // 2 x nblk partials by block id
   return 0;
 }
@@ -1339,7 +1367,10 @@ int launch_energy(const Ctx& c, const double* m, const double* U, const double B
 int launch_local(const Ctx& c, const double* v, double* out, const double* gate, hipStream_t s) {
   const Part& p = c.part_mag;
   if (part_len(p) == 0) return 0;
-  k_local<<<grid_for(p), kBlock, 0, s>>>(p, c.geo, v, 1.0 / c.V_mag, out, gate);
+  if (c.precision >= 1)
+    k_local<float><<<grid_for(p), kBlock, 0, s>>>(p, c.geo, v, 1.0 / c.V_mag, out, gate);
+  else
+    k_local<double><<<grid_for(p), kBlock, 0, s>>>(p, c.geo, v, 1.0 / c.V_mag, out, gate);
   return 0;
 }
 

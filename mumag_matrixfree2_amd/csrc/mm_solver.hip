@@ -1185,7 +1185,7 @@ int pcg_dev(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it
   hipStream_t s = c.stream;
   const long N = c.N;
   MM_CHECK(c.lv.size() >= 1 && c.lv[0].M != nullptr, "preconditioner not set up (call mm_amg_upload or mm_precond_jacobi)");
-  if (c.precision == 2) {
+  if (c.precision >= 2) {
     MM_CHECK(c.lv.size() > 1 && c.lv[0].Mf != nullptr && c.rxf != nullptr && c.geom_mode == kGeomCoords &&
                  c.apply_clusters,
              "the FP32 PCG needs the AMG V-cycle, the on-the-fly cluster apply and mm_set_precision(ctx, 2)");

@@ -148,3 +148,22 @@ def test_fp32_pcg_field_step_energy(mixed_ctx, oracle_c0, method):
           f"{out['fp32pcg']['demag_iters']}", flush=True)
     assert out["fp32pcg"]["converged"]
     assert abs(out["fp32pcg"]["E"] - out["fp64"]["E"]) <= 1e-10 * scale
+
+
+def test_fp32_energy_densities_to_fp32_accuracy(mixed_ctx, oracle_c0):
+    """Form 'fp32all' computes the element energy densities and gradient terms in FP32 (FP64
+    accumulation and compensated sum): energy and gradient agree with the oracle to FP32
+    accuracy (1e-6 of the energy scale, 1e-5 relative gradient), not to the FP64 bar."""
+    ctx = mixed_ctx
+    m = tilted_m0(ctx.N, seed=36, sigma=0.3)
+    U = np.random.default_rng(36).standard_normal(ctx.N)
+    B = np.array([0.01, 0.0, -0.3])
+    ctx.set_precision("fp32all")
+    try:
+        E, g = ctx.energy_grad(m, U, B)
+    finally:
+        ctx.set_precision("fp64")
+    E_ref, g_ref = oracle_c0.energy_and_grad(m, U, B)
+    scale = sum(abs(v) for v in oracle_c0.energy_terms(m, U, B).values())
+    print(f"[fp32 energy] |dE|/scale={abs(E - E_ref) / scale:.2e} |dg|/|g|={rel(g, g_ref):.2e}", flush=True)
+    assert abs(E - E_ref) <= 1e-6 * scale and rel(g, g_ref) <= 1e-5
