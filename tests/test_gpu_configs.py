@@ -10,6 +10,8 @@
   against the FP64 GPU sweep and the oracle's sweep, same switching step, energies within 1e-8.
 - Scatter determinism: the FP64 atomic scatter of the apply and a full solve repeated in one
   process differ by no more than a stated bound.
+- The 24.6M-tet polycrystal at its real size through size-independent properties (symmetry,
+  linear exactness, the solver tolerance in all three precision forms).
 """
 
 from __future__ import annotations
@@ -145,3 +147,60 @@ def test_scatter_spread_is_bounded(cube_c0):
     print(f"[scatter] apply spread {spread:.3e} (|y|max {np.abs(ys).max():.3e}), solve spread {du:.3e}", flush=True)
     assert spread <= 4 * np.spacing(np.abs(ys).max())
     assert du <= 1e-13
+
+
+def test_full_size_polycrystal_size_independent_properties():
+    """The 24.6M-tet polycrystal at its real size, checked through properties that hold at any
+    size: A is symmetric (x.Ay = y.Ax), A (a.x + c) = reg (a.x + c) on nodes whose neighbours are all
+    interior (the stiffness annihilates linear functions), and the FP64, mixed and FP32-PCG solves
+    all meet the 1e-10 tolerance on the FP64 true residual and agree with each other to it."""
+    from mumag_matrixfree2_amd.autotune import tune_apply
+    from mumag_matrixfree2_amd.context import MeshContext
+    from mumag_matrixfree2_amd.mesh import polycrystal_cube, random_easy_axes
+
+    mesh = polycrystal_cube(200.0, 150, 5, n_grains=27, gb_width=2.0, seed=4)
+    Kd = JS**2 / (2 * MU0)
+    G = 29
+    A, K1, Js, k = np.zeros(G), np.zeros(G), np.zeros(G), np.tile([0.0, 0.0, 1.0], (G, 1))
+    A[:27], K1[:27], Js[:27] = 7.7e-12 * 1e18 / Kd, 4.3e6 / Kd, 1.0
+    k[:27] = random_easy_axes(27, 40)
+    A[27], Js[27] = 1.5e-12 * 1e18 / Kd, 1.0
+    ctx = MeshContext(mesh["knt"], mesh["ijk"][:, :4], mesh["ijk"][:, 4], A, K1, Js, k,
+                      boundary_mask=mesh["boundary_mask"])
+    print(f"[poly25M] N={ctx.N} E={ctx.E}", flush=True)
+    assert ctx.E == 24_576_000
+    tune_apply(ctx, reps=5)
+    rng = np.random.default_rng(80)
+    x, y = rng.standard_normal(ctx.N), rng.standard_normal(ctx.N)
+    Ax, Ay = ctx.apply_A(x), ctx.apply_A(y)
+    assert abs(x @ Ay - y @ Ax) <= 1e-12 * np.linalg.norm(x) * np.linalg.norm(Ay)
+    # linear functions: rows whose whole node star is interior
+    knt = mesh["knt"]
+    lin = knt @ np.array([0.3, -0.2, 0.5]) + 1.7
+    conn = mesh["ijk"][:, :4]
+    bad = np.zeros(ctx.N, dtype=bool)
+    on_bnd = mesh["boundary_mask"] == 0
+    touches_bnd = on_bnd[conn].any(axis=1)
+    bad[conn[touches_bnd].ravel()] = True
+    inner = ~bad
+    Al = ctx.apply_A(lin)
+    assert np.max(np.abs(Al[inner] - 1e-12 * lin[inner])) <= 1e-12 * np.max(np.abs(Al))
+    # solves in the three precision forms
+    ctx.setup_precond("amgcl")
+    m = np.zeros((ctx.N, 3))
+    m[:, 2] = 1.0
+    m += 0.3 * rng.standard_normal((ctx.N, 3))
+    m /= np.linalg.norm(m, axis=1, keepdims=True)
+    Us = {}
+    for prec in ("fp64", "mixed", "fp32pcg"):
+        ctx.set_precision(prec)
+        U, it, rr = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+        print(f"[poly25M solve {prec}] it={it} rel_res={rr:.2e}", flush=True)
+        assert rr <= 1e-10
+        Us[prec] = U
+    ctx.set_precision("fp64")
+    b = ctx.rhs(m)
+    for prec, U in Us.items():
+        assert np.linalg.norm(b - ctx.apply_A(U)) <= 1.01e-10 * np.linalg.norm(b), prec
+        assert np.linalg.norm(U - Us["fp64"]) <= 1e-8 * np.linalg.norm(Us["fp64"]), prec
+    ctx.close()
