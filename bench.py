@@ -122,17 +122,24 @@ def amg_kwargs(args) -> dict:
     return kw
 
 
-def measured_traffic(workload):
+def measured_traffic(workload, cluster_block_nodes):
     """HBM bytes per launch of the PCG's FP64 apply from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (``profiles/k_apply_cl_<workload>_fp64.md``, line
-    ``traffic_bytes_per_launch: N``), or None if there is none."""
+    ``traffic_bytes_per_launch: N``), or None if there is none or if the summary was taken at a
+    cluster block size (line ``traffic_cluster_block_nodes: B``) other than this run's."""
     f = ROOT / "profiles" / f"k_apply_cl_{workload}_fp64.md"
     if not f.exists():
         return None
+    fields = {}
     for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return {"bytes": int(line.split(":")[1]), "source": str(f.relative_to(ROOT))}
-    return None
+        key, sep, val = line.partition(":")
+        if sep and key in ("traffic_bytes_per_launch", "traffic_cluster_block_nodes"):
+            fields.setdefault(key, int(val))
+    if "traffic_bytes_per_launch" not in fields:
+        return None
+    if fields.get("traffic_cluster_block_nodes", cluster_block_nodes) != cluster_block_nodes:
+        return None
+    return {"bytes": fields["traffic_bytes_per_launch"], "source": str(f.relative_to(ROOT))}
 
 
 def cpu_baseline(mesh, mats, B_vals, budget_s, log=None, max_iter=2000):
@@ -592,7 +599,7 @@ def main():
     avg_apply_s = apply_ms / max(apply_n, 1) * 1e-3
     algo = apply_algorithmic_bytes(ctx, fp32=args.precision in ("fp32pcg", "fp32all"))
     achieved = algo / avg_apply_s / 1e9
-    traffic = measured_traffic(args.workload)
+    traffic = measured_traffic(args.workload, ctx.cluster_block_nodes)
     label = {"fp64": "FP64", "mixed": "mixed: FP32 V-cycle, FP64 PCG, FP64 energies",
              "fp32pcg": "FP32 PCG and V-cycle with FP64 dots and FP64 true-residual refinement, FP64 energies",
              "fp32all": "FP32 PCG and V-cycle with FP64 dots and FP64 true-residual refinement, FP32 energy "
