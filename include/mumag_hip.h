@@ -77,10 +77,10 @@ int mm_ctx_create_segmented(int N, int E, const double* knt, const int32_t* conn
                             const double* bnd_mask, double reg, int flags, int nseg, const int32_t* seg_node_off,
                             int64_t* out_ctx);
 
-/* info[0..12] = V_mag, N, E, element blocks (all), element blocks (magnetic),
+/* info[0..13] = V_mag, N, E, element blocks (all), element blocks (magnetic),
  * largest node list of a block, magnetic elements, node-list entries of all blocks,
  * cluster blocks, node-list entries of the cluster blocks, edge-star clusters, block node cap,
- * cluster block node cap */
+ * cluster block node cap, cluster staging mode (mm_set_cluster_staging) */
 int mm_ctx_info(int64_t ctx, double* info);
 
 /* Oriented connectivity (E,4), Ve (E,) and shape-function gradients of corners 1..3 (E,3,3)
@@ -121,6 +121,12 @@ int mm_set_apply_shape(int64_t ctx, int unit, int param);
  * grid. Rebuilds the cluster partition (not allowed on segmented contexts). Results agree to
  * rounding; a tuning knob with no reference counterpart. */
 int mm_set_cluster_block_nodes(int64_t ctx, int nb);
+/* How the FP32 applies (V-cycle, FP32 PCG) on cluster blocks of 512 or 1024 nodes stage their
+ * nodes: 0 = load the node ids, then gather the FP64 coordinates; 1 (default) = stream per-entry
+ * records (block-relative FP32 coordinates with the node id, in block order, 16 bytes per
+ * node-list entry kept on the device). Same staged values, same results; the FP64 apply always
+ * gathers. A tuning knob with no reference counterpart. */
+int mm_set_cluster_staging(int64_t ctx, int mode);
 
 /* y = A x: assembled-mode apply_A (src/poisson_solve.py:166-171) computed matrix-free. */
 int mm_poisson_apply(int64_t ctx, const double* x, double* y);

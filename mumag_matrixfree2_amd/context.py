@@ -18,6 +18,7 @@ from ._hip import call, f64, i32, ptr
 GEOM_MODES = {"on_the_fly": 0, "stored_f64": 1, "stored_f32": 2, "on_the_fly_f32": 3}
 METHODS = {"pcohen_hs": 0, "lbfgs": 1}
 APPLY_UNITS = {"element": 0, "cluster": 1, "persistent": 2}
+CLUSTER_STAGING = {"gather": 0, "records": 1}
 PRECISIONS = {"fp64": 0, "mixed": 1, "fp32pcg": 2, "fp32all": 3}
 
 
@@ -73,7 +74,7 @@ class MeshContext:
                  ptr(K1_red), ptr(Js_red), ptr(k_easy), ptr(bnd), self.reg, flags, self.segments.size - 1,
                  ptr(self.segments), C.byref(h))
         self.handle = h.value
-        info = np.zeros(13)
+        info = np.zeros(14)
         call("mm_ctx_info", self.handle, ptr(info))
         self.V_mag = float(info[0])
         self.n_blocks_all = int(info[3])
@@ -86,6 +87,7 @@ class MeshContext:
         self.n_clusters = int(info[10])
         self.block_nodes = int(info[11])
         self.cluster_block_nodes = int(info[12])
+        self.cluster_staging = "records" if info[13] else "gather"
         self.apply_shape = {"unit": "cluster", "param": 256}
         self.solve_guess = 0
         self.precond_type = None
@@ -155,7 +157,7 @@ class MeshContext:
         """Distinct nodes per cluster block of the Poisson apply (256, 512 or 1024); rebuilds the
         cluster partition and refreshes the partition counts."""
         call("mm_set_cluster_block_nodes", self.handle, int(nb))
-        info = np.zeros(13)
+        info = np.zeros(14)
         call("mm_ctx_info", self.handle, ptr(info))
         self.n_blocks_cl = int(info[8])
         self.cluster_node_list_entries = int(info[9])
@@ -164,6 +166,14 @@ class MeshContext:
             self.apply_shape = {"unit": "cluster", "param": 256}
         if self.cluster_block_nodes != 256 and self.apply_shape["unit"] == "cluster":
             self.apply_shape = {"unit": "cluster", "param": 256}
+
+    def set_cluster_staging(self, mode: str):
+        """How the FP32 applies on cluster blocks of 512 or 1024 nodes stage their nodes: "gather"
+        (node ids, then the FP64 coordinates gathered by id) or "records" (the default: per-entry
+        block-relative FP32 coordinate records streamed in block order). The staged values, and
+        so the results, are the same; the FP64 apply always gathers."""
+        call("mm_set_cluster_staging", self.handle, CLUSTER_STAGING[mode])
+        self.cluster_staging = mode
 
     def set_geometry_mode(self, geom: str):
         call("mm_set_geometry_mode", self.handle, GEOM_MODES[geom])

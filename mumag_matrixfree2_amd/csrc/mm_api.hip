@@ -431,17 +431,45 @@ static void build_cluster_partition(int N, std::vector<HostCluster>& cl, const s
 }
 
 // Cut the context's clusters into blocks of at most `cap` distinct nodes and upload them as
-// part_cl, freeing a previous cluster partition. hc receives the host partition.
-static int upload_cluster_part(Ctx& c, int cap, const std::vector<int>& seg_of, HostCPart& hc) {
+// part_cl with their entry records, freeing a previous cluster partition. hc receives the host
+// partition; knt_host: the node coordinates, or null to read them back from the device.
+static int upload_cluster_part(Ctx& c, int cap, const std::vector<int>& seg_of, HostCPart& hc,
+                               const double* knt_host) {
   std::vector<HostCluster> cl(c.h_cl_tets.size());
   for (size_t i = 0; i < cl.size(); ++i) {
     for (int q = 0; q < 8; ++q) cl[i].node[q] = c.h_cl_nodes[8 * i + q];
     cl[i].tets = c.h_cl_tets[i];
   }
   build_cluster_partition(c.N, cl, c.h_is_bnd, hc, cap, seg_of);
+  // entry records of the FP32 kernels in block order: coordinates relative to the block's first
+  // node (subtracted in FP64, then rounded, as the gathering staging does) with id | flags
+  std::vector<double> hk;
+  if (!knt_host) {
+    hk.resize(3 * (size_t)c.N);
+    MM_HIP(hipMemcpy(hk.data(), c.knt, hk.size() * sizeof(double), hipMemcpyDeviceToHost));
+    knt_host = hk.data();
+  }
+  const double* kh = knt_host;
+  const size_t n_ent = hc.nodes.size();
+  std::vector<float4> r32(n_ent);
+  for (size_t b = 0; b + 1 < hc.nptr.size(); ++b) {
+    if (hc.nptr[b] == hc.nptr[b + 1]) continue;
+    const size_t id0 = hc.nodes[hc.nptr[b]] & kIdMask;
+    const double ox = kh[3 * id0], oy = kh[3 * id0 + 1], oz = kh[3 * id0 + 2];
+    for (int n = hc.nptr[b]; n < hc.nptr[b + 1]; ++n) {
+      const uint32_t f = hc.nodes[n];
+      const size_t id = f & kIdMask;
+      float4 g;
+      g.x = (float)(kh[3 * id] - ox);
+      g.y = (float)(kh[3 * id + 1] - oy);
+      g.z = (float)(kh[3 * id + 2] - oz);
+      std::memcpy(&g.w, &f, sizeof(float));
+      r32[n] = g;
+    }
+  }
   Part& q = c.part_cl;
   for (const void* old : {(const void*)q.eptr, (const void*)q.nptr, (const void*)q.nodes, (const void*)q.lconn,
-                          (const void*)q.lconn2, (const void*)q.cmask})
+                          (const void*)q.lconn2, (const void*)q.cmask, (const void*)q.rec32})
     if (old) {
       auto it = std::find(c.allocs.begin(), c.allocs.end(), old);
       if (it != c.allocs.end()) {
@@ -460,12 +488,15 @@ static int upload_cluster_part(Ctx& c, int cap, const std::vector<int>& seg_of, 
   MM_TRY(upload(c, &w0, hc.w0.data(), hc.w0.size()));
   MM_TRY(upload(c, &w1, hc.w1.data(), hc.w1.size()));
   MM_TRY(upload(c, &wm, hc.wm.data(), hc.wm.size()));
+  float4* d32;
+  MM_TRY(upload(c, &d32, r32.data(), r32.size()));
   q.eptr = cp;
   q.nptr = np;
   q.nodes = nodes;
   q.lconn = w0;
   q.lconn2 = w1;
   q.cmask = wm;
+  q.rec32 = d32;
   c.nodes_cl = (long long)hc.nodes.size();
   c.cl_cap = cap;
   return 0;
@@ -622,7 +653,7 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
     }
     c.h_is_bnd = is_bnd;
     HostCPart hc;
-    MM_TRY(upload_cluster_part(c, node_cap, seg_of, hc));
+    MM_TRY(upload_cluster_part(c, node_cap, seg_of, hc, knt));
     c.h_seg_blk_cl = seg_blocks(hc.nptr, hc.nodes);
   }
   c.h_seg_blk_mag = seg_blocks(hp_mag.nptr, hp_mag.nodes);
@@ -884,6 +915,7 @@ int mm_ctx_info(int64_t h, double* info) {
   info[10] = c.part_cl.E;
   info[11] = c.node_cap;
   info[12] = c.cl_cap;
+  info[13] = c.cl_records ? 1 : 0;
   return 0;
 }
 
@@ -935,12 +967,19 @@ int mm_set_cluster_block_nodes(int64_t h, int nb) {
   if (nb == c.cl_cap) return 0;
   MM_HIP(hipStreamSynchronize(c.stream));
   HostCPart hc;
-  MM_TRY(upload_cluster_part(c, nb, std::vector<int>(c.N, 0), hc));
+  MM_TRY(upload_cluster_part(c, nb, std::vector<int>(c.N, 0), hc, nullptr));
   if (nb != kMaxBlockNodes && c.apply_persist > 0) {
     c.apply_persist = 0;  // the persistent grid stages 256-node blocks
     c.elem_threads = 256;
   }
   if (nb != kMaxBlockNodes) c.elem_threads = 256;
+  return 0;
+}
+
+int mm_set_cluster_staging(int64_t h, int mode) {
+  GET_CTX(h);
+  MM_CHECK(mode == 0 || mode == 1, "cluster staging must be 0 (gathers) or 1 (entry records)");
+  c.cl_records = mode == 1;
   return 0;
 }
 

@@ -88,6 +88,9 @@ struct Part {
   // and bits 16 + 2q, 17 + 2q the high bits of slot q's index (blocks of up to 1024 nodes)
   const uint32_t* lconn2 = nullptr;
   const uint32_t* cmask = nullptr;
+  // cluster partition only, one record per node-list entry in block order for the FP32
+  // kernels: x, y, z relative to the block's first node, id | flags as the bits of w
+  const float4* rec32 = nullptr;
   // optional subset of the blocks (the blocks of some segments of a segmented context):
   // workgroup position i < nlist works on block blist[i]; null: every block
   const int* blist = nullptr;
@@ -174,6 +177,7 @@ struct Ctx {
   int elem_threads = 256;             // workgroup size of the Poisson apply (64, 128, 256)
   int node_cap = kMaxBlockNodes;      // distinct nodes per element block
   int cl_cap = kMaxBlockNodes;        // distinct nodes per cluster block (256, 512 or 1024)
+  bool cl_records = true;             // FP32 applies on 512 / 1024-node cluster blocks stage entry records
   std::vector<int> h_cl_nodes;        // host copy of the clusters: 8 slots each (-1 unused)
   std::vector<uint32_t> h_cl_tets;    // tet mask of each cluster
   std::vector<char> h_is_bnd;         // Dirichlet flag per node

@@ -322,18 +322,24 @@ __device__ __forceinline__ void cluster_apply(uint32_t w0, uint32_t w1, uint32_t
 // SCALED: input w .* r from the FP64 residual r.
 // NB: distinct nodes per block (256, 512 or 1024; indices above 255 carry their high bits in
 // cmask), NB / NT staged nodes per thread.
+// REC (FP32 only): the staging streams the block's entry records (block-relative FP32
+// coordinates with id | flags, in block order) instead of loading the node ids and then
+// gathering the FP64 coordinates; the staged values are the same, so are the results. (FP64
+// records, 32 bytes per entry, were measured slower than the gathers: 151 vs 117 us on the
+// 24.6M-tet polycrystal, where the gathered coordinates of shared nodes hit the caches.)
 template <typename T>
 struct same_type {  // a non-deduced parameter type: the template argument is given or defaulted
   using type = T;
 };
 
-template <typename T, bool SCALED, bool DOT, int NT, int NB, typename TR = double>
+template <typename T, bool SCALED, bool DOT, int NT, int NB, bool REC, typename TR = double>
 __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __restrict__ x,
                                                  const typename same_type<TR>::type* __restrict__ r,
                                                  const T* __restrict__ w,
                                                  T* __restrict__ xo, T* __restrict__ y, T reg,
                                                  double* __restrict__ dot_part) {
   static_assert(NB % NT == 0 && NB <= kMaxClusterBlockNodes, "block nodes per thread");
+  static_assert(!REC || sizeof(T) == 4, "entry records carry FP32 coordinates");
   __shared__ T sx[NB];
   __shared__ double sy[NB];
   __shared__ T sX[3][NB];
@@ -342,6 +348,7 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
   const int tid = threadIdx.x;
   constexpr int kNode = NB / NT;
   uint32_t fv[kNode];
+  T rc[kNode][3], xg[kNode];
   const int c0 = bi.e0 + tid;
   uint32_t cw0 = 0u, cw1 = 0u, cwm = 0u;
   if (c0 < bi.e1) {
@@ -349,13 +356,23 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
     cw1 = p.lconn2[c0];
     cwm = p.cmask[c0];
   }
+  // Every lane reads a valid entry (the block's last one past its end; a block holds at least
+  // one node), so each round of loads issues without branches and waits once.
 #pragma unroll
   for (int q = 0; q < kNode; ++q) {
-    const int i = tid + q * NT;
-    fv[q] = i < bi.nb ? p.nodes[bi.n0 + i] : 0u;
+    const int e = bi.n0 + min(tid + q * NT, bi.nb - 1);
+    if constexpr (REC) {
+      const float4 rv = p.rec32[e];
+      rc[q][0] = rv.x;
+      rc[q][1] = rv.y;
+      rc[q][2] = rv.z;
+      fv[q] = __float_as_uint(rv.w);
+    } else {
+      fv[q] = p.nodes[e];
+    }
   }
   double ox = 0.0, oy = 0.0, oz = 0.0;
-  if constexpr (sizeof(T) == 4) {
+  if constexpr (sizeof(T) == 4 && !REC) {
     const uint32_t id0 = p.nodes[bi.n0] & kIdMask;
     ox = geo.knt[3 * id0];
     oy = geo.knt[3 * id0 + 1];
@@ -363,23 +380,29 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
   }
 #pragma unroll
   for (int q = 0; q < kNode; ++q) {
-    const int i = tid + q * NT;
-    if (i < bi.nb) {
-      const uint32_t f = fv[q];
-      const uint32_t id = f & kIdMask;
-      T xv;
-      if constexpr (SCALED) {
-        xv = (T)(w[id] * r[id]);
-        if (f & kOwner) xo[id] = xv;
-      } else {
-        xv = x[id];
-      }
-      sx[i] = (f & kBoundary) ? (T)0 : xv;
-      sy[i] = 0.0;
-      sX[0][i] = (T)(geo.knt[3 * id + 0] - ox);
-      sX[1][i] = (T)(geo.knt[3 * id + 1] - oy);
-      sX[2][i] = (T)(geo.knt[3 * id + 2] - oz);
+    const uint32_t id = fv[q] & kIdMask;
+    if constexpr (SCALED)
+      xg[q] = (T)(w[id] * r[id]);
+    else
+      xg[q] = x[id];
+    if constexpr (!REC) {
+      rc[q][0] = (T)(geo.knt[3 * id + 0] - ox);
+      rc[q][1] = (T)(geo.knt[3 * id + 1] - oy);
+      rc[q][2] = (T)(geo.knt[3 * id + 2] - oz);
     }
+  }
+  // slots past the block's end receive copies of its last entry, which no cluster reads
+#pragma unroll
+  for (int q = 0; q < kNode; ++q) {
+    const int i = tid + q * NT;
+    const uint32_t f = fv[q];
+    if constexpr (SCALED)
+      if ((f & kOwner) && i < bi.nb) xo[f & kIdMask] = xg[q];
+    sx[i] = (f & kBoundary) ? (T)0 : xg[q];
+    sy[i] = 0.0;
+    sX[0][i] = rc[q][0];
+    sX[1][i] = rc[q][1];
+    sX[2][i] = rc[q][2];
   }
   __syncthreads();
   double acc_e = 0.0;
@@ -1161,14 +1184,25 @@ static void with_nt(int nt, F&& f) {
 
 // workgroup size and block node count of the cluster apply: blocks of more than 256 nodes run
 // 256 threads
+// blocks of more than 256 nodes stage either way in the FP32 kernels (gathers or entry records)
 template <typename F>
 static void with_cl_shape(const Ctx& c, F&& f) {
-  if (c.cl_cap == 1024)
-    f(std::integral_constant<int, 256>{}, std::integral_constant<int, 1024>{});
-  else if (c.cl_cap == 512)
-    f(std::integral_constant<int, 256>{}, std::integral_constant<int, 512>{});
-  else
-    with_nt(c.elem_threads, [&](auto nt) { f(nt, std::integral_constant<int, kMaxBlockNodes>{}); });
+  using nt256 = std::integral_constant<int, 256>;
+  using rec = std::true_type;
+  using gather = std::false_type;
+  if (c.cl_cap == 1024) {
+    if (c.cl_records)
+      f(nt256{}, std::integral_constant<int, 1024>{}, rec{});
+    else
+      f(nt256{}, std::integral_constant<int, 1024>{}, gather{});
+  } else if (c.cl_cap == 512) {
+    if (c.cl_records)
+      f(nt256{}, std::integral_constant<int, 512>{}, rec{});
+    else
+      f(nt256{}, std::integral_constant<int, 512>{}, gather{});
+  } else {
+    with_nt(c.elem_threads, [&](auto nt) { f(nt, std::integral_constant<int, kMaxBlockNodes>{}, gather{}); });
+  }
 }
 
 // persistent grid: apply_persist workgroups per CU, a multiple of 8 (one share per XCD)
@@ -1196,14 +1230,14 @@ int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t
     const Part& p = c.part_cl;
     if (part_len(p) == 0) return 0;
     const int grid = grid_for(p);
-    with_cl_shape(c, [&](auto nt, auto nb) {
+    with_cl_shape(c, [&](auto nt, auto nb, auto) {  // FP64 blocks always gather
       constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
       if (dot_part)
-        k_apply_cl<double, false, true, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y, c.reg,
-                                                                     dot_part);
+        k_apply_cl<double, false, true, NT, NB, false><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
+                                                                          c.reg, dot_part);
       else
-        k_apply_cl<double, false, false, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
-                                                                      c.reg, nullptr);
+        k_apply_cl<double, false, false, NT, NB, false><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
+                                                                           c.reg, nullptr);
     });
     return 0;
   }
@@ -1234,9 +1268,9 @@ int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* 
   }
   const int grid = grid_for(p);
   if (c.apply_clusters) {
-    with_cl_shape(c, [&](auto nt, auto nb) {
+    with_cl_shape(c, [&](auto nt, auto nb, auto) {  // FP64 blocks always gather
       constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
-      k_apply_cl<double, true, false, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, c.reg, nullptr);
+      k_apply_cl<double, true, false, NT, NB, false><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, c.reg, nullptr);
     });
     return 0;
   }
@@ -1252,9 +1286,10 @@ int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s) {
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
   if (c.apply_clusters) {
-    with_cl_shape(c, [&](auto nt, auto nb) {
+    with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
       constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
-      k_apply_cl<float, false, false, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
+      constexpr bool REC = decltype(rec)::value;
+      k_apply_cl<float, false, false, NT, NB, REC><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
                                                                   (float)c.reg, nullptr);
     });
     return 0;
@@ -1298,9 +1333,10 @@ int launch_apply_f32_dot(const Ctx& c, const float* x, float* y, double* dot_par
   const Part& p = c.part_cl;
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
-  with_cl_shape(c, [&](auto nt, auto nb) {
+  with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
     constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
-    k_apply_cl<float, false, true, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
+    constexpr bool REC = decltype(rec)::value;
+    k_apply_cl<float, false, true, NT, NB, REC><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
                                                                (float)c.reg, dot_part);
   });
   return 0;
@@ -1312,9 +1348,10 @@ int launch_apply_f32_scaled_f(const Ctx& c, const float* r, const float* w, floa
   const Part& p = c.part_cl;
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
-  with_cl_shape(c, [&](auto nt, auto nb) {
+  with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
     constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
-    k_apply_cl<float, true, false, NT, NB, float><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg,
+    constexpr bool REC = decltype(rec)::value;
+    k_apply_cl<float, true, false, NT, NB, REC, float><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg,
                                                                       nullptr);
   });
   return 0;
@@ -1325,9 +1362,10 @@ int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
   if (c.apply_clusters) {
-    with_cl_shape(c, [&](auto nt, auto nb) {
+    with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
       constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
-      k_apply_cl<float, true, false, NT, NB><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg,
+      constexpr bool REC = decltype(rec)::value;
+      k_apply_cl<float, true, false, NT, NB, REC><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg,
                                                                  nullptr);
     });
     return 0;

@@ -169,7 +169,8 @@ def test_full_size_polycrystal_size_independent_properties():
                       boundary_mask=mesh["boundary_mask"])
     print(f"[poly25M] N={ctx.N} E={ctx.E}", flush=True)
     assert ctx.E == 24_576_000
-    tune_apply(ctx, reps=5)
+    tuned = tune_apply(ctx, reps=5)
+    print(f"[poly25M] apply shape {tuned['unit']}:{tuned['param']}:{tuned['cluster_block_nodes']}", flush=True)
     rng = np.random.default_rng(80)
     x, y = rng.standard_normal(ctx.N), rng.standard_normal(ctx.N)
     Ax, Ay = ctx.apply_A(x), ctx.apply_A(y)

@@ -46,6 +46,31 @@ def test_fp32_vcycle_is_fp64_vcycle_to_fp32_accuracy(mixed_ctx, oracle_c0):
     assert float(r @ z) > 0
 
 
+@pytest.mark.parametrize("nb", [512, 1024])
+def test_fp32_vcycle_same_under_both_stagings(mixed_ctx, oracle_c0, nb):
+    """The FP32 applies of the V-cycle stage the same values from entry records as from gathers:
+    the two V-cycles agree to the rounding of the atomic summation order, and both are the FP64
+    oracle's V-cycle to FP32 accuracy."""
+    from oracle.solvers import vcycle
+
+    ctx = mixed_ctx
+    r = np.random.default_rng(32).standard_normal(ctx.N)
+    z_ref = vcycle(oracle_levels(ctx.amg_levels, "amgcl"), oracle_c0.apply_A, r, "amgcl")
+    ctx.set_cluster_block_nodes(nb)
+    ctx.set_precision("mixed")
+    try:
+        z = {}
+        for staging in ("gather", "records"):
+            ctx.set_cluster_staging(staging)
+            z[staging] = ctx.precond_apply(r)
+    finally:
+        ctx.set_precision("fp64")
+        ctx.set_cluster_staging("records")
+        ctx.set_cluster_block_nodes(256)
+    assert rel(z["records"], z["gather"]) <= 1e-6
+    assert rel(z["records"], z_ref) <= 1e-5 and rel(z["gather"], z_ref) <= 1e-5
+
+
 def test_mixed_solve_meets_fp64_tolerance(mixed_ctx, oracle_c0):
     ctx = mixed_ctx
     m = tilted_m0(ctx.N, seed=32, sigma=0.4)
@@ -103,20 +128,27 @@ def test_mixed_sweep_same_switching_step(small_cube):
     assert np.max(np.abs(Emx - E64) / np.maximum(np.abs(E64), 1.0)) <= 1e-8
 
 
-def test_fp32_pcg_meets_the_fp64_tolerance(mixed_ctx, oracle_c0):
+@pytest.mark.parametrize("blocks", ["256", "1024:records", "1024:gather"])
+def test_fp32_pcg_meets_the_fp64_tolerance(mixed_ctx, oracle_c0, blocks):
     """The FP32 PCG with FP64 refinement stops on the FP64 true residual: r.r <= tol^2 b.b
     checked by the oracle in FP64, and the solution agrees with the FP64 solve to the tolerance
-    (from zero and from a warm start close to the solution)."""
+    (from zero and from a warm start close to the solution); on 256-node cluster blocks and on
+    1024-node blocks with either staging of the FP32 kernels."""
     ctx = mixed_ctx
+    nb, *staging = blocks.split(":")
     m = tilted_m0(ctx.N, seed=34, sigma=0.4)
     U64, it64, _ = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
     b = oracle_c0.rhs(m)
+    ctx.set_cluster_block_nodes(int(nb))
+    ctx.set_cluster_staging(staging[0] if staging else "records")
     ctx.set_precision("fp32pcg")
     try:
         U, it, rr = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
         U2, it2, rr2 = ctx.solve_U(m, U64 * (1 + 1e-4), 1e-12)
     finally:
         ctx.set_precision("fp64")
+        ctx.set_cluster_staging("records")
+        ctx.set_cluster_block_nodes(256)
     true_rr = np.linalg.norm(b - oracle_c0.A @ U) / np.linalg.norm(b)
     true_rr2 = np.linalg.norm(b - oracle_c0.A @ U2) / np.linalg.norm(b)
     print(f"[fp32pcg solve] it={it} (fp64 {it64}) rr={rr:.3e} true={true_rr:.3e}; warm it={it2} true={true_rr2:.3e}",

@@ -53,18 +53,24 @@ def test_poisson_apply(ctx_c0, oracle_c0, geom):
 
 
 @pytest.mark.parametrize("shape", ["persistent:1", "persistent:3", "cluster:256", "cluster:64", "element:128",
-                                   "element:64", "cluster:256:512", "cluster:256:1024"])
+                                   "element:64", "cluster:256:512", "cluster:256:1024", "cluster:256:512:gather",
+                                   "cluster:256:1024:gather"])
 def test_poisson_apply_every_work_shape(ctx_c0, oracle_c0, shape):
     """Every apply shape the autotuner can pick gives the oracle's A x, in one apply and inside
     a PCG solve (the persistent grid writes its x.Ax partials per block); the third field is the
-    cluster block size (slot indices above 255 carry their high bits in the cluster mask)."""
+    cluster block size (slot indices above 255 carry their high bits in the cluster mask), the
+    fourth the staging of those blocks in the FP32 kernels (entry records instead of gathers; the
+    FP32 PCG's kernels with it are checked in the mixed-precision tests)."""
     from oracle.solvers import solve_U as oracle_solve_U
 
-    unit, param, *nb = shape.split(":")
+    unit, param, *rest = shape.split(":")
+    nb = int(rest[0]) if rest else 256
+    staging = rest[1] if len(rest) > 1 else "records"
     rng = np.random.default_rng(5)
     x = rng.standard_normal(ctx_c0.N)
-    ctx_c0.set_cluster_block_nodes(int(nb[0]) if nb else 256)
+    ctx_c0.set_cluster_block_nodes(nb)
     ctx_c0.set_apply_shape(unit, int(param))
+    ctx_c0.set_cluster_staging(staging)
     try:
         assert rel(ctx_c0.apply_A(x), oracle_c0.A @ x) <= TOL
         if ctx_c0.precond_type is None:
@@ -73,8 +79,8 @@ def test_poisson_apply_every_work_shape(ctx_c0, oracle_c0, shape):
         U, it, rr = ctx_c0.solve_U(m, np.zeros(ctx_c0.N), 1e-10)
         U_ref, _, _ = oracle_solve_U(oracle_c0, m, np.zeros(ctx_c0.N), 1e-12, precond="direct")
         assert rr <= 1e-10 and rel(U, U_ref) <= 1e-7
-        if nb:
-            assert ctx_c0.cluster_block_nodes == int(nb[0]) and ctx_c0.n_blocks_cl >= 1
+        if rest:
+            assert ctx_c0.cluster_block_nodes == nb and ctx_c0.n_blocks_cl >= 1
             ctx_c0.set_precision("mixed")
             try:
                 Um, _, rrm = ctx_c0.solve_U(m, np.zeros(ctx_c0.N), 1e-10)
@@ -82,6 +88,7 @@ def test_poisson_apply_every_work_shape(ctx_c0, oracle_c0, shape):
                 ctx_c0.set_precision("fp64")
             assert rrm <= 1e-10 and rel(Um, U_ref) <= 1e-7
     finally:
+        ctx_c0.set_cluster_staging("records")
         ctx_c0.set_cluster_block_nodes(256)
         ctx_c0.set_apply_shape("cluster", 256)
 

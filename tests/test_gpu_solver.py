@@ -26,12 +26,19 @@ def solver_ctx(request, cube_c0):
     return request.param, ctx
 
 
-def test_vcycle_matches_oracle(solver_ctx, oracle_c0):
+@pytest.mark.parametrize("nb", [256, 512, 1024])
+def test_vcycle_matches_oracle(solver_ctx, oracle_c0, nb):
+    """The FP64 V-cycle on every cluster block size (its first apply stages x = M r from the
+    residual, the second the corrected x)."""
     from oracle.solvers import vcycle
 
     kind, ctx = solver_ctx
     r = np.random.default_rng(11).standard_normal(ctx.N)
-    z = ctx.precond_apply(r)
+    ctx.set_cluster_block_nodes(nb)
+    try:
+        z = ctx.precond_apply(r)
+    finally:
+        ctx.set_cluster_block_nodes(256)
     z_ref = vcycle(oracle_levels(ctx.amg_levels, kind), oracle_c0.apply_A, r, kind)
     assert rel(z, z_ref) <= 1e-11
     assert float(r @ z) > 0.0
