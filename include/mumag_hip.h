@@ -116,12 +116,12 @@ int mm_set_geometry_mode(int64_t ctx, int mode);
 int mm_set_apply_shape(int64_t ctx, int unit, int param);
 
 /* Distinct nodes per block of the edge-star cluster partition of the Poisson apply: 256 (the
- * default), 512 or 1024. Larger blocks stage fewer node entries and share fewer nodes with other
- * blocks (fewer global atomics); they run 256 threads per workgroup and not on the persistent
- * grid. Rebuilds the cluster partition (not allowed on segmented contexts). Results agree to
- * rounding; a tuning knob with no reference counterpart. */
+ * default), 512, 1024 or 2048. Larger blocks stage fewer node entries and share fewer nodes with
+ * other blocks (fewer global atomics); they run 256 threads per workgroup (512 for 2048-node
+ * blocks) and not on the persistent grid. Rebuilds the cluster partition (not allowed on
+ * segmented contexts). Results agree to rounding; a tuning knob with no reference counterpart. */
 int mm_set_cluster_block_nodes(int64_t ctx, int nb);
-/* How the FP32 applies (V-cycle, FP32 PCG) on cluster blocks of 512 or 1024 nodes stage their
+/* How the FP32 applies (V-cycle, FP32 PCG) on cluster blocks of 512 to 2048 nodes stage their
  * nodes: 0 = load the node ids, then gather the FP64 coordinates; 1 (default) = stream per-entry
  * records (block-relative FP32 coordinates with the node id, in block order, 16 bytes per
  * node-list entry kept on the device). Same staged values, same results; the FP64 apply always

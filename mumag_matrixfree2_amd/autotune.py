@@ -9,9 +9,9 @@ result beyond rounding:
 * the workgroup size (64, 128 or 256 threads per element block), or for the persistent grid
   the workgroups per CU.
 
-A third knob is the number of distinct nodes per block of the cluster partition (256, 512 or
-1024; larger blocks stage fewer node entries and leave fewer shared nodes to the write-back
-atomics): ``tune_apply`` rebuilds the cluster partition for each value it tries. The node cap
+A third knob is the number of distinct nodes per block of the cluster partition (256, 512,
+1024 or 2048; larger blocks stage fewer node entries and leave fewer shared nodes to the
+write-back atomics): ``tune_apply`` rebuilds the cluster partition for each value it tries. The node cap
 of the element blocks (the per-tet kernels) is fixed when a context is created;
 ``tune_block_nodes`` times contexts built with each candidate value.
 
@@ -26,7 +26,7 @@ SHAPES = (("persistent", 2), ("persistent", 3), ("cluster", 256), ("cluster", 12
           ("element", 256), ("element", 128), ("element", 64))
 
 
-CLUSTER_BLOCK_NODES = (512, 1024)
+CLUSTER_BLOCK_NODES = (512, 1024, 2048)
 
 
 def tune_apply(ctx, shapes=SHAPES, reps: int = 40, cluster_block_nodes=CLUSTER_BLOCK_NODES) -> dict:

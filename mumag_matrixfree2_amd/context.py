@@ -154,8 +154,8 @@ class MeshContext:
         self.apply_shape = {"unit": unit, "param": int(param)}
 
     def set_cluster_block_nodes(self, nb: int):
-        """Distinct nodes per cluster block of the Poisson apply (256, 512 or 1024); rebuilds the
-        cluster partition and refreshes the partition counts."""
+        """Distinct nodes per cluster block of the Poisson apply (256, 512, 1024 or 2048);
+        rebuilds the cluster partition and refreshes the partition counts."""
         call("mm_set_cluster_block_nodes", self.handle, int(nb))
         info = np.zeros(14)
         call("mm_ctx_info", self.handle, ptr(info))
@@ -168,7 +168,7 @@ class MeshContext:
             self.apply_shape = {"unit": "cluster", "param": 256}
 
     def set_cluster_staging(self, mode: str):
-        """How the FP32 applies on cluster blocks of 512 or 1024 nodes stage their nodes: "gather"
+        """How the FP32 applies on cluster blocks of 512 to 2048 nodes stage their nodes: "gather"
         (node ids, then the FP64 coordinates gathered by id) or "records" (the default: per-entry
         block-relative FP32 coordinate records streamed in block order). The staged values, and
         so the results, are the same; the FP64 apply always gathers."""

@@ -332,8 +332,10 @@ static void build_clusters(const double* knt, int E, const std::vector<int>& con
 
 // Greedy cut of the clusters (ordered by smallest, then largest node id) into blocks touching
 // at most node_cap distinct nodes, with the flags of build_partition. Per cluster: w0/w1 hold
-// the block-local indices of slots 0-3 / 4-7 (uint8 each), wm bits 0-5 the tets and bits 8-15
-// the used slots.
+// the block-local indices of slots 0-3 / 4-7 (low 8 bits each), wm bits 0-5 the tets and bits
+// 8 + 3q .. 10 + 3q the high bits of slot q's index; the kernels derive the used slots from the
+// tets (slots 0, 1 and ring slot 2 + k of tets k and k - 1 mod 6), which upload_cluster_part
+// checks against the slots that hold nodes.
 struct HostCPart {
   std::vector<int> cptr, nptr;
   std::vector<uint32_t> nodes, w0, w1, wm;
@@ -375,8 +377,7 @@ static void build_cluster_partition(int N, std::vector<HostCluster>& cl, const s
         if (h.node[q] >= 0) {
           const uint32_t li = (uint32_t)remap[local[h.node[q]]];
           w[q >> 2] |= (li & 255u) << (8 * (q & 3));
-          m |= 1u << (8 + q);
-          m |= (li >> 8) << (16 + 2 * q);
+          m |= (li >> 8) << (8 + 3 * q);
         }
       hp.w0.push_back(w[0]);
       hp.w1.push_back(w[1]);
@@ -439,6 +440,13 @@ static int upload_cluster_part(Ctx& c, int cap, const std::vector<int>& seg_of, 
   for (size_t i = 0; i < cl.size(); ++i) {
     for (int q = 0; q < 8; ++q) cl[i].node[q] = c.h_cl_nodes[8 * i + q];
     cl[i].tets = c.h_cl_tets[i];
+  }
+  for (const HostCluster& h : cl) {
+    uint32_t held = 0u;
+    for (int q = 0; q < 8; ++q)
+      if (h.node[q] >= 0) held |= 1u << q;
+    const uint32_t t = h.tets & 63u, ring = (t | (t << 1) | (t >> 5)) & 63u;
+    MM_CHECK(t != 0u && held == (3u | (ring << 2)), "cluster slots do not follow from its tets");
   }
   build_cluster_partition(c.N, cl, c.h_is_bnd, hc, cap, seg_of);
   // entry records of the FP32 kernels in block order: coordinates relative to the block's first
@@ -962,7 +970,7 @@ int mm_set_geometry_mode(int64_t h, int mode) {
 
 int mm_set_cluster_block_nodes(int64_t h, int nb) {
   GET_CTX(h);
-  MM_CHECK(nb == 256 || nb == 512 || nb == 1024, "cluster block nodes must be 256, 512 or 1024");
+  MM_CHECK(nb == 256 || nb == 512 || nb == 1024 || nb == 2048, "cluster block nodes must be 256, 512, 1024 or 2048");
   MM_CHECK(c.nseg == 1, "segmented contexts keep 256-node cluster blocks");
   if (nb == c.cl_cap) return 0;
   MM_HIP(hipStreamSynchronize(c.stream));

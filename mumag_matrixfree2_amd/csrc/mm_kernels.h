@@ -28,7 +28,7 @@ namespace mm {
 constexpr int kBlock = 256;              // threads per workgroup
 constexpr int kMaxBlockNodes = 256;      // distinct nodes per element block (uint8 indices)
 constexpr int kMaxBlockElems = 2048;     // elements per block cap
-constexpr int kMaxClusterBlockNodes = 1024;  // distinct nodes per cluster block (10-bit slot indices)
+constexpr int kMaxClusterBlockNodes = 2048;  // distinct nodes per cluster block (11-bit slot indices)
 constexpr uint32_t kOwner = 1u << 31;    // this block adds reg*x / writes the boundary row / Zeeman
 constexpr uint32_t kPrivate = 1u << 30;  // no other block touches the node: plain store
 constexpr uint32_t kBoundary = 1u << 29; // Dirichlet node (outer surface)
@@ -84,8 +84,9 @@ struct Part {
   const uint32_t* lconn = nullptr;    // E packed local corners
   const uint8_t* mat = nullptr;       // E material index (0-based)
   // edge-star cluster partition only: eptr are cluster offsets, lconn / lconn2 the low 8 bits
-  // of the local indices of slots 0-3 / 4-7, cmask bits 0-5 the tets, bits 8-15 the used slots
-  // and bits 16 + 2q, 17 + 2q the high bits of slot q's index (blocks of up to 1024 nodes)
+  // of the local indices of slots 0-3 / 4-7, cmask bits 0-5 the tets and bits 8 + 3q .. 10 + 3q
+  // the high bits of slot q's index (blocks of up to 2048 nodes); the used slots follow from
+  // the tets (cluster_slots)
   const uint32_t* lconn2 = nullptr;
   const uint32_t* cmask = nullptr;
   // cluster partition only, one record per node-list entry in block order for the FP32
@@ -176,8 +177,8 @@ struct Ctx {
   int geom_mode = kGeomCoords;        // geometry source of the Poisson apply
   int elem_threads = 256;             // workgroup size of the Poisson apply (64, 128, 256)
   int node_cap = kMaxBlockNodes;      // distinct nodes per element block
-  int cl_cap = kMaxBlockNodes;        // distinct nodes per cluster block (256, 512 or 1024)
-  bool cl_records = true;             // FP32 applies on 512 / 1024-node cluster blocks stage entry records
+  int cl_cap = kMaxBlockNodes;        // distinct nodes per cluster block (256, 512, 1024 or 2048)
+  bool cl_records = true;             // FP32 applies on 512..2048-node cluster blocks stage entry records
   std::vector<int> h_cl_nodes;        // host copy of the clusters: 8 slots each (-1 unused)
   std::vector<uint32_t> h_cl_tets;    // tet mask of each cluster
   std::vector<char> h_is_bnd;         // Dirichlet flag per node

@@ -243,6 +243,14 @@ __global__ __launch_bounds__(NT) void k_apply_f32(Part p, Geo geo, const float* 
 // from LDS once, the edge vector c1 = x1 - x0 and the cross products c1 x (x_s - x0) are
 // shared by the tets, and the slot sums leave through 8 LDS atomics. Adds the tets' energies
 // sum_a x_a y_a to acc_e when DOT.
+// Slots of a cluster's tets (bit q: slot q): tet j has corners 0, 1, 2 + j and 2 + (j + 1) % 6,
+// so ring slot 2 + k belongs to tets k and k - 1 (mod 6); no tets, no slots.
+__device__ __forceinline__ uint32_t cluster_slots(uint32_t wm) {
+  const uint32_t t = wm & 63u;
+  const uint32_t ring = (t | (t << 1) | (t >> 5)) & 63u;
+  return t ? 3u | (ring << 2) : 0u;
+}
+
 template <typename T, bool DOT, int NB>
 __device__ __forceinline__ void cluster_apply(uint32_t w0, uint32_t w1, uint32_t wm, const T (*sX)[NB],
                                               const T* sx, double* sy, double& acc_e) {
@@ -254,12 +262,13 @@ __device__ __forceinline__ void cluster_apply(uint32_t w0, uint32_t w1, uint32_t
   }
   if constexpr (NB > 256) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) li[q] |= (int)((wm >> (16 + 2 * q)) & 3u) << 8;
+    for (int q = 0; q < 8; ++q) li[q] |= (int)((wm >> (8 + 3 * q)) & 7u) << 8;
   }
+  const uint32_t used = cluster_slots(wm);
   T X[8][3], v[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    if (wm & (1u << (8 + q))) {
+    if (used & (1u << q)) {
       X[q][0] = sX[0][li[q]];
       X[q][1] = sX[1][li[q]];
       X[q][2] = sX[2][li[q]];
@@ -313,7 +322,7 @@ __device__ __forceinline__ void cluster_apply(uint32_t w0, uint32_t w1, uint32_t
   }
 #pragma unroll
   for (int q = 0; q < 8; ++q)
-    if (wm & (1u << (8 + q))) atomicAdd(&sy[li[q]], (double)acc[q]);
+    if (used & (1u << q)) atomicAdd(&sy[li[q]], (double)acc[q]);
 }
 
 // Same operator as k_apply, one workgroup per block of clusters. T = double: the FP64
@@ -485,6 +494,7 @@ __global__ __launch_bounds__(NT) void k_apply_cl_multi(Part p, Geo geo, const do
   for (int r = 0; r < K; ++r) acc_e[r] = 0.0;
   for (int cc = bi.e0 + tid; cc < bi.e1; cc += NT) {
     const uint32_t w0 = p.lconn[cc], w1 = p.lconn2[cc], wm = p.cmask[cc];
+    const uint32_t used_slots = cluster_slots(wm);
     int li[8];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -496,7 +506,7 @@ __global__ __launch_bounds__(NT) void k_apply_cl_multi(Part p, Geo geo, const do
       double X0[3] = {sX[0][li[0]], sX[1][li[0]], sX[2][li[0]]};
 #pragma unroll
       for (int q = 1; q < 8; ++q) {
-        const bool used = wm & (1u << (8 + q));
+        const bool used = used_slots & (1u << q);
 #pragma unroll
         for (int k = 0; k < 3; ++k) c[q][k] = used ? sX[k][li[q]] - X0[k] : 0.0;
       }
@@ -512,7 +522,7 @@ __global__ __launch_bounds__(NT) void k_apply_cl_multi(Part p, Geo geo, const do
       double v[8], acc[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        v[q] = (wm & (1u << (8 + q))) ? sx[r][li[q]] : 0.0;
+        v[q] = (used_slots & (1u << q)) ? sx[r][li[q]] : 0.0;
         acc[q] = 0.0;
       }
 #pragma unroll
@@ -541,7 +551,7 @@ __global__ __launch_bounds__(NT) void k_apply_cl_multi(Part p, Geo geo, const do
       }
 #pragma unroll
       for (int q = 0; q < 8; ++q)
-        if (wm & (1u << (8 + q))) atomicAdd(&sy[r][li[q]], acc[q]);
+        if (used_slots & (1u << q)) atomicAdd(&sy[r][li[q]], acc[q]);
     }
   }
   __syncthreads();
@@ -1182,15 +1192,21 @@ static void with_nt(int nt, F&& f) {
   }
 }
 
-// workgroup size and block node count of the cluster apply: blocks of more than 256 nodes run
-// 256 threads
-// blocks of more than 256 nodes stage either way in the FP32 kernels (gathers or entry records)
+// workgroup size and block node count of the cluster apply: blocks of 512 or 1024 nodes run
+// 256 threads, blocks of 2048 nodes 512 threads; blocks of more than 256 nodes stage either way
+// in the FP32 kernels (gathers or entry records)
 template <typename F>
 static void with_cl_shape(const Ctx& c, F&& f) {
   using nt256 = std::integral_constant<int, 256>;
   using rec = std::true_type;
   using gather = std::false_type;
-  if (c.cl_cap == 1024) {
+  if (c.cl_cap == 2048) {
+    using nt512 = std::integral_constant<int, 512>;
+    if (c.cl_records)
+      f(nt512{}, std::integral_constant<int, 2048>{}, rec{});
+    else
+      f(nt512{}, std::integral_constant<int, 2048>{}, gather{});
+  } else if (c.cl_cap == 1024) {
     if (c.cl_records)
       f(nt256{}, std::integral_constant<int, 1024>{}, rec{});
     else

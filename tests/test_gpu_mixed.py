@@ -46,7 +46,7 @@ def test_fp32_vcycle_is_fp64_vcycle_to_fp32_accuracy(mixed_ctx, oracle_c0):
     assert float(r @ z) > 0
 
 
-@pytest.mark.parametrize("nb", [512, 1024])
+@pytest.mark.parametrize("nb", [512, 1024, 2048])
 def test_fp32_vcycle_same_under_both_stagings(mixed_ctx, oracle_c0, nb):
     """The FP32 applies of the V-cycle stage the same values from entry records as from gathers:
     the two V-cycles agree to the rounding of the atomic summation order, and both are the FP64
@@ -128,7 +128,7 @@ def test_mixed_sweep_same_switching_step(small_cube):
     assert np.max(np.abs(Emx - E64) / np.maximum(np.abs(E64), 1.0)) <= 1e-8
 
 
-@pytest.mark.parametrize("blocks", ["256", "1024:records", "1024:gather"])
+@pytest.mark.parametrize("blocks", ["256", "1024:records", "1024:gather", "2048:records"])
 def test_fp32_pcg_meets_the_fp64_tolerance(mixed_ctx, oracle_c0, blocks):
     """The FP32 PCG with FP64 refinement stops on the FP64 true residual: r.r <= tol^2 b.b
     checked by the oracle in FP64, and the solution agrees with the FP64 solve to the tolerance

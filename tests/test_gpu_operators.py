@@ -53,8 +53,9 @@ def test_poisson_apply(ctx_c0, oracle_c0, geom):
 
 
 @pytest.mark.parametrize("shape", ["persistent:1", "persistent:3", "cluster:256", "cluster:64", "element:128",
-                                   "element:64", "cluster:256:512", "cluster:256:1024", "cluster:256:512:gather",
-                                   "cluster:256:1024:gather"])
+                                   "element:64", "cluster:256:512", "cluster:256:1024", "cluster:256:2048",
+                                   "cluster:256:512:gather", "cluster:256:1024:gather",
+                                   "cluster:256:2048:gather"])
 def test_poisson_apply_every_work_shape(ctx_c0, oracle_c0, shape):
     """Every apply shape the autotuner can pick gives the oracle's A x, in one apply and inside
     a PCG solve (the persistent grid writes its x.Ax partials per block); the third field is the

@@ -26,7 +26,7 @@ def solver_ctx(request, cube_c0):
     return request.param, ctx
 
 
-@pytest.mark.parametrize("nb", [256, 512, 1024])
+@pytest.mark.parametrize("nb", [256, 512, 1024, 2048])
 def test_vcycle_matches_oracle(solver_ctx, oracle_c0, nb):
     """The FP64 V-cycle on every cluster block size (its first apply stages x = M r from the
     residual, the second the corrected x)."""
