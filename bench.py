@@ -490,7 +490,7 @@ def main():
                       boundary_mask=mesh["boundary_mask"], block_nodes=args.block_nodes)
     log(f"[bench] context {time.perf_counter() - t0:.1f}s")
     ctx.setup_precond(args.precond, log=log, **amg_kwargs(args))
-    from mumag_matrixfree2_amd.autotune import tune_apply
+    from mumag_matrixfree2_amd.autotune import tune_apply, tune_spmv_lanes
 
     if args.apply_shape == "auto":
         shape = tune_apply(ctx)
@@ -501,6 +501,9 @@ def main():
         ctx.set_apply_shape(parts[0], int(parts[1]))
         shape = {"unit": parts[0], "param": int(parts[1])}
     log(f"[bench] apply shape {shape}")
+    ctx.set_precision(args.precision)
+    lanes = tune_spmv_lanes(ctx)
+    log(f"[bench] V-cycle SpMV lanes {lanes}")
     log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: N={ctx.N} E={ctx.E} "
         f"AMG levels={[l['A'].shape[0] for l in ctx.amg_levels]}")
 
@@ -624,6 +627,7 @@ def main():
             "amg_levels": [int(l["A"].shape[0]) for l in ctx.amg_levels],
             "apply_shape": {"unit": shape["unit"], "param": shape["param"], "block_nodes": ctx.block_nodes,
                             "cluster_block_nodes": ctx.cluster_block_nodes},
+            "spmv_lanes": lanes["lanes"],
             "solve_guess": (f"projection onto the last {args.solve_guess} solutions" if args.solve_guess
                             else "previous solution (reference warm start)"),
             "pcg_tol": 1e-10, "tau_f": 1e-10,

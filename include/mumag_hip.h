@@ -150,6 +150,11 @@ int mm_amg_set_level(int64_t ctx, int l, int n, int n_next, const int32_t* Ap, c
                      const int32_t* Ri, const double* Rx, const double* dense_inv);
 /* kind 1 = amgcl (SPAI0), 2 = amg (damped Jacobi); nlev levels set above. */
 int mm_amg_commit(int64_t ctx, int kind, int nlev);
+/* Lanes per row (4, 8, 16, 32 or 64) of the CSR products of level l: which = 0 its A, 1 its P,
+ * 2 its R. mm_amg_set_level chooses them from the average row length; the SpMV tuner sets them
+ * from measured V-cycle times. Results agree to rounding; no reference counterpart. */
+int mm_amg_set_lanes(int64_t ctx, int l, int which, int lanes);
+int mm_amg_get_lanes(int64_t ctx, int l, int32_t* out3);
 /* Jacobi preconditioner on diag(A) (precond_type 'jacobi', src/poisson_solve.py:409-421). */
 int mm_precond_jacobi(int64_t ctx);
 

@@ -80,6 +80,8 @@ SIGNATURES = {
     "mm_amg_set_level": [C.c_int64, C.c_int, C.c_int, C.c_int, i32p, i32p, f64p, f64p, i32p, i32p, f64p, i32p, i32p,
                          f64p, f64p],
     "mm_amg_commit": [C.c_int64, C.c_int, C.c_int],
+    "mm_amg_set_lanes": [C.c_int64, C.c_int, C.c_int, C.c_int],
+    "mm_amg_get_lanes": [C.c_int64, C.c_int, i32p],
     "mm_precond_jacobi": [C.c_int64],
     "mm_precond_apply": [C.c_int64, f64p, f64p],
     "mm_ctx_get_exchange_diag": [C.c_int64, f64p],

@@ -100,3 +100,41 @@ def tune_rhs_width(ctx, widths=RHS_WIDTHS, reps: int = 20) -> dict:
     best = min(per, key=per.get)
     width = best if per[best] < single else 1
     return {"width": width, "us_per_vector": {k: round(v, 3) for k, v in per.items()}, "single_us": round(single, 3)}
+
+
+SPMV_LANES = (4, 8, 16, 32, 64)
+
+
+def tune_spmv_lanes(ctx, lanes=SPMV_LANES, reps: int = 10, rounds: int = 1) -> dict:
+    """Lanes per row of every CSR product of the V-cycle (each level's A, P and R), chosen by
+    coordinate descent on the measured time of a whole preconditioner application in the
+    context's current precision form: for each product in turn, every lane count is timed with
+    the others fixed and the fastest kept. Returns ``{"lanes": ctx.amg_lanes(), "us_before",
+    "us_after"}``."""
+    levels = ctx.amg_levels or []
+    ctx.bench_precond(reps=2)
+    before = ctx.bench_precond(reps=reps) * 1e3
+    best_us = before
+    for _ in range(rounds):
+        for lev, L in enumerate(levels):
+            for which in ("A", "P", "R"):
+                if which == "A" and (lev == 0 or L.get("A") is None):
+                    continue  # level 0 applies A matrix-free
+                if which in ("P", "R") and L.get("P") is None:
+                    continue  # the coarsest level has no transfer operators
+                if which == "A" and lev == len(levels) - 1:
+                    continue  # the coarsest level is solved with its dense inverse
+                cur = ctx.amg_lanes()[lev][which]
+                times = {}
+                for ln in lanes:
+                    ctx.set_amg_lanes(lev, which, ln)
+                    times[ln] = ctx.bench_precond(reps=reps) * 1e3
+                best = min(times, key=times.get)
+                if times[best] < times.get(cur, float("inf")):
+                    cur = best
+                ctx.set_amg_lanes(lev, which, cur)
+                best_us = times[cur]
+    after = ctx.bench_precond(reps=reps) * 1e3
+    return {"lanes": ctx.amg_lanes(), "us_before": round(before, 2), "us_after": round(after, 2),
+            "us_best_seen": round(best_us, 2)}
+

@@ -364,6 +364,26 @@ class MeshContext:
         call("mm_bench_apply", self.handle, GEOM_MODES[geom], int(reps), C.byref(ms))
         return float(ms.value)
 
+    def bench_precond(self, reps=20, use_graph=False):
+        """Mean ms of one preconditioner application on device-resident vectors (in the context's
+        precision form)."""
+        ms = C.c_double(0.0)
+        call("mm_bench_precond", self.handle, int(bool(use_graph)), int(reps), C.byref(ms))
+        return float(ms.value)
+
+    def amg_lanes(self) -> list[dict]:
+        """Lanes per row of each level's CSR products: [{"A": a, "P": p, "R": r}, ...]."""
+        out = []
+        for lev in range(len(self.amg_levels or [])):
+            v = np.zeros(3, dtype=np.int32)
+            call("mm_amg_get_lanes", self.handle, lev, v.ctypes.data_as(C.POINTER(C.c_int32)))
+            out.append({"A": int(v[0]), "P": int(v[1]), "R": int(v[2])})
+        return out
+
+    def set_amg_lanes(self, level: int, which: str, lanes: int):
+        """Lanes per row (4..64) of level ``level``'s ``which`` product ("A", "P" or "R")."""
+        call("mm_amg_set_lanes", self.handle, int(level), {"A": 0, "P": 1, "R": 2}[which], int(lanes))
+
 
 def _info_dict(info) -> dict:
     return {

@@ -1200,6 +1200,26 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
   return 0;
 }
 
+int mm_amg_set_lanes(int64_t h, int l, int which, int lanes) {
+  GET_CTX(h);
+  MM_CHECK(l >= 0 && l < (int)c.lv.size(), "level index out of range");
+  MM_CHECK(which >= 0 && which <= 2, "which must be 0 (A), 1 (P) or 2 (R)");
+  MM_CHECK(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64,
+           "lanes per row must be 4, 8, 16, 32 or 64");
+  Level& L = c.lv[l];
+  (which == 0 ? L.lanesA : which == 1 ? L.lanesP : L.lanesR) = lanes;
+  return 0;
+}
+
+int mm_amg_get_lanes(int64_t h, int l, int32_t* out3) {
+  GET_CTX(h);
+  MM_CHECK(l >= 0 && l < (int)c.lv.size() && out3, "level index out of range");
+  out3[0] = c.lv[l].lanesA;
+  out3[1] = c.lv[l].lanesP;
+  out3[2] = c.lv[l].lanesR;
+  return 0;
+}
+
 int mm_amg_commit(int64_t h, int kind, int nlev) {
   GET_CTX(h);
   c.blist_key.clear();  // segment subsets recompute the row runs of the new levels
@@ -1681,6 +1701,7 @@ int mm_bench_precond(int64_t h, int use_graph, int reps, double* ms) {
   if (use_graph) {
     MM_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     vfill(c.pt, 0.0, c.N, c.stream);
+    if (c.ptf) vfill_f(c.ptf, 0.0f, c.N, c.stream);
     MM_TRY(precond(c, c.pr, c.pz, c.part_rz, c.stream));
     MM_HIP(hipStreamEndCapture(c.stream, &graph));
     MM_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1692,6 +1713,7 @@ int mm_bench_precond(int64_t h, int use_graph, int reps, double* ms) {
       MM_HIP(hipGraphLaunch(exec, c.stream));
     } else {
       vfill(c.pt, 0.0, c.N, c.stream);
+      if (c.ptf) vfill_f(c.ptf, 0.0f, c.N, c.stream);
       MM_TRY(precond(c, c.pr, c.pz, c.part_rz, c.stream));
     }
   }

@@ -328,6 +328,7 @@ int launch_jpar(const Ctx& c, const double* m, const double h[3], double* out, h
 
 // vector kernels (mm_vec.hip)
 int vfill(double* x, double v, long n, hipStream_t s);
+int vfill_f(float* x, float v, long n, hipStream_t s);
 int vcopy(double* dst, const double* src, long n, hipStream_t s);
 int launch_copy_calib(double* dst, const double* src, long n, hipStream_t s);
 // out[0] = a.b (deterministic)

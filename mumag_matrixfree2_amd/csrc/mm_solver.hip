@@ -24,6 +24,10 @@ __global__ void k_fill(double* __restrict__ x, double v, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
 }
 
+__global__ void k_fill_f(float* __restrict__ x, float v, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
+}
+
 __global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) d[i] = s[i];
 }
@@ -49,6 +53,11 @@ static inline int vgrid(long n) {
 
 int vfill(double* x, double v, long n, hipStream_t s) {
   if (n > 0) k_fill<<<vgrid(n), kBlock, 0, s>>>(x, v, n);
+  return 0;
+}
+
+int vfill_f(float* x, float v, long n, hipStream_t s) {
+  if (n > 0) k_fill_f<<<vgrid(n), kBlock, 0, s>>>(x, v, n);
   return 0;
 }
 int launch_copy_calib(double* d, const double* src, long n, hipStream_t s) {

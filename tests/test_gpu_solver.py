@@ -44,6 +44,49 @@ def test_vcycle_matches_oracle(solver_ctx, oracle_c0, nb):
     assert float(r @ z) > 0.0
 
 
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+def test_vcycle_matches_oracle_at_every_lane_count(solver_ctx, oracle_c0, lanes):
+    """The CSR products of the coarse levels give the same V-cycle with any lanes per row (the
+    values the SpMV tuner chooses between)."""
+    from oracle.solvers import vcycle
+
+    kind, ctx = solver_ctx
+    saved = ctx.amg_lanes()
+    r = np.random.default_rng(13).standard_normal(ctx.N)
+    try:
+        for lev, L in enumerate(saved):
+            for which in ("A", "P", "R"):
+                ctx.set_amg_lanes(lev, which, lanes)
+        assert all(v == lanes for d in ctx.amg_lanes() for v in d.values())
+        z = ctx.precond_apply(r)
+    finally:
+        for lev, L in enumerate(saved):
+            for which, v in L.items():
+                ctx.set_amg_lanes(lev, which, v)
+    z_ref = vcycle(oracle_levels(ctx.amg_levels, kind), oracle_c0.apply_A, r, kind)
+    assert rel(z, z_ref) <= 1e-11
+
+
+def test_spmv_lane_tuner_keeps_the_vcycle(solver_ctx, oracle_c0):
+    from mumag_matrixfree2_amd.autotune import tune_spmv_lanes
+    from oracle.solvers import vcycle
+
+    kind, ctx = solver_ctx
+    saved = ctx.amg_lanes()
+    try:
+        res = tune_spmv_lanes(ctx, reps=3)
+        print(f"[{kind}] spmv lanes {res}", flush=True)
+        assert len(res["lanes"]) == len(ctx.amg_levels)
+        r = np.random.default_rng(14).standard_normal(ctx.N)
+        z = ctx.precond_apply(r)
+    finally:
+        for lev, L in enumerate(saved):
+            for which, v in L.items():
+                ctx.set_amg_lanes(lev, which, v)
+    z_ref = vcycle(oracle_levels(ctx.amg_levels, kind), oracle_c0.apply_A, r, kind)
+    assert rel(z, z_ref) <= 1e-11
+
+
 def test_solve_U_matches_oracle_pcg(solver_ctx, oracle_c0):
     from oracle.solvers import solve_U
 
