@@ -150,7 +150,7 @@ int mm_amg_set_level(int64_t ctx, int l, int n, int n_next, const int32_t* Ap, c
                      const int32_t* Ri, const double* Rx, const double* dense_inv);
 /* kind 1 = amgcl (SPAI0), 2 = amg (damped Jacobi); nlev levels set above. */
 int mm_amg_commit(int64_t ctx, int kind, int nlev);
-/* Lanes per row (4, 8, 16, 32 or 64) of the CSR products of level l: which = 0 its A, 1 its P,
+/* Lanes per row (1, 2, 4, 8, 16, 32 or 64) of the CSR products of level l: which = 0 its A, 1 its P,
  * 2 its R. mm_amg_set_level chooses them from the average row length; the SpMV tuner sets them
  * from measured V-cycle times. Results agree to rounding; no reference counterpart. */
 int mm_amg_set_lanes(int64_t ctx, int l, int which, int lanes);

@@ -102,7 +102,7 @@ def tune_rhs_width(ctx, widths=RHS_WIDTHS, reps: int = 20) -> dict:
     return {"width": width, "us_per_vector": {k: round(v, 3) for k, v in per.items()}, "single_us": round(single, 3)}
 
 
-SPMV_LANES = (4, 8, 16, 32, 64)
+SPMV_LANES = (1, 2, 4, 8, 16, 32, 64)
 
 
 def tune_spmv_lanes(ctx, lanes=SPMV_LANES, reps: int = 10, rounds: int = 1) -> dict:

@@ -381,7 +381,7 @@ class MeshContext:
         return out
 
     def set_amg_lanes(self, level: int, which: str, lanes: int):
-        """Lanes per row (4..64) of level ``level``'s ``which`` product ("A", "P" or "R")."""
+        """Lanes per row (1..64) of level ``level``'s ``which`` product ("A", "P" or "R")."""
         call("mm_amg_set_lanes", self.handle, int(level), {"A": 0, "P": 1, "R": 2}[which], int(lanes))
 
 

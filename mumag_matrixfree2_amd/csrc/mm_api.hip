@@ -1204,8 +1204,8 @@ int mm_amg_set_lanes(int64_t h, int l, int which, int lanes) {
   GET_CTX(h);
   MM_CHECK(l >= 0 && l < (int)c.lv.size(), "level index out of range");
   MM_CHECK(which >= 0 && which <= 2, "which must be 0 (A), 1 (P) or 2 (R)");
-  MM_CHECK(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64,
-           "lanes per row must be 4, 8, 16, 32 or 64");
+  MM_CHECK(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64,
+           "lanes per row must be 1, 2, 4, 8, 16, 32 or 64");
   Level& L = c.lv[l];
   (which == 0 ? L.lanesA : which == 1 ? L.lanesP : L.lanesR) = lanes;
   return 0;

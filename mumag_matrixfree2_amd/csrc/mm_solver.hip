@@ -338,6 +338,8 @@ static void spmv_lanes(int lanes, int row0, int row1, const int* Ap, const int* 
                        const TX2* x2, TY* y, TY* z, const TY* zw, hipStream_t s) {
   const int grid = (int)(((long)(row1 - row0) * lanes + kBlock - 1) / kBlock);
   switch (lanes) {
+    case 1: k_spmv<MODE, 1, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    case 2: k_spmv<MODE, 2, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
     case 4: k_spmv<MODE, 4, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
     case 8: k_spmv<MODE, 8, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
     case 16: k_spmv<MODE, 16, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
@@ -378,6 +380,8 @@ static void spmv_post(int lanes, int nrows, const int* Ap, const int* Ai, const 
   for_runs(runs, nrows, [&](int r0, int r1) {
     const int grid = (int)(((long)(r1 - r0) * lanes + kBlock - 1) / kBlock);
     switch (lanes) {
+      case 1: k_spmv_post<1, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      case 2: k_spmv_post<2, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
       case 4: k_spmv_post<4, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
       case 8: k_spmv_post<8, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
       case 16: k_spmv_post<16, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;

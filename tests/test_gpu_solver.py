@@ -44,7 +44,7 @@ def test_vcycle_matches_oracle(solver_ctx, oracle_c0, nb):
     assert float(r @ z) > 0.0
 
 
-@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32, 64])
 def test_vcycle_matches_oracle_at_every_lane_count(solver_ctx, oracle_c0, lanes):
     """The CSR products of the coarse levels give the same V-cycle with any lanes per row (the
     values the SpMV tuner chooses between)."""
