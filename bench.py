@@ -451,6 +451,9 @@ def main():
     ap.add_argument("--solve-guess", type=int, default=8,
                     help="initial guess of the minimiser's Poisson solves: projection onto the last K solutions "
                          "(0: the previous solution, as the reference)")
+    ap.add_argument("--spmv-lanes", default="auto",
+                    help='lanes per row of the V-cycle CSR products: "auto" (timed at set-up) or a JSON list '
+                         'of per-level {"A": a, "P": p, "R": r}')
     ap.add_argument("--max-iter", type=int, default=2000,
                     help="minimiser iterations per field step at most (the workload's is 2000; lower only to "
                          "bound profiler runs)")
@@ -502,7 +505,13 @@ def main():
         shape = {"unit": parts[0], "param": int(parts[1])}
     log(f"[bench] apply shape {shape}")
     ctx.set_precision(args.precision)
-    lanes = tune_spmv_lanes(ctx)
+    if args.spmv_lanes == "auto":
+        lanes = tune_spmv_lanes(ctx)
+    else:
+        for lev, d in enumerate(json.loads(args.spmv_lanes)):
+            for which, v in d.items():
+                ctx.set_amg_lanes(lev, which, int(v))
+        lanes = {"lanes": ctx.amg_lanes()}
     log(f"[bench] V-cycle SpMV lanes {lanes}")
     log(f"[bench] set-up {time.perf_counter() - t0:.1f}s: N={ctx.N} E={ctx.E} "
         f"AMG levels={[l['A'].shape[0] for l in ctx.amg_levels]}")
