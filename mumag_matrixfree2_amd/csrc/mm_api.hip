@@ -55,23 +55,36 @@ static Ctx* get_ctx(int64_t h) {
   return g_ctx[h - 1];
 }
 
+// Every fill or copy into a new buffer runs on the context's stream and has completed before the
+// buffer is handed out. The stream is created non-blocking, so it does not wait for null-stream
+// work: a null-stream hipMemset (asynchronous to the host) issued here could land after the
+// stream's first kernel had written the buffer and wipe its results.
 template <typename T>
-static int dalloc(Ctx& c, T** p, size_t n) {
+static int dalloc_raw(Ctx& c, T** p, size_t n) {
   if (n == 0) n = 1;
   void* q = nullptr;
   MM_HIP(hipMalloc(&q, n * sizeof(T)));
   c.allocs.push_back(q);
-  // zeroed once: a segmented solve leaves the entries of segments it does not iterate
-  // untouched, and they must hold finite values (the coarsest level multiplies whole vectors)
-  MM_HIP(hipMemset(q, 0, n * sizeof(T)));
   *p = (T*)q;
   return 0;
 }
 
 template <typename T>
+static int dalloc(Ctx& c, T** p, size_t n) {
+  MM_TRY(dalloc_raw(c, p, n));
+  // zeroed once: a segmented solve leaves the entries of segments it does not iterate
+  // untouched, and they must hold finite values (the coarsest level multiplies whole vectors)
+  MM_HIP(hipMemsetAsync(*p, 0, std::max<size_t>(n, 1) * sizeof(T), c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+template <typename T>
 static int upload(Ctx& c, T** p, const T* h, size_t n) {
-  MM_TRY(dalloc(c, p, n));
-  if (n) MM_HIP(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  if (n == 0) return dalloc(c, p, n);
+  MM_TRY(dalloc_raw(c, p, n));
+  MM_HIP(hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
   return 0;
 }
 
@@ -454,7 +467,8 @@ static int upload_cluster_part(Ctx& c, int cap, const std::vector<int>& seg_of, 
   std::vector<double> hk;
   if (!knt_host) {
     hk.resize(3 * (size_t)c.N);
-    MM_HIP(hipMemcpy(hk.data(), c.knt, hk.size() * sizeof(double), hipMemcpyDeviceToHost));
+    MM_HIP(hipMemcpyAsync(hk.data(), c.knt, hk.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    MM_HIP(hipStreamSynchronize(c.stream));
     knt_host = hk.data();
   }
   const double* kh = knt_host;
@@ -768,7 +782,6 @@ static int create(int N, int E, const double* knt, const int32_t* conn_in, const
   const int max_grid = std::max({kRedBlocks, 8 * ((c.part_all.nblk + 7) / 8), 8 * ((c.part_mag.nblk + 7) / 8)});
   MM_TRY(dalloc(c, &c.partial, 4 * (size_t)max_grid));
   MM_TRY(dalloc(c, &c.dscal, kNumScal));
-  MM_HIP(hipMemset(c.dscal, 0, kNumScal * sizeof(double)));
   MM_HIP(hipHostMalloc((void**)&c.hscal, kNumScal * sizeof(double), hipHostMallocDefault));
   MM_TRY(dalloc(c, &c.part_pq, (size_t)8 * ((std::max(c.part_all.nblk, c.part_cl.nblk) + 7) / 8) + kRedGridMax));
   MM_TRY(dalloc(c, &c.part_rr, 2 * (size_t)kRedGridMax));
@@ -1242,7 +1255,8 @@ int mm_precond_jacobi(int64_t h) {
   GET_CTX(h);
   MM_TRY(mm_poisson_diag(h, nullptr));
   std::vector<double> d(c.N);
-  MM_HIP(hipMemcpy(d.data(), c.diagA, c.N * sizeof(double), hipMemcpyDeviceToHost));
+  MM_HIP(hipMemcpyAsync(d.data(), c.diagA, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
   for (auto& v : d) v = 1.0 / (v + 1e-30);
   c.lv.clear();
   c.lv.resize(1);

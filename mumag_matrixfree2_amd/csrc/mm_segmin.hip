@@ -532,7 +532,8 @@ static int seg_min_alloc(Ctx& c) {
   }
   MM_HIP(hipMalloc((void**)&c.sm_tab, (size_t)c.nseg * kSegSlots * sizeof(double)));
   c.allocs.push_back(c.sm_tab);
-  MM_HIP(hipMemset(c.sm_tab, 0, (size_t)c.nseg * kSegSlots * sizeof(double)));
+  // on the context's stream (non-blocking: a null-stream fill would not be ordered with its kernels)
+  MM_HIP(hipMemsetAsync(c.sm_tab, 0, (size_t)c.nseg * kSegSlots * sizeof(double), c.stream));
   MM_HIP(hipMalloc((void**)&c.sm_in, (size_t)c.nseg * kSegIn * sizeof(double)));
   c.allocs.push_back(c.sm_in);
   return 0;

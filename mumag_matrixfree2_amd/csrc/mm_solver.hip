@@ -632,7 +632,9 @@ int ensure_seg(Ctx& c) {
   auto up = [&](int** d, const std::vector<int>& h) -> int {
     MM_HIP(hipMalloc((void**)d, h.size() * sizeof(int)));
     c.allocs.push_back(*d);
-    MM_HIP(hipMemcpy(*d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+    // on the context's stream and completed here (the stream does not wait for null-stream copies)
+    MM_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    MM_HIP(hipStreamSynchronize(c.stream));
     return 0;
   };
   MM_TRY(up(&c.d_seg_node_off, c.h_seg_node_off));

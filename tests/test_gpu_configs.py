@@ -169,6 +169,12 @@ def test_full_size_polycrystal_size_independent_properties():
                       boundary_mask=mesh["boundary_mask"])
     print(f"[poly25M] N={ctx.N} E={ctx.E}", flush=True)
     assert ctx.E == 24_576_000
+    # the geometry the context computed on the device and copied back (the AMG set-up assembles
+    # its matrix from it) holds every element: the same volumes as the host's own determinants
+    _, Ve_ctx, _ = ctx.geometry()
+    _, Ve_host, _ = ctx.element_table()
+    assert np.array_equal(np.sort(Ve_ctx) > 0, np.ones(ctx.E, dtype=bool))
+    assert np.max(np.abs(np.sort(Ve_ctx) - np.sort(Ve_host))) <= 1e-12 * Ve_host.max()
     tuned = tune_apply(ctx, reps=5)
     print(f"[poly25M] apply shape {tuned['unit']}:{tuned['param']}:{tuned['cluster_block_nodes']}", flush=True)
     rng = np.random.default_rng(80)

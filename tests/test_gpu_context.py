@@ -56,3 +56,22 @@ def test_malformed_mesh_is_rejected(small_cube, case):
         knt[ijk[13, 1]] = np.nan
     with pytest.raises(RuntimeError, match=msg):
         _create(knt, ijk, small_cube["boundary_mask"])
+
+
+def test_geometry_copy_holds_every_element_on_recycled_memory():
+    """Contexts created one after another, each on device memory the previous one freed: the
+    element volumes each context computes on the device and copies back (the AMG set-up assembles
+    its matrix from them) equal the host's own determinants, element for element."""
+    from mumag_matrixfree2_amd.context import MeshContext
+    from mumag_matrixfree2_amd.mesh import config_cube_mesh
+
+    mesh = config_cube_mesh(40.0, 20, 3)
+    A, K1, Js, k = ndfeb_materials()
+    for rep in range(4):
+        ctx = MeshContext(mesh["knt"], mesh["ijk"][:, :4], mesh["ijk"][:, 4], A, K1, Js, k,
+                          boundary_mask=mesh["boundary_mask"])
+        _, Ve_ctx, _ = ctx.geometry()
+        _, Ve_host, _ = ctx.element_table()
+        ctx.close()
+        assert np.all(Ve_ctx > 0), rep
+        assert np.max(np.abs(np.sort(Ve_ctx) - np.sort(Ve_host))) <= 1e-12 * Ve_host.max(), rep
