@@ -446,8 +446,8 @@ def main():
     ap.add_argument("--block-nodes", type=int, default=256,
                     help="distinct nodes per element block (16..256): the LDS staging size of the element kernels")
     ap.add_argument("--apply-shape", default="auto",
-                    help="'auto' (autotuned at set-up) or unit:param[:cluster block nodes], e.g. cluster:256, "
-                         "cluster:256:1024, element:128, persistent:2")
+                    help="'auto' (autotuned at set-up) or unit:param[:cluster block nodes[:scatter]], e.g. "
+                         "cluster:256, cluster:256:1024, cluster:256:1024:colour, element:128, persistent:2")
     ap.add_argument("--solve-guess", type=int, default=8,
                     help="initial guess of the minimiser's Poisson solves: projection onto the last K solutions "
                          "(0: the previous solution, as the reference)")
@@ -499,9 +499,11 @@ def main():
         shape = tune_apply(ctx)
     else:
         parts = args.apply_shape.split(":")
-        if len(parts) == 3:
+        if len(parts) >= 3:
             ctx.set_cluster_block_nodes(int(parts[2]))
         ctx.set_apply_shape(parts[0], int(parts[1]))
+        if len(parts) == 4:
+            ctx.set_apply_scatter(parts[3])
         shape = {"unit": parts[0], "param": int(parts[1])}
     log(f"[bench] apply shape {shape}")
     ctx.set_precision(args.precision)
@@ -635,7 +637,7 @@ def main():
             "tets": ctx.E, "nodes": ctx.N, "method": args.method, "precond": f"SA-AMG V-cycle ({args.precond})",
             "amg_levels": [int(l["A"].shape[0]) for l in ctx.amg_levels],
             "apply_shape": {"unit": shape["unit"], "param": shape["param"], "block_nodes": ctx.block_nodes,
-                            "cluster_block_nodes": ctx.cluster_block_nodes},
+                            "cluster_block_nodes": ctx.cluster_block_nodes, "scatter": ctx.apply_scatter},
             "spmv_lanes": lanes["lanes"],
             "solve_guess": (f"projection onto the last {args.solve_guess} solutions" if args.solve_guess
                             else "previous solution (reference warm start)"),

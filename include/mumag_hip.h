@@ -77,10 +77,11 @@ int mm_ctx_create_segmented(int N, int E, const double* knt, const int32_t* conn
                             const double* bnd_mask, double reg, int flags, int nseg, const int32_t* seg_node_off,
                             int64_t* out_ctx);
 
-/* info[0..13] = V_mag, N, E, element blocks (all), element blocks (magnetic),
+/* info[0..14] = V_mag, N, E, element blocks (all), element blocks (magnetic),
  * largest node list of a block, magnetic elements, node-list entries of all blocks,
  * cluster blocks, node-list entries of the cluster blocks, edge-star clusters, block node cap,
- * cluster block node cap, cluster staging mode (mm_set_cluster_staging) */
+ * cluster block node cap, cluster staging mode (mm_set_cluster_staging), block colours of the
+ * colour-sequenced scatter (0 when off; mm_set_apply_scatter) */
 int mm_ctx_info(int64_t ctx, double* info);
 
 /* Oriented connectivity (E,4), Ve (E,) and shape-function gradients of corners 1..3 (E,3,3)
@@ -127,6 +128,13 @@ int mm_set_cluster_block_nodes(int64_t ctx, int nb);
  * node-list entry kept on the device). Same staged values, same results; the FP64 apply always
  * gathers. A tuning knob with no reference counterpart. */
 int mm_set_cluster_staging(int64_t ctx, int mode);
+/* How the cluster apply adds the block sums of nodes that several blocks touch: 0 (default) =
+ * hardware FP64 atomics (global_atomic_add_f64) in one launch; 1 = colour-sequenced: the blocks
+ * are coloured so that no two blocks of a colour share a node and run as one launch per colour,
+ * adding shared-node sums with plain loads and stores (no global atomics; the sums inside a block
+ * still come from LDS atomics). Not on segmented contexts or the persistent grid. A tuning knob with no reference
+ * counterpart: the two scatters the path may use, kept selectable so either can be measured. */
+int mm_set_apply_scatter(int64_t ctx, int mode);
 
 /* y = A x: assembled-mode apply_A (src/poisson_solve.py:166-171) computed matrix-free. */
 int mm_poisson_apply(int64_t ctx, const double* x, double* y);

@@ -29,11 +29,18 @@ SHAPES = (("persistent", 2), ("persistent", 3), ("cluster", 256), ("cluster", 12
 CLUSTER_BLOCK_NODES = (512, 1024, 2048)
 
 
-def tune_apply(ctx, shapes=SHAPES, reps: int = 40, cluster_block_nodes=CLUSTER_BLOCK_NODES) -> dict:
+SCATTER_MODES = ("atomic", "colour")
+
+
+def tune_apply(ctx, shapes=SHAPES, reps: int = 40, cluster_block_nodes=CLUSTER_BLOCK_NODES,
+               scatter_modes=SCATTER_MODES) -> dict:
     """Time every (unit, param) shape on ``ctx`` with 256-node cluster blocks, then 256-thread
     cluster applies on ``cluster_block_nodes``-node blocks (not on segmented contexts); keep the
-    fastest on the context and return ``{"unit", "param", "cluster_block_nodes",
-    "us_per_apply": {"unit:param[:nodes]": microseconds}}``."""
+    fastest on the context; when it is a cluster shape, also time the colour-sequenced scatter on
+    it and keep whichever scatter is faster. Returns ``{"unit", "param", "cluster_block_nodes",
+    "scatter", "us_per_apply": {"unit:param[:nodes][:scatter]": microseconds}}``."""
+    if getattr(ctx, "apply_scatter", "atomic") != "atomic":
+        ctx.set_apply_scatter("atomic")
     times = {}
     if getattr(ctx, "cluster_block_nodes", 256) != 256:
         ctx.set_cluster_block_nodes(256)
@@ -51,9 +58,23 @@ def tune_apply(ctx, shapes=SHAPES, reps: int = 40, cluster_block_nodes=CLUSTER_B
     if ctx.segments is None:
         ctx.set_cluster_block_nodes(best[2])
     ctx.set_apply_shape(best[0], best[1])
-    return {"unit": best[0], "param": best[1], "cluster_block_nodes": best[2],
-            "us_per_apply": {f"{u}:{q}" + ("" if nb == 256 else f":{nb}"): round(v, 3)
-                             for (u, q, nb), v in times.items()}}
+    us = {f"{u}:{q}" + ("" if nb == 256 else f":{nb}"): round(v, 3) for (u, q, nb), v in times.items()}
+    scatter = "atomic"
+    if scatter_modes and ctx.segments is None and best[0] == "cluster":
+        # the colour-sequenced scatter against the FP64 atomics, on the chosen shape
+        key = f"{best[0]}:{best[1]}" + ("" if best[2] == 256 else f":{best[2]}")
+        for mode in scatter_modes:
+            if mode == "atomic":
+                continue
+            ctx.set_apply_scatter(mode)
+            ctx.bench_apply(reps=3)
+            t = ctx.bench_apply(reps=reps) * 1e3
+            us[f"{key}:{mode}"] = round(t, 3)
+            if t < times[best]:
+                scatter, times[best] = mode, t
+        ctx.set_apply_scatter(scatter)
+    return {"unit": best[0], "param": best[1], "cluster_block_nodes": best[2], "scatter": scatter,
+            "us_per_apply": us}
 
 
 def tune_block_nodes(make_ctx, caps=(128, 192, 256), reps: int = 40):

@@ -74,7 +74,7 @@ class MeshContext:
                  ptr(K1_red), ptr(Js_red), ptr(k_easy), ptr(bnd), self.reg, flags, self.segments.size - 1,
                  ptr(self.segments), C.byref(h))
         self.handle = h.value
-        info = np.zeros(14)
+        info = np.zeros(15)
         call("mm_ctx_info", self.handle, ptr(info))
         self.V_mag = float(info[0])
         self.n_blocks_all = int(info[3])
@@ -88,6 +88,8 @@ class MeshContext:
         self.block_nodes = int(info[11])
         self.cluster_block_nodes = int(info[12])
         self.cluster_staging = "records" if info[13] else "gather"
+        self.apply_scatter = "atomic"
+        self.n_block_colours = 0
         self.apply_shape = {"unit": "cluster", "param": 256}
         self.solve_guess = 0
         self.precond_type = None
@@ -157,15 +159,28 @@ class MeshContext:
         """Distinct nodes per cluster block of the Poisson apply (256, 512, 1024 or 2048);
         rebuilds the cluster partition and refreshes the partition counts."""
         call("mm_set_cluster_block_nodes", self.handle, int(nb))
-        info = np.zeros(14)
-        call("mm_ctx_info", self.handle, ptr(info))
-        self.n_blocks_cl = int(info[8])
-        self.cluster_node_list_entries = int(info[9])
-        self.cluster_block_nodes = int(info[12])
+        self._refresh_cluster_info()
         if self.cluster_block_nodes != 256 and self.apply_shape["unit"] == "persistent":
             self.apply_shape = {"unit": "cluster", "param": 256}
         if self.cluster_block_nodes != 256 and self.apply_shape["unit"] == "cluster":
             self.apply_shape = {"unit": "cluster", "param": 256}
+
+    def set_apply_scatter(self, mode: str):
+        """How the cluster apply adds the sums of nodes that several blocks touch: "atomic" (the
+        default: hardware FP64 atomics, one launch) or "colour" (blocks coloured so that no two
+        blocks of a colour share a node, one launch per colour, plain loads and stores). Same
+        operator to rounding."""
+        call("mm_set_apply_scatter", self.handle, {"atomic": 0, "colour": 1}[mode])
+        self.apply_scatter = mode
+        self._refresh_cluster_info()
+
+    def _refresh_cluster_info(self):
+        info = np.zeros(15)
+        call("mm_ctx_info", self.handle, ptr(info))
+        self.n_blocks_cl = int(info[8])
+        self.cluster_node_list_entries = int(info[9])
+        self.cluster_block_nodes = int(info[12])
+        self.n_block_colours = int(info[14])
 
     def set_cluster_staging(self, mode: str):
         """How the FP32 applies on cluster blocks of 512 to 2048 nodes stage their nodes: "gather"

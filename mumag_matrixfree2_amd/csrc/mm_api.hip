@@ -524,6 +524,68 @@ static int upload_cluster_part(Ctx& c, int cap, const std::vector<int>& seg_of, 
   return 0;
 }
 
+// Greedy colouring of the cluster blocks for the colour-sequenced scatter: blocks that share a
+// node get different colours (each block takes the smallest colour none of its already coloured
+// neighbours holds, in block order, so a colour's blocks stay in spatial order). The blocks are
+// uploaded colour by colour; the colours run as consecutive launches on the context's stream.
+int build_block_colours(Ctx& c) {
+  const Part& p = c.part_cl;
+  const int nb = p.nblk;
+  std::vector<int> nptr(nb + 1);
+  std::vector<uint32_t> nodes((size_t)c.nodes_cl);
+  MM_HIP(hipStreamSynchronize(c.stream));
+  MM_HIP(hipMemcpyAsync(nptr.data(), p.nptr, nptr.size() * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  if (!nodes.empty())
+    MM_HIP(hipMemcpyAsync(nodes.data(), p.nodes, nodes.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+  MM_HIP(hipStreamSynchronize(c.stream));
+  // blocks touching each shared node, in block order (CSR over the nodes)
+  std::vector<long long> off((size_t)c.N + 1, 0);
+  for (int b = 0; b < nb; ++b)
+    for (int e = nptr[b]; e < nptr[b + 1]; ++e)
+      if (!(nodes[e] & kPrivate)) off[(nodes[e] & kIdMask) + 1]++;
+  for (int v = 0; v < c.N; ++v) off[v + 1] += off[v];
+  std::vector<int> blk((size_t)off[c.N]);
+  std::vector<long long> fill(off.begin(), off.end() - 1);
+  for (int b = 0; b < nb; ++b)
+    for (int e = nptr[b]; e < nptr[b + 1]; ++e)
+      if (!(nodes[e] & kPrivate)) blk[fill[nodes[e] & kIdMask]++] = b;
+  std::vector<int> colour(nb, -1), stamp;
+  int ncol = 0;
+  for (int b = 0; b < nb; ++b) {
+    for (int e = nptr[b]; e < nptr[b + 1]; ++e) {
+      if (nodes[e] & kPrivate) continue;
+      const uint32_t v = nodes[e] & kIdMask;
+      for (long long k = off[v]; k < off[v + 1]; ++k) {
+        const int cb = colour[blk[k]];
+        if (cb >= 0) stamp[cb] = b;  // a neighbour coloured earlier
+      }
+    }
+    int col = 0;
+    while (col < ncol && stamp[col] == b) ++col;
+    if (col == ncol) {
+      stamp.push_back(-1);
+      ++ncol;
+    }
+    colour[b] = col;
+  }
+  std::vector<int> coff(ncol + 1, 0), order(nb);
+  for (int b = 0; b < nb; ++b) coff[colour[b] + 1]++;
+  for (int k = 0; k < ncol; ++k) coff[k + 1] += coff[k];
+  std::vector<int> at(coff.begin(), coff.end() - 1);
+  for (int b = 0; b < nb; ++b) order[at[colour[b]]++] = b;
+  if (c.d_colour_blk) {
+    auto it = std::find(c.allocs.begin(), c.allocs.end(), (void*)c.d_colour_blk);
+    if (it != c.allocs.end()) {
+      (void)hipFree(*it);
+      c.allocs.erase(it);
+    }
+    c.d_colour_blk = nullptr;
+  }
+  MM_TRY(upload(c, &c.d_colour_blk, order.data(), order.size()));
+  c.h_colour_off = coff;
+  return 0;
+}
+
 static int upload_part(Ctx& c, const HostPart& hp, int E, Part& p) {
   p.nblk = (int)hp.eptr.size() - 1;
   p.E = E;
@@ -937,6 +999,7 @@ int mm_ctx_info(int64_t h, double* info) {
   info[11] = c.node_cap;
   info[12] = c.cl_cap;
   info[13] = c.cl_records ? 1 : 0;
+  info[14] = c.apply_seq ? (double)c.h_colour_off.size() - 1 : 0.0;
   return 0;
 }
 
@@ -989,6 +1052,7 @@ int mm_set_cluster_block_nodes(int64_t h, int nb) {
   MM_HIP(hipStreamSynchronize(c.stream));
   HostCPart hc;
   MM_TRY(upload_cluster_part(c, nb, std::vector<int>(c.N, 0), hc, nullptr));
+  if (c.apply_seq) MM_TRY(build_block_colours(c));
   if (nb != kMaxBlockNodes && c.apply_persist > 0) {
     c.apply_persist = 0;  // the persistent grid stages 256-node blocks
     c.elem_threads = 256;
@@ -1001,6 +1065,15 @@ int mm_set_cluster_staging(int64_t h, int mode) {
   GET_CTX(h);
   MM_CHECK(mode == 0 || mode == 1, "cluster staging must be 0 (gathers) or 1 (entry records)");
   c.cl_records = mode == 1;
+  return 0;
+}
+
+int mm_set_apply_scatter(int64_t h, int mode) {
+  GET_CTX(h);
+  MM_CHECK(mode == 0 || mode == 1, "scatter must be 0 (FP64 atomics) or 1 (colour-sequenced)");
+  MM_CHECK(mode == 0 || c.nseg == 1, "segmented contexts keep the atomic scatter");
+  c.apply_seq = mode == 1;
+  if (c.apply_seq) MM_TRY(build_block_colours(c));
   return 0;
 }
 

@@ -96,6 +96,10 @@ struct Part {
   // workgroup position i < nlist works on block blist[i]; null: every block
   const int* blist = nullptr;
   int nlist = 0;
+  // colour-sequenced launch (cluster apply): no two blocks of the launch share a node, so a
+  // shared node's block sum is added with a plain load and store instead of an FP64 atomic;
+  // the launches of the colours follow each other on one stream
+  int seq = 0;
 };
 
 // Workgroup tables of a segmented vector kernel over a subset of the segments: workgroup w
@@ -179,6 +183,11 @@ struct Ctx {
   int node_cap = kMaxBlockNodes;      // distinct nodes per element block
   int cl_cap = kMaxBlockNodes;        // distinct nodes per cluster block (256, 512, 1024 or 2048)
   bool cl_records = true;             // FP32 applies on 512..2048-node cluster blocks stage entry records
+  // colour-sequenced scatter of the cluster apply (instead of FP64 atomics on shared nodes): the
+  // cluster blocks grouped by colour (no two blocks of a colour share a node), one launch each
+  bool apply_seq = false;
+  int* d_colour_blk = nullptr;        // part_cl.nblk block ids, colour by colour, block order inside
+  std::vector<int> h_colour_off;      // colours + 1 offsets into d_colour_blk
   std::vector<int> h_cl_nodes;        // host copy of the clusters: 8 slots each (-1 unused)
   std::vector<uint32_t> h_cl_tets;    // tet mask of each cluster
   std::vector<char> h_is_bnd;         // Dirichlet flag per node
@@ -306,6 +315,10 @@ int resolve_apply_timing(Ctx& c);
 int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t s, double* dot_part = nullptr);
 // number of x . (A x) partials the on-the-fly apply writes
 int apply_dot_count(const Ctx& c);
+// greedy colouring of the cluster blocks (blocks sharing a node get different colours), uploaded
+// to d_colour_blk / h_colour_off; called when the colour-sequenced scatter is switched on and
+// whenever the cluster partition is rebuilt while it is on
+int build_block_colours(Ctx& c);
 int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s);
 // y = A (w .* r) with xo = w .* r written by the node owners (fused pre-smoothing); y zeroed first
 int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* xo, double* y, hipStream_t s);

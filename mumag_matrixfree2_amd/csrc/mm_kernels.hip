@@ -440,6 +440,8 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
     }
     if (f & kPrivate)
       y[id] = (T)v;
+    else if (p.seq)
+      y[id] = y[id] + (T)v;  // the node's only writer in this launch (colour-sequenced blocks)
     else
       unsafeAtomicAdd(&y[id], (T)v);
   }
@@ -1182,6 +1184,24 @@ __global__ __launch_bounds__(kBlock) void k_jpar(Part p, Geo geo, const double* 
 static inline int part_len(const Part& p) { return p.blist ? p.nlist : p.nblk; }
 static inline int grid_for(const Part& p) { return 8 * ((part_len(p) + 7) / 8); }
 
+// Calls f(part, grid) for one launch of the cluster apply over p, or, with the colour-sequenced
+// scatter on, for one launch per colour (the blocks of that colour as p's block list), in colour
+// order on the caller's stream.
+template <typename F>
+static void for_cluster_launches(const Ctx& c, const Part& p, F&& f) {
+  if (!c.apply_seq || p.blist != nullptr || c.h_colour_off.size() < 2) {
+    f(p, grid_for(p));
+    return;
+  }
+  for (size_t k = 0; k + 1 < c.h_colour_off.size(); ++k) {
+    Part q = p;
+    q.blist = c.d_colour_blk + c.h_colour_off[k];
+    q.nlist = c.h_colour_off[k + 1] - c.h_colour_off[k];
+    q.seq = 1;
+    if (q.nlist > 0) f(q, grid_for(q));
+  }
+}
+
 // Calls f(std::integral_constant<int, NT>) for the context's workgroup size NT (64, 128, 256).
 template <typename F>
 static void with_nt(int nt, F&& f) {
@@ -1243,17 +1263,17 @@ int launch_apply(const Ctx& c, int geom, const double* x, double* y, hipStream_t
     return 0;
   }
   if (geom == kGeomCoords && c.apply_clusters) {
-    const Part& p = c.part_cl;
-    if (part_len(p) == 0) return 0;
-    const int grid = grid_for(p);
-    with_cl_shape(c, [&](auto nt, auto nb, auto) {  // FP64 blocks always gather
-      constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
-      if (dot_part)
-        k_apply_cl<double, false, true, NT, NB, false><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
-                                                                          c.reg, dot_part);
-      else
-        k_apply_cl<double, false, false, NT, NB, false><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
-                                                                           c.reg, nullptr);
+    if (part_len(c.part_cl) == 0) return 0;
+    for_cluster_launches(c, c.part_cl, [&](const Part& p, int grid) {
+      with_cl_shape(c, [&](auto nt, auto nb, auto) {  // FP64 blocks always gather
+        constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
+        if (dot_part)
+          k_apply_cl<double, false, true, NT, NB, false><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr,
+                                                                            y, c.reg, dot_part);
+        else
+          k_apply_cl<double, false, false, NT, NB, false><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr,
+                                                                             y, c.reg, nullptr);
+      });
     });
     return 0;
   }
@@ -1284,9 +1304,11 @@ int launch_apply_scaled(const Ctx& c, const double* r, const double* w, double* 
   }
   const int grid = grid_for(p);
   if (c.apply_clusters) {
-    with_cl_shape(c, [&](auto nt, auto nb, auto) {  // FP64 blocks always gather
-      constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
-      k_apply_cl<double, true, false, NT, NB, false><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, c.reg, nullptr);
+    for_cluster_launches(c, p, [&](const Part& q, int g) {
+      with_cl_shape(c, [&](auto nt, auto nb, auto) {  // FP64 blocks always gather
+        constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
+        k_apply_cl<double, true, false, NT, NB, false><<<g, NT, 0, s>>>(q, c.geo, nullptr, r, w, xo, y, c.reg, nullptr);
+      });
     });
     return 0;
   }
@@ -1302,11 +1324,13 @@ int launch_apply_f32(const Ctx& c, const float* x, float* y, hipStream_t s) {
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
   if (c.apply_clusters) {
-    with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
-      constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
-      constexpr bool REC = decltype(rec)::value;
-      k_apply_cl<float, false, false, NT, NB, REC><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
-                                                                  (float)c.reg, nullptr);
+    for_cluster_launches(c, p, [&](const Part& q, int g) {
+      with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
+        constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
+        constexpr bool REC = decltype(rec)::value;
+        k_apply_cl<float, false, false, NT, NB, REC><<<g, NT, 0, s>>>(q, c.geo, x, nullptr, nullptr, nullptr, y,
+                                                                   (float)c.reg, nullptr);
+      });
     });
     return 0;
   }
@@ -1346,14 +1370,14 @@ int launch_apply_multi(const Ctx& c, int K, const double* x, double* y, double* 
 // the FP32 element energies) in dot_part
 int launch_apply_f32_dot(const Ctx& c, const float* x, float* y, double* dot_part, hipStream_t s) {
   MM_CHECK(c.apply_clusters, "the FP32 PCG needs the cluster apply");
-  const Part& p = c.part_cl;
-  if (part_len(p) == 0) return 0;
-  const int grid = grid_for(p);
-  with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
-    constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
-    constexpr bool REC = decltype(rec)::value;
-    k_apply_cl<float, false, true, NT, NB, REC><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
-                                                               (float)c.reg, dot_part);
+  if (part_len(c.part_cl) == 0) return 0;
+  for_cluster_launches(c, c.part_cl, [&](const Part& p, int grid) {
+    with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
+      constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
+      constexpr bool REC = decltype(rec)::value;
+      k_apply_cl<float, false, true, NT, NB, REC><<<grid, NT, 0, s>>>(p, c.geo, x, nullptr, nullptr, nullptr, y,
+                                                                 (float)c.reg, dot_part);
+    });
   });
   return 0;
 }
@@ -1361,14 +1385,14 @@ int launch_apply_f32_dot(const Ctx& c, const float* x, float* y, double* dot_par
 // y = A (w .* r) with FP32 r (the V-cycle of the FP32 PCG)
 int launch_apply_f32_scaled_f(const Ctx& c, const float* r, const float* w, float* xo, float* y, hipStream_t s) {
   MM_CHECK(c.apply_clusters, "the FP32 PCG needs the cluster apply");
-  const Part& p = c.part_cl;
-  if (part_len(p) == 0) return 0;
-  const int grid = grid_for(p);
-  with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
-    constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
-    constexpr bool REC = decltype(rec)::value;
-    k_apply_cl<float, true, false, NT, NB, REC, float><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg,
-                                                                      nullptr);
+  if (part_len(c.part_cl) == 0) return 0;
+  for_cluster_launches(c, c.part_cl, [&](const Part& p, int grid) {
+    with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
+      constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
+      constexpr bool REC = decltype(rec)::value;
+      k_apply_cl<float, true, false, NT, NB, REC, float><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y,
+                                                                        (float)c.reg, nullptr);
+    });
   });
   return 0;
 }
@@ -1378,11 +1402,13 @@ int launch_apply_f32_scaled(const Ctx& c, const double* r, const float* w, float
   if (part_len(p) == 0) return 0;
   const int grid = grid_for(p);
   if (c.apply_clusters) {
-    with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
-      constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
-      constexpr bool REC = decltype(rec)::value;
-      k_apply_cl<float, true, false, NT, NB, REC><<<grid, NT, 0, s>>>(p, c.geo, nullptr, r, w, xo, y, (float)c.reg,
-                                                                 nullptr);
+    for_cluster_launches(c, p, [&](const Part& q, int g) {
+      with_cl_shape(c, [&](auto nt, auto nb, auto rec) {
+        constexpr int NT = decltype(nt)::value, NB = decltype(nb)::value;
+        constexpr bool REC = decltype(rec)::value;
+        k_apply_cl<float, true, false, NT, NB, REC><<<g, NT, 0, s>>>(q, c.geo, nullptr, r, w, xo, y, (float)c.reg,
+                                                                  nullptr);
+      });
     });
     return 0;
   }

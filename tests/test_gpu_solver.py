@@ -87,6 +87,47 @@ def test_spmv_lane_tuner_keeps_the_vcycle(solver_ctx, oracle_c0):
     assert rel(z, z_ref) <= 1e-11
 
 
+@pytest.mark.parametrize("nb", [256, 1024, 2048])
+def test_colour_sequenced_scatter_matches_oracle(solver_ctx, oracle_c0, nb):
+    """The colour-sequenced scatter of the cluster apply (no two blocks of a launch share a node;
+    one launch per colour; shared-node sums added with plain loads and stores): the oracle's
+    A x and V-cycle, and FP64 and mixed solves to the solver tolerance against the oracle's
+    direct solution. The block sums themselves still come from LDS atomics in arrival order, so
+    repeated applies agree to a few ulp (the bound of the atomic scatter), not bit for bit."""
+    from oracle.solvers import solve_U as oracle_solve_U
+    from oracle.solvers import vcycle
+
+    kind, ctx = solver_ctx
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(ctx.N)
+    m = tilted_m0(ctx.N, seed=22)
+    U_ref, _, _ = oracle_solve_U(oracle_c0, m, np.zeros(ctx.N), 1e-12, precond="direct")
+    ctx.set_cluster_block_nodes(nb)
+    ctx.set_apply_scatter("colour")
+    try:
+        assert ctx.n_block_colours >= 2 or ctx.n_blocks_cl == 1
+        ys = [ctx.apply_A(x) for _ in range(10)]
+        assert rel(ys[0], oracle_c0.A @ x) <= 1e-12
+        assert max(float(np.max(np.abs(y - ys[0]))) for y in ys) <= 4 * np.spacing(np.abs(ys[0]).max())
+        z = ctx.precond_apply(x)
+        z_ref = vcycle(oracle_levels(ctx.amg_levels, kind), oracle_c0.apply_A, x, kind)
+        assert rel(z, z_ref) <= 1e-11
+        U1, it1, rr1 = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+        U2, it2, rr2 = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+        assert rr1 <= 1e-10 and rel(U1, U_ref) <= 1e-7
+        assert abs(it1 - it2) <= 1 and rel(U1, U2) <= 1e-13
+        ctx.set_precision("mixed")
+        try:
+            Um, _, rrm = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+        finally:
+            ctx.set_precision("fp64")
+        assert rrm <= 1e-10 and rel(Um, U_ref) <= 1e-7
+    finally:
+        ctx.set_apply_scatter("atomic")
+        ctx.set_cluster_block_nodes(256)
+    assert ctx.n_block_colours == 0
+
+
 def test_solve_U_matches_oracle_pcg(solver_ctx, oracle_c0):
     from oracle.solvers import solve_U
 
