@@ -325,6 +325,87 @@ __device__ __forceinline__ void cluster_apply(uint32_t w0, uint32_t w1, uint32_t
     if (used & (1u << q)) atomicAdd(&sy[li[q]], (double)acc[q]);
 }
 
+// cluster_apply walking the ring: ring slot r (slot 2 + r) is loaded when tet r - 1 needs it
+// and its LDS atomic issues as soon as its two tets (r - 1, r) are done, so only two ring slots
+// (and slot 0's, which waits for tet 5) are live at a time. Same arithmetic and the same
+// summation order per slot as cluster_apply. The FP64 apply uses it: 110-116 VGPRs instead of
+// 140-142, so four 1024-node workgroups fit a CU (their LDS allows four) without spills.
+template <typename T, bool DOT, int NB>
+__device__ __forceinline__ void cluster_apply_ring(uint32_t w0, uint32_t w1, uint32_t wm, const T (*sX)[NB],
+                                                   const T* sx, double* sy, double& acc_e) {
+  int li[8];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    li[q] = (w0 >> (8 * q)) & 255u;
+    li[4 + q] = (w1 >> (8 * q)) & 255u;
+  }
+  if constexpr (NB > 256) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) li[q] |= (int)((wm >> (8 + 3 * q)) & 7u) << 8;
+  }
+  const uint32_t t = wm & 63u;
+  const uint32_t ring = (t | (t << 1) | (t >> 5)) & 63u;
+  const T x0 = sX[0][li[0]], y0 = sX[1][li[0]], z0 = sX[2][li[0]], v0 = sx[li[0]];
+  const T c1[3] = {sX[0][li[1]] - x0, sX[1][li[1]] - y0, sX[2][li[1]] - z0};
+  const T d1 = sx[li[1]] - v0;
+  auto load = [&](int r, T c[3], T& d, T m[3]) {
+    if (ring & (1u << r)) {
+      const int l = li[2 + r];
+      c[0] = sX[0][l] - x0;
+      c[1] = sX[1][l] - y0;
+      c[2] = sX[2][l] - z0;
+      d = sx[l] - v0;
+    } else {
+      c[0] = c[1] = c[2] = d = (T)0;
+    }
+    m[0] = c1[1] * c[2] - c1[2] * c[1];
+    m[1] = c1[2] * c[0] - c1[0] * c[2];
+    m[2] = c1[0] * c[1] - c1[1] * c[0];
+  };
+  T a0 = (T)0, a1 = (T)0;
+  T cF[3], dF, mF[3], aF = (T)0;
+  load(0, cF, dF, mF);
+  T cA[3] = {cF[0], cF[1], cF[2]}, dA = dF, mA[3] = {mF[0], mF[1], mF[2]}, aA = (T)0;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    T cB[3], dB, mB[3], aB = (T)0;
+    if (j == 5) {
+      cB[0] = cF[0], cB[1] = cF[1], cB[2] = cF[2], dB = dF, mB[0] = mF[0], mB[1] = mF[1], mB[2] = mF[2];
+    } else {
+      load(j + 1, cB, dB, mB);
+    }
+    if (t & (1u << j)) {
+      T n[3][3];
+      // corners (0, 1, s2, s3): n_1 = c_s2 x c_s3, n_2 = c_s3 x c_1 = -m_s3, n_3 = c_1 x c_s2 = m_s2
+      n[0][0] = cA[1] * cB[2] - cA[2] * cB[1];
+      n[0][1] = cA[2] * cB[0] - cA[0] * cB[2];
+      n[0][2] = cA[0] * cB[1] - cA[1] * cB[0];
+      n[1][0] = -mB[0];
+      n[1][1] = -mB[1];
+      n[1][2] = -mB[2];
+      n[2][0] = mA[0];
+      n[2][1] = mA[1];
+      n[2][2] = mA[2];
+      const T det = c1[0] * n[0][0] + c1[1] * n[0][1] + c1[2] * n[0][2];
+      T ya[4];
+      const T en = tet_laplace(n, det, d1, dA, dB, ya);
+      if constexpr (DOT) acc_e += (double)en;
+      a0 += ya[0];
+      a1 += ya[1];
+      aA += ya[2];
+      aB += ya[3];
+    }
+    if (j == 0)
+      aF = aA;  // ring slot 0 also belongs to tet 5
+    else if (ring & (1u << j))
+      atomicAdd(&sy[li[2 + j]], (double)aA);
+    cA[0] = cB[0], cA[1] = cB[1], cA[2] = cB[2], dA = dB, mA[0] = mB[0], mA[1] = mB[1], mA[2] = mB[2], aA = aB;
+  }
+  if (ring & 1u) atomicAdd(&sy[li[2]], (double)(aF + aA));
+  atomicAdd(&sy[li[0]], (double)a0);
+  atomicAdd(&sy[li[1]], (double)a1);
+}
+
 // Same operator as k_apply, one workgroup per block of clusters. T = double: the FP64
 // operator (x, y FP64, DOT as in k_apply); T = float: the FP32 operator of the
 // mixed-precision V-cycle (block-relative FP32 coordinates, FP64 LDS accumulation).
@@ -415,9 +496,18 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
   }
   __syncthreads();
   double acc_e = 0.0;
-  if (c0 < bi.e1) cluster_apply<T, DOT, NB>(cw0, cw1, cwm, sX, sx, sy, acc_e);
-  for (int cc = c0 + NT; cc < bi.e1; cc += NT)
-    cluster_apply<T, DOT, NB>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
+  if (c0 < bi.e1) {
+    if constexpr (sizeof(T) == 8)
+      cluster_apply_ring<T, DOT, NB>(cw0, cw1, cwm, sX, sx, sy, acc_e);
+    else
+      cluster_apply<T, DOT, NB>(cw0, cw1, cwm, sX, sx, sy, acc_e);
+  }
+  for (int cc = c0 + NT; cc < bi.e1; cc += NT) {
+    if constexpr (sizeof(T) == 8)
+      cluster_apply_ring<T, DOT, NB>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
+    else
+      cluster_apply<T, DOT, NB>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
+  }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kNode; ++q) {
@@ -445,8 +535,19 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
     else
       unsafeAtomicAdd(&y[id], (T)v);
   }
-  if constexpr (DOT)
-    if (bi.b >= 0) block_sum_to<NT>(acc_e, dot_part + bi.b);
+  if constexpr (DOT) {
+    // block sum through sy, which every thread is done with after the barrier: no extra LDS, so
+    // the DOT kernel fits as many workgroups per CU as the plain one
+    acc_e = wave_red<false>(acc_e);
+    __syncthreads();
+    if ((tid & 63) == 0) sy[tid >> 6] = acc_e;
+    __syncthreads();
+    if (tid == 0 && bi.b >= 0) {
+      double s = sy[0];
+      for (int w = 1; w < NT / 64; ++w) s += sy[w];
+      dot_part[bi.b] = s;
+    }
+  }
 }
 
 // ------------------------------- multi-RHS Poisson apply over edge-star clusters (K vectors)
