@@ -154,6 +154,40 @@ def test_local_grad(ctx_c0, oracle_c0):
     assert rel(ctx_c0.local_grad(v), oracle_c0.local_grad(v)) <= TOL
 
 
+def test_local_grad_per_tet_materials():
+    """The local Hessian action on an 8-grain polycrystal with a grain-boundary phase (per-grain
+    easy axes, A and K1 changing across the boundaries, an airbox) against the oracle: in FP64 to
+    the parity bar, and in the FP32 element arithmetic of the mixed forms to FP32 accuracy."""
+    from mumag_matrixfree2_amd.context import MeshContext
+    from mumag_matrixfree2_amd.mesh import polycrystal_cube, random_easy_axes
+    from oracle.fem import Problem
+
+    from conftest import MU0
+
+    mesh = polycrystal_cube(30.0, 12, 3, n_grains=8, gb_width=1.5, seed=4)
+    Kd = 1.61**2 / (2 * MU0)
+    G = 10
+    A, K1, Js, k = np.zeros(G), np.zeros(G), np.zeros(G), np.tile([0.0, 0.0, 1.0], (G, 1))
+    A[:8], K1[:8], Js[:8] = 7.7e-12 * 1e18 / Kd, 4.3e6 / Kd, 1.0
+    k[:8] = random_easy_axes(8, 5)
+    A[8], Js[8] = 1.5e-12 * 1e18 / Kd, 1.0
+    ctx = MeshContext(mesh["knt"], mesh["ijk"][:, :4], mesh["ijk"][:, 4], A, K1, Js, k,
+                      boundary_mask=mesh["boundary_mask"])
+    prob = Problem(mesh["knt"], mesh["ijk"], A, K1, Js, k, boundary_mask=mesh["boundary_mask"])
+    try:
+        v = np.random.default_rng(16).standard_normal((ctx.N, 3))
+        ref = prob.local_grad(v)
+        out = {}
+        for prec in ("fp64", "mixed"):
+            ctx.set_precision(prec)
+            out[prec] = ctx.local_grad(v)
+        ctx.set_precision("fp64")
+        assert rel(out["fp64"], ref) <= TOL
+        assert rel(out["mixed"], ref) <= 1e-5
+    finally:
+        ctx.close()
+
+
 def test_j_parallel(ctx_c0, oracle_c0):
     m = tilted_m0(ctx_c0.N, seed=7, sigma=0.5)
     h = np.array([0.017452, 0.0, 0.999848])
