@@ -125,21 +125,17 @@ def amg_kwargs(args) -> dict:
 def measured_traffic(workload, cluster_block_nodes):
     """HBM bytes per launch of the PCG's FP64 apply from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (``profiles/k_apply_cl_<workload>_fp64.md``, line
-    ``traffic_bytes_per_launch: N``), or None if there is none or if the summary was taken at a
-    cluster block size (line ``traffic_cluster_block_nodes: B``) other than this run's."""
+    ``traffic_bytes_per_launch_<B>: N`` for cluster blocks of B nodes), or None if the summary has
+    no line for this run's block size."""
     f = ROOT / "profiles" / f"k_apply_cl_{workload}_fp64.md"
     if not f.exists():
         return None
-    fields = {}
+    key = f"traffic_bytes_per_launch_{cluster_block_nodes}"
     for line in f.read_text().splitlines():
-        key, sep, val = line.partition(":")
-        if sep and key in ("traffic_bytes_per_launch", "traffic_cluster_block_nodes"):
-            fields.setdefault(key, int(val))
-    if "traffic_bytes_per_launch" not in fields:
-        return None
-    if fields.get("traffic_cluster_block_nodes", cluster_block_nodes) != cluster_block_nodes:
-        return None
-    return {"bytes": fields["traffic_bytes_per_launch"], "source": str(f.relative_to(ROOT))}
+        k, sep, val = line.partition(":")
+        if sep and k.strip() == key:
+            return {"bytes": int(val), "source": str(f.relative_to(ROOT))}
+    return None
 
 
 def cpu_baseline(mesh, mats, B_vals, budget_s, log=None, max_iter=2000):
@@ -523,16 +519,21 @@ def main():
     m0 = initial_m(ctx.N)
     steps = max(1, min(args.steps, len(B_vals)))
 
-    def sweep(precision, timed, guess=None):
+    def sweep(precision, timed, guess=None, keep_states=False):
         ctx.set_precision(precision)
         ctx.set_solve_guess(args.solve_guess if guess is None else guess)
         ctx.state_set(m0, np.zeros(ctx.N))
         if timed:
             ctx.apply_timing(True)
         c0 = ctx.counters()
-        J, E, iters = [], [], []
+        J, E, iters, states = [], [], [], []
+        t_copy = 0.0  # host copies of the states (kept for the worst-step control), not counted
         t_start = time.perf_counter()
         for k in range(steps):
+            if keep_states:
+                tc = time.perf_counter()
+                states.append(ctx.state_get())
+                t_copy += time.perf_counter() - tc
             ts = time.perf_counter()
             info = ctx.minimize_resident(p, B_vals[k] * H_DIR)
             J.append(ctx.state_j_parallel(H_DIR))
@@ -541,12 +542,12 @@ def main():
             log(f"[bench:{precision}] step {k:2d} B={B_vals[k] * JS:+.3f} T J={J[-1] * JS:+.5f} T "
                 f"E={info['E']:+.10e} it={info['iters']} nf={info['evals']} icg={info['demag_iters']} "
                 f"t={time.perf_counter() - ts:.3f}s")
-        el = time.perf_counter() - t_start
+        el = time.perf_counter() - t_start - t_copy
         c1 = ctx.counters()
         timing = ctx.apply_timing(False) if timed else None
         return {"seconds": el, "J": np.array(J), "E": np.array(E), "iters": iters,
                 "pcg_iters": c1["pcg_iters"] - c0["pcg_iters"], "applies": c1["apply"] - c0["apply"],
-                "timing": timing}
+                "timing": timing, "states": states}
 
     # warm-up on a separate state
     ctx.set_precision(args.precision)
@@ -573,7 +574,7 @@ def main():
         B_last = B_vals[steps - 1] * H_DIR
         m_mx, U_mx = ctx.state_get()
         _, g_mx = ctx.energy_grad(m_mx, U_mx, B_last)
-        r64 = sweep("fp64", timed=False)
+        r64 = sweep("fp64", timed=False, keep_states=True)
         m64, U64 = ctx.state_get()
         _, g64 = ctx.energy_grad(m64, U64, B_last)
         U_same64, _, _ = ctx.solve_U(m64, np.zeros(ctx.N), 1e-10)
@@ -584,6 +585,33 @@ def main():
             return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
         err = np.abs(head["E"] - r64["E"]) / np.maximum(np.abs(r64["E"]), 1.0)
+        # The worst step once more from the FP64 sweep's state before it: the headline form, the FP64
+        # form, and the FP64 form with its sums in another order (the colour-sequenced scatter), so
+        # the line shows how far FP64 rounding alone moves that step's result
+        kw = int(np.argmax(err))
+        m_s, U_s = r64["states"][kw]
+        scatter0 = ctx.apply_scatter
+
+        def rerun(precision, scatter):
+            ctx.set_precision(precision)
+            ctx.set_apply_scatter(scatter)
+            ctx.set_solve_guess(args.solve_guess)
+            ctx.state_set(m_s, U_s)
+            info = ctx.minimize_resident(p, B_vals[kw] * H_DIR)
+            return info["E"], info["iters"]
+
+        E_a, it_a = rerun("fp64", "atomic")
+        E_c, it_c = rerun("fp64", "colour")
+        E_m, it_m = rerun(args.precision, "atomic")
+        ctx.set_apply_scatter(scatter0)
+        ctx.set_precision(args.precision)
+        sc = max(abs(E_a), 1.0)
+        worst = {"step": kw, "B_T": float(B_vals[kw] * JS), "sweep_rel_energy_err": float(err[kw]),
+                 "from_the_fp64_state_before_it": {
+                     "mixed_vs_fp64_rel_energy_err": abs(E_m - E_a) / sc,
+                     "fp64_vs_fp64_other_sum_order_rel_energy_diff": abs(E_c - E_a) / sc,
+                     "minimizer_iters": {"mixed": it_m, "fp64": it_a, "fp64_other_sum_order": it_c}}}
+        log(f"[bench] worst step {kw}: {worst}")
         compare = {
             "fp64_field_steps_per_s": steps / r64["seconds"],
             "speedup_vs_fp64": r64["seconds"] / elapsed,
@@ -595,6 +623,7 @@ def main():
             "same_m_potential_rel_err": rel(U_samemx, U_same64),
             "minimizer_iters": [int(sum(head["iters"])), int(sum(r64["iters"]))],
             "pcg_iters": [head["pcg_iters"], r64["pcg_iters"]],
+            "worst_step": worst,
         }
 
     guess_cmp = None
