@@ -119,6 +119,8 @@ def amg_kwargs(args) -> dict:
         kw["theta"] = tuple(float(t) for t in args.amg_theta.split(","))
     if args.amg_omega is not None:
         kw["omega"] = args.amg_omega
+    if args.amg_filter is not None:
+        kw["filter_levels"] = () if args.amg_filter == "none" else tuple(int(v) for v in args.amg_filter.split(","))
     return kw
 
 
@@ -439,6 +441,9 @@ def main():
                     help="strength thresholds of the SA set-up, fine level first, comma-separated (default 0.1,0.05)")
     ap.add_argument("--amg-omega", type=float, default=None,
                     help="prolongator smoothing weight omega / rho(D^-1 A) of the SA set-up (default 1.5)")
+    ap.add_argument("--amg-filter", default=None,
+                    help="levels whose prolongator is smoothed with the filtered matrix, comma-separated, or "
+                         "'none' (default 0: the fine level)")
     ap.add_argument("--block-nodes", type=int, default=256,
                     help="distinct nodes per element block (16..256): the LDS staging size of the element kernels")
     ap.add_argument("--apply-shape", default="auto",
