@@ -124,15 +124,16 @@ def amg_kwargs(args) -> dict:
     return kw
 
 
-def measured_traffic(workload, cluster_block_nodes):
-    """HBM bytes per launch of the PCG's FP64 apply from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (``profiles/k_apply_cl_<workload>_fp64.md``, line
-    ``traffic_bytes_per_launch_<B>: N`` for cluster blocks of B nodes), or None if the summary has
-    no line for this run's block size."""
-    f = ROOT / "profiles" / f"k_apply_cl_{workload}_fp64.md"
+def measured_traffic(workload, cluster_block_nodes, fp32=False):
+    """HBM bytes per launch of the PCG's apply (FP64, or FP32 in the FP32-PCG forms) from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
+    (``profiles/k_apply_cl_<workload>.md``, line ``traffic_bytes_per_launch_<fp64|fp32>_<B>: N`` for
+    cluster blocks of B nodes), or None if the summary has no line for this run's kernel and block
+    size."""
+    f = ROOT / "profiles" / f"k_apply_cl_{workload}.md"
     if not f.exists():
         return None
-    key = f"traffic_bytes_per_launch_{cluster_block_nodes}"
+    key = f"traffic_bytes_per_launch_{'fp32' if fp32 else 'fp64'}_{cluster_block_nodes}"
     for line in f.read_text().splitlines():
         k, sep, val = line.partition(":")
         if sep and k.strip() == key:
@@ -432,15 +433,20 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="time budget of the CPU baseline (field steps from the start of the sweep)")
-    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed", "fp32pcg", "fp32all"],
-                    help="headline form: 'mixed' (FP32 V-cycle under the FP64 PCG) or 'fp32pcg' (FP32 PCG with "
-                         "FP64 refinement); the FP64 form runs the same steps for the comparison")
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed", "fp32pcg", "fp32all"],
+                    help="headline form: 'fp32pcg' (the default on one mesh: FP32 PCG and V-cycle with FP64 dots "
+                         "and FP64 true-residual replacement), 'mixed' (FP32 V-cycle under the FP64 PCG; the "
+                         "default on the grain set), 'fp32all' or 'fp64'; the FP64 form runs the same steps for "
+                         "the comparison")
     ap.add_argument("--no-fp64-compare", action="store_true", help="mixed headline without the FP64 sweep")
     ap.add_argument("--max-coarse", type=int, default=None)
     ap.add_argument("--amg-theta", default=None,
                     help="strength thresholds of the SA set-up, fine level first, comma-separated (default 0.1,0.05)")
     ap.add_argument("--amg-omega", type=float, default=None,
                     help="prolongator smoothing weight omega / rho(D^-1 A) of the SA set-up (default 1.5)")
+    ap.add_argument("--ref-eta", type=float, default=None,
+                    help="FP32-PCG forms: replace the recursive residual by the true one after this reduction "
+                         "(default: the library's)")
     ap.add_argument("--amg-filter", default=None,
                     help="levels whose prolongator is smoothed with the filtered matrix, comma-separated, or "
                          "'none' (default 0: the fine level)")
@@ -473,6 +479,8 @@ def main():
     ap.add_argument("--no-segmented", action="store_true",
                     help="grains64: skip the segmented vs threaded Poisson solve comparison")
     args = ap.parse_args()
+    if args.precision is None:
+        args.precision = "mixed" if args.workload == "grains64" else "fp32pcg"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the hysteresis path runs on a single MI355X here "
               "(element-block partitions would shard across GPUs with a halo exchange per apply); "
@@ -507,6 +515,8 @@ def main():
             ctx.set_apply_scatter(parts[3])
         shape = {"unit": parts[0], "param": int(parts[1])}
     log(f"[bench] apply shape {shape}")
+    if args.ref_eta is not None:
+        ctx.set_refinement(args.ref_eta)
     ctx.set_precision(args.precision)
     if args.spmv_lanes == "auto":
         lanes = tune_spmv_lanes(ctx)
@@ -647,7 +657,7 @@ def main():
     avg_apply_s = apply_ms / max(apply_n, 1) * 1e-3
     algo = apply_algorithmic_bytes(ctx, fp32=args.precision in ("fp32pcg", "fp32all"))
     achieved = algo / avg_apply_s / 1e9
-    traffic = measured_traffic(args.workload, ctx.cluster_block_nodes)
+    traffic = measured_traffic(args.workload, ctx.cluster_block_nodes, fp32=args.precision in ("fp32pcg", "fp32all"))
     label = {"fp64": "FP64", "mixed": "mixed: FP32 V-cycle, FP64 PCG, FP64 energies",
              "fp32pcg": "FP32 PCG and V-cycle with FP64 dots and FP64 true-residual refinement, FP64 energies",
              "fp32all": "FP32 PCG and V-cycle with FP64 dots and FP64 true-residual refinement, FP32 energy "
@@ -686,7 +696,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None if traffic is None or args.precision in ("fp32pcg", "fp32all") else traffic["bytes"],
+            "traffic": None if traffic is None else traffic["bytes"],
             "kernel": ("k_apply_cl<float, false, true> (FP32 matrix-free Poisson apply of the PCG, edge-star "
                        "clusters)" if args.precision in ("fp32pcg", "fp32all") else
                        "k_apply_cl<double, false, true> (FP64 matrix-free Poisson apply of the PCG, edge-star "

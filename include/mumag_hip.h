@@ -135,6 +135,11 @@ int mm_set_cluster_staging(int64_t ctx, int mode);
  * still come from LDS atomics). Not on segmented contexts or the persistent grid. A tuning knob with no reference
  * counterpart: the two scatters the path may use, kept selectable so either can be measured. */
 int mm_set_apply_scatter(int64_t ctx, int mode);
+/* FP32 PCG of precision forms 2 and 3: the recursive FP32 residual is replaced by the true FP64
+ * residual b - A U (and the FP32 correction added to U) each time its norm has dropped by the
+ * factor eta, in (0, 1), since the last replacement; the Krylov direction carries on. The stopping
+ * test is the true residual's, as in the FP64 PCG. A tuning knob with no reference counterpart. */
+int mm_set_refinement(int64_t ctx, double eta);
 
 /* y = A x: assembled-mode apply_A (src/poisson_solve.py:166-171) computed matrix-free. */
 int mm_poisson_apply(int64_t ctx, const double* x, double* y);

@@ -75,6 +75,7 @@ SIGNATURES = {
     "mm_set_cluster_block_nodes": [C.c_int64, C.c_int],
     "mm_set_cluster_staging": [C.c_int64, C.c_int],
     "mm_set_apply_scatter": [C.c_int64, C.c_int],
+    "mm_set_refinement": [C.c_int64, C.c_double],
     "mm_poisson_apply": [C.c_int64, f64p, f64p],
     "mm_poisson_rhs": [C.c_int64, f64p, f64p],
     "mm_poisson_diag": [C.c_int64, f64p],

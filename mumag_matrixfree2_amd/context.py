@@ -174,6 +174,11 @@ class MeshContext:
         self.apply_scatter = mode
         self._refresh_cluster_info()
 
+    def set_refinement(self, eta: float):
+        """FP32 PCG ('fp32pcg', 'fp32all'): replace the recursive FP32 residual by the true FP64
+        residual each time its norm has dropped by ``eta`` since the last replacement."""
+        call("mm_set_refinement", self.handle, float(eta))
+
     def _refresh_cluster_info(self):
         info = np.zeros(15)
         call("mm_ctx_info", self.handle, ptr(info))

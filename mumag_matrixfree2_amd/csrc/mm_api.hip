@@ -1068,6 +1068,13 @@ int mm_set_cluster_staging(int64_t h, int mode) {
   return 0;
 }
 
+int mm_set_refinement(int64_t h, double eta) {
+  GET_CTX(h);
+  MM_CHECK(eta > 0.0 && eta < 1.0, "residual replacement factor must be in (0, 1)");
+  c.ref_eta = eta;
+  return 0;
+}
+
 int mm_set_apply_scatter(int64_t h, int mode) {
   GET_CTX(h);
   MM_CHECK(mode == 0 || mode == 1, "scatter must be 0 (FP64 atomics) or 1 (colour-sequenced)");

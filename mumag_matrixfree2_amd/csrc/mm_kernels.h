@@ -225,7 +225,8 @@ struct Ctx {
   float *px0f = nullptr, *ptf = nullptr;  // level-0 FP32 work vectors
   // FP32 PCG with FP64 refinement (precision 2): correction, residual, direction, A p, V-cycle output
   float *rxf = nullptr, *rrf = nullptr, *rpf = nullptr, *rqf = nullptr, *rzf = nullptr;
-  int ref_cycles = 0;                 // refinement cycles of the last solve
+  int ref_cycles = 0;                 // residual replacements of the last solve
+  double ref_eta = 1e-2;              // FP32 PCG: replace the residual after this reduction of its norm
 
   // minimiser state (3N unless noted)
   double *m = nullptr, *U = nullptr /*N*/, *g_raw = nullptr, *g_tan = nullptr, *g_ext = nullptr, *g_prev = nullptr;

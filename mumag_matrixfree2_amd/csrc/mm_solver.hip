@@ -1024,9 +1024,12 @@ int precond_apply(Ctx& c, const double* r, double* z, double* rz_part, hipStream
 // ------------------------------------------------- FP32 PCG with FP64 iterative refinement
 // precision 2: the PCG runs on FP32 vectors (correction, residual, direction, A p, V-cycle
 // output) with the FP32 matrix-free apply and the FP32 V-cycle, while its dot products and the
-// alpha / beta recurrences stay FP64. An outer FP64 loop recomputes the true residual
-// r = b - A U with the FP64 apply after every inner solve and restarts the inner PCG on it until
-// r.r <= tol^2 b.b: the stopping test, and so the solution's accuracy, are those of the FP64 PCG.
+// alpha / beta recurrences stay FP64. Whenever the FP32 recursion has reduced the residual by
+// the factor ref_eta since the last true residual (or reaches the tolerance), the correction is
+// added to U in FP64 and the true residual r = b - A U is recomputed with the FP64 apply and
+// replaces the recursive one; the search direction and the CG scalars carry on (residual
+// replacement), so the Krylov iteration is not restarted. The stopping test uses the true
+// residual only: r.r <= tol^2 b.b, as in the FP64 PCG, so the solution's accuracy is its.
 
 // rf = (float) r, xc = 0, tf = 0
 __global__ void k_ref_start(const double* __restrict__ r, float* __restrict__ rf, float* __restrict__ xc,
@@ -1127,7 +1130,10 @@ static int timed_apply_f32_dot(Ctx& c, const float* x, float* y, double* dot_par
 }
 
 // Inner reduction per refinement cycle: about what an FP32 residual can carry
-constexpr double kRefEta = 1e-6;
+// The recursion hands over to the true-residual test once its r.r is below this share of tol^2 b.b
+// (the replaced residuals carry little drift, so the true one is then almost always below tol^2 b.b
+// too; when it is not, the iteration goes on)
+constexpr double kRefFinal = 0.81;
 
 static int pcg_ref(Ctx& c, const double* b, double* U, double tol, int maxiter, int* it_out, double* r2_out,
                    double* bb_out, bool q_ready) {
@@ -1150,35 +1156,38 @@ static int pcg_ref(Ctx& c, const double* b, double* U, double tol, int maxiter, 
   }
   const double tol2 = tol * tol * bb;
   int it = 0, cycles = 0;
-  while (it < maxiter && r2 > tol2) {
-    // inner FP32 PCG on A xc = r, to 0.5 tol of b or a 1e-6 reduction of this cycle's residual
-    const double target = std::max(0.25 * tol2, kRefEta * kRefEta * r2);
-    k_ref_start<<<vgrid(N), kBlock, 0, s>>>(c.pr, c.rrf, c.rxf, c.ptf, N);
+  if (it < maxiter && r2 > tol2) {
+    k_ref_start<<<vgrid(N), kBlock, 0, s>>>(c.pr, c.rrf, c.rxf, c.ptf, N);  // r32 = r, xc = 0
     MM_TRY(precond_ff(c, c.rrf, c.rzf, c.part_rz, s));
     k_ref_p<<<gv, kBlock, 0, s>>>(sc, c.part_rz, gv, 0, 0, 1, c.rzf, c.rpf, c.rqf, N);
-    int cur = 0;
-    double r2in = r2;
-    while (it < maxiter && r2in > target) {
-      MM_TRY(timed_apply_f32_dot(c, c.rpf, c.rqf, c.part_pq, s));  // q was cleared by k_ref_p
-      c.n_apply++;
-      k_ref_xr<<<gv, kBlock, 0, s>>>(sc, cur, c.part_pq, n_pq, c.rxf, c.rrf, c.rpf, c.rqf, c.ptf, N, c.hpart_dev);
-      MM_HIP(hipEventRecord(c.ev, s));
-      MM_TRY(precond_ff(c, c.rrf, c.rzf, c.part_rz, s));
-      k_ref_p<<<gv, kBlock, 0, s>>>(sc, c.part_rz, gv, 1 - cur, cur, 0, c.rzf, c.rpf, c.rqf, N);
-      cur = 1 - cur;
-      ++it;
-      MM_HIP(hipEventSynchronize(c.ev));
-      r2in = 0.0;
-      for (int i = 0; i < gv; ++i) r2in += c.hpart[i];
-    }
-    // U += xc and the true residual in FP64
+  }
+  int cur = 0;
+  double target = std::max(kRefFinal * tol2, c.ref_eta * c.ref_eta * r2);
+  while (it < maxiter && r2 > tol2) {
+    MM_TRY(timed_apply_f32_dot(c, c.rpf, c.rqf, c.part_pq, s));  // q was cleared by k_ref_p
+    c.n_apply++;
+    k_ref_xr<<<gv, kBlock, 0, s>>>(sc, cur, c.part_pq, n_pq, c.rxf, c.rrf, c.rpf, c.rqf, c.ptf, N, c.hpart_dev);
+    MM_HIP(hipEventRecord(c.ev, s));
+    // the next direction is formed while the host waits for the recursive r.r
+    MM_TRY(precond_ff(c, c.rrf, c.rzf, c.part_rz, s));
+    k_ref_p<<<gv, kBlock, 0, s>>>(sc, c.part_rz, gv, 1 - cur, cur, 0, c.rzf, c.rpf, c.rqf, N);
+    cur = 1 - cur;
+    ++it;
+    MM_HIP(hipEventSynchronize(c.ev));
+    double r2in = 0.0;
+    for (int i = 0; i < gv; ++i) r2in += c.hpart[i];
+    if (r2in > target && it < maxiter) continue;
+    // residual replacement: U += xc in FP64, the true residual b - A U, loaded into the FP32
+    // residual (xc restarts at zero); the direction p just formed and the scalars are kept
     k_ref_update<<<vgrid(N), kBlock, 0, s>>>(U, c.rxf, c.pq, N);
     MM_TRY(launch_apply(c, kGeomCoords, U, c.pq, s));
     c.n_apply++;
     k_pcg_init<<<gv, kBlock, 0, s>>>(b, c.pq, c.pr, nullptr, nullptr, N, c.hpart_dev);
+    k_ref_start<<<vgrid(N), kBlock, 0, s>>>(c.pr, c.rrf, c.rxf, c.ptf, N);
     MM_HIP(hipStreamSynchronize(s));
     r2 = 0.0;
     for (int i = 0; i < gv; ++i) r2 += c.hpart[2 * i];
+    target = std::max(kRefFinal * tol2, c.ref_eta * c.ref_eta * r2);
     ++cycles;
   }
   MM_HIP(hipStreamSynchronize(s));

@@ -159,6 +159,32 @@ def test_fp32_pcg_meets_the_fp64_tolerance(mixed_ctx, oracle_c0, blocks):
     assert it <= 2 * it64 + 5
 
 
+@pytest.mark.parametrize("eta", [1e-2, 1e-3])
+def test_fp32_pcg_residual_replacement_keeps_the_krylov_iteration(mixed_ctx, oracle_c0, eta):
+    """The FP32 PCG replaces its recursive residual by the FP64 true residual after each eta
+    reduction and keeps its search direction: it stops on the FP64 true residual, reaches the FP64
+    solution to the tolerance, and needs no more than two iterations beyond the FP64 PCG (a
+    restarted refinement loses the Krylov space at every cycle). An eta outside (0, 1) is an error."""
+    ctx = mixed_ctx
+    m = tilted_m0(ctx.N, seed=35, sigma=0.4)
+    U64, it64, _ = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+    b = oracle_c0.rhs(m)
+    ctx.set_precision("fp32pcg")
+    ctx.set_refinement(eta)
+    try:
+        U, it, rr = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+        with pytest.raises(RuntimeError, match="replacement factor"):
+            ctx.set_refinement(1.0)
+    finally:
+        ctx.set_refinement(1e-2)
+        ctx.set_precision("fp64")
+    true_rr = np.linalg.norm(b - oracle_c0.A @ U) / np.linalg.norm(b)
+    print(f"[fp32pcg eta={eta}] it={it} (fp64 {it64}) rr={rr:.3e} true={true_rr:.3e}", flush=True)
+    assert rr <= 1e-10 and true_rr <= 1.001e-10
+    assert rel(U, U64) <= 1e-8
+    assert it <= it64 + 2
+
+
 @pytest.mark.parametrize("method", ["pcohen_hs", "lbfgs"])
 def test_fp32_pcg_field_step_energy(mixed_ctx, oracle_c0, method):
     from mumag_matrixfree2_amd.context import make_params
