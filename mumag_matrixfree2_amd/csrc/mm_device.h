@@ -325,6 +325,26 @@ __device__ void block_sum_to(double v, double* __restrict__ out) {
   }
 }
 
+// Grid-stride loop over [0, n) that gives each thread kChunk items per round: the loads of a
+// round (load(i) returns the item's operands) are all issued before the first body(i, operands)
+// runs, so a wave keeps kChunk loads per stream in flight instead of waiting once per item. Each
+// thread still visits its items i, i + S, i + 2 S, ... (S = grid threads) in increasing order, so
+// a per-thread accumulation in body adds in the same order as a plain grid-stride loop.
+constexpr int kChunk = 4;
+template <int U = kChunk, typename L, typename B>
+__device__ __forceinline__ void grid_chunks(long n, L&& load, B&& body) {
+  const long S = (long)gridDim.x * blockDim.x;
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  for (; i + (U - 1) * S < n; i += U * S) {
+    decltype(load(i)) v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = load(i + u * S);
+#pragma unroll
+    for (int u = 0; u < U; ++u) body(i + u * S, v[u]);
+  }
+  for (; i < n; i += S) body(i, load(i));
+}
+
 // Grid size for grid-stride vector kernels with reductions: at most 256 workgroups (one per CU),
 // so a consumer sums at most 256 partials per value and each thread loops over more items.
 constexpr int kRedGridMax = 256;

@@ -328,8 +328,10 @@ __device__ __forceinline__ void cluster_apply(uint32_t w0, uint32_t w1, uint32_t
 // cluster_apply walking the ring: ring slot r (slot 2 + r) is loaded when tet r - 1 needs it
 // and its LDS atomic issues as soon as its two tets (r - 1, r) are done, so only two ring slots
 // (and slot 0's, which waits for tet 5) are live at a time. Same arithmetic and the same
-// summation order per slot as cluster_apply. The FP64 apply uses it: 110-116 VGPRs instead of
-// 140-142, so four 1024-node workgroups fit a CU (their LDS allows four) without spills.
+// summation order per slot as cluster_apply. k_apply_cl uses it in both precisions: FP64 110-116
+// VGPRs instead of 140-142, so four 1024-node workgroups fit a CU (their LDS allows four) without
+// spills; FP32 66-75 instead of 79-88, so three 512-thread 2048-node workgroups fit a CU (two
+// before) and the FP32 apply on the 24.6M-tet polycrystal runs 65 instead of 72 us back-to-back.
 template <typename T, bool DOT, int NB>
 __device__ __forceinline__ void cluster_apply_ring(uint32_t w0, uint32_t w1, uint32_t wm, const T (*sX)[NB],
                                                    const T* sx, double* sy, double& acc_e) {
@@ -497,17 +499,10 @@ __global__ __launch_bounds__(NT) void k_apply_cl(Part p, Geo geo, const T* __res
   __syncthreads();
   double acc_e = 0.0;
   if (c0 < bi.e1) {
-    if constexpr (sizeof(T) == 8)
-      cluster_apply_ring<T, DOT, NB>(cw0, cw1, cwm, sX, sx, sy, acc_e);
-    else
-      cluster_apply<T, DOT, NB>(cw0, cw1, cwm, sX, sx, sy, acc_e);
+    cluster_apply_ring<T, DOT, NB>(cw0, cw1, cwm, sX, sx, sy, acc_e);
   }
-  for (int cc = c0 + NT; cc < bi.e1; cc += NT) {
-    if constexpr (sizeof(T) == 8)
-      cluster_apply_ring<T, DOT, NB>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
-    else
-      cluster_apply<T, DOT, NB>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
-  }
+  for (int cc = c0 + NT; cc < bi.e1; cc += NT)
+    cluster_apply_ring<T, DOT, NB>(p.lconn[cc], p.lconn2[cc], p.cmask[cc], sX, sx, sy, acc_e);
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kNode; ++q) {

@@ -120,6 +120,16 @@ __global__ __launch_bounds__(kBlock) void k_pc_init(const double* __restrict__ g
   block_partial<1>(v, part);
 }
 
+// operands of one node for the chunked loops of the Steihaug kernels: up to four 3-vectors and a
+// scalar
+struct Node4x3 {
+  double3 a, b, c, d;
+  double s;
+};
+__device__ __forceinline__ double3 load3(const double* __restrict__ v, long i) {
+  return double3{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+}
+
 __device__ __forceinline__ double pc_active(const double* bank, int max_iter) {
   const double rho = bank[B_RHO];
   const bool act = (bank[B_IT] < (double)max_iter) && (rho > bank[B_TGT]) && (rho > 1e-25) && (rho < 1e20) &&
@@ -152,18 +162,22 @@ __global__ __launch_bounds__(kBlock) void k_pc_ap(const double* __restrict__ ban
                                                   long N, double* __restrict__ part) {
   if (bank[B_ACT] == 0.0) return;
   double v[1] = {0.0};
-  NODE_LOOP(i, N) {
-    const double* mi = m + 3 * i;
-    const double mc = mi[0] * cv[3 * i] + mi[1] * cv[3 * i + 1] + mi[2] * cv[3 * i + 2];
-    const double mg = mi[0] * g[3 * i] + mi[1] * g[3 * i + 1] + mi[2] * g[3 * i + 2];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const double pk = p[3 * i + k];
-      const double qk = cv[3 * i + k] - mc * mi[k] - mg * pk + reg * pk;
-      q[3 * i + k] = qk;
-      v[0] += pk * qk;
-    }
-  }
+  grid_chunks<2>(
+      N, [&](long i) { return Node4x3{load3(m, i), load3(cv, i), load3(g, i), load3(p, i)}; },
+      [&](long i, const Node4x3& u) {
+        const double3 mi = u.a, c = u.b, gi = u.c, pi = u.d;
+        const double mc = mi.x * c.x + mi.y * c.y + mi.z * c.z;
+        const double mg = mi.x * gi.x + mi.y * gi.y + mi.z * gi.z;
+        const double q0 = c.x - mc * mi.x - mg * pi.x + reg * pi.x;
+        const double q1 = c.y - mc * mi.y - mg * pi.y + reg * pi.y;
+        const double q2 = c.z - mc * mi.z - mg * pi.z + reg * pi.z;
+        q[3 * i] = q0;
+        q[3 * i + 1] = q1;
+        q[3 * i + 2] = q2;
+        v[0] += pi.x * q0;
+        v[0] += pi.y * q1;
+        v[0] += pi.z * q2;
+      });
   block_partial<1>(v, part);
 }
 
@@ -189,21 +203,30 @@ __global__ __launch_bounds__(kBlock) void k_pc_yr(double* __restrict__ sc, const
     sc[S_DN] = dn ? 1.0 : 0.0;
   }
   double v[1] = {0.0};
-  NODE_LOOP(i, N) {
-    const double s = Mp[i];
+  const double3 zero3 = {0.0, 0.0, 0.0};
+  grid_chunks<2>(
+      N,
+      [&](long i) {
+        return Node4x3{load3(r, i), dn ? zero3 : load3(y, i), dn ? zero3 : load3(p, i),
+                       dn ? zero3 : load3(q, i), Mp[i]};
+      },
+      [&](long i, const Node4x3& u) {
+        const double s = u.s;
+        const double rr[3] = {u.a.x, u.a.y, u.a.z}, yy[3] = {u.b.x, u.b.y, u.b.z};
+        const double pp[3] = {u.c.x, u.c.y, u.c.z}, qq[3] = {u.d.x, u.d.y, u.d.z};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double rk = r[3 * i + k];
-      if (!dn) {
-        y[3 * i + k] += alpha * p[3 * i + k];
-        rk -= alpha * q[3 * i + k];
-        r[3 * i + k] = rk;
-      }
-      const double zk = rk * s;
-      z[3 * i + k] = zk;
-      v[0] += rk * zk;
-    }
-  }
+        for (int k = 0; k < 3; ++k) {
+          double rk = rr[k];
+          if (!dn) {
+            y[3 * i + k] = yy[k] + alpha * pp[k];
+            rk -= alpha * qq[k];
+            r[3 * i + k] = rk;
+          }
+          const double zk = rk * s;
+          z[3 * i + k] = zk;
+          v[0] += rk * zk;
+        }
+      });
   block_partial<1>(v, part_b);
 }
 
@@ -220,13 +243,15 @@ __global__ __launch_bounds__(kBlock) void k_pc_p(const double* __restrict__ sc, 
   const double rhon = sum_partials_all<kBlock>(part_b, n_part);
   const bool upd = sc[S_DN] == 0.0;
   const double beta = rhon / (bank[B_RHO] + 1e-30);
-  NODE_LOOP(i, N) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      if (upd) p[3 * i + k] = z[3 * i + k] + beta * p[3 * i + k];
-      cv[3 * i + k] = 0.0;
-    }
-  }
+  if (upd)
+    grid_chunks(
+        3 * N, [&](long j) { return double2{z[j], p[j]}; }, [&](long j, double2 u) {
+          p[j] = u.x + beta * u.y;
+          cv[j] = 0.0;
+        });
+  else
+    grid_chunks(
+        3 * N, [&](long) { return 0.0; }, [&](long j, double) { cv[j] = 0.0; });
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     next[B_RHO] = rhon;
     next[B_Q] = bank[B_Q] + sc[S_DQ];

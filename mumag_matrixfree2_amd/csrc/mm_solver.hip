@@ -41,7 +41,8 @@ __global__ void k_copy_calib(double* __restrict__ d, const double* __restrict__ 
 __global__ __launch_bounds__(kBlock) void k_dot_part(const double* __restrict__ a, const double* __restrict__ b, long n,
                                                      double* __restrict__ part) {
   double v[1] = {0.0};
-  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) v[0] += a[i] * b[i];
+  grid_chunks(
+      n, [&](long i) { return double2{a[i], b[i]}; }, [&](long, double2 u) { v[0] += u.x * u.y; });
   block_partial<1>(v, part);
 }
 
@@ -77,14 +78,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(const double* __restrict__ 
                                                      double* __restrict__ r, double* __restrict__ t,
                                                      float* __restrict__ tf, long n, double* __restrict__ part_rr) {
   double v[2] = {0.0, 0.0};
-  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
-    const double ri = b[i] - q[i];
-    r[i] = ri;
-    if (t) t[i] = 0.0;
-    if (tf) tf[i] = 0.0f;
-    v[0] += ri * ri;
-    v[1] += b[i] * b[i];
-  }
+  grid_chunks(
+      n, [&](long i) { return double2{b[i], q[i]}; },
+      [&](long i, double2 u) {
+        const double ri = u.x - u.y;
+        r[i] = ri;
+        if (t) t[i] = 0.0;
+        if (tf) tf[i] = 0.0f;
+        v[0] += ri * ri;
+        v[1] += u.x * u.x;
+      });
   block_partial<2>(v, part_rr);
 }
 
@@ -98,14 +101,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xr(const double* __restrict__ sc
                                                    double* __restrict__ part_rr) {
   const double alpha = sc[rz_slot] / (sum_partials_all<kBlock>(part_pq, n_pq) + 1e-30);
   double v[1] = {0.0};
-  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
-    if (t) t[i] = 0.0;
-    if (tf) tf[i] = 0.0f;
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    v[0] += ri * ri;
-  }
+  grid_chunks(
+      n, [&](long i) { return double4{x[i], p[i], r[i], q[i]}; },
+      [&](long i, double4 u) {
+        if (t) t[i] = 0.0;
+        if (tf) tf[i] = 0.0f;
+        x[i] = u.x + alpha * u.y;
+        const double ri = u.z - alpha * u.w;
+        r[i] = ri;
+        v[0] += ri * ri;
+      });
   block_partial<1>(v, part_rr);
 }
 
@@ -118,10 +123,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p(double* __restrict__ sc, const
   const double rz = sum_partials_all<kBlock>(part_rz, n_rz);
   const double beta = first ? 0.0 : rz / (sc[old_slot] + 1e-30);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[new_slot] = rz;
-  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
-    p[i] = first ? z[i] : z[i] + beta * p[i];
-    q[i] = 0.0;
-  }
+  grid_chunks(
+      n, [&](long i) { return double2{z[i], first ? 0.0 : p[i]}; },
+      [&](long i, double2 u) {
+        p[i] = first ? u.x : u.x + beta * u.y;
+        q[i] = 0.0;
+      });
 }
 
 // x = M .* b (and t = 0 when t != nullptr)
@@ -138,11 +145,13 @@ __global__ __launch_bounds__(kBlock) void k_post_dot(const double* __restrict__ 
                                                      const double* __restrict__ b, const double* __restrict__ t,
                                                      double* __restrict__ z, long n, double* __restrict__ part) {
   double v[1] = {0.0};
-  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
-    const double zi = x[i] + M[i] * (b[i] - t[i]);
-    z[i] = zi;
-    v[0] += b[i] * zi;
-  }
+  grid_chunks(
+      n, [&](long i) { return double4{x[i], M[i], b[i], t[i]}; },
+      [&](long i, double4 u) {
+        const double zi = u.x + u.y * (u.z - u.w);
+        z[i] = zi;
+        v[0] += u.z * zi;
+      });
   block_partial<1>(v, part);
 }
 
@@ -412,11 +421,13 @@ __global__ __launch_bounds__(kBlock) void k_post_dot_mixed(const float* __restri
                                                            const double* __restrict__ r, const float* __restrict__ t,
                                                            double* __restrict__ z, long n, double* __restrict__ part) {
   double v[1] = {0.0};
-  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
-    const double zi = (double)x[i] + (double)M[i] * (r[i] - (double)t[i]);
-    z[i] = zi;
-    v[0] += r[i] * zi;
-  }
+  grid_chunks(
+      n, [&](long i) { return double4{(double)x[i], (double)M[i], r[i], (double)t[i]}; },
+      [&](long i, double4 u) {
+        const double zi = u.x + u.y * (u.z - u.w);
+        z[i] = zi;
+        v[0] += u.z * zi;
+      });
   block_partial<1>(v, part);
 }
 
@@ -1046,11 +1057,13 @@ __global__ __launch_bounds__(kBlock) void k_post_dot_ff(const float* __restrict_
                                                         const float* __restrict__ r, const float* __restrict__ t,
                                                         float* __restrict__ z, long n, double* __restrict__ part) {
   double v[1] = {0.0};
-  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
-    const float zi = x[i] + M[i] * (r[i] - t[i]);
-    z[i] = zi;
-    v[0] += (double)r[i] * (double)zi;
-  }
+  grid_chunks(
+      n, [&](long i) { return float4{x[i], M[i], r[i], t[i]}; },
+      [&](long i, float4 u) {
+        const float zi = u.x + u.y * (u.z - u.w);
+        z[i] = zi;
+        v[0] += (double)u.z * (double)zi;
+      });
   block_partial<1>(v, part);
 }
 
@@ -1063,14 +1076,15 @@ __global__ __launch_bounds__(kBlock) void k_ref_xr(const double* __restrict__ sc
                                                    float* __restrict__ tf, long n, double* __restrict__ part_rr) {
   const double alpha = sc[rz_slot] / (sum_partials_all<kBlock>(part_pq, n_pq) + 1e-30);
   double v[1] = {0.0};
-  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
-    tf[i] = 0.0f;
-    xc[i] = (float)((double)xc[i] + alpha * (double)p[i]);
-    const double ri = (double)r[i] - alpha * (double)q[i];
-    const float rf = (float)ri;
-    r[i] = rf;
-    v[0] += (double)rf * (double)rf;
-  }
+  grid_chunks(
+      n, [&](long i) { return float4{xc[i], p[i], r[i], q[i]}; },
+      [&](long i, float4 a) {
+        tf[i] = 0.0f;
+        xc[i] = (float)((double)a.x + alpha * (double)a.y);
+        const float rf = (float)((double)a.z - alpha * (double)a.w);
+        r[i] = rf;
+        v[0] += (double)rf * (double)rf;
+      });
   block_partial<1>(v, part_rr);
 }
 
@@ -1083,10 +1097,12 @@ __global__ __launch_bounds__(kBlock) void k_ref_p(double* __restrict__ sc, const
   const double rz = sum_partials_all<kBlock>(part_rz, n_rz);
   const double beta = first ? 0.0 : rz / (sc[old_slot] + 1e-30);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[new_slot] = rz;
-  for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
-    p[i] = first ? z[i] : (float)((double)z[i] + beta * (double)p[i]);
-    q[i] = 0.0f;
-  }
+  grid_chunks(
+      n, [&](long i) { return float2{z[i], first ? 0.0f : p[i]}; },
+      [&](long i, float2 u) {
+        p[i] = first ? u.x : (float)((double)u.x + beta * (double)u.y);
+        q[i] = 0.0f;
+      });
 }
 
 // U += xc; q = 0 for the FP64 apply of the true residual
