@@ -39,6 +39,16 @@ static inline int vg(long n) {
 
 #define NODE_LOOP(i, n) for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < (n); i += (long)gridDim.x * kBlock)
 
+// operands of one node for the chunked loops (grid_chunks) of the node kernels: up to four 3-vectors and a
+// scalar
+struct Node4x3 {
+  double3 a, b, c, d;
+  double s;
+};
+__device__ __forceinline__ double3 load3(const double* __restrict__ v, long i) {
+  return double3{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+}
+
 __device__ __forceinline__ void tangent3(const double* m, const double* g, double s, double* out) {
   const double md = m[0] * g[0] + m[1] * g[1] + m[2] * g[2];
   out[0] = s * (g[0] - md * m[0]);
@@ -63,20 +73,23 @@ __global__ __launch_bounds__(kBlock) void k_tangents(const double* __restrict__ 
                                                      const double* __restrict__ invMr, double* __restrict__ g_tan,
                                                      double* __restrict__ g_ext, long N, double* __restrict__ part) {
   double v[1] = {0.0};
-  NODE_LOOP(i, N) {
-    double t[3];
-    tangent3(m + 3 * i, g + 3 * i, 1.0, t);
-    g_ext[3 * i] = t[0];
-    g_ext[3 * i + 1] = t[1];
-    g_ext[3 * i + 2] = t[2];
-    const double s = invMr[i];
-    double gs[3] = {g[3 * i] * s, g[3 * i + 1] * s, g[3 * i + 2] * s};
-    tangent3(m + 3 * i, gs, 1.0, t);
-    g_tan[3 * i] = t[0];
-    g_tan[3 * i + 1] = t[1];
-    g_tan[3 * i + 2] = t[2];
-    v[0] = fmax(v[0], fmax(fabs(t[0]), fmax(fabs(t[1]), fabs(t[2]))));
-  }
+  grid_chunks<2>(
+      N, [&](long i) { return Node4x3{load3(m, i), load3(g, i), {}, {}, invMr[i]}; },
+      [&](long i, const Node4x3& u) {
+        const double mi[3] = {u.a.x, u.a.y, u.a.z}, gi[3] = {u.b.x, u.b.y, u.b.z};
+        double t[3];
+        tangent3(mi, gi, 1.0, t);
+        g_ext[3 * i] = t[0];
+        g_ext[3 * i + 1] = t[1];
+        g_ext[3 * i + 2] = t[2];
+        const double s = u.s;
+        double gs[3] = {gi[0] * s, gi[1] * s, gi[2] * s};
+        tangent3(mi, gs, 1.0, t);
+        g_tan[3 * i] = t[0];
+        g_tan[3 * i + 1] = t[1];
+        g_tan[3 * i + 2] = t[2];
+        v[0] = fmax(v[0], fmax(fabs(t[0]), fmax(fabs(t[1]), fabs(t[2]))));
+      });
   block_partial<1, kBlock, true>(v, part);
 }
 
@@ -84,7 +97,8 @@ __global__ __launch_bounds__(kBlock) void k_tangents(const double* __restrict__ 
 __global__ __launch_bounds__(kBlock) void k_maxabs_part(const double* __restrict__ a, long n,
                                                         double* __restrict__ part) {
   double v[1] = {0.0};
-  NODE_LOOP(i, n) v[0] = fmax(v[0], fabs(a[i]));
+  grid_chunks(
+      n, [&](long i) { return a[i]; }, [&](long, double u) { v[0] = fmax(v[0], fabs(u)); });
   block_partial<1, kBlock, true>(v, part);
 }
 
@@ -104,30 +118,23 @@ __global__ __launch_bounds__(kBlock) void k_pc_init(const double* __restrict__ g
                                                     double* __restrict__ z, double* __restrict__ p,
                                                     double* __restrict__ cv, long N, double* __restrict__ part) {
   double v[1] = {0.0};
-  NODE_LOOP(i, N) {
-    const double s = Mp[i];
+  grid_chunks(
+      N, [&](long i) { return Node4x3{load3(g_ext, i), {}, {}, {}, Mp[i]}; },
+      [&](long i, const Node4x3& u) {
+        const double s = u.s;
+        const double rr[3] = {u.a.x, u.a.y, u.a.z};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const double ri = g_ext[3 * i + k];
-      y[3 * i + k] = 0.0;
-      r[3 * i + k] = ri;
-      z[3 * i + k] = ri * s;
-      p[3 * i + k] = ri * s;
-      cv[3 * i + k] = 0.0;
-      v[0] += ri * ri * s;
-    }
-  }
+        for (int k = 0; k < 3; ++k) {
+          const double ri = rr[k];
+          y[3 * i + k] = 0.0;
+          r[3 * i + k] = ri;
+          z[3 * i + k] = ri * s;
+          p[3 * i + k] = ri * s;
+          cv[3 * i + k] = 0.0;
+          v[0] += ri * ri * s;
+        }
+      });
   block_partial<1>(v, part);
-}
-
-// operands of one node for the chunked loops of the Steihaug kernels: up to four 3-vectors and a
-// scalar
-struct Node4x3 {
-  double3 a, b, c, d;
-  double s;
-};
-__device__ __forceinline__ double3 load3(const double* __restrict__ v, long i) {
-  return double3{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
 }
 
 __device__ __forceinline__ double pc_active(const double* bank, int max_iter) {
@@ -203,30 +210,38 @@ __global__ __launch_bounds__(kBlock) void k_pc_yr(double* __restrict__ sc, const
     sc[S_DN] = dn ? 1.0 : 0.0;
   }
   double v[1] = {0.0};
-  const double3 zero3 = {0.0, 0.0, 0.0};
-  grid_chunks<2>(
-      N,
-      [&](long i) {
-        return Node4x3{load3(r, i), dn ? zero3 : load3(y, i), dn ? zero3 : load3(p, i),
-                       dn ? zero3 : load3(q, i), Mp[i]};
-      },
-      [&](long i, const Node4x3& u) {
-        const double s = u.s;
-        const double rr[3] = {u.a.x, u.a.y, u.a.z}, yy[3] = {u.b.x, u.b.y, u.b.z};
-        const double pp[3] = {u.c.x, u.c.y, u.c.z}, qq[3] = {u.d.x, u.d.y, u.d.z};
+  // z = Mp r and the r.z partials, after the update of y and r unless done (the branch on dn
+  // stays outside the loops: inside, the compiler waits for each load)
+  auto zr = [&](long i, const double (&rr)[3], double s) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          double rk = rr[k];
-          if (!dn) {
+    for (int k = 0; k < 3; ++k) {
+      const double zk = rr[k] * s;
+      z[3 * i + k] = zk;
+      v[0] += rr[k] * zk;
+    }
+  };
+  if (dn)
+    grid_chunks<2>(
+        N, [&](long i) { return Node4x3{load3(r, i), {}, {}, {}, Mp[i]}; },
+        [&](long i, const Node4x3& u) {
+          const double rr[3] = {u.a.x, u.a.y, u.a.z};
+          zr(i, rr, u.s);
+        });
+  else
+    grid_chunks<2>(
+        N, [&](long i) { return Node4x3{load3(r, i), load3(y, i), load3(p, i), load3(q, i), Mp[i]}; },
+        [&](long i, const Node4x3& u) {
+          const double yy[3] = {u.b.x, u.b.y, u.b.z}, pp[3] = {u.c.x, u.c.y, u.c.z};
+          const double r0[3] = {u.a.x, u.a.y, u.a.z}, qq[3] = {u.d.x, u.d.y, u.d.z};
+          double rr[3];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
             y[3 * i + k] = yy[k] + alpha * pp[k];
-            rk -= alpha * qq[k];
-            r[3 * i + k] = rk;
+            rr[k] = r0[k] - alpha * qq[k];
+            r[3 * i + k] = rr[k];
           }
-          const double zk = rk * s;
-          z[3 * i + k] = zk;
-          v[0] += rk * zk;
-        }
-      });
+          zr(i, rr, u.s);
+        });
   block_partial<1>(v, part_b);
 }
 
@@ -267,16 +282,19 @@ __global__ __launch_bounds__(kBlock) void k_pc_norms(const double* __restrict__ 
                                                      const double* __restrict__ Mp, long N,
                                                      double* __restrict__ part) {
   double v[3] = {0.0, 0.0, 0.0};
-  NODE_LOOP(i, N) {
-    const double s = Mp[i];
+  grid_chunks<2>(
+      N, [&](long i) { return Node4x3{load3(y, i), load3(g, i), {}, {}, Mp[i]}; },
+      [&](long, const Node4x3& u) {
+        const double s = u.s;
+        const double yy[3] = {u.a.x, u.a.y, u.a.z}, gg[3] = {u.b.x, u.b.y, u.b.z};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const double yk = y[3 * i + k], gk = g[3 * i + k];
-      v[0] += yk * yk;
-      v[1] += gk * s * gk * s;
-      v[2] += yk * gk;
-    }
-  }
+        for (int k = 0; k < 3; ++k) {
+          const double yk = yy[k], gk = gg[k];
+          v[0] += yk * yk;
+          v[1] += gk * s * gk * s;
+          v[2] += yk * gk;
+        }
+      });
   block_partial<3>(v, part);
 }
 
@@ -309,11 +327,17 @@ __global__ __launch_bounds__(kBlock) void k_pc_finish(double* __restrict__ sc, c
   const double yn = sqrt(yy), zn = sqrt(zz);
   const double scale = (yn > 10.0 * zn) ? (10.0 * zn / (yn + 1e-30)) : 1.0;
   const bool fallback = !(scale * yg > 1e-12);
-  NODE_LOOP(i, N) {
-    const double s = Mp[i];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) y[3 * i + k] = fallback ? g[3 * i + k] * s : y[3 * i + k] * scale;
-  }
+  if (fallback)  // uniform branches outside the loops, so the loads of a round go out together
+    grid_chunks(
+        N, [&](long i) { return Node4x3{load3(g, i), {}, {}, {}, Mp[i]}; },
+        [&](long i, const Node4x3& u) {
+          y[3 * i] = u.a.x * u.s;
+          y[3 * i + 1] = u.a.y * u.s;
+          y[3 * i + 2] = u.a.z * u.s;
+        });
+  else
+    grid_chunks(
+        3 * N, [&](long j) { return y[j]; }, [&](long j, double u) { y[j] = u * scale; });
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc[S_YY] = yy;
     sc[S_ZZ] = zz;
@@ -328,11 +352,13 @@ __global__ __launch_bounds__(kBlock) void k_beta_dots(const double* __restrict__
                                                       const double* __restrict__ gp, const double* __restrict__ dp,
                                                       long n3, double* __restrict__ part) {
   double v[2] = {0.0, 0.0};
-  NODE_LOOP(i, n3) {
-    const double dg = ge[i] - gp[i];
-    v[0] += y[i] * dg;
-    v[1] += dp[i] * dg;
-  }
+  grid_chunks(
+      n3, [&](long i) { return double4{ge[i], gp[i], y[i], dp[i]}; },
+      [&](long, double4 u) {
+        const double dg = u.x - u.y;
+        v[0] += u.z * dg;
+        v[1] += u.w * dg;
+      });
   block_partial<2>(v, part);
 }
 
@@ -347,16 +373,20 @@ __global__ __launch_bounds__(kBlock) void k_direction(int restart, const double*
   sum_partials_k<2, kBlock>(part_nd, n_nd, nd);
   const double beta = restart ? 0.0 : fmax(0.0, nd[0] / (nd[1] + 1e-30));
   double v[1] = {0.0};
-  NODE_LOOP(i, N) {
-    double t[3];
-    tangent3(m + 3 * i, dp + 3 * i, 1.0, t);
+  grid_chunks<2>(
+      N, [&](long i) { return Node4x3{load3(m, i), load3(dp, i), load3(y, i), load3(ge, i)}; },
+      [&](long i, const Node4x3& u) {
+        const double mi[3] = {u.a.x, u.a.y, u.a.z}, di[3] = {u.b.x, u.b.y, u.b.z};
+        const double yi[3] = {u.c.x, u.c.y, u.c.z}, gi[3] = {u.d.x, u.d.y, u.d.z};
+        double t[3];
+        tangent3(mi, di, 1.0, t);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const double dk = -y[3 * i + k] + beta * t[k];
-      d[3 * i + k] = dk;
-      v[0] += dk * ge[3 * i + k];
-    }
-  }
+        for (int k = 0; k < 3; ++k) {
+          const double dk = -yi[k] + beta * t[k];
+          d[3 * i + k] = dk;
+          v[0] += dk * gi[k];
+        }
+      });
   block_partial<1>(v, part_dg);
 }
 
@@ -369,19 +399,24 @@ __global__ __launch_bounds__(kBlock) void k_direction_fix(int force, const doubl
                                                           double* __restrict__ part_pg) {
   const bool flip = force || sum_partials_all<kBlock>(part_dg, n_dg) > 0.0;
   double v[1] = {0.0};
-  NODE_LOOP(i, N) {
-    double dd[3];
+  // the uniform flip is applied after the loads, which all go out first
+  const double* src = flip ? alt : d;
+  const double sgn = flip ? -1.0 : 1.0;
+  grid_chunks<2>(
+      N, [&](long i) { return Node4x3{load3(src, i), load3(g, i), load3(m, i), {}}; },
+      [&](long i, const Node4x3& u) {
+        const double dd[3] = {sgn * u.a.x, sgn * u.a.y, sgn * u.a.z};
+        const double gi[3] = {u.b.x, u.b.y, u.b.z};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      dd[k] = flip ? -alt[3 * i + k] : d[3 * i + k];
-      d[3 * i + k] = dd[k];
-      v[0] += g[3 * i + k] * dd[k];
-    }
-    const double* mi = m + 3 * i;
-    H[3 * i + 0] = mi[1] * dd[2] - mi[2] * dd[1];
-    H[3 * i + 1] = mi[2] * dd[0] - mi[0] * dd[2];
-    H[3 * i + 2] = mi[0] * dd[1] - mi[1] * dd[0];
-  }
+        for (int k = 0; k < 3; ++k) {
+          d[3 * i + k] = dd[k];
+          v[0] += gi[k] * dd[k];
+        }
+        const double mi[3] = {u.c.x, u.c.y, u.c.z};
+        H[3 * i + 0] = mi[1] * dd[2] - mi[2] * dd[1];
+        H[3 * i + 1] = mi[2] * dd[0] - mi[0] * dd[2];
+        H[3 * i + 2] = mi[0] * dd[1] - mi[1] * dd[0];
+      });
   block_partial<1>(v, part_pg);
 }
 
@@ -408,7 +443,8 @@ __global__ void k_cayley(const double* __restrict__ m, const double* __restrict_
 __global__ __launch_bounds__(kBlock) void k_maxdiff(const double* __restrict__ a, const double* __restrict__ b,
                                                     long n, double* __restrict__ part) {
   double v[1] = {0.0};
-  NODE_LOOP(i, n) v[0] = fmax(v[0], fabs(a[i] - b[i]));
+  grid_chunks(
+      n, [&](long i) { return double2{a[i], b[i]}; }, [&](long, double2 u) { v[0] = fmax(v[0], fabs(u.x - u.y)); });
   block_partial<1, kBlock, true>(v, part);
 }
 

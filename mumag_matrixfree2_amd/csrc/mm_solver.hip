@@ -123,12 +123,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p(double* __restrict__ sc, const
   const double rz = sum_partials_all<kBlock>(part_rz, n_rz);
   const double beta = first ? 0.0 : rz / (sc[old_slot] + 1e-30);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[new_slot] = rz;
-  grid_chunks(
-      n, [&](long i) { return double2{z[i], first ? 0.0 : p[i]}; },
-      [&](long i, double2 u) {
-        p[i] = first ? u.x : u.x + beta * u.y;
-        q[i] = 0.0;
-      });
+  // the branch on first stays outside the loops: inside, the compiler waits for each load
+  if (first)
+    grid_chunks(
+        n, [&](long i) { return z[i]; },
+        [&](long i, double u) {
+          p[i] = u;
+          q[i] = 0.0;
+        });
+  else
+    grid_chunks(
+        n, [&](long i) { return double2{z[i], p[i]}; },
+        [&](long i, double2 u) {
+          p[i] = u.x + beta * u.y;
+          q[i] = 0.0;
+        });
 }
 
 // x = M .* b (and t = 0 when t != nullptr)
@@ -1097,12 +1106,20 @@ __global__ __launch_bounds__(kBlock) void k_ref_p(double* __restrict__ sc, const
   const double rz = sum_partials_all<kBlock>(part_rz, n_rz);
   const double beta = first ? 0.0 : rz / (sc[old_slot] + 1e-30);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[new_slot] = rz;
-  grid_chunks(
-      n, [&](long i) { return float2{z[i], first ? 0.0f : p[i]}; },
-      [&](long i, float2 u) {
-        p[i] = first ? u.x : (float)((double)u.x + beta * (double)u.y);
-        q[i] = 0.0f;
-      });
+  if (first)  // outside the loops, as in k_pcg_p
+    grid_chunks(
+        n, [&](long i) { return z[i]; },
+        [&](long i, float u) {
+          p[i] = u;
+          q[i] = 0.0f;
+        });
+  else
+    grid_chunks(
+        n, [&](long i) { return float2{z[i], p[i]}; },
+        [&](long i, float2 u) {
+          p[i] = (float)((double)u.x + beta * (double)u.y);
+          q[i] = 0.0f;
+        });
 }
 
 // U += xc; q = 0 for the FP64 apply of the true residual
