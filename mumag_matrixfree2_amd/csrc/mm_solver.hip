@@ -214,44 +214,6 @@ __device__ __forceinline__ TA row_products(int j0, int j1, int lane, const int* 
   return s;
 }
 
-// Products of two rows (a0..a1, b0..b1) in one lane, their index/value loads issued together
-// and then their gathers together: a one-lane-per-row product (the prolongation's ~3-entry rows)
-// then keeps twice the loads in flight per wave and needs half the waves. Each row still adds
-// its entries in increasing order, as row_products does with one lane. Padding entries re-read
-// an entry of a non-empty row of the pair and add an exact zero.
-template <int MODE, typename TA, typename TX, typename TX2>
-__device__ __forceinline__ void row_products_2(int a0, int a1, int b0, int b1, const int* __restrict__ Ai,
-                                               const TA* __restrict__ Ax, const TX* __restrict__ x,
-                                               const TX2* __restrict__ x2, TA& sa, TA& sb) {
-  constexpr int U = spmv_unroll<1>();
-  sa = 0;
-  sb = 0;
-  const int safe = a1 > a0 ? a0 : b0;  // a valid entry whenever the loop runs
-  for (int k = 0; a0 + k < a1 || b0 + k < b1; k += U) {
-    int cu[2 * U];
-    TA au[2 * U];
-    bool ok[2 * U];
-#pragma unroll
-    for (int u = 0; u < 2 * U; ++u) {
-      const int j = (u < U ? a0 : b0) + k + (u % U);
-      ok[u] = j < (u < U ? a1 : b1);
-      const int jj = ok[u] ? j : safe;
-      cu[u] = Ai[jj];
-      au[u] = Ax[jj];
-    }
-    TA xv[2 * U];
-#pragma unroll
-    for (int u = 0; u < 2 * U; ++u) {
-      const int c = cu[u];
-      xv[u] = (MODE == 2) ? (TA)((TA)x[c] - (TA)x2[c]) : (TA)x[c];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) sa += ok[u] ? au[u] * xv[u] : (TA)0;
-#pragma unroll
-    for (int u = U; u < 2 * U; ++u) sb += ok[u] ? au[u] * xv[u] : (TA)0;
-  }
-}
-
 // CSR SpMV with LANES lanes per row, accumulation in the matrix value type TA.
 // MODE 0: y = A x; 1: y += A x (and z = 0 on the same rows when z != nullptr); 2: y = A (x - x2)
 // and, when z != nullptr, z = zw .* y on the same rows (a restriction that also forms the next
@@ -262,41 +224,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int row0, int nrows, const int*
                                                  const int* __restrict__ Ai, const TA* __restrict__ Ax,
                                                  const TX* __restrict__ x, const TX2* __restrict__ x2,
                                                  TY* __restrict__ y, TY* __restrict__ z, const TY* __restrict__ zw) {
-  if constexpr (LANES == 1) {
-    // two rows per lane, kBlock rows apart (each wave's 64 rows stay consecutive)
-    const int ra = row0 + (int)((long)blockIdx.x * 2 * kBlock + threadIdx.x), rb = ra + kBlock;
-    const bool la = ra < nrows, lb = rb < nrows;
-    if (!la) return;
-    TY ya = 0, yb = 0, wa = 0, wb = 0;
-    if (MODE == 1) {
-      ya = y[ra];
-      if (lb) yb = y[rb];
-    }
-    if (MODE == 2 && z) {
-      wa = zw[ra];
-      if (lb) wb = zw[rb];
-    }
-    const int a0 = Ap[ra], a1 = Ap[ra + 1];
-    const int b0 = lb ? Ap[rb] : 0, b1 = lb ? Ap[rb + 1] : 0;
-    TA sa, sb;
-    row_products_2<MODE>(a0, a1, b0, b1, Ai, Ax, x, x2, sa, sb);
-    if (MODE == 1) {
-      y[ra] = ya + (TY)sa;
-      if (z) z[ra] = 0;
-      if (lb) {
-        y[rb] = yb + (TY)sb;
-        if (z) z[rb] = 0;
-      }
-    } else {
-      y[ra] = (TY)sa;
-      if (MODE == 2 && z) z[ra] = wa * (TY)sa;
-      if (lb) {
-        y[rb] = (TY)sb;
-        if (MODE == 2 && z) z[rb] = wb * (TY)sb;
-      }
-    }
-    return;
-  }
   const int lane = threadIdx.x & (LANES - 1);
   const int row = row0 + (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
   const bool live = row < nrows;
@@ -434,9 +361,7 @@ static void symv(const Level& L, const T* tiles, const T* b, T* y, hipStream_t s
 template <int MODE, typename TA, typename TX, typename TX2, typename TY>
 static void spmv_lanes(int lanes, int row0, int row1, const int* Ap, const int* Ai, const TA* Ax, const TX* x,
                        const TX2* x2, TY* y, TY* z, const TY* zw, hipStream_t s) {
-  // one lane per row: a workgroup covers 2 kBlock rows (two rows per lane, k_spmv)
-  const long rows = (long)(row1 - row0);
-  const int grid = (int)(lanes == 1 ? (rows + 2 * kBlock - 1) / (2 * kBlock) : (rows * lanes + kBlock - 1) / kBlock);
+  const int grid = (int)(((long)(row1 - row0) * lanes + kBlock - 1) / kBlock);
   switch (lanes) {
     case 1: k_spmv<MODE, 1, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
     case 2: k_spmv<MODE, 2, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
