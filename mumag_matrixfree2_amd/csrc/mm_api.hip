@@ -887,6 +887,8 @@ int alloc_min_state(Ctx& c) {
     MM_TRY(dalloc(c, v, n3));
   MM_TRY(dalloc(c, &c.U, n));
   MM_TRY(dalloc(c, &c.U_t, n));
+  if (c.precision >= 1)  // the mixed forms' FP32 Steihaug vectors (else allocated by mm_set_precision)
+    for (float** v : {&c.r3f, &c.z3f, &c.p3f, &c.q3f, &c.cv3f}) MM_TRY(dalloc(c, v, n3));
   MM_TRY(dalloc(c, &c.mpart, 9 * 2 * (size_t)kRedGridMax));
   MM_HIP(hipHostMalloc((void**)&c.hmpart, 9 * 2 * kRedGridMax * sizeof(double), hipHostMallocDefault));
   return 0;
@@ -1033,6 +1035,8 @@ int mm_set_precision(int64_t h, int precision) {
            "(2 with FP32 energy densities)");
   if (precision >= 2 && c.rxf == nullptr)
     for (float** v : {&c.rxf, &c.rrf, &c.rpf, &c.rqf, &c.rzf}) MM_TRY(dalloc(c, v, (size_t)c.N));
+  if (precision >= 1 && c.r3f == nullptr && c.r3 != nullptr)  // contexts with minimiser state (else alloc_min_state)
+    for (float** v : {&c.r3f, &c.z3f, &c.p3f, &c.q3f, &c.cv3f}) MM_TRY(dalloc(c, v, 3 * (size_t)c.N));
   c.precision = precision;
   return 0;
 }

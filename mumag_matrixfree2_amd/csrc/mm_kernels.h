@@ -233,6 +233,8 @@ struct Ctx {
   double *y = nullptr, *d = nullptr, *d_prev = nullptr, *H = nullptr;
   double *m_t = nullptr, *g_t = nullptr, *U_t = nullptr /*N*/;
   double *r3 = nullptr, *z3 = nullptr, *p3 = nullptr, *q3 = nullptr, *cv3 = nullptr;
+  // the Steihaug loop's r, z, p, q and C p in FP32 (the mixed-precision forms; allocated with them)
+  float *r3f = nullptr, *z3f = nullptr, *p3f = nullptr, *q3f = nullptr, *cv3f = nullptr;
   int mem = 10;
   double *S = nullptr, *Yh = nullptr;  // mem x 3N (FP64 history)
   float *Sf = nullptr, *Yf = nullptr;  // mem x 3N (FP32 history of the mixed-precision forms)
@@ -336,6 +338,8 @@ int launch_energy_buf(const Ctx& c, const double* m, const double* U, const doub
 int launch_energy(const Ctx& c, const double* m, const double* U, const double B[3], double* g, double* E_out,
                   hipStream_t s);
 int launch_local(const Ctx& c, const double* v, double* out, const double* gate, hipStream_t s);
+// the same on FP32 vectors (the mixed-precision forms' Steihaug loop)
+int launch_local(const Ctx& c, const float* v, float* out, const double* gate, hipStream_t s);
 int launch_geometry(const Ctx& c, const int* conn_dev, hipStream_t s);
 int launch_diag(const Ctx& c, double* d, hipStream_t s);
 int launch_jpar(const Ctx& c, const double* m, const double h[3], double* out, hipStream_t s);

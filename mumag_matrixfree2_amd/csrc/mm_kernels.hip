@@ -1103,10 +1103,11 @@ __global__ __launch_bounds__(kBlock) void k_seg_block_sums(const int* __restrict
 // --------------------------------------------------- local Hessian action C v (K8)
 // C v = (K_ex + K_an) v / V_mag: exchange and anisotropy parts of K7 without demag/Zeeman.
 // T = float: element arithmetic in FP32 on block-relative FP32 coordinates and FP32 copies of v
-// (the mixed-precision forms' Steihaug loop), accumulation and output in FP64.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const double* __restrict__ vin,
-                                                  double inv_V, double* __restrict__ out,
+// (the mixed-precision forms' Steihaug loop), accumulation in FP64. TV: the type of v and of the
+// output (FP32 in the mixed forms' Steihaug loop, whose vectors are FP32).
+template <typename T, typename TV = double>
+__global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const TV* __restrict__ vin,
+                                                  double inv_V, TV* __restrict__ out,
                                                   const double* __restrict__ gate) {
   if (gate != nullptr && *gate == 0.0) return;
   __shared__ T sm[3][kMaxBlockNodes];
@@ -1172,10 +1173,10 @@ __global__ __launch_bounds__(kBlock) void k_local(Part p, Geo geo, const double*
     const uint32_t id = fl & kIdMask;
     if (fl & kPrivate) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) out[3 * id + k] = sg[k][tid] * iv;
+      for (int k = 0; k < 3; ++k) out[3 * id + k] = (TV)(sg[k][tid] * iv);
     } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&out[3 * id + k], sg[k][tid] * iv);
+      for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&out[3 * id + k], (TV)(sg[k][tid] * iv));
     }
   }
 }
@@ -1547,6 +1548,13 @@ int launch_local(const Ctx& c, const double* v, double* out, const double* gate,
     k_local<float><<<grid_for(p), kBlock, 0, s>>>(p, c.geo, v, 1.0 / c.V_mag, out, gate);
   else
     k_local<double><<<grid_for(p), kBlock, 0, s>>>(p, c.geo, v, 1.0 / c.V_mag, out, gate);
+  return 0;
+}
+
+int launch_local(const Ctx& c, const float* v, float* out, const double* gate, hipStream_t s) {
+  const Part& p = c.part_mag;
+  if (part_len(p) == 0) return 0;
+  k_local<float, float><<<grid_for(p), kBlock, 0, s>>>(p, c.geo, v, 1.0 / c.V_mag, out, gate);
   return 0;
 }
 

@@ -11,6 +11,7 @@
 // device scalar block.
 #include <chrono>
 #include <cmath>
+#include <type_traits>
 #include <utility>
 
 #include "mm_device.h"
@@ -45,8 +46,9 @@ struct Node4x3 {
   double3 a, b, c, d;
   double s;
 };
-__device__ __forceinline__ double3 load3(const double* __restrict__ v, long i) {
-  return double3{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+template <typename TV>
+__device__ __forceinline__ double3 load3(const TV* __restrict__ v, long i) {
+  return double3{(double)v[3 * i], (double)v[3 * i + 1], (double)v[3 * i + 2]};
 }
 
 __device__ __forceinline__ void tangent3(const double* m, const double* g, double s, double* out) {
@@ -112,11 +114,17 @@ enum PcBank { B_RHO = 0, B_Q = 1, B_IT = 2, B_DONE = 3, B_ACT = 4, B_TGT = 5 };
 constexpr int kPcBank0 = 224;  // banks at slots 224..229 and 232..237
 __host__ __device__ constexpr int pc_bank(int k) { return kPcBank0 + 8 * (k & 1); }
 
+// The loop's vectors r, z, p, q and C p are stored as TV: FP64, or FP32 in the mixed-precision
+// forms (whose Hessian action is FP32 arithmetic already); y, g_ext, m, g and the scalars stay
+// FP64, and every product is formed in FP64 from the stored values, so with TV = double the
+// arithmetic is exactly the FP64 loop's.
+
 // y = 0; r = g_ext; z = Mp r; p = z; partials of rho = r.z
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_pc_init(const double* __restrict__ g_ext, const double* __restrict__ Mp,
-                                                    double* __restrict__ y, double* __restrict__ r,
-                                                    double* __restrict__ z, double* __restrict__ p,
-                                                    double* __restrict__ cv, long N, double* __restrict__ part) {
+                                                    double* __restrict__ y, TV* __restrict__ r,
+                                                    TV* __restrict__ z, TV* __restrict__ p,
+                                                    TV* __restrict__ cv, long N, double* __restrict__ part) {
   double v[1] = {0.0};
   grid_chunks(
       N, [&](long i) { return Node4x3{load3(g_ext, i), {}, {}, {}, Mp[i]}; },
@@ -125,13 +133,17 @@ __global__ __launch_bounds__(kBlock) void k_pc_init(const double* __restrict__ g
         const double rr[3] = {u.a.x, u.a.y, u.a.z};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          const double ri = rr[k];
+          const double ri = (double)(TV)rr[k];
+          const double zi = (double)(TV)(ri * s);
           y[3 * i + k] = 0.0;
-          r[3 * i + k] = ri;
-          z[3 * i + k] = ri * s;
-          p[3 * i + k] = ri * s;
-          cv[3 * i + k] = 0.0;
-          v[0] += ri * ri * s;
+          r[3 * i + k] = (TV)ri;
+          z[3 * i + k] = (TV)zi;
+          p[3 * i + k] = (TV)zi;
+          cv[3 * i + k] = (TV)0;
+          if constexpr (sizeof(TV) == 8)
+            v[0] += ri * ri * s;  // the FP64 loop's own order
+          else
+            v[0] += ri * zi;  // r.z of the stored vectors
         }
       });
   block_partial<1>(v, part);
@@ -163,9 +175,10 @@ __global__ __launch_bounds__(kBlock) void k_pc_state_init(double* __restrict__ s
 }
 
 // q = P p = Cv - (m.Cv) m - (m.g) p + reg p; partials of p.q
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_pc_ap(const double* __restrict__ bank, const double* __restrict__ m,
-                                                  const double* __restrict__ g, const double* __restrict__ p,
-                                                  const double* __restrict__ cv, double* __restrict__ q, double reg,
+                                                  const double* __restrict__ g, const TV* __restrict__ p,
+                                                  const TV* __restrict__ cv, TV* __restrict__ q, double reg,
                                                   long N, double* __restrict__ part) {
   if (bank[B_ACT] == 0.0) return;
   double v[1] = {0.0};
@@ -175,12 +188,12 @@ __global__ __launch_bounds__(kBlock) void k_pc_ap(const double* __restrict__ ban
         const double3 mi = u.a, c = u.b, gi = u.c, pi = u.d;
         const double mc = mi.x * c.x + mi.y * c.y + mi.z * c.z;
         const double mg = mi.x * gi.x + mi.y * gi.y + mi.z * gi.z;
-        const double q0 = c.x - mc * mi.x - mg * pi.x + reg * pi.x;
-        const double q1 = c.y - mc * mi.y - mg * pi.y + reg * pi.y;
-        const double q2 = c.z - mc * mi.z - mg * pi.z + reg * pi.z;
-        q[3 * i] = q0;
-        q[3 * i + 1] = q1;
-        q[3 * i + 2] = q2;
+        const double q0 = (double)(TV)(c.x - mc * mi.x - mg * pi.x + reg * pi.x);
+        const double q1 = (double)(TV)(c.y - mc * mi.y - mg * pi.y + reg * pi.y);
+        const double q2 = (double)(TV)(c.z - mc * mi.z - mg * pi.z + reg * pi.z);
+        q[3 * i] = (TV)q0;
+        q[3 * i + 1] = (TV)q1;
+        q[3 * i + 2] = (TV)q2;
         v[0] += pi.x * q0;
         v[0] += pi.y * q1;
         v[0] += pi.z * q2;
@@ -190,11 +203,12 @@ __global__ __launch_bounds__(kBlock) void k_pc_ap(const double* __restrict__ ban
 
 // p.q from the partials -> alpha, dq, negative curvature / stagnation (workgroup 0 stores them);
 // y += alpha p; r -= alpha q (unless done now); z = Mp r; partials of rho_n = r.z
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_pc_yr(double* __restrict__ sc, const double* __restrict__ bank,
                                                   const double* __restrict__ part_a, int n_part, double nu,
-                                                  double* __restrict__ y, double* __restrict__ r,
-                                                  double* __restrict__ z, const double* __restrict__ p,
-                                                  const double* __restrict__ q, const double* __restrict__ Mp, long N,
+                                                  double* __restrict__ y, TV* __restrict__ r,
+                                                  TV* __restrict__ z, const TV* __restrict__ p,
+                                                  const TV* __restrict__ q, const double* __restrict__ Mp, long N,
                                                   double* __restrict__ part_b) {
   if (bank[B_ACT] == 0.0) return;
   const double pAp = sum_partials_all<kBlock>(part_a, n_part);
@@ -215,8 +229,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_yr(double* __restrict__ sc, const
   auto zr = [&](long i, const double (&rr)[3], double s) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const double zk = rr[k] * s;
-      z[3 * i + k] = zk;
+      const double zk = (double)(TV)(rr[k] * s);
+      z[3 * i + k] = (TV)zk;
       v[0] += rr[k] * zk;
     }
   };
@@ -237,8 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_yr(double* __restrict__ sc, const
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
             y[3 * i + k] = yy[k] + alpha * pp[k];
-            rr[k] = r0[k] - alpha * qq[k];
-            r[3 * i + k] = rr[k];
+            rr[k] = (double)(TV)(r0[k] - alpha * qq[k]);
+            r[3 * i + k] = (TV)rr[k];
           }
           zr(i, rr, u.s);
         });
@@ -247,10 +261,11 @@ __global__ __launch_bounds__(kBlock) void k_pc_yr(double* __restrict__ sc, const
 
 // rho_n from the partials; p = z + beta p (unless done now); cv = 0 for the next iteration;
 // workgroup 0 writes the next bank (a copy of this one when the loop is no longer active)
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_pc_p(const double* __restrict__ sc, const double* __restrict__ bank,
                                                  double* __restrict__ next, const double* __restrict__ part_b,
-                                                 int n_part, const double* __restrict__ z, double* __restrict__ p,
-                                                 double* __restrict__ cv, int max_iter, long N) {
+                                                 int n_part, const TV* __restrict__ z, TV* __restrict__ p,
+                                                 TV* __restrict__ cv, int max_iter, long N) {
   if (bank[B_ACT] == 0.0) {
     if (blockIdx.x == 0 && threadIdx.x < 6) next[threadIdx.x] = bank[threadIdx.x];
     return;
@@ -260,13 +275,13 @@ __global__ __launch_bounds__(kBlock) void k_pc_p(const double* __restrict__ sc, 
   const double beta = rhon / (bank[B_RHO] + 1e-30);
   if (upd)
     grid_chunks(
-        3 * N, [&](long j) { return double2{z[j], p[j]}; }, [&](long j, double2 u) {
-          p[j] = u.x + beta * u.y;
-          cv[j] = 0.0;
+        3 * N, [&](long j) { return double2{(double)z[j], (double)p[j]}; }, [&](long j, double2 u) {
+          p[j] = (TV)(u.x + beta * u.y);
+          cv[j] = (TV)0;
         });
   else
     grid_chunks(
-        3 * N, [&](long) { return 0.0; }, [&](long j, double) { cv[j] = 0.0; });
+        3 * N, [&](long) { return 0.0; }, [&](long j, double) { cv[j] = (TV)0; });
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     next[B_RHO] = rhon;
     next[B_Q] = bank[B_Q] + sc[S_DQ];
@@ -641,18 +656,27 @@ static int solve_P(Ctx& c, const MinParams& p) {
   const int gr = red_grid(N);
   double* part_a = c.part_rz;  // p.q partials (the PCG is not running meanwhile)
   double* part_b = c.part_rr;  // r.z partials
-  k_pc_init<<<gr, kBlock, 0, s>>>(c.g_ext, c.inv_M_prec, c.y, c.r3, c.z3, c.p3, c.cv3, N, part_b);
-  k_pc_state_init<<<1, kBlock, 0, s>>>(sc, part_b, gr, rbank(c, R_GN), red_grid(N), p.pc_auto, p.pc_force_eta,
-                                       p.pc_force_alpha, p.pc_iters);
-  for (int k = 0; k < p.pc_iters; ++k) {
-    double* bank = sc + pc_bank(k);
-    MM_TRY(launch_local(c, c.p3, c.cv3, bank + B_ACT, s));
-    c.n_local++;
-    k_pc_ap<<<gr, kBlock, 0, s>>>(bank, c.m, c.g_raw, c.p3, c.cv3, c.q3, p.pc_reg, N, part_a);
-    k_pc_yr<<<gr, kBlock, 0, s>>>(sc, bank, part_a, gr, p.pc_stagnation_nu, c.y, c.r3, c.z3, c.p3, c.q3,
-                                   c.inv_M_prec, N, part_b);
-    k_pc_p<<<gr, kBlock, 0, s>>>(sc, bank, sc + pc_bank(k + 1), part_b, gr, c.z3, c.p3, c.cv3, p.pc_iters, N);
-  }
+  // the loop on FP32 vectors in the mixed-precision forms, on FP64 vectors otherwise
+  auto loop = [&](auto* r, auto* z, auto* pv, auto* q, auto* cv) -> int {
+    using TV = std::remove_pointer_t<decltype(r)>;
+    k_pc_init<TV><<<gr, kBlock, 0, s>>>(c.g_ext, c.inv_M_prec, c.y, r, z, pv, cv, N, part_b);
+    k_pc_state_init<<<1, kBlock, 0, s>>>(sc, part_b, gr, rbank(c, R_GN), red_grid(N), p.pc_auto, p.pc_force_eta,
+                                         p.pc_force_alpha, p.pc_iters);
+    for (int k = 0; k < p.pc_iters; ++k) {
+      double* bank = sc + pc_bank(k);
+      MM_TRY(launch_local(c, pv, cv, bank + B_ACT, s));
+      c.n_local++;
+      k_pc_ap<TV><<<gr, kBlock, 0, s>>>(bank, c.m, c.g_raw, pv, cv, q, p.pc_reg, N, part_a);
+      k_pc_yr<TV><<<gr, kBlock, 0, s>>>(sc, bank, part_a, gr, p.pc_stagnation_nu, c.y, r, z, pv, q, c.inv_M_prec, N,
+                                         part_b);
+      k_pc_p<TV><<<gr, kBlock, 0, s>>>(sc, bank, sc + pc_bank(k + 1), part_b, gr, z, pv, cv, p.pc_iters, N);
+    }
+    return 0;
+  };
+  if (c.precision >= 1 && c.r3f != nullptr)
+    MM_TRY(loop(c.r3f, c.z3f, c.p3f, c.q3f, c.cv3f));
+  else
+    MM_TRY(loop(c.r3, c.z3, c.p3, c.q3, c.cv3));
   k_pc_norms<<<gr, kBlock, 0, s>>>(c.y, c.g_ext, c.inv_M_prec, N, c.partial);
   k_pc_finish<<<gr, kBlock, 0, s>>>(sc, sc + pc_bank(p.pc_iters), c.partial, gr, c.y, c.g_ext, c.inv_M_prec, N);
   return 0;
