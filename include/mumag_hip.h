@@ -96,14 +96,16 @@ int mm_ctx_get_exchange_diag(int64_t ctx, double* d);
 
 /* Precision of the Poisson solve: 0 = FP64; 1 = mixed: the whole V-cycle in FP32 (FP32
  * matrix-free applies and coarse levels) while the PCG recurrence, its operator apply, the dot
- * products and alpha/beta stay FP64; 2 = FP32 PCG with FP64 iterative refinement: the PCG runs on
- * FP32 vectors with the FP32 apply and V-cycle (dots, alpha and beta in FP64) and an outer loop
- * recomputes the true residual b - A U with the FP64 apply and restarts the inner PCG on it until
+ * products and alpha/beta stay FP64; 2 = FP32 PCG with FP64 residual replacement: the PCG runs on
+ * FP32 vectors with the FP32 apply and V-cycle (dots, alpha and beta in FP64); its recursive
+ * residual is replaced by the true residual b - A U of the FP64 apply after each reduction by
+ * mm_set_refinement's factor, the Krylov direction carrying on, and it stops when the true
  * r.r <= tol^2 b.b (needs the AMG V-cycle and the on-the-fly cluster apply); 3 = form 2 with the
  * energy densities and gradient terms of mm_energy_grad and of the minimiser in FP32 element
  * arithmetic (FP64 accumulation, FP64 Zeeman term, compensated FP64 energy sum). Every form meets
- * the FP64 solver tolerance; forms 1-3 run the minimiser's local Hessian action (the Steihaug loop)
- * in FP32 element arithmetic and keep the L-BFGS history in FP32. */
+ * the FP64 solver tolerance; forms 1-3 run the minimiser's Steihaug loop on FP32 vectors (r, z,
+ * p, q and the local Hessian action C p, which is FP32 element arithmetic; FP64 dot products,
+ * FP64 result y) and keep the L-BFGS history in FP32. */
 int mm_set_precision(int64_t ctx, int precision);
 
 /* Select the geometry source of the Poisson apply (see flags of mm_ctx_create). */
@@ -137,8 +139,9 @@ int mm_set_cluster_staging(int64_t ctx, int mode);
 int mm_set_apply_scatter(int64_t ctx, int mode);
 /* FP32 PCG of precision forms 2 and 3: the recursive FP32 residual is replaced by the true FP64
  * residual b - A U (and the FP32 correction added to U) each time its norm has dropped by the
- * factor eta, in (0, 1), since the last replacement; the Krylov direction carries on. The stopping
- * test is the true residual's, as in the FP64 PCG. A tuning knob with no reference counterpart. */
+ * factor eta, in (0, 1), since the last replacement (default 1e-3); the Krylov direction carries
+ * on. The stopping test is the true residual's, as in the FP64 PCG. A tuning knob with no reference
+ * counterpart. */
 int mm_set_refinement(int64_t ctx, double eta);
 
 /* y = A x: assembled-mode apply_A (src/poisson_solve.py:166-171) computed matrix-free. */

@@ -226,7 +226,7 @@ struct Ctx {
   // FP32 PCG with FP64 refinement (precision 2): correction, residual, direction, A p, V-cycle output
   float *rxf = nullptr, *rrf = nullptr, *rpf = nullptr, *rqf = nullptr, *rzf = nullptr;
   int ref_cycles = 0;                 // residual replacements of the last solve
-  double ref_eta = 1e-2;              // FP32 PCG: replace the residual after this reduction of its norm
+  double ref_eta = 1e-3;              // FP32 PCG: replace the residual after this reduction of its norm
 
   // minimiser state (3N unless noted)
   double *m = nullptr, *U = nullptr /*N*/, *g_raw = nullptr, *g_tan = nullptr, *g_ext = nullptr, *g_prev = nullptr;

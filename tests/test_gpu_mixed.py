@@ -176,7 +176,7 @@ def test_fp32_pcg_residual_replacement_keeps_the_krylov_iteration(mixed_ctx, ora
         with pytest.raises(RuntimeError, match="replacement factor"):
             ctx.set_refinement(1.0)
     finally:
-        ctx.set_refinement(1e-2)
+        ctx.set_refinement(1e-3)  # the default
         ctx.set_precision("fp64")
     true_rr = np.linalg.norm(b - oracle_c0.A @ U) / np.linalg.norm(b)
     print(f"[fp32pcg eta={eta}] it={it} (fp64 {it64}) rr={rr:.3e} true={true_rr:.3e}", flush=True)
