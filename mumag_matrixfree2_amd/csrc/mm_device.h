@@ -325,9 +325,10 @@ __device__ void block_sum_to(double v, double* __restrict__ out) {
   }
 }
 
-// Grid-stride loop over [0, n) that gives each thread kChunk items per round: the loads of a
-// round (load(i) returns the item's operands) are all issued before the first body(i, operands)
-// runs, so a wave keeps kChunk loads per stream in flight instead of waiting once per item. Each
+// Grid-stride loop over [0, n) that gives each thread U items per round (kChunk unless given): the
+// loads of a round (load(i) returns the item's operands by value) are all issued before the first
+// body(i, operands) runs, so a wave keeps U loads per stream in flight instead of waiting once per
+// item. Each
 // thread still visits its items i, i + S, i + 2 S, ... (S = grid threads) in increasing order, so
 // a per-thread accumulation in body adds in the same order as a plain grid-stride loop.
 constexpr int kChunk = 4;
