@@ -1246,6 +1246,19 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
     MM_TRY(upload(c, &L.Rxf, to_f32(Rx, Rp[n_next]).data(), Rp[n_next]));
   }
   if (dense_inv) MM_TRY(upload(c, &L.dense_invf_t, to_f32(tiles64.data(), tiles64.size()).data(), tiles64.size()));
+  // FP16 copies for the FP32 PCG's V-cycle, when every value is finite and within the FP16 range
+  auto up_f16 = [&](__half** dst, const double* a, size_t k) -> int {
+    for (size_t i = 0; i < k; ++i)
+      if (!(std::fabs(a[i]) <= 6.0e4)) return 0;
+    std::vector<__half> h(k);
+    for (size_t i = 0; i < k; ++i) h[i] = __float2half((float)a[i]);
+    return upload(c, dst, h.data(), k);
+  };
+  if (l > 0 && Ap) MM_TRY(up_f16(&L.Axh, Ax, Ap[n]));
+  if (n_next > 0) {
+    MM_TRY(up_f16(&L.Pxh, Px, Pp[n]));
+    MM_TRY(up_f16(&L.Rxh, Rx, Rp[n_next]));
+  }
   // SpMV lanes per row: power of two near a quarter of the average row length, in [4, 64]
   auto lanes = [](long nnz, long rows) {
     const double avg = rows > 0 ? (double)nnz / (double)rows : 0.0;
@@ -1332,6 +1345,9 @@ int mm_amg_commit(int64_t h, int kind, int nlev) {
     }
   }
   c.amg_kind = kind;
+  c.lv_half = nlev >= 2;
+  for (int l = 0; l + 1 < nlev; ++l)
+    c.lv_half = c.lv_half && c.lv[l].Pxh && c.lv[l].Rxh && (l == 0 || c.lv[l].Axh);
   return 0;
 }
 
@@ -1347,6 +1363,7 @@ int mm_precond_jacobi(int64_t h) {
   c.lv[0].n = c.N;
   MM_TRY(upload(c, &c.lv[0].M, d.data(), c.N));
   c.amg_kind = 0;
+  c.lv_half = false;
   return 0;
 }
 

@@ -16,6 +16,7 @@
 //     FP64 and FP32, structure-of-arrays, for the streamed-geometry apply variant.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -154,6 +155,9 @@ struct Level {
   double *b = nullptr, *x = nullptr, *t = nullptr, *xs = nullptr;  // xs: post-smoothed result
   // FP32 copies for the mixed-precision V-cycle
   float *Axf = nullptr, *Pxf = nullptr, *Rxf = nullptr, *Mf = nullptr;
+  // FP16 copies of the matrix values for the FP32 PCG's V-cycle (products and sums in FP32);
+  // null where a value is outside the FP16 range
+  __half *Axh = nullptr, *Pxh = nullptr, *Rxh = nullptr;
   float *bf = nullptr, *xf = nullptr, *tf = nullptr, *xsf = nullptr;
   // segmented contexts: the rows of segment g are seg_off[g] .. seg_off[g + 1] - 1 (empty when
   // the level's unknowns are not ordered by segment); runs: the row ranges of the segments a
@@ -222,6 +226,7 @@ struct Ctx {
   std::vector<std::vector<int>> amg_seg_off;  // per level: rows of each segment (set-up)
   double* diagA = nullptr;            // N
   int precision = 0;                  // 0: FP64; 1: FP32 V-cycle; 2: FP32 PCG with FP64 refinement
+  bool lv_half = false;               // every level holds its FP16 matrix values (the FP32 PCG uses them)
   float *px0f = nullptr, *ptf = nullptr;  // level-0 FP32 work vectors
   // FP32 PCG with FP64 refinement (precision 2): correction, residual, direction, A p, V-cycle output
   float *rxf = nullptr, *rrf = nullptr, *rpf = nullptr, *rqf = nullptr, *rzf = nullptr;

@@ -10,6 +10,7 @@
 // that copy overlaps the V-cycle of the same iteration.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include <functional>
@@ -184,9 +185,9 @@ constexpr int spmv_unroll() {
 // entries. No branch per entry: a padding entry re-reads the row's last entry (j1 - 1 >= jb is a
 // valid entry inside the loop) and contributes an exact zero through the select. MODE as in
 // k_spmv.
-template <int MODE, int LANES, typename TA, typename TX, typename TX2>
+template <int MODE, int LANES, typename TA, typename TV, typename TX, typename TX2>
 __device__ __forceinline__ TA row_products(int j0, int j1, int lane, const int* __restrict__ Ai,
-                                           const TA* __restrict__ Ax, const TX* __restrict__ x,
+                                           const TV* __restrict__ Ax, const TX* __restrict__ x,
                                            const TX2* __restrict__ x2) {
   constexpr int UNR = spmv_unroll<LANES>();
   TA s = 0;
@@ -200,7 +201,7 @@ __device__ __forceinline__ TA row_products(int j0, int j1, int lane, const int* 
       ok[u] = j < j1;
       const int jj = ok[u] ? j : j1 - 1;
       cu[u] = Ai[jj];
-      au[u] = Ax[jj];
+      au[u] = (TA)Ax[jj];
     }
     TA xv[UNR];
 #pragma unroll
@@ -214,14 +215,18 @@ __device__ __forceinline__ TA row_products(int j0, int j1, int lane, const int* 
   return s;
 }
 
-// CSR SpMV with LANES lanes per row, accumulation in the matrix value type TA.
+// Accumulation type of a matrix value type: FP16 values are multiplied and summed in FP32.
+template <typename TV>
+using acc_t = typename std::conditional<std::is_same<TV, __half>::value, float, TV>::type;
+
+// CSR SpMV with LANES lanes per row; matrix values stored as TV, products and sums in TA.
 // MODE 0: y = A x; 1: y += A x (and z = 0 on the same rows when z != nullptr); 2: y = A (x - x2)
 // and, when z != nullptr, z = zw .* y on the same rows (a restriction that also forms the next
 // level's pre-smoothed guess M b). The row's own entries of the epilogue are loaded first, so
 // their latency overlaps the products.
-template <int MODE, int LANES, typename TA, typename TX, typename TX2, typename TY>
+template <int MODE, int LANES, typename TA, typename TX, typename TX2, typename TY, typename TV = TA>
 __global__ __launch_bounds__(kBlock) void k_spmv(int row0, int nrows, const int* __restrict__ Ap,
-                                                 const int* __restrict__ Ai, const TA* __restrict__ Ax,
+                                                 const int* __restrict__ Ai, const TV* __restrict__ Ax,
                                                  const TX* __restrict__ x, const TX2* __restrict__ x2,
                                                  TY* __restrict__ y, TY* __restrict__ z, const TY* __restrict__ zw) {
   const int lane = threadIdx.x & (LANES - 1);
@@ -233,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int row0, int nrows, const int*
     if (MODE == 2 && z) w = zw[row];
   }
   TA s = 0;
-  if (live) s = row_products<MODE, LANES>(Ap[row], Ap[row + 1], lane, Ai, Ax, x, x2);
+  if (live) s = row_products<MODE, LANES, TA>(Ap[row], Ap[row + 1], lane, Ai, Ax, x, x2);
 #pragma unroll
   for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LANES);
   if (live && lane == 0) {
@@ -249,9 +254,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int row0, int nrows, const int*
 }
 
 // Post-smoothing step as one SpMV: xs = x + M .* (b - A x), written to a separate vector.
-template <int LANES, typename T>
+template <int LANES, typename T, typename TV = T>
 __global__ __launch_bounds__(kBlock) void k_spmv_post(int row0, int nrows, const int* __restrict__ Ap,
-                                                      const int* __restrict__ Ai, const T* __restrict__ Ax,
+                                                      const int* __restrict__ Ai, const TV* __restrict__ Ax,
                                                       const T* __restrict__ x, const T* __restrict__ M,
                                                       const T* __restrict__ b, T* __restrict__ xs) {
   const int lane = threadIdx.x & (LANES - 1);
@@ -264,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_post(int row0, int nrows, const
     br = b[row];
   }
   T s = 0;
-  if (live) s = row_products<0, LANES>(Ap[row], Ap[row + 1], lane, Ai, Ax, x, (const T*)nullptr);
+  if (live) s = row_products<0, LANES, T>(Ap[row], Ap[row + 1], lane, Ai, Ax, x, (const T*)nullptr);
 #pragma unroll
   for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LANES);
   if (live && lane == 0) xs[row] = xr + mr * (br - s);
@@ -358,18 +363,19 @@ static void symv(const Level& L, const T* tiles, const T* b, T* y, hipStream_t s
   k_symv_reduce<T><<<T_, 256, 0, s>>>(L.n, T_, L.symv_part, y);
 }
 
-template <int MODE, typename TA, typename TX, typename TX2, typename TY>
-static void spmv_lanes(int lanes, int row0, int row1, const int* Ap, const int* Ai, const TA* Ax, const TX* x,
+template <int MODE, typename TV, typename TX, typename TX2, typename TY>
+static void spmv_lanes(int lanes, int row0, int row1, const int* Ap, const int* Ai, const TV* Ax, const TX* x,
                        const TX2* x2, TY* y, TY* z, const TY* zw, hipStream_t s) {
+  using TA = acc_t<TV>;
   const int grid = (int)(((long)(row1 - row0) * lanes + kBlock - 1) / kBlock);
   switch (lanes) {
-    case 1: k_spmv<MODE, 1, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    case 2: k_spmv<MODE, 2, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    case 4: k_spmv<MODE, 4, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    case 8: k_spmv<MODE, 8, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    case 16: k_spmv<MODE, 16, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    case 32: k_spmv<MODE, 32, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    default: k_spmv<MODE, 64, TA, TX, TX2, TY><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    case 1: k_spmv<MODE, 1, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    case 2: k_spmv<MODE, 2, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    case 4: k_spmv<MODE, 4, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    case 8: k_spmv<MODE, 8, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    case 16: k_spmv<MODE, 16, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    case 32: k_spmv<MODE, 32, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    default: k_spmv<MODE, 64, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
   }
 }
 
@@ -385,8 +391,8 @@ static inline void for_runs(const Runs* runs, int nrows, const std::function<voi
 }
 
 // y = A x (mode 0), y += A x and z = 0 (mode 1), y = A (x - x2) and z = zw .* y (mode 2; z optional)
-template <typename TA, typename TX, typename TX2, typename TY>
-static void spmv(int mode, int lanes, int nrows, const int* Ap, const int* Ai, const TA* Ax, const TX* x,
+template <typename TV, typename TX, typename TX2, typename TY>
+static void spmv(int mode, int lanes, int nrows, const int* Ap, const int* Ai, const TV* Ax, const TX* x,
                  const TX2* x2, TY* y, hipStream_t s, TY* z = nullptr, const Runs* runs = nullptr,
                  const TY* zw = nullptr) {
   for_runs(runs, nrows, [&](int r0, int r1) {
@@ -399,19 +405,19 @@ static void spmv(int mode, int lanes, int nrows, const int* Ap, const int* Ai, c
   });
 }
 
-template <typename T>
-static void spmv_post(int lanes, int nrows, const int* Ap, const int* Ai, const T* Ax, const T* x, const T* M,
+template <typename T, typename TV>
+static void spmv_post(int lanes, int nrows, const int* Ap, const int* Ai, const TV* Ax, const T* x, const T* M,
                       const T* b, T* xs, hipStream_t s, const Runs* runs = nullptr) {
   for_runs(runs, nrows, [&](int r0, int r1) {
     const int grid = (int)(((long)(r1 - r0) * lanes + kBlock - 1) / kBlock);
     switch (lanes) {
-      case 1: k_spmv_post<1, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      case 2: k_spmv_post<2, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      case 4: k_spmv_post<4, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      case 8: k_spmv_post<8, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      case 16: k_spmv_post<16, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      case 32: k_spmv_post<32, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      default: k_spmv_post<64, T><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      case 1: k_spmv_post<1, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      case 2: k_spmv_post<2, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      case 4: k_spmv_post<4, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      case 8: k_spmv_post<8, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      case 16: k_spmv_post<16, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      case 32: k_spmv_post<32, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      default: k_spmv_post<64, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
     }
   });
 }
@@ -969,8 +975,28 @@ static void vcycle_coarse(Ctx& c, int l, hipStream_t s) {
   spmv_post(L.lanesA, L.n, L.Ap, L.Ai, L.Ax, L.x, L.M, L.b, L.xs, s, &L.runs);      // xs = x + M (b - A x)
 }
 
-// FP32 coarse part of the V-cycle (levels >= 1), same structure as vcycle_coarse.
+// Matrix values of a level as the FP32 V-cycle reads them: FP32 copies, or FP16 copies (the FP32
+// PCG's V-cycle, when every level holds them).
+template <typename TV>
+struct LevelVals;
+template <>
+struct LevelVals<float> {
+  static const float* A(const Level& L) { return L.Axf; }
+  static const float* P(const Level& L) { return L.Pxf; }
+  static const float* R(const Level& L) { return L.Rxf; }
+};
+template <>
+struct LevelVals<__half> {
+  static const __half* A(const Level& L) { return L.Axh; }
+  static const __half* P(const Level& L) { return L.Pxh; }
+  static const __half* R(const Level& L) { return L.Rxh; }
+};
+
+// FP32 coarse part of the V-cycle (levels >= 1), same structure as vcycle_coarse; matrix values
+// read as TV.
+template <typename TV = float>
 static void vcycle_coarse_f(Ctx& c, int l, hipStream_t s) {
+  using V = LevelVals<TV>;
   Level& L = c.lv[l];
   const int last = (int)c.lv.size() - 1;
   if (l == last) {
@@ -979,12 +1005,12 @@ static void vcycle_coarse_f(Ctx& c, int l, hipStream_t s) {
   }
   Level& Nx = c.lv[l + 1];
   const bool nx_smooth = l + 1 < last;
-  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.xf, (const float*)nullptr, L.tf, s, (float*)nullptr, &L.runs);
-  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, L.Rxf, L.bf, L.tf, Nx.bf, s, nx_smooth ? Nx.xf : (float*)nullptr, &Nx.runs,
+  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, V::A(L), L.xf, (const float*)nullptr, L.tf, s, (float*)nullptr, &L.runs);
+  spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, V::R(L), L.bf, L.tf, Nx.bf, s, nx_smooth ? Nx.xf : (float*)nullptr, &Nx.runs,
        nx_smooth ? (const float*)Nx.Mf : (const float*)nullptr);
-  vcycle_coarse_f(c, l + 1, s);
-  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, L.Pxf, Nx.xsf, (const float*)nullptr, L.xf, s, (float*)nullptr, &L.runs);
-  spmv_post(L.lanesA, L.n, L.Ap, L.Ai, L.Axf, L.xf, L.Mf, L.bf, L.xsf, s, &L.runs);
+  vcycle_coarse_f<TV>(c, l + 1, s);
+  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, V::P(L), Nx.xsf, (const float*)nullptr, L.xf, s, (float*)nullptr, &L.runs);
+  spmv_post(L.lanesA, L.n, L.Ap, L.Ai, V::A(L), L.xf, L.Mf, L.bf, L.xsf, s, &L.runs);
 }
 
 // Mixed-precision preconditioner: the whole V-cycle in FP32 (FP32 matrix-free fine applies,
@@ -1138,21 +1164,29 @@ __global__ void k_ref_update(double* __restrict__ U, const float* __restrict__ x
 }
 
 // The FP32 V-cycle on an FP32 residual: z (FP32) and FP64 partials of r.z. The caller has
-// cleared ptf.
-static int precond_ff(Ctx& c, const float* r, float* z, double* rz_part, hipStream_t s) {
+// cleared ptf. The transfer and coarse-level matrices are read from their FP16 copies when every
+// level holds them (products and sums in FP32): this V-cycle is the preconditioner of a PCG whose
+// stopping test is the FP64 true residual.
+template <typename TV>
+static int precond_ff_v(Ctx& c, const float* r, float* z, double* rz_part, hipStream_t s) {
+  using V = LevelVals<TV>;
   const long N = c.N;
   Level& L0 = c.lv[0];
   Level& L1 = c.lv[1];
   MM_TRY(launch_apply_f32_scaled_f(c, r, L0.Mf, c.px0f, c.ptf, s));               // x = M0 r, t = A x
   const bool l1_smooth = c.lv.size() > 2;
-  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, L0.Rxf, r, c.ptf, L1.bf, s, l1_smooth ? L1.xf : (float*)nullptr, &L1.runs,
-       l1_smooth ? (const float*)L1.Mf : (const float*)nullptr);  // b1 = R0 (r - t)
-  vcycle_coarse_f(c, 1, s);
-  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, L0.Pxf, L1.xsf, (const float*)nullptr, c.px0f, s, c.ptf, &L0.runs);
+  spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, V::R(L0), r, c.ptf, L1.bf, s, l1_smooth ? L1.xf : (float*)nullptr,
+       &L1.runs, l1_smooth ? (const float*)L1.Mf : (const float*)nullptr);  // b1 = R0 (r - t)
+  vcycle_coarse_f<TV>(c, 1, s);
+  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, V::P(L0), L1.xsf, (const float*)nullptr, c.px0f, s, c.ptf, &L0.runs);
   MM_TRY(launch_apply_f32(c, c.px0f, c.ptf, s));                                  // t = A x
   k_post_dot_ff<<<red_grid(N), kBlock, 0, s>>>(c.px0f, L0.Mf, r, c.ptf, z, N, rz_part);
   c.n_vcycle += 1;
   return 0;
+}
+
+static int precond_ff(Ctx& c, const float* r, float* z, double* rz_part, hipStream_t s) {
+  return c.lv_half ? precond_ff_v<__half>(c, r, z, rz_part, s) : precond_ff_v<float>(c, r, z, rz_part, s);
 }
 
 static int timed_apply_f32_dot(Ctx& c, const float* x, float* y, double* dot_part, hipStream_t s) {
