@@ -15,6 +15,7 @@
 // masked to the segments that are still iterating or taking a trial.
 #include <chrono>
 #include <cmath>
+#include <type_traits>
 #include <utility>
 
 #include "mm_device.h"
@@ -147,10 +148,13 @@ __global__ __launch_bounds__(kBlock) void k_sg_tangents(SegM S, const double* __
 }
 
 // ------------------------------------------------------------------ Steihaug-PCG per segment
+// The loop's r, z, p, q and C p are stored as TV: FP64, or FP32 in the mixed-precision forms
+// (products in FP64 from the stored values; with TV = double exactly the FP64 loop).
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_sg_pc_init(SegM S, const double* __restrict__ g_ext,
                                                        const double* __restrict__ Mp, double* __restrict__ y,
-                                                       double* __restrict__ r, double* __restrict__ z,
-                                                       double* __restrict__ p, double* __restrict__ cv,
+                                                       TV* __restrict__ r, TV* __restrict__ z,
+                                                       TV* __restrict__ p, TV* __restrict__ cv,
                                                        double* __restrict__ part) {
   int sg;
   long lo, hi;
@@ -159,17 +163,21 @@ __global__ __launch_bounds__(kBlock) void k_sg_pc_init(SegM S, const double* __r
   double v[1] = {0.0};
   for (long i = lo + threadIdx.x; i < hi; i += kBlock) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) cv[3 * i + k] = 0.0;  // every segment: k_local adds into cv
+    for (int k = 0; k < 3; ++k) cv[3 * i + k] = (TV)0;  // every segment: k_local adds into cv
     if (!act) continue;
     const double s = Mp[i];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const double ri = g_ext[3 * i + k];
+      const double ri = (double)(TV)g_ext[3 * i + k];
+      const double zi = (double)(TV)(ri * s);
       y[3 * i + k] = 0.0;
-      r[3 * i + k] = ri;
-      z[3 * i + k] = ri * s;
-      p[3 * i + k] = ri * s;
-      v[0] += ri * ri * s;
+      r[3 * i + k] = (TV)ri;
+      z[3 * i + k] = (TV)zi;
+      p[3 * i + k] = (TV)zi;
+      if constexpr (sizeof(TV) == 8)
+        v[0] += ri * ri * s;  // the FP64 loop's own order
+      else
+        v[0] += ri * zi;  // r.z of the stored vectors
     }
   }
   block_partial<1>(v, part);
@@ -202,9 +210,10 @@ __global__ __launch_bounds__(kBlock) void k_sg_pc_state_init(SegM S, const doubl
 }
 
 // q = P p = Cv - (m.Cv) m - (m.g) p + reg p; partials of p.q
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_sg_pc_ap(SegM S, int bank_slot, const double* __restrict__ m,
-                                                     const double* __restrict__ g, const double* __restrict__ p,
-                                                     const double* __restrict__ cv, double* __restrict__ q, double reg,
+                                                     const double* __restrict__ g, const TV* __restrict__ p,
+                                                     const TV* __restrict__ cv, TV* __restrict__ q, double reg,
                                                      double* __restrict__ part) {
   int sg;
   long lo, hi;
@@ -213,13 +222,15 @@ __global__ __launch_bounds__(kBlock) void k_sg_pc_ap(SegM S, int bank_slot, cons
   double v[1] = {0.0};
   for (long i = lo + threadIdx.x; i < hi; i += kBlock) {
     const double* mi = m + 3 * i;
-    const double mc = mi[0] * cv[3 * i] + mi[1] * cv[3 * i + 1] + mi[2] * cv[3 * i + 2];
+    const double c0 = (double)cv[3 * i], c1 = (double)cv[3 * i + 1], c2 = (double)cv[3 * i + 2];
+    const double ci[3] = {c0, c1, c2};
+    const double mc = mi[0] * c0 + mi[1] * c1 + mi[2] * c2;
     const double mg = mi[0] * g[3 * i] + mi[1] * g[3 * i + 1] + mi[2] * g[3 * i + 2];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const double pk = p[3 * i + k];
-      const double qk = cv[3 * i + k] - mc * mi[k] - mg * pk + reg * pk;
-      q[3 * i + k] = qk;
+      const double pk = (double)p[3 * i + k];
+      const double qk = (double)(TV)(ci[k] - mc * mi[k] - mg * pk + reg * pk);
+      q[3 * i + k] = (TV)qk;
       v[0] += pk * qk;
     }
   }
@@ -228,10 +239,11 @@ __global__ __launch_bounds__(kBlock) void k_sg_pc_ap(SegM S, int bank_slot, cons
 
 // p.q of the segment -> alpha, dq, negative curvature / stagnation; y += alpha p; r -= alpha q
 // (unless done now); z = Mp r; partials of r.z
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_sg_pc_yr(SegM S, int bank_slot, const double* __restrict__ part_a,
-                                                     double nu, double* __restrict__ y, double* __restrict__ r,
-                                                     double* __restrict__ z, const double* __restrict__ p,
-                                                     const double* __restrict__ q, const double* __restrict__ Mp,
+                                                     double nu, double* __restrict__ y, TV* __restrict__ r,
+                                                     TV* __restrict__ z, const TV* __restrict__ p,
+                                                     const TV* __restrict__ q, const double* __restrict__ Mp,
                                                      double* __restrict__ part_b) {
   int sg;
   long lo, hi;
@@ -258,14 +270,14 @@ __global__ __launch_bounds__(kBlock) void k_sg_pc_yr(SegM S, int bank_slot, cons
     const double s = Mp[i];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      double rk = r[3 * i + k];
+      double rk = (double)r[3 * i + k];
       if (!dn) {
-        y[3 * i + k] += alpha * p[3 * i + k];
-        rk -= alpha * q[3 * i + k];
-        r[3 * i + k] = rk;
+        y[3 * i + k] += alpha * (double)p[3 * i + k];
+        rk = (double)(TV)(rk - alpha * (double)q[3 * i + k]);
+        r[3 * i + k] = (TV)rk;
       }
-      const double zk = rk * s;
-      z[3 * i + k] = zk;
+      const double zk = (double)(TV)(rk * s);
+      z[3 * i + k] = (TV)zk;
       v[0] += rk * zk;
     }
   }
@@ -274,9 +286,10 @@ __global__ __launch_bounds__(kBlock) void k_sg_pc_yr(SegM S, int bank_slot, cons
 
 // r.z of the segment; p = z + beta p (unless done now); cv = 0; the segment's first workgroup
 // writes the next bank (a copy of this one once the segment's loop has stopped)
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_sg_pc_p(SegM S, int bank_slot, int next_slot,
-                                                    const double* __restrict__ part_b, const double* __restrict__ z,
-                                                    double* __restrict__ p, double* __restrict__ cv, int max_iter) {
+                                                    const double* __restrict__ part_b, const TV* __restrict__ z,
+                                                    TV* __restrict__ p, TV* __restrict__ cv, int max_iter) {
   int sg;
   long lo, hi;
   segm_range(S, sg, lo, hi);
@@ -295,8 +308,8 @@ __global__ __launch_bounds__(kBlock) void k_sg_pc_p(SegM S, int bank_slot, int n
   for (long i = lo + threadIdx.x; i < hi; i += kBlock)
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      if (upd) p[3 * i + k] = z[3 * i + k] + beta * p[3 * i + k];
-      cv[3 * i + k] = 0.0;
+      if (upd) p[3 * i + k] = (TV)((double)z[3 * i + k] + beta * (double)p[3 * i + k]);
+      cv[3 * i + k] = (TV)0;
     }
   if (first_wg(S, sg)) {
     next[SB_RHO] = rhon;
@@ -636,17 +649,26 @@ int minimize_segmented(Ctx& c, const MinParams& p, const double B[3], MinInfo* i
     MM_TRY(use(want));
     k_sg_tangents<<<W, kBlock, 0, s>>>(S, c.m, c.g_raw, c.inv_M_rel, c.g_tan, c.g_ext, c.sm_pa);
     k_sg_reduce<1, true><<<G, kBlock, 0, s>>>(S, c.sm_pa, T_GN);
-    // Steihaug-PCG solve of P y = g_ext per segment
-    k_sg_pc_init<<<W, kBlock, 0, s>>>(S, c.g_ext, c.inv_M_prec, c.y, c.r3, c.z3, c.p3, c.cv3, c.sm_pb);
-    k_sg_pc_state_init<<<G, kBlock, 0, s>>>(S, c.sm_pb, p.pc_auto, p.pc_force_eta, p.pc_force_alpha, p.pc_iters);
-    for (int k = 0; k < p.pc_iters; ++k) {
-      MM_TRY(launch_local(c, c.p3, c.cv3, nullptr, s));
-      c.n_local++;
-      k_sg_pc_ap<<<W, kBlock, 0, s>>>(S, seg_bank(k), c.m, c.g_raw, c.p3, c.cv3, c.q3, p.pc_reg, c.sm_pa);
-      k_sg_pc_yr<<<W, kBlock, 0, s>>>(S, seg_bank(k), c.sm_pa, p.pc_stagnation_nu, c.y, c.r3, c.z3, c.p3, c.q3,
-                                       c.inv_M_prec, c.sm_pb);
-      k_sg_pc_p<<<W, kBlock, 0, s>>>(S, seg_bank(k), seg_bank(k + 1), c.sm_pb, c.z3, c.p3, c.cv3, p.pc_iters);
-    }
+    // Steihaug-PCG solve of P y = g_ext per segment, on FP32 vectors in the mixed-precision forms
+    auto pc_loop = [&](auto* r3, auto* z3, auto* p3, auto* q3, auto* cv3) -> int {
+      using TV = std::remove_pointer_t<decltype(r3)>;
+      k_sg_pc_init<TV><<<W, kBlock, 0, s>>>(S, c.g_ext, c.inv_M_prec, c.y, r3, z3, p3, cv3, c.sm_pb);
+      k_sg_pc_state_init<<<G, kBlock, 0, s>>>(S, c.sm_pb, p.pc_auto, p.pc_force_eta, p.pc_force_alpha,
+                                              p.pc_iters);
+      for (int k = 0; k < p.pc_iters; ++k) {
+        MM_TRY(launch_local(c, p3, cv3, nullptr, s));
+        c.n_local++;
+        k_sg_pc_ap<TV><<<W, kBlock, 0, s>>>(S, seg_bank(k), c.m, c.g_raw, p3, cv3, q3, p.pc_reg, c.sm_pa);
+        k_sg_pc_yr<TV><<<W, kBlock, 0, s>>>(S, seg_bank(k), c.sm_pa, p.pc_stagnation_nu, c.y, r3, z3, p3, q3,
+                                             c.inv_M_prec, c.sm_pb);
+        k_sg_pc_p<TV><<<W, kBlock, 0, s>>>(S, seg_bank(k), seg_bank(k + 1), c.sm_pb, z3, p3, cv3, p.pc_iters);
+      }
+      return 0;
+    };
+    if (c.precision >= 1 && c.r3f != nullptr)
+      MM_TRY(pc_loop(c.r3f, c.z3f, c.p3f, c.q3f, c.cv3f));
+    else
+      MM_TRY(pc_loop(c.r3, c.z3, c.p3, c.q3, c.cv3));
     k_sg_pc_norms<<<W, kBlock, 0, s>>>(S, c.y, c.g_ext, c.inv_M_prec, c.sm_pa);
     k_sg_pc_finish<<<W, kBlock, 0, s>>>(S, seg_bank(p.pc_iters), c.sm_pa, c.y, c.g_ext, c.inv_M_prec, c.sm_pb);
     k_sg_reduce<1, true><<<G, kBlock, 0, s>>>(S, c.sm_pb, T_GS);
