@@ -1246,18 +1246,29 @@ int mm_amg_set_level(int64_t h, int l, int n, int n_next, const int32_t* Ap, con
     MM_TRY(upload(c, &L.Rxf, to_f32(Rx, Rp[n_next]).data(), Rp[n_next]));
   }
   if (dense_inv) MM_TRY(upload(c, &L.dense_invf_t, to_f32(tiles64.data(), tiles64.size()).data(), tiles64.size()));
-  // FP16 copies for the FP32 PCG's V-cycle, when every value is finite and within the FP16 range
-  auto up_f16 = [&](__half** dst, const double* a, size_t k) -> int {
-    for (size_t i = 0; i < k; ++i)
-      if (!(std::fabs(a[i]) <= 6.0e4)) return 0;
+  // FP16 copies for the FP32 PCG's V-cycle, when every value is finite: each matrix is scaled by
+  // 2^e2 so that its largest magnitude lands in [2^13, 2^14) (below the FP16 maximum 65504, and
+  // as far above the FP16 subnormals as it goes), whatever the mesh's length unit; *inv = 2^-e2.
+  auto up_f16 = [&](__half** dst, float* inv, const double* a, size_t k) -> int {
+    double amax = 0.0;
+    for (size_t i = 0; i < k; ++i) {
+      if (!std::isfinite(a[i])) return 0;
+      amax = std::max(amax, std::fabs(a[i]));
+    }
+    int e = 0;
+    if (amax > 0.0) std::frexp(amax, &e);  // amax in [2^(e-1), 2^e)
+    const int e2 = 14 - e;
+    if (e2 < -100 || e2 > 100) return 0;
+    const double sc = std::ldexp(1.0, e2);
     std::vector<__half> h(k);
-    for (size_t i = 0; i < k; ++i) h[i] = __float2half((float)a[i]);
+    for (size_t i = 0; i < k; ++i) h[i] = __float2half((float)(a[i] * sc));
+    *inv = std::ldexp(1.0f, -e2);
     return upload(c, dst, h.data(), k);
   };
-  if (l > 0 && Ap) MM_TRY(up_f16(&L.Axh, Ax, Ap[n]));
+  if (l > 0 && Ap) MM_TRY(up_f16(&L.Axh, &L.Axh_s, Ax, Ap[n]));
   if (n_next > 0) {
-    MM_TRY(up_f16(&L.Pxh, Px, Pp[n]));
-    MM_TRY(up_f16(&L.Rxh, Rx, Rp[n_next]));
+    MM_TRY(up_f16(&L.Pxh, &L.Pxh_s, Px, Pp[n]));
+    MM_TRY(up_f16(&L.Rxh, &L.Rxh_s, Rx, Rp[n_next]));
   }
   // SpMV lanes per row: power of two near a quarter of the average row length, in [4, 64]
   auto lanes = [](long nnz, long rows) {

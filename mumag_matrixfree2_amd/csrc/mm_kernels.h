@@ -155,9 +155,11 @@ struct Level {
   double *b = nullptr, *x = nullptr, *t = nullptr, *xs = nullptr;  // xs: post-smoothed result
   // FP32 copies for the mixed-precision V-cycle
   float *Axf = nullptr, *Pxf = nullptr, *Rxf = nullptr, *Mf = nullptr;
-  // FP16 copies of the matrix values for the FP32 PCG's V-cycle (products and sums in FP32);
-  // null where a value is outside the FP16 range
+  // FP16 copies of the matrix values for the FP32 PCG's V-cycle (products and sums in FP32),
+  // each matrix scaled by a power of two that puts its largest magnitude in [2^13, 2^14); the
+  // *_s fields hold the inverse scales, applied to each row sum. Null where a value is not finite.
   __half *Axh = nullptr, *Pxh = nullptr, *Rxh = nullptr;
+  float Axh_s = 1.0f, Pxh_s = 1.0f, Rxh_s = 1.0f;
   float *bf = nullptr, *xf = nullptr, *tf = nullptr, *xsf = nullptr;
   // segmented contexts: the rows of segment g are seg_off[g] .. seg_off[g + 1] - 1 (empty when
   // the level's unknowns are not ordered by segment); runs: the row ranges of the segments a

@@ -228,7 +228,8 @@ template <int MODE, int LANES, typename TA, typename TX, typename TX2, typename 
 __global__ __launch_bounds__(kBlock) void k_spmv(int row0, int nrows, const int* __restrict__ Ap,
                                                  const int* __restrict__ Ai, const TV* __restrict__ Ax,
                                                  const TX* __restrict__ x, const TX2* __restrict__ x2,
-                                                 TY* __restrict__ y, TY* __restrict__ z, const TY* __restrict__ zw) {
+                                                 TY* __restrict__ y, TY* __restrict__ z, const TY* __restrict__ zw,
+                                                 TA vs) {
   const int lane = threadIdx.x & (LANES - 1);
   const int row = row0 + (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
   const bool live = row < nrows;
@@ -241,6 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int row0, int nrows, const int*
   if (live) s = row_products<MODE, LANES, TA>(Ap[row], Ap[row + 1], lane, Ai, Ax, x, x2);
 #pragma unroll
   for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LANES);
+  if constexpr (std::is_same<TV, __half>::value) s *= vs;  // undo the FP16 copy's scale
   if (live && lane == 0) {
     if (MODE == 1) {
       y[row] = y_old + (TY)s;
@@ -258,7 +260,7 @@ template <int LANES, typename T, typename TV = T>
 __global__ __launch_bounds__(kBlock) void k_spmv_post(int row0, int nrows, const int* __restrict__ Ap,
                                                       const int* __restrict__ Ai, const TV* __restrict__ Ax,
                                                       const T* __restrict__ x, const T* __restrict__ M,
-                                                      const T* __restrict__ b, T* __restrict__ xs) {
+                                                      const T* __restrict__ b, T* __restrict__ xs, T vs) {
   const int lane = threadIdx.x & (LANES - 1);
   const int row = row0 + (int)(((long)blockIdx.x * kBlock + threadIdx.x) / LANES);
   const bool live = row < nrows;
@@ -272,6 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_post(int row0, int nrows, const
   if (live) s = row_products<0, LANES, T>(Ap[row], Ap[row + 1], lane, Ai, Ax, x, (const T*)nullptr);
 #pragma unroll
   for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_down(s, o, LANES);
+  if constexpr (std::is_same<TV, __half>::value) s *= vs;  // undo the FP16 copy's scale
   if (live && lane == 0) xs[row] = xr + mr * (br - s);
 }
 
@@ -365,17 +368,17 @@ static void symv(const Level& L, const T* tiles, const T* b, T* y, hipStream_t s
 
 template <int MODE, typename TV, typename TX, typename TX2, typename TY>
 static void spmv_lanes(int lanes, int row0, int row1, const int* Ap, const int* Ai, const TV* Ax, const TX* x,
-                       const TX2* x2, TY* y, TY* z, const TY* zw, hipStream_t s) {
+                       const TX2* x2, TY* y, TY* z, const TY* zw, hipStream_t s, acc_t<TV> vs) {
   using TA = acc_t<TV>;
   const int grid = (int)(((long)(row1 - row0) * lanes + kBlock - 1) / kBlock);
   switch (lanes) {
-    case 1: k_spmv<MODE, 1, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    case 2: k_spmv<MODE, 2, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    case 4: k_spmv<MODE, 4, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    case 8: k_spmv<MODE, 8, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    case 16: k_spmv<MODE, 16, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    case 32: k_spmv<MODE, 32, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
-    default: k_spmv<MODE, 64, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw); break;
+    case 1: k_spmv<MODE, 1, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw, vs); break;
+    case 2: k_spmv<MODE, 2, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw, vs); break;
+    case 4: k_spmv<MODE, 4, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw, vs); break;
+    case 8: k_spmv<MODE, 8, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw, vs); break;
+    case 16: k_spmv<MODE, 16, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw, vs); break;
+    case 32: k_spmv<MODE, 32, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw, vs); break;
+    default: k_spmv<MODE, 64, TA, TX, TX2, TY, TV><<<grid, kBlock, 0, s>>>(row0, row1, Ap, Ai, Ax, x, x2, y, z, zw, vs); break;
   }
 }
 
@@ -394,30 +397,30 @@ static inline void for_runs(const Runs* runs, int nrows, const std::function<voi
 template <typename TV, typename TX, typename TX2, typename TY>
 static void spmv(int mode, int lanes, int nrows, const int* Ap, const int* Ai, const TV* Ax, const TX* x,
                  const TX2* x2, TY* y, hipStream_t s, TY* z = nullptr, const Runs* runs = nullptr,
-                 const TY* zw = nullptr) {
+                 const TY* zw = nullptr, acc_t<TV> vs = 1) {
   for_runs(runs, nrows, [&](int r0, int r1) {
     if (mode == 0)
-      spmv_lanes<0>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, (const TY*)nullptr, s);
+      spmv_lanes<0>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, (TY*)nullptr, (const TY*)nullptr, s, vs);
     else if (mode == 1)
-      spmv_lanes<1>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, z, (const TY*)nullptr, s);
+      spmv_lanes<1>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, z, (const TY*)nullptr, s, vs);
     else
-      spmv_lanes<2>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, zw ? z : (TY*)nullptr, zw, s);
+      spmv_lanes<2>(lanes, r0, r1, Ap, Ai, Ax, x, x2, y, zw ? z : (TY*)nullptr, zw, s, vs);
   });
 }
 
 template <typename T, typename TV>
 static void spmv_post(int lanes, int nrows, const int* Ap, const int* Ai, const TV* Ax, const T* x, const T* M,
-                      const T* b, T* xs, hipStream_t s, const Runs* runs = nullptr) {
+                      const T* b, T* xs, hipStream_t s, const Runs* runs = nullptr, T vs = 1) {
   for_runs(runs, nrows, [&](int r0, int r1) {
     const int grid = (int)(((long)(r1 - r0) * lanes + kBlock - 1) / kBlock);
     switch (lanes) {
-      case 1: k_spmv_post<1, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      case 2: k_spmv_post<2, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      case 4: k_spmv_post<4, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      case 8: k_spmv_post<8, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      case 16: k_spmv_post<16, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      case 32: k_spmv_post<32, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
-      default: k_spmv_post<64, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs); break;
+      case 1: k_spmv_post<1, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs, vs); break;
+      case 2: k_spmv_post<2, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs, vs); break;
+      case 4: k_spmv_post<4, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs, vs); break;
+      case 8: k_spmv_post<8, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs, vs); break;
+      case 16: k_spmv_post<16, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs, vs); break;
+      case 32: k_spmv_post<32, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs, vs); break;
+      default: k_spmv_post<64, T, TV><<<grid, kBlock, 0, s>>>(r0, r1, Ap, Ai, Ax, x, M, b, xs, vs); break;
     }
   });
 }
@@ -984,12 +987,18 @@ struct LevelVals<float> {
   static const float* A(const Level& L) { return L.Axf; }
   static const float* P(const Level& L) { return L.Pxf; }
   static const float* R(const Level& L) { return L.Rxf; }
+  static float As(const Level&) { return 1.0f; }
+  static float Ps(const Level&) { return 1.0f; }
+  static float Rs(const Level&) { return 1.0f; }
 };
 template <>
 struct LevelVals<__half> {
   static const __half* A(const Level& L) { return L.Axh; }
   static const __half* P(const Level& L) { return L.Pxh; }
   static const __half* R(const Level& L) { return L.Rxh; }
+  static float As(const Level& L) { return L.Axh_s; }
+  static float Ps(const Level& L) { return L.Pxh_s; }
+  static float Rs(const Level& L) { return L.Rxh_s; }
 };
 
 // FP32 coarse part of the V-cycle (levels >= 1), same structure as vcycle_coarse; matrix values
@@ -1005,12 +1014,14 @@ static void vcycle_coarse_f(Ctx& c, int l, hipStream_t s) {
   }
   Level& Nx = c.lv[l + 1];
   const bool nx_smooth = l + 1 < last;
-  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, V::A(L), L.xf, (const float*)nullptr, L.tf, s, (float*)nullptr, &L.runs);
+  spmv(0, L.lanesA, L.n, L.Ap, L.Ai, V::A(L), L.xf, (const float*)nullptr, L.tf, s, (float*)nullptr, &L.runs,
+       (const float*)nullptr, V::As(L));
   spmv(2, L.lanesR, Nx.n, L.Rp, L.Ri, V::R(L), L.bf, L.tf, Nx.bf, s, nx_smooth ? Nx.xf : (float*)nullptr, &Nx.runs,
-       nx_smooth ? (const float*)Nx.Mf : (const float*)nullptr);
+       nx_smooth ? (const float*)Nx.Mf : (const float*)nullptr, V::Rs(L));
   vcycle_coarse_f<TV>(c, l + 1, s);
-  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, V::P(L), Nx.xsf, (const float*)nullptr, L.xf, s, (float*)nullptr, &L.runs);
-  spmv_post(L.lanesA, L.n, L.Ap, L.Ai, V::A(L), L.xf, L.Mf, L.bf, L.xsf, s, &L.runs);
+  spmv(1, L.lanesP, L.n, L.Pp, L.Pi, V::P(L), Nx.xsf, (const float*)nullptr, L.xf, s, (float*)nullptr, &L.runs,
+       (const float*)nullptr, V::Ps(L));
+  spmv_post(L.lanesA, L.n, L.Ap, L.Ai, V::A(L), L.xf, L.Mf, L.bf, L.xsf, s, &L.runs, V::As(L));
 }
 
 // Mixed-precision preconditioner: the whole V-cycle in FP32 (FP32 matrix-free fine applies,
@@ -1176,9 +1187,10 @@ static int precond_ff_v(Ctx& c, const float* r, float* z, double* rz_part, hipSt
   MM_TRY(launch_apply_f32_scaled_f(c, r, L0.Mf, c.px0f, c.ptf, s));               // x = M0 r, t = A x
   const bool l1_smooth = c.lv.size() > 2;
   spmv(2, L0.lanesR, L1.n, L0.Rp, L0.Ri, V::R(L0), r, c.ptf, L1.bf, s, l1_smooth ? L1.xf : (float*)nullptr,
-       &L1.runs, l1_smooth ? (const float*)L1.Mf : (const float*)nullptr);  // b1 = R0 (r - t)
+       &L1.runs, l1_smooth ? (const float*)L1.Mf : (const float*)nullptr, V::Rs(L0));  // b1 = R0 (r - t)
   vcycle_coarse_f<TV>(c, 1, s);
-  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, V::P(L0), L1.xsf, (const float*)nullptr, c.px0f, s, c.ptf, &L0.runs);
+  spmv(1, L0.lanesP, (int)N, L0.Pp, L0.Pi, V::P(L0), L1.xsf, (const float*)nullptr, c.px0f, s, c.ptf, &L0.runs,
+       (const float*)nullptr, V::Ps(L0));
   MM_TRY(launch_apply_f32(c, c.px0f, c.ptf, s));                                  // t = A x
   k_post_dot_ff<<<red_grid(N), kBlock, 0, s>>>(c.px0f, L0.Mf, r, c.ptf, z, N, rz_part);
   c.n_vcycle += 1;

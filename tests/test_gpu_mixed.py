@@ -185,6 +185,37 @@ def test_fp32_pcg_residual_replacement_keeps_the_krylov_iteration(mixed_ctx, ora
     assert it <= it64 + 2
 
 
+@pytest.mark.parametrize("h", [1e-3, 1e3])
+def test_fp32_pcg_vcycle_independent_of_the_length_unit(cube_c0, h):
+    """The FP32 PCG's V-cycle keeps FP16 copies of its level matrices, each scaled by a power of
+    two: the same mesh in a length unit h times the nanometre (the stiffness scales as h, the
+    right-hand side as h^2, so the coarse matrices at h = 1e3 exceed the FP16 maximum and at
+    h = 1e-3 sit near its subnormals unless scaled) takes the iterations it takes in nanometres
+    on a hierarchy of three or more levels, and its solution is the nanometre one times h."""
+    from mumag_matrixfree2_amd.context import MeshContext
+
+    A, K1, Js, k = ndfeb_materials()
+    msh = cube_c0
+    out = {}
+    for unit in (1.0, h):
+        ctx = MeshContext(msh["knt"] * unit, msh["ijk"][:, :4], msh["ijk"][:, 4], A, K1, Js, k,
+                          boundary_mask=msh["boundary_mask"])
+        try:
+            ctx.setup_precond("amgcl", max_coarse=60)
+            levels = [lv["A"].shape[0] for lv in ctx.amg_levels]
+            m = tilted_m0(ctx.N, seed=36, sigma=0.4)
+            U64, it64, _ = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+            ctx.set_precision("fp32pcg")
+            U, it, rr = ctx.solve_U(m, np.zeros(ctx.N), 1e-10)
+        finally:
+            ctx.close()
+        print(f"[length unit {unit:g}] levels={levels} it={it} (fp64 {it64}) rr={rr:.3e}", flush=True)
+        assert len(levels) >= 3 and rr <= 1e-10 and rel(U, U64) <= 1e-8
+        out[unit] = (U, it)
+    assert abs(out[h][1] - out[1.0][1]) <= 1
+    assert rel(out[h][0], h * out[1.0][0]) <= 1e-8
+
+
 @pytest.mark.parametrize("method", ["pcohen_hs", "lbfgs"])
 def test_fp32_pcg_field_step_energy(mixed_ctx, oracle_c0, method):
     from mumag_matrixfree2_amd.context import make_params
