@@ -101,8 +101,9 @@ int mm_ctx_get_exchange_diag(int64_t ctx, double* d);
  * residual is replaced by the true residual b - A U of the FP64 apply after each reduction by
  * mm_set_refinement's factor, the Krylov direction carrying on, and it stops when the true
  * r.r <= tol^2 b.b (needs the AMG V-cycle and the on-the-fly cluster apply); its V-cycle reads
- * the transfer and coarse-level matrix values from FP16 copies when they all fit the FP16 range
- * (products and sums in FP32); 3 = form 2 with the
+ * the transfer and coarse-level matrix values from FP16 copies, each matrix scaled by a power of
+ * two that puts its largest magnitude in [2^13, 2^14) (products and sums in FP32, the row sums
+ * scaled back; made when every value is finite); 3 = form 2 with the
  * energy densities and gradient terms of mm_energy_grad and of the minimiser in FP32 element
  * arithmetic (FP64 accumulation, FP64 Zeeman term, compensated FP64 energy sum). Every form meets
  * the FP64 solver tolerance; forms 1-3 run the minimiser's Steihaug loop on FP32 vectors (r, z,
